@@ -1,0 +1,163 @@
+"""ctypes mirror of include/aph.h, the C-ABI of libaph.so.
+
+The structures below have exactly the C layouts of aph.h; numpy structured
+dtypes with the same layouts are provided for arrays of rays, hits and
+instances.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libaph.so")
+
+APH_OK = 0
+APH_E_INVALID = -1
+APH_E_DEVICE = -2
+APH_E_MEMORY = -3
+APH_E_STATE = -4
+APH_E_UNSUPPORTED = -5
+APH_E_OVERFLOW = -6
+
+APH_BUILDER_LBVH = 0
+APH_BUILDER_SAH = 1
+APH_MODE_MEGAKERNEL = 0
+APH_MODE_WAVEFRONT = 1
+
+RAY_DTYPE = np.dtype([("org", "<f4", 3), ("tmin", "<f4"), ("dir", "<f4", 3), ("tmax", "<f4")])
+HIT_DTYPE = np.dtype([("t", "<f4"), ("inst", "<u4"), ("prim", "<u4"), ("u", "<f4"), ("v", "<f4")])
+INSTANCE_DTYPE = np.dtype([("o2w", "<f4", 12), ("w2o", "<f4", 12), ("mesh", "<u4"), ("material", "<u4"),
+                           ("mask", "<u4"), ("flags", "<u4")])
+ALIAS_DTYPE = np.dtype([("alias", "<u4"), ("prob", "<f4"), ("pdf", "<f4"), ("pad", "<f4")])
+assert RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 20 and INSTANCE_DTYPE.itemsize == 112
+
+
+class Material(C.Structure):
+    """aph_material: MaterialConstants of DefaultMaterialUniformBuffer.slang:3-72 (400 B)."""
+    _fields_ = [
+        ("base", C.c_float), ("base_color", C.c_float * 3), ("diffuse_roughness", C.c_float),
+        ("metalness", C.c_float), ("specular", C.c_float), ("_pad0", C.c_int32),
+        ("specular_color", C.c_float * 3), ("specular_roughness", C.c_float), ("specular_ior", C.c_float),
+        ("specular_anisotropy", C.c_float), ("specular_rotation", C.c_float), ("transmission", C.c_float),
+        ("transmission_color", C.c_float * 3), ("subsurface", C.c_float), ("subsurface_color", C.c_float * 3),
+        ("_pad1", C.c_int32), ("subsurface_radius", C.c_float * 3), ("subsurface_scale", C.c_float),
+        ("subsurface_anisotropy", C.c_float), ("sheen", C.c_float), ("_pad2", C.c_int32), ("_pad3", C.c_int32),
+        ("sheen_color", C.c_float * 3), ("sheen_roughness", C.c_float), ("coat", C.c_float),
+        ("coat_color", C.c_float * 3), ("coat_roughness", C.c_float), ("coat_anisotropy", C.c_float),
+        ("coat_rotation", C.c_float), ("coat_ior", C.c_float), ("coat_affect_color", C.c_float),
+        ("coat_affect_roughness", C.c_float), ("emission", C.c_float), ("_pad4", C.c_int32),
+        ("emission_color", C.c_float * 3), ("_pad5", C.c_int32), ("opacity", C.c_float * 3),
+        ("thin_walled", C.c_int32), ("texture_block", C.c_int32 * 44),
+    ]
+
+
+assert C.sizeof(Material) == 400
+
+
+class Environment(C.Structure):
+    _fields_ = [
+        ("light_top", C.c_float * 3), ("light_bottom", C.c_float * 3),
+        ("background_top", C.c_float * 3), ("background_bottom", C.c_float * 3),
+        ("light_transform", C.c_float * 16), ("background_transform", C.c_float * 16),
+        ("background_use_screen", C.c_int32), ("filter", C.c_int32),
+        ("light_rgb", C.POINTER(C.c_float)), ("light_width", C.c_uint32), ("light_height", C.c_uint32),
+        ("background_rgb", C.POINTER(C.c_float)), ("background_width", C.c_uint32),
+        ("background_height", C.c_uint32),
+    ]
+
+
+class DistantLight(C.Structure):
+    _fields_ = [("color_intensity", C.c_float * 4), ("direction", C.c_float * 3), ("cos_radius", C.c_float)]
+
+
+class Frame(C.Structure):
+    _fields_ = [
+        ("inv_view", C.c_float * 16), ("view_size", C.c_float * 2), ("jitter", C.c_float * 2),
+        ("is_ortho", C.c_int32), ("focal_distance", C.c_float), ("lens_radius", C.c_float),
+        ("trace_depth", C.c_int32), ("max_luminance", C.c_float), ("rr_enabled", C.c_int32),
+        ("rr_start_depth", C.c_int32), ("force_opaque_shadows", C.c_int32), ("display_errors", C.c_int32),
+        ("temporal_jitter", C.c_int32), ("light_count", C.c_int32), ("lights", DistantLight * 4),
+    ]
+
+
+class Request(C.Structure):
+    _fields_ = [("target", C.c_uint64), ("tlas", C.c_uint64), ("frame", Frame), ("sample_start", C.c_uint32),
+                ("sample_count", C.c_uint32), ("seed_offset", C.c_uint32), ("_pad", C.c_uint32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("node_visits", C.c_uint64),
+                ("tri_tests", C.c_uint64), ("inst_visits", C.c_uint64), ("paths", C.c_uint64)]
+
+
+_P = C.c_void_p
+_u32 = C.c_uint32
+_u64 = C.c_uint64
+_i = C.c_int
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "aph_version": (_i, []),
+    "aph_create": (_i, [_i, C.POINTER(_P)]),
+    "aph_destroy": (None, [_P]),
+    "aph_last_error": (C.c_char_p, [_P]),
+    "aph_get_stream": (_i, [_P, C.POINTER(_P)]),
+    "aph_mesh_upload": (_i, [_P, _P, _u32, _P, _u32, _P, _P, C.POINTER(_u64)]),
+    "aph_blas_build": (_i, [_P, _u64, _i]),
+    "aph_tlas_build": (_i, [_P, _P, _u32, C.POINTER(_u64)]),
+    "aph_inverse3x4": (_i, [_P, _P]),
+    "aph_materials_upload": (_i, [_P, _P, _u32]),
+    "aph_material_defaults": (None, [C.POINTER(Material)]),
+    "aph_environment_upload": (_i, [_P, C.POINTER(Environment)]),
+    "aph_alias_build": (_i, [_P, _u32, _u32, _P, C.POINTER(C.c_float)]),
+    "aph_trace_closest": (_i, [_P, _u64, _P, _u32, _P]),
+    "aph_trace_any": (_i, [_P, _u64, _P, _u32, _P]),
+    "aph_trace_closest_device": (_i, [_P, _u64, _P, _u32, _P]),
+    "aph_trace_any_device": (_i, [_P, _u64, _P, _u32, _P]),
+    "aph_target_create": (_i, [_P, _u32, _u32, C.POINTER(_u64)]),
+    "aph_target_destroy": (_i, [_P, _u64]),
+    "aph_target_device_ptr": (_i, [_P, _u64, C.POINTER(_P)]),
+    "aph_render": (_i, [_P, _P, _u32, _i]),
+    "aph_wait": (_i, [_P]),
+    "aph_readback": (_i, [_P, _u64, _P, C.c_size_t]),
+    "aph_post": (_i, [_P, _u64, _P, _i, _i, _P, C.c_size_t]),
+    "aph_set_stats_level": (_i, [_P, _i]),
+    "aph_stats_get": (_i, [_P, C.POINTER(Stats)]),
+    "aph_stats_reset": (_i, [_P]),
+    "aph_timing_enable": (_i, [_P, _i]),
+    "aph_timing_get": (_i, [_P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(_u64)]),
+    "aph_timing_reset": (_i, [_P]),
+}
+
+_lib = None
+
+
+def load():
+    """Loads libaph.so; raises when the HIP backend has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"HIP backend library missing: {LIB_PATH} (run __graft_entry__.build())")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def ptr(a):
+    """Pointer to a contiguous numpy array (or None)."""
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def default_material():
+    m = Material()
+    load().aph_material_defaults(C.byref(m))
+    return m

@@ -1,0 +1,885 @@
+// C-ABI of the gfx950 path-tracing backend (include/aph.h): context,
+// device-resident scene objects, ray queries, render requests, readback.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/aph.h"
+#include "alias.h"
+#include "bvh_build.h"
+#include "kernels.h"
+#include "wavefront.h"
+
+using namespace aph;
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() { reset(); }
+    void reset()
+    {
+        if (p) hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    hipError_t alloc(size_t n)
+    {
+        reset();
+        if (n == 0) n = 16;
+        hipError_t e = hipMalloc(&p, n);
+        if (e == hipSuccess) bytes = n;
+        else p = nullptr;
+        return e;
+    }
+    // Grows (never shrinks) a scratch buffer.
+    hipError_t reserve(size_t n)
+    {
+        if (bytes >= n && p) return hipSuccess;
+        return alloc(n);
+    }
+    template <class T> T* as() const { return (T*)p; }
+};
+
+struct Mesh {
+    uint32_t nv = 0, nt = 0;
+    bool hasNrm = false;
+    DevBuf pos, nrm, uv, idx, nodes, tris;
+    bool built = false;
+    BuildInfo info{};
+};
+
+struct Tlas {
+    uint32_t n = 0, nvalid = 0;
+    DevBuf nodes, trav, shade;
+    TraceScene sc{};
+    uint32_t depth = 0;
+};
+
+struct Target {
+    uint32_t w = 0, h = 0;
+    DevBuf accum;
+};
+
+struct Timing {
+    double ms = 0.0;
+    uint64_t launches = 0;
+};
+
+}  // namespace
+
+struct aph_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    uint64_t nextHandle = 1;
+    std::map<uint64_t, std::unique_ptr<Mesh>> meshes;
+    std::map<uint64_t, std::unique_ptr<Tlas>> tlases;
+    std::map<uint64_t, std::unique_ptr<Target>> targets;
+    DevBuf mats;
+    uint32_t nmat = 0;
+    DevBuf envLight, envAlias, envBg;
+    DevEnv env{};
+    DevBuf counters;
+    int statsLevel = 0;
+    DevBuf scratchA, scratchB;
+    WavefrontState wf;
+    bool timing = false;
+    std::map<std::string, Timing> timings;
+    std::vector<PendingEvent> pending;
+};
+
+namespace {
+
+int fail(aph_ctx* c, int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+int hipFail(aph_ctx* c, hipError_t e, const char* what)
+{
+    return fail(c, e == hipErrorOutOfMemory ? APH_E_MEMORY : APH_E_DEVICE, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define HIPCALL(ctx, x, what)                              \
+    do {                                                   \
+        hipError_t e_ = (x);                               \
+        if (e_ != hipSuccess) return hipFail(ctx, e_, what); \
+    } while (0)
+
+int enter(aph_ctx* c)
+{
+    if (!c) return APH_E_INVALID;
+    hipError_t e = hipSetDevice(c->device);
+    if (e != hipSuccess) return hipFail(c, e, "hipSetDevice");
+    c->err.clear();
+    return APH_OK;
+}
+
+void resolveTimings(aph_ctx* c)
+{
+    for (auto& p : c->pending) {
+        hipEventSynchronize(p.b);
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, p.a, p.b);
+        Timing& t = c->timings[p.name];
+        t.ms += ms;
+        t.launches++;
+        hipEventDestroy(p.a);
+        hipEventDestroy(p.b);
+    }
+    c->pending.clear();
+}
+
+struct TimedScope {
+    aph_ctx* c;
+    const char* name;
+    hipEvent_t a = nullptr, b = nullptr;
+    TimedScope(aph_ctx* ctx, const char* n) : c(ctx), name(n)
+    {
+        if (c->timing) {
+            hipEventCreate(&a);
+            hipEventCreate(&b);
+            hipEventRecord(a, c->stream);
+        }
+    }
+    ~TimedScope()
+    {
+        if (c->timing) {
+            hipEventRecord(b, c->stream);
+            c->pending.push_back(PendingEvent{name, a, b});
+        }
+    }
+};
+
+// Double-precision 4x4 inverse (row-major), rounded to float.
+bool inverse4(const float* mf, float* out)
+{
+    double m[16], inv[16];
+    for (int i = 0; i < 16; i++) m[i] = mf[i];
+    inv[0] = m[5] * m[10] * m[15] - m[5] * m[11] * m[14] - m[9] * m[6] * m[15] + m[9] * m[7] * m[14] +
+             m[13] * m[6] * m[11] - m[13] * m[7] * m[10];
+    inv[4] = -m[4] * m[10] * m[15] + m[4] * m[11] * m[14] + m[8] * m[6] * m[15] - m[8] * m[7] * m[14] -
+             m[12] * m[6] * m[11] + m[12] * m[7] * m[10];
+    inv[8] = m[4] * m[9] * m[15] - m[4] * m[11] * m[13] - m[8] * m[5] * m[15] + m[8] * m[7] * m[13] +
+             m[12] * m[5] * m[11] - m[12] * m[7] * m[9];
+    inv[12] = -m[4] * m[9] * m[14] + m[4] * m[10] * m[13] + m[8] * m[5] * m[14] - m[8] * m[6] * m[13] -
+              m[12] * m[5] * m[10] + m[12] * m[6] * m[9];
+    inv[1] = -m[1] * m[10] * m[15] + m[1] * m[11] * m[14] + m[9] * m[2] * m[15] - m[9] * m[3] * m[14] -
+             m[13] * m[2] * m[11] + m[13] * m[3] * m[10];
+    inv[5] = m[0] * m[10] * m[15] - m[0] * m[11] * m[14] - m[8] * m[2] * m[15] + m[8] * m[3] * m[14] +
+             m[12] * m[2] * m[11] - m[12] * m[3] * m[10];
+    inv[9] = -m[0] * m[9] * m[15] + m[0] * m[11] * m[13] + m[8] * m[1] * m[15] - m[8] * m[3] * m[13] -
+             m[12] * m[1] * m[11] + m[12] * m[3] * m[9];
+    inv[13] = m[0] * m[9] * m[14] - m[0] * m[10] * m[13] - m[8] * m[1] * m[14] + m[8] * m[2] * m[13] +
+              m[12] * m[1] * m[10] - m[12] * m[2] * m[9];
+    inv[2] = m[1] * m[6] * m[15] - m[1] * m[7] * m[14] - m[5] * m[2] * m[15] + m[5] * m[3] * m[14] +
+             m[13] * m[2] * m[7] - m[13] * m[3] * m[6];
+    inv[6] = -m[0] * m[6] * m[15] + m[0] * m[7] * m[14] + m[4] * m[2] * m[15] - m[4] * m[3] * m[14] -
+             m[12] * m[2] * m[7] + m[12] * m[3] * m[6];
+    inv[10] = m[0] * m[5] * m[15] - m[0] * m[7] * m[13] - m[4] * m[1] * m[15] + m[4] * m[3] * m[13] +
+              m[12] * m[1] * m[7] - m[12] * m[3] * m[5];
+    inv[14] = -m[0] * m[5] * m[14] + m[0] * m[6] * m[13] + m[4] * m[1] * m[14] - m[4] * m[2] * m[13] -
+              m[12] * m[1] * m[6] + m[12] * m[2] * m[5];
+    inv[3] = -m[1] * m[6] * m[11] + m[1] * m[7] * m[10] + m[5] * m[2] * m[11] - m[5] * m[3] * m[10] -
+             m[9] * m[2] * m[7] + m[9] * m[3] * m[6];
+    inv[7] = m[0] * m[6] * m[11] - m[0] * m[7] * m[10] - m[4] * m[2] * m[11] + m[4] * m[3] * m[10] +
+             m[8] * m[2] * m[7] - m[8] * m[3] * m[6];
+    inv[11] = -m[0] * m[5] * m[11] + m[0] * m[7] * m[9] + m[4] * m[1] * m[11] - m[4] * m[3] * m[9] -
+              m[8] * m[1] * m[7] + m[8] * m[3] * m[5];
+    inv[15] = m[0] * m[5] * m[10] - m[0] * m[6] * m[9] - m[4] * m[1] * m[10] + m[4] * m[2] * m[9] +
+              m[8] * m[1] * m[6] - m[8] * m[2] * m[5];
+    double det = m[0] * inv[0] + m[1] * inv[4] + m[2] * inv[8] + m[3] * inv[12];
+    if (det == 0.0) return false;
+    for (int i = 0; i < 16; i++) out[i] = (float)(inv[i] / det);
+    return true;
+}
+
+void identity4(float* m)
+{
+    for (int i = 0; i < 16; i++) m[i] = (i % 5 == 0) ? 1.0f : 0.0f;
+}
+
+template <class T> T* lookup(std::map<uint64_t, std::unique_ptr<T>>& m, uint64_t h)
+{
+    auto it = m.find(h);
+    return it == m.end() ? nullptr : it->second.get();
+}
+
+void setDefaultEnv(aph_ctx* c)
+{
+    // EnvironmentBase.cpp:33-45 defaults.
+    DevEnv& e = c->env;
+    std::memset(&e, 0, sizeof(e));
+    e.lightTop = f3{1.0f, 1.0f, 1.0f};
+    e.lightBottom = f3{0.0f, 0.0f, 0.0f};
+    e.bgTop = f3{0.02f, 0.25f, 0.60f};
+    e.bgBottom = f3{0.32f, 0.79f, 1.0f};
+    identity4(e.lightXf);
+    identity4(e.lightXfInv);
+    identity4(e.bgXf);
+}
+
+DevMaterial toDev(const aph_material& a)
+{
+    DevMaterial m;
+    std::memset(&m, 0, sizeof(m));
+    auto v = [](const float* p) { return f3{p[0], p[1], p[2]}; };
+    m.base = a.base; m.baseColor = v(a.base_color);
+    m.diffuseRoughness = a.diffuse_roughness; m.metalness = a.metalness; m.specular = a.specular;
+    m.specularRoughness = a.specular_roughness; m.specularColor = v(a.specular_color); m.specularIOR = a.specular_ior;
+    m.specularAnisotropy = a.specular_anisotropy; m.specularRotation = a.specular_rotation;
+    m.transmission = a.transmission; m.subsurface = a.subsurface; m.transmissionColor = v(a.transmission_color);
+    m.sheen = a.sheen; m.subsurfaceColor = v(a.subsurface_color); m.sheenRoughness = a.sheen_roughness;
+    m.sheenColor = v(a.sheen_color); m.coat = a.coat; m.coatColor = v(a.coat_color);
+    m.coatRoughness = a.coat_roughness; m.coatAnisotropy = a.coat_anisotropy; m.coatRotation = a.coat_rotation;
+    m.coatIOR = a.coat_ior; m.coatAffectColor = a.coat_affect_color; m.coatAffectRoughness = a.coat_affect_roughness;
+    m.emission = a.emission; m.thinWalled = a.thin_walled; m.emissionColor = v(a.emission_color);
+    m.opacity = v(a.opacity);
+    return m;
+}
+
+}  // namespace
+
+extern "C" {
+
+int aph_version(void) { return APH_VERSION; }
+
+int aph_create(int device, aph_ctx_t* out)
+{
+    if (!out) return APH_E_INVALID;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return APH_E_DEVICE;
+    if (hipSetDevice(device) != hipSuccess) return APH_E_DEVICE;
+    std::unique_ptr<aph_ctx> c(new aph_ctx());
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return APH_E_DEVICE;
+    if (c->counters.alloc(sizeof(DevCounters)) != hipSuccess) return APH_E_MEMORY;
+    if (hipMemset(c->counters.p, 0, sizeof(DevCounters)) != hipSuccess) return APH_E_DEVICE;
+    setDefaultEnv(c.get());
+    aph_material dm;
+    aph_material_defaults(&dm);
+    DevMaterial d = toDev(dm);
+    if (c->mats.alloc(sizeof(DevMaterial)) != hipSuccess) return APH_E_MEMORY;
+    if (hipMemcpy(c->mats.p, &d, sizeof(d), hipMemcpyHostToDevice) != hipSuccess) return APH_E_DEVICE;
+    c->nmat = 1;
+    *out = c.release();
+    return APH_OK;
+}
+
+void aph_destroy(aph_ctx_t c)
+{
+    if (!c) return;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    resolveTimings(c);
+    c->meshes.clear();
+    c->tlases.clear();
+    c->targets.clear();
+    wavefrontRelease(c->wf);
+    hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* aph_last_error(aph_ctx_t c) { return c ? c->err.c_str() : "null context"; }
+
+int aph_get_stream(aph_ctx_t c, void** stream)
+{
+    if (!c || !stream) return APH_E_INVALID;
+    *stream = (void*)c->stream;
+    return APH_OK;
+}
+
+int aph_inverse3x4(const float mf[12], float out[12])
+{
+    if (!mf || !out) return APH_E_INVALID;
+    double m[12];
+    for (int i = 0; i < 12; i++) m[i] = (double)mf[i];
+    double a = m[0], b = m[1], cc = m[2], d = m[4], e = m[5], f = m[6], g = m[8], h = m[9], k = m[10];
+    double A = e * k - f * h, B = f * g - d * k, C = d * h - e * g;
+    double det = a * A + b * B + cc * C;
+    if (det == 0.0) {
+        std::memset(out, 0, 12 * sizeof(float));
+        return APH_E_INVALID;
+    }
+    double r[9] = {A / det,           (cc * h - b * k) / det, (b * f - cc * e) / det,
+                   B / det,           (a * k - cc * g) / det, (cc * d - a * f) / det,
+                   C / det,           (b * g - a * h) / det,  (a * e - b * d) / det};
+    double tx = m[3], ty = m[7], tz = m[11];
+    for (int i = 0; i < 3; i++) {
+        out[4 * i + 0] = (float)r[3 * i + 0];
+        out[4 * i + 1] = (float)r[3 * i + 1];
+        out[4 * i + 2] = (float)r[3 * i + 2];
+        out[4 * i + 3] = (float)(-((r[3 * i + 0] * tx + r[3 * i + 1] * ty) + r[3 * i + 2] * tz));
+    }
+    return APH_OK;
+}
+
+int aph_mesh_upload(aph_ctx_t c, const float* pos, uint32_t nv, const uint32_t* idx, uint32_t nt, const float* nrm,
+                    const float* uv, uint64_t* mesh)
+{
+    int rc = enter(c);
+    if (rc) return rc;
+    if (!mesh) return fail(c, APH_E_INVALID, "mesh output pointer is null");
+    *mesh = 0;
+    if (!pos || nv == 0) return fail(c, APH_E_INVALID, "positions are required (GeometryBase.cpp:84-87)");
+    if (nt > (kLeafFirstMask + 1u)) return fail(c, APH_E_INVALID, "too many triangles (%u)", nt);
+    std::vector<uint32_t> seq;
+    if (!idx) {
+        // No indices: sequential triangles (HGIGeometry.cpp:37-47).
+        if ((uint64_t)nt * 3u > nv) return fail(c, APH_E_INVALID, "non-indexed mesh needs 3*nt <= nv");
+        seq.resize((size_t)nt * 3);
+        for (size_t i = 0; i < seq.size(); i++) seq[i] = (uint32_t)i;
+        idx = seq.data();
+    }
+    for (size_t i = 0; i < (size_t)nt * 3; i++)
+        if (idx[i] >= nv) return fail(c, APH_E_INVALID, "index %zu = %u out of range (nv = %u)", i, idx[i], nv);
+    std::unique_ptr<Mesh> m(new Mesh());
+    m->nv = nv;
+    m->nt = nt;
+    m->hasNrm = nrm != nullptr;
+    HIPCALL(c, m->pos.alloc((size_t)nv * 12), "alloc positions");
+    HIPCALL(c, hipMemcpyAsync(m->pos.p, pos, (size_t)nv * 12, hipMemcpyHostToDevice, c->stream), "copy positions");
+    HIPCALL(c, m->idx.alloc((size_t)nt * 12), "alloc indices");
+    if (nt) HIPCALL(c, hipMemcpyAsync(m->idx.p, idx, (size_t)nt * 12, hipMemcpyHostToDevice, c->stream), "copy indices");
+    if (nrm) {
+        HIPCALL(c, m->nrm.alloc((size_t)nv * 12), "alloc normals");
+        HIPCALL(c, hipMemcpyAsync(m->nrm.p, nrm, (size_t)nv * 12, hipMemcpyHostToDevice, c->stream), "copy normals");
+    }
+    if (uv) {
+        HIPCALL(c, m->uv.alloc((size_t)nv * 8), "alloc uv");
+        HIPCALL(c, hipMemcpyAsync(m->uv.p, uv, (size_t)nv * 8, hipMemcpyHostToDevice, c->stream), "copy uv");
+    }
+    HIPCALL(c, hipStreamSynchronize(c->stream), "mesh upload");
+    uint64_t h = c->nextHandle++;
+    c->meshes[h] = std::move(m);
+    *mesh = h;
+    return APH_OK;
+}
+
+int aph_blas_build(aph_ctx_t c, uint64_t mesh, int builder)
+{
+    int rc = enter(c);
+    if (rc) return rc;
+    Mesh* m = lookup(c->meshes, mesh);
+    if (!m) return fail(c, APH_E_INVALID, "unknown mesh handle %llu", (unsigned long long)mesh);
+    if (builder != APH_BUILDER_LBVH && builder != APH_BUILDER_SAH)
+        return fail(c, APH_E_INVALID, "unknown builder %d", builder);
+    uint32_t n = m->nt;
+    TimedScope ts(c, "blas_build");
+    HIPCALL(c, m->nodes.alloc((size_t)(n > 1 ? n - 1 : 1) * sizeof(BvhNode)), "alloc BLAS nodes");
+    HIPCALL(c, m->tris.alloc((size_t)(n ? n : 1) * sizeof(Tri)), "alloc BLAS triangles");
+    if (n == 0) {
+        m->built = true;
+        m->info = BuildInfo{};
+        return APH_OK;
+    }
+    DevBuf lo, hi, sorted;
+    HIPCALL(c, lo.alloc((size_t)n * 16), "alloc boxes");
+    HIPCALL(c, hi.alloc((size_t)n * 16), "alloc boxes");
+    HIPCALL(c, sorted.alloc((size_t)n * 4), "alloc order");
+    HIPCALL(c, triBoxes(m->pos.as<float>(), m->idx.as<uint32_t>(), n, c->stream, lo.as<float4>(), hi.as<float4>()),
+            "triangle boxes");
+    BuildInfo info{};
+    HIPCALL(c, lbvhBuild(lo.as<float4>(), hi.as<float4>(), n, false, nullptr, c->stream, m->nodes.as<BvhNode>(),
+                         sorted.as<uint32_t>(), &info),
+            "BLAS build");
+    HIPCALL(c, reorderTris(m->pos.as<float>(), m->idx.as<uint32_t>(), sorted.as<uint32_t>(), n, c->stream,
+                           m->tris.as<Tri>()),
+            "reorder triangles");
+    HIPCALL(c, hipStreamSynchronize(c->stream), "BLAS build");
+    m->info = info;
+    m->built = true;
+    return APH_OK;
+}
+
+int aph_tlas_build(aph_ctx_t c, const aph_instance* inst, uint32_t n, uint64_t* tlas)
+{
+    int rc = enter(c);
+    if (rc) return rc;
+    if (!tlas) return fail(c, APH_E_INVALID, "tlas output pointer is null");
+    *tlas = 0;
+    if (n && !inst) return fail(c, APH_E_INVALID, "instances pointer is null");
+    std::unique_ptr<Tlas> t(new Tlas());
+    t->n = n;
+    std::vector<InstTrav> trav(n ? n : 1);
+    std::vector<InstShade> shade(n ? n : 1);
+    std::vector<float4> blo, bhi;
+    std::vector<uint32_t> remap;
+    double extent = 0.0, kappa = 1.0;
+    uint32_t maxBlasDepth = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        const aph_instance& in = inst[i];
+        Mesh* m = lookup(c->meshes, (uint64_t)in.mesh);
+        if (!m) return fail(c, APH_E_INVALID, "instance %u: unknown mesh handle %u", i, in.mesh);
+        if (!m->built) return fail(c, APH_E_STATE, "instance %u: BLAS of mesh %u not built", i, in.mesh);
+        float w2o[12];
+        bool zero = true;
+        for (int k = 0; k < 12; k++) zero = zero && in.w2o[k] == 0.0f;
+        if (zero) {
+            if (aph_inverse3x4(in.o2w, w2o) != APH_OK) return fail(c, APH_E_INVALID, "instance %u: singular transform", i);
+        } else {
+            std::memcpy(w2o, in.w2o, sizeof(w2o));
+        }
+        InstTrav& tv = trav[i];
+        std::memcpy(tv.w2o, w2o, sizeof(w2o));
+        tv.nodes = m->nodes.as<BvhNode>();
+        tv.tris = m->tris.as<Tri>();
+        InstShade& sh = shade[i];
+        std::memcpy(sh.o2w, in.o2w, sizeof(sh.o2w));
+        sh.pos = m->pos.as<float>();
+        sh.nrm = m->hasNrm ? m->nrm.as<float>() : nullptr;
+        sh.idx = m->idx.as<uint32_t>();
+        sh.material = in.material;
+        sh.flags = in.flags;
+        if (m->nt == 0) continue;
+        // condition number bound (infinity norm) of the linear part
+        double na = 0.0, nb = 0.0;
+        for (int r = 0; r < 3; r++) {
+            na = std::fmax(na, std::fabs(in.o2w[4 * r]) + std::fabs(in.o2w[4 * r + 1]) + std::fabs(in.o2w[4 * r + 2]));
+            nb = std::fmax(nb, std::fabs(w2o[4 * r]) + std::fabs(w2o[4 * r + 1]) + std::fabs(w2o[4 * r + 2]));
+        }
+        kappa = std::fmax(kappa, na * nb);
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int cn = 0; cn < 8; cn++) {
+            double p[3] = {(cn & 1) ? m->info.rootHi[0] : m->info.rootLo[0], (cn & 2) ? m->info.rootHi[1] : m->info.rootLo[1],
+                           (cn & 4) ? m->info.rootHi[2] : m->info.rootLo[2]};
+            for (int k = 0; k < 3; k++) {
+                const float* r = in.o2w + 4 * k;
+                double w = (double)r[0] * p[0] + (double)r[1] * p[1] + (double)r[2] * p[2] + (double)r[3];
+                lo[k] = std::fmin(lo[k], std::nextafter((float)w, -INFINITY));
+                hi[k] = std::fmax(hi[k], std::nextafter((float)w, INFINITY));
+            }
+        }
+        for (int k = 0; k < 3; k++) extent = std::fmax(extent, std::fmax(std::fabs(lo[k]), std::fabs(hi[k])));
+        blo.push_back(make_float4(lo[0], lo[1], lo[2], 0.0f));
+        bhi.push_back(make_float4(hi[0], hi[1], hi[2], 0.0f));
+        remap.push_back(i);
+        maxBlasDepth = std::max(maxBlasDepth, m->info.depth);
+    }
+    uint32_t nv = (uint32_t)blo.size();
+    t->nvalid = nv;
+    HIPCALL(c, t->trav.alloc(sizeof(InstTrav) * (n ? n : 1)), "alloc instances");
+    HIPCALL(c, t->shade.alloc(sizeof(InstShade) * (n ? n : 1)), "alloc instances");
+    HIPCALL(c, hipMemcpyAsync(t->trav.p, trav.data(), sizeof(InstTrav) * (n ? n : 1), hipMemcpyHostToDevice, c->stream),
+            "copy instances");
+    HIPCALL(c, hipMemcpyAsync(t->shade.p, shade.data(), sizeof(InstShade) * (n ? n : 1), hipMemcpyHostToDevice,
+                              c->stream),
+            "copy instances");
+    HIPCALL(c, t->nodes.alloc(sizeof(BvhNode) * (nv > 1 ? nv - 1 : 1)), "alloc TLAS nodes");
+    if (nv) {
+        TimedScope ts(c, "tlas_build");
+        DevBuf lo, hi, sorted, rm;
+        HIPCALL(c, lo.alloc((size_t)nv * 16), "alloc");
+        HIPCALL(c, hi.alloc((size_t)nv * 16), "alloc");
+        HIPCALL(c, sorted.alloc((size_t)nv * 4), "alloc");
+        HIPCALL(c, rm.alloc((size_t)nv * 4), "alloc");
+        HIPCALL(c, hipMemcpyAsync(lo.p, blo.data(), (size_t)nv * 16, hipMemcpyHostToDevice, c->stream), "copy");
+        HIPCALL(c, hipMemcpyAsync(hi.p, bhi.data(), (size_t)nv * 16, hipMemcpyHostToDevice, c->stream), "copy");
+        HIPCALL(c, hipMemcpyAsync(rm.p, remap.data(), (size_t)nv * 4, hipMemcpyHostToDevice, c->stream), "copy");
+        BuildInfo info{};
+        HIPCALL(c, lbvhBuild(lo.as<float4>(), hi.as<float4>(), nv, true, rm.as<uint32_t>(), c->stream,
+                             t->nodes.as<BvhNode>(), sorted.as<uint32_t>(), &info),
+                "TLAS build");
+        HIPCALL(c, hipStreamSynchronize(c->stream), "TLAS build");
+        t->depth = info.depth;
+        // Every pending sibling needs a stack slot: TLAS path + exit marker + BLAS path.
+        if (info.depth + 1 + maxBlasDepth >= (uint32_t)kStackCap)
+            return fail(c, APH_E_OVERFLOW, "BVH too deep for the traversal stack (TLAS %u + BLAS %u >= %d)",
+                        info.depth, maxBlasDepth, kStackCap - 1);
+    }
+    HIPCALL(c, hipStreamSynchronize(c->stream), "TLAS upload");
+    t->sc.tlas = t->nodes.as<BvhNode>();
+    t->sc.inst = t->trav.as<InstTrav>();
+    t->sc.extent = (float)(3.0 * extent);
+    t->sc.padScale = (float)(1e-5 * kappa);
+    t->sc.ninst = nv;
+    uint64_t h = c->nextHandle++;
+    c->tlases[h] = std::move(t);
+    *tlas = h;
+    return APH_OK;
+}
+
+void aph_material_defaults(aph_material* m)
+{
+    // Standard Surface defaults, MaterialBase.cpp:109-167.
+    if (!m) return;
+    std::memset(m, 0, sizeof(*m));
+    m->base = 0.8f;
+    for (int i = 0; i < 3; i++) {
+        m->base_color[i] = 1.0f;
+        m->specular_color[i] = 1.0f;
+        m->transmission_color[i] = 1.0f;
+        m->subsurface_color[i] = 1.0f;
+        m->subsurface_radius[i] = 1.0f;
+        m->sheen_color[i] = 1.0f;
+        m->coat_color[i] = 1.0f;
+        m->emission_color[i] = 1.0f;
+        m->opacity[i] = 1.0f;
+    }
+    m->specular = 1.0f;
+    m->specular_roughness = 0.2f;
+    m->specular_ior = 1.5f;
+    m->subsurface_scale = 1.0f;
+    m->sheen_roughness = 0.3f;
+    m->coat_roughness = 0.1f;
+    m->coat_ior = 1.5f;
+}
+
+int aph_materials_upload(aph_ctx_t c, const aph_material* mats, uint32_t n)
+{
+    int rc = enter(c);
+    if (rc) return rc;
+    if (!mats || n == 0) return fail(c, APH_E_INVALID, "need at least one material");
+    std::vector<DevMaterial> d(n);
+    for (uint32_t i = 0; i < n; i++) {
+        // has_*_image flags at byte offsets 224, 260, 292, 324, 356
+        const int texFlags[5] = {0, 9, 17, 25, 33};
+        for (int f : texFlags)
+            if (mats[i].texture_block[f] != 0)
+                return fail(c, APH_E_UNSUPPORTED, "material %u: image textures are not supported yet", i);
+        d[i] = toDev(mats[i]);
+    }
+    HIPCALL(c, hipStreamSynchronize(c->stream), "materials");
+    HIPCALL(c, c->mats.alloc(sizeof(DevMaterial) * n), "alloc materials");
+    HIPCALL(c, hipMemcpy(c->mats.p, d.data(), sizeof(DevMaterial) * n, hipMemcpyHostToDevice), "copy materials");
+    c->nmat = n;
+    return APH_OK;
+}
+
+int aph_alias_build(const float* rgb, uint32_t w, uint32_t h, void* entries, float* integral)
+{
+    if (!rgb || !entries || !integral || w == 0 || h == 0) return APH_E_INVALID;
+    aliasBuild(rgb, w, h, (AliasEntry*)entries, integral);
+    return APH_OK;
+}
+
+int aph_environment_upload(aph_ctx_t c, const aph_environment* e)
+{
+    int rc = enter(c);
+    if (rc) return rc;
+    if (!e) return fail(c, APH_E_INVALID, "environment is null");
+    if (e->filter != 0 && e->filter != 1) return fail(c, APH_E_INVALID, "filter must be 0 or 1");
+    HIPCALL(c, hipStreamSynchronize(c->stream), "environment");
+    DevEnv d;
+    std::memset(&d, 0, sizeof(d));
+    d.lightTop = f3{e->light_top[0], e->light_top[1], e->light_top[2]};
+    d.lightBottom = f3{e->light_bottom[0], e->light_bottom[1], e->light_bottom[2]};
+    d.bgTop = f3{e->background_top[0], e->background_top[1], e->background_top[2]};
+    d.bgBottom = f3{e->background_bottom[0], e->background_bottom[1], e->background_bottom[2]};
+    std::memcpy(d.lightXf, e->light_transform, sizeof(d.lightXf));
+    std::memcpy(d.bgXf, e->background_transform, sizeof(d.bgXf));
+    if (!inverse4(e->light_transform, d.lightXfInv)) return fail(c, APH_E_INVALID, "light transform is singular");
+    d.bgUseScreen = e->background_use_screen;
+    d.filter = e->filter;
+    if (e->light_rgb) {
+        uint32_t w = e->light_width, h = e->light_height;
+        if (w == 0 || h == 0) return fail(c, APH_E_INVALID, "light image has zero size");
+        size_t n = (size_t)w * h;
+        std::vector<float4> tex(n);
+        for (size_t i = 0; i < n; i++)
+            tex[i] = make_float4(e->light_rgb[3 * i], e->light_rgb[3 * i + 1], e->light_rgb[3 * i + 2], 1.0f);
+        std::vector<AliasEntry> alias(n);
+        float integral = 0.0f;
+        aliasBuild(e->light_rgb, w, h, alias.data(), &integral);
+        HIPCALL(c, c->envLight.alloc(n * 16), "alloc light image");
+        HIPCALL(c, c->envAlias.alloc(n * 16), "alloc alias map");
+        HIPCALL(c, hipMemcpy(c->envLight.p, tex.data(), n * 16, hipMemcpyHostToDevice), "copy light image");
+        HIPCALL(c, hipMemcpy(c->envAlias.p, alias.data(), n * 16, hipMemcpyHostToDevice), "copy alias map");
+        d.hasLightTex = 1;
+        d.lightTex = c->envLight.as<float4>();
+        d.alias = c->envAlias.as<AliasEntry>();
+        d.lw = w;
+        d.lh = h;
+        d.lightIntegral = integral;
+    }
+    if (e->background_rgb) {
+        uint32_t w = e->background_width, h = e->background_height;
+        if (w == 0 || h == 0) return fail(c, APH_E_INVALID, "background image has zero size");
+        size_t n = (size_t)w * h;
+        std::vector<float4> tex(n);
+        for (size_t i = 0; i < n; i++)
+            tex[i] = make_float4(e->background_rgb[3 * i], e->background_rgb[3 * i + 1], e->background_rgb[3 * i + 2],
+                                 1.0f);
+        HIPCALL(c, c->envBg.alloc(n * 16), "alloc background image");
+        HIPCALL(c, hipMemcpy(c->envBg.p, tex.data(), n * 16, hipMemcpyHostToDevice), "copy background image");
+        d.hasBgTex = 1;
+        d.bgTex = c->envBg.as<float4>();
+        d.bw = w;
+        d.bh = h;
+    }
+    c->env = d;
+    return APH_OK;
+}
+
+static int traceCommon(aph_ctx_t c, uint64_t tlas, const aph_ray* rays, uint32_t n, aph_hit* hits, uint8_t* occ,
+                       bool any, bool device)
+{
+    int rc = enter(c);
+    if (rc) return rc;
+    Tlas* t = lookup(c->tlases, tlas);
+    if (!t) return fail(c, APH_E_INVALID, "unknown tlas handle %llu", (unsigned long long)tlas);
+    if (n == 0) return APH_OK;
+    if (!rays || (any ? !occ : !hits)) return fail(c, APH_E_INVALID, "null ray or result pointer");
+    const aph_ray* dr = rays;
+    aph_hit* dh = hits;
+    uint8_t* doc = occ;
+    if (!device) {
+        HIPCALL(c, c->scratchA.reserve((size_t)n * sizeof(aph_ray)), "alloc rays");
+        HIPCALL(c, c->scratchB.reserve((size_t)n * (any ? 1 : sizeof(aph_hit))), "alloc hits");
+        HIPCALL(c, hipMemcpyAsync(c->scratchA.p, rays, (size_t)n * sizeof(aph_ray), hipMemcpyHostToDevice, c->stream),
+                "copy rays");
+        dr = c->scratchA.as<aph_ray>();
+        dh = any ? nullptr : c->scratchB.as<aph_hit>();
+        doc = any ? c->scratchB.as<uint8_t>() : nullptr;
+    }
+    {
+        TimedScope ts(c, any ? "trace_any" : "trace_closest");
+        HIPCALL(c, launchTrace(t->sc, dr, n, dh, doc, c->counters.as<DevCounters>(), any, c->statsLevel > 0, c->stream),
+                "trace launch");
+    }
+    if (!device) {
+        if (any) HIPCALL(c, hipMemcpyAsync(occ, doc, n, hipMemcpyDeviceToHost, c->stream), "copy occlusion");
+        else HIPCALL(c, hipMemcpyAsync(hits, dh, (size_t)n * sizeof(aph_hit), hipMemcpyDeviceToHost, c->stream),
+                     "copy hits");
+        HIPCALL(c, hipStreamSynchronize(c->stream), "trace");
+        DevCounters k;
+        HIPCALL(c, hipMemcpy(&k, c->counters.p, sizeof(k), hipMemcpyDeviceToHost), "counters");
+        if (k.overflow) return fail(c, APH_E_OVERFLOW, "traversal stack overflow on %llu rays", k.overflow);
+    }
+    return APH_OK;
+}
+
+int aph_trace_closest(aph_ctx_t c, uint64_t tlas, const aph_ray* rays, uint32_t n, aph_hit* hits)
+{
+    return traceCommon(c, tlas, rays, n, hits, nullptr, false, false);
+}
+int aph_trace_any(aph_ctx_t c, uint64_t tlas, const aph_ray* rays, uint32_t n, uint8_t* occluded)
+{
+    return traceCommon(c, tlas, rays, n, nullptr, occluded, true, false);
+}
+int aph_trace_closest_device(aph_ctx_t c, uint64_t tlas, const aph_ray* d_rays, uint32_t n, aph_hit* d_hits)
+{
+    return traceCommon(c, tlas, d_rays, n, d_hits, nullptr, false, true);
+}
+int aph_trace_any_device(aph_ctx_t c, uint64_t tlas, const aph_ray* d_rays, uint32_t n, uint8_t* d_occluded)
+{
+    return traceCommon(c, tlas, d_rays, n, nullptr, d_occluded, true, true);
+}
+
+int aph_target_create(aph_ctx_t c, uint32_t width, uint32_t height, uint64_t* target)
+{
+    int rc = enter(c);
+    if (rc) return rc;
+    if (!target) return fail(c, APH_E_INVALID, "target output pointer is null");
+    *target = 0;
+    if (width == 0 || height == 0 || width > 16384 || height > 16384)
+        return fail(c, APH_E_INVALID, "bad target size %ux%u", width, height);
+    std::unique_ptr<Target> t(new Target());
+    t->w = width;
+    t->h = height;
+    HIPCALL(c, t->accum.alloc((size_t)width * height * 16), "alloc target");
+    HIPCALL(c, hipMemsetAsync(t->accum.p, 0, (size_t)width * height * 16, c->stream), "clear target");
+    uint64_t h = c->nextHandle++;
+    c->targets[h] = std::move(t);
+    *target = h;
+    return APH_OK;
+}
+
+int aph_target_destroy(aph_ctx_t c, uint64_t target)
+{
+    int rc = enter(c);
+    if (rc) return rc;
+    if (!lookup(c->targets, target)) return fail(c, APH_E_INVALID, "unknown target");
+    HIPCALL(c, hipStreamSynchronize(c->stream), "target destroy");
+    c->targets.erase(target);
+    return APH_OK;
+}
+
+int aph_target_device_ptr(aph_ctx_t c, uint64_t target, void** ptr)
+{
+    int rc = enter(c);
+    if (rc) return rc;
+    Target* t = lookup(c->targets, target);
+    if (!t || !ptr) return fail(c, APH_E_INVALID, "unknown target or null pointer");
+    *ptr = t->accum.p;
+    return APH_OK;
+}
+
+int aph_render(aph_ctx_t c, const aph_request* reqs, uint32_t nreq, int mode)
+{
+    int rc = enter(c);
+    if (rc) return rc;
+    if (nreq == 0) return APH_OK;
+    if (!reqs) return fail(c, APH_E_INVALID, "requests pointer is null");
+    if (mode != APH_MODE_MEGAKERNEL && mode != APH_MODE_WAVEFRONT) return fail(c, APH_E_INVALID, "unknown mode %d", mode);
+    std::vector<WavefrontRequest> wreq;
+    for (uint32_t i = 0; i < nreq; i++) {
+        const aph_request& r = reqs[i];
+        Target* tg = lookup(c->targets, r.target);
+        Tlas* tl = lookup(c->tlases, r.tlas);
+        if (!tg) return fail(c, APH_E_INVALID, "request %u: unknown target", i);
+        if (!tl) return fail(c, APH_E_INVALID, "request %u: unknown tlas", i);
+        if ((uint64_t)r.sample_start + r.sample_count > 0xFFFFFFFFull)
+            return fail(c, APH_E_INVALID, "request %u: sample range overflows", i);
+        if (r.frame.light_count < 0 || r.frame.light_count > 4)
+            return fail(c, APH_E_INVALID, "request %u: light_count must be in [0,4]", i);
+        aph_frame fp = r.frame;
+        fp.trace_depth = fp.trace_depth < 1 ? 1 : (fp.trace_depth > 10 ? 10 : fp.trace_depth);  // RendererBase.cpp:265
+        if (fp.rr_start_depth < 0) fp.rr_start_depth = 0;                                         // :274-275
+        RenderScene rs;
+        rs.sc = tl->sc;
+        rs.ishade = tl->shade.as<InstShade>();
+        rs.mats = c->mats.as<DevMaterial>();
+        rs.nmat = c->nmat;
+        rs.pad_ = 0;
+        rs.env = c->env;
+        if (mode == APH_MODE_MEGAKERNEL) {
+            TimedScope ts(c, "render_megakernel");
+            HIPCALL(c, launchRenderMega(rs, fp, tg->w, tg->h, r.sample_start, r.sample_count, r.seed_offset,
+                                        tg->accum.as<float4>(), c->counters.as<DevCounters>(), c->statsLevel > 0,
+                                        c->stream),
+                    "megakernel launch");
+        } else {
+            WavefrontRequest w;
+            w.rs = rs;
+            w.frame = fp;
+            w.width = tg->w;
+            w.height = tg->h;
+            w.sampleStart = r.sample_start;
+            w.sampleCount = r.sample_count;
+            w.seedOffset = r.seed_offset;
+            w.accum = tg->accum.as<float4>();
+            wreq.push_back(w);
+        }
+    }
+    if (mode == APH_MODE_WAVEFRONT) {
+        hipError_t e = wavefrontRender(c->wf, wreq.data(), (uint32_t)wreq.size(), c->counters.as<DevCounters>(),
+                                       c->statsLevel > 0, c->stream, c->timing ? &c->pending : nullptr);
+        if (e != hipSuccess) return hipFail(c, e, "wavefront render");
+    }
+    return APH_OK;
+}
+
+int aph_wait(aph_ctx_t c)
+{
+    int rc = enter(c);
+    if (rc) return rc;
+    HIPCALL(c, hipStreamSynchronize(c->stream), "wait");
+    DevCounters k;
+    HIPCALL(c, hipMemcpy(&k, c->counters.p, sizeof(k), hipMemcpyDeviceToHost), "counters");
+    if (k.overflow) return fail(c, APH_E_OVERFLOW, "traversal stack overflow on %llu rays", k.overflow);
+    return APH_OK;
+}
+
+int aph_readback(aph_ctx_t c, uint64_t target, void* dst, size_t bytes)
+{
+    int rc = enter(c);
+    if (rc) return rc;
+    Target* t = lookup(c->targets, target);
+    if (!t || !dst) return fail(c, APH_E_INVALID, "unknown target or null destination");
+    size_t need = (size_t)t->w * t->h * 16;
+    if (bytes < need) return fail(c, APH_E_INVALID, "destination too small (%zu < %zu)", bytes, need);
+    HIPCALL(c, hipMemcpyAsync(dst, t->accum.p, need, hipMemcpyDeviceToHost, c->stream), "readback");
+    HIPCALL(c, hipStreamSynchronize(c->stream), "readback");
+    return APH_OK;
+}
+
+int aph_post(aph_ctx_t c, uint64_t target, const float brightness[3], int tone_map, int gamma, float* dst, size_t bytes)
+{
+    int rc = enter(c);
+    if (rc) return rc;
+    Target* t = lookup(c->targets, target);
+    if (!t || !dst || !brightness) return fail(c, APH_E_INVALID, "unknown target or null pointer");
+    size_t n = (size_t)t->w * t->h;
+    if (bytes < n * 16) return fail(c, APH_E_INVALID, "destination too small");
+    HIPCALL(c, c->scratchB.reserve(n * 16), "alloc post");
+    {
+        TimedScope ts(c, "post");
+        HIPCALL(c, launchPost(t->accum.as<float4>(), (uint32_t)n, brightness, tone_map, gamma, c->scratchB.as<float4>(),
+                              c->stream),
+                "post launch");
+    }
+    HIPCALL(c, hipMemcpyAsync(dst, c->scratchB.p, n * 16, hipMemcpyDeviceToHost, c->stream), "post readback");
+    HIPCALL(c, hipStreamSynchronize(c->stream), "post");
+    return APH_OK;
+}
+
+int aph_set_stats_level(aph_ctx_t c, int level)
+{
+    if (!c) return APH_E_INVALID;
+    if (level < 0 || level > 1) return fail(c, APH_E_INVALID, "stats level must be 0 or 1");
+    c->statsLevel = level;
+    return APH_OK;
+}
+
+int aph_stats_get(aph_ctx_t c, aph_stats* out)
+{
+    int rc = enter(c);
+    if (rc) return rc;
+    if (!out) return fail(c, APH_E_INVALID, "null stats pointer");
+    HIPCALL(c, hipStreamSynchronize(c->stream), "stats");
+    DevCounters k;
+    HIPCALL(c, hipMemcpy(&k, c->counters.p, sizeof(k), hipMemcpyDeviceToHost), "stats");
+    out->extension_rays = k.ext;
+    out->shadow_rays = k.shadow;
+    out->node_visits = k.nodes;
+    out->tri_tests = k.tris;
+    out->inst_visits = k.insts;
+    out->paths = k.paths;
+    return APH_OK;
+}
+
+int aph_stats_reset(aph_ctx_t c)
+{
+    int rc = enter(c);
+    if (rc) return rc;
+    HIPCALL(c, hipStreamSynchronize(c->stream), "stats reset");
+    HIPCALL(c, hipMemset(c->counters.p, 0, sizeof(DevCounters)), "stats reset");
+    return APH_OK;
+}
+
+int aph_timing_enable(aph_ctx_t c, int enable)
+{
+    if (!c) return APH_E_INVALID;
+    c->timing = enable != 0;
+    return APH_OK;
+}
+
+int aph_timing_get(aph_ctx_t c, const char* kernel, double* total_ms, uint64_t* launches)
+{
+    int rc = enter(c);
+    if (rc) return rc;
+    if (!kernel || !total_ms || !launches) return fail(c, APH_E_INVALID, "null pointer");
+    resolveTimings(c);
+    auto it = c->timings.find(kernel);
+    *total_ms = it == c->timings.end() ? 0.0 : it->second.ms;
+    *launches = it == c->timings.end() ? 0 : it->second.launches;
+    return APH_OK;
+}
+
+int aph_timing_reset(aph_ctx_t c)
+{
+    int rc = enter(c);
+    if (rc) return rc;
+    resolveTimings(c);
+    c->timings.clear();
+    return APH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,260 @@
+// One pixel-sample of the Aurora path tracer on gfx950: the RayGenShader
+// loop (Shaders/MainEntryPoints.slang:70-334) with the closest-hit
+// (Shaders/HitShaderEntryPoints.slang:35-486) and miss (:354-426) programs
+// inlined and traversal from trace.h.  HGI semantics: opaque shadows, no
+// layers, no ground plane, no AOVs.
+#pragma once
+
+#include "../../include/aph.h"
+#include "shade.h"
+#include "trace.h"
+
+namespace aph {
+
+struct RenderScene {
+    TraceScene sc;
+    const InstShade* ishade;
+    const DevMaterial* mats;
+    uint32_t nmat;
+    uint32_t pad_;
+    DevEnv env;
+};
+
+struct PathCounters {
+    uint32_t ext, shadow;
+    TraceStats ts;
+};
+
+template <bool STATS>
+__device__ __forceinline__ float traceShadow(const RenderScene& rs, f3 org, f3 dir, ShortStack& stk, PathCounters& pc)
+{
+    HitRec h;
+    pc.shadow++;
+    bool occ = traceRay<true, STATS>(rs.sc, org, dir, kRayTMin, INFINITY, stk, h, pc.ts);
+    return occ ? 0.0f : 1.0f;
+}
+
+// computeCameraRay, RayTrace.slang:130-182
+__device__ __forceinline__ void cameraRay(const aph_frame& fp, uint32_t x, uint32_t y, uint32_t W, uint32_t H, Rng& rng,
+                                          f3& org, f3& dir)
+{
+    float sx = (float)x, sy = (float)y;
+    if (fp.temporal_jitter) {
+        sx += 0.5f + fp.jitter[0];
+        sy += 0.5f + fp.jitter[1];
+    } else {
+        f2 r = random2D(rng);
+        sx += r.x;
+        sy += r.y;
+    }
+    float nx = (sx / (float)W) * 2.0f - 1.0f, ny = (sy / (float)H) * 2.0f - 1.0f;
+    ny = -ny;
+    f3 right = f3{fp.inv_view[0], fp.inv_view[1], fp.inv_view[2]};
+    f3 up = f3{fp.inv_view[4], fp.inv_view[5], fp.inv_view[6]};
+    f3 front = -f3{fp.inv_view[8], fp.inv_view[9], fp.inv_view[10]};
+    f3 eye = f3{fp.inv_view[12], fp.inv_view[13], fp.inv_view[14]};
+    f3 off = right * (fp.view_size[0] * 0.5f * nx) + up * (fp.view_size[1] * 0.5f * ny);
+    if (fp.is_ortho) {
+        dir = front;
+        org = eye + off;
+    } else {
+        dir = normalize(front + off);
+        org = eye;
+    }
+    if (fp.lens_radius > 0.0f) {
+        f3 focal = org + dir * fp.focal_distance;
+        f2 r = random2D(rng);
+        float rr = fp.lens_radius * sqrtf(r.x), phi = 2.0f * kPi * r.y;
+        org = org + right * (rr * cosf(phi)) + up * (rr * sinf(phi));
+        dir = normalize(focal - org);
+    }
+}
+
+// shadeDirectionalLight, ShadeFunctions.slang:51-89
+__device__ __forceinline__ f3 shadeDirectional(const aph_distant_light& l, const DevMaterial& m, const Shading& sh, f3 V,
+                                               Rng& rng, f3& shadowDir)
+{
+    f2 r = random2D(rng);
+    f3 L = sampleCone(r, f3{l.direction[0], l.direction[1], l.direction[2]}, l.cos_radius);
+    f3 f = evalMaterial(m, sh, V, L);
+    if (isBlack(f)) {
+        shadowDir = splat(0.0f);
+        return splat(0.0f);
+    }
+    shadowDir = L;
+    return f3{l.color_intensity[0], l.color_intensity[1], l.color_intensity[2]} * l.color_intensity[3] * f;
+}
+
+// The whole path of one pixel-sample; returns RGBA before accumulation.
+template <bool STATS>
+__device__ float4 pathSample(const RenderScene& rs, const aph_frame& fp, uint32_t W, uint32_t H, uint32_t px,
+                             uint32_t py, uint32_t sampleIndex, ShortStack& stk, PathCounters& pc)
+{
+    const DevEnv& env = rs.env;
+    const int maxDepth = fp.trace_depth;
+    Rng rng = rngInit(sampleIndex, W, px, py);
+    f3 contrib = splat(1.0f);
+    int rayType = kCameraRay, nextRayType = kCameraRay;
+    bool isPrimary = true, pathEnded = false;
+    float outputAlpha = 0.0f;
+    f3 org, dir;
+    cameraRay(fp, px, py, W, H, rng, org, dir);
+    f3 outDirect = splat(0.0f), outDiffuse = splat(0.0f), outGlossy = splat(0.0f);
+    int misLobeID = kNoSecondaryRay;
+    f3 direct = splat(0.0f), indirect = splat(0.0f);
+    float alpha = 0.0f;
+    for (int depth = 0; depth <= maxDepth && !pathEnded; depth++) {
+        if (depth == maxDepth) {
+            bool bg = isPrimary || rayType == kTransmissionLobe || rayType == kGroundPlaneReflectionRay;
+            direct = evalEnvironment(env, dir, bg, px, py, W, H) * contrib;
+            pathEnded = true;
+            // `indirect` keeps the previous segment's value: the reference does not clear it here.
+        } else {
+            float minT = depth == 0 ? 0.0f : kRayTMin;
+            direct = splat(0.0f);
+            indirect = splat(0.0f);
+            HitRec hit;
+            pc.ext++;
+            bool h = traceRay<false, STATS>(rs.sc, org, dir, minT, INFINITY, stk, hit, pc.ts);
+            bool skip = false;
+            if (!h) {
+                bool bg = isPrimary || rayType == kTransmissionLobe || rayType == kGroundPlaneReflectionRay;
+                if (bg) direct = evalEnvironment(env, dir, true, px, py, W, H) * contrib;
+                pathEnded = true;
+            } else {
+                // ---- closest hit ----
+                f3 nextContrib = splat(1.0f);
+                f3 V = -dir;
+                const InstShade& ins = rs.ishade[hit.inst];
+                Shading sh;
+                computeShading(ins, hit.prim, (float)hit.u, (float)hit.v, sh);
+                uint32_t mi = ins.material < rs.nmat ? ins.material : 0u;
+                const DevMaterial m = rs.mats[mi];
+                f3 transparency = splat(1.0f) - m.opacity;
+                float P = luminance(transparency);
+                bool hitLayer = random2D(rng).x >= P;
+                if (hitLayer) {
+                    direct = direct + m.emissionColor * m.emission * lerp(splat(1.0f), m.coatColor, m.coat);
+                    f3 dirShadow = splat(0.0f), dirColor = splat(0.0f);
+                    bool hasDir = false;
+                    if (fp.light_count > 1) {
+                        f3 o2 = dot(V, sh.normal) < 0.0f ? sh.geomPosition : sh.position;
+                        for (int li = 0; li < fp.light_count; li++) {
+                            if (fp.lights[li].color_intensity[3] <= 0.0f) continue;
+                            f3 sd;
+                            f3 lr = shadeDirectional(fp.lights[li], m, sh, V, rng, sd);
+                            if (isBlack(lr)) continue;
+                            direct = direct + lr * traceShadow<STATS>(rs, o2, sd, stk, pc);
+                        }
+                    } else if (fp.light_count > 0) {
+                        // one light: the selection still draws a random number (HitShaderEntryPoints.slang:322-341)
+                        const aph_distant_light& l0 = fp.lights[0];
+                        float w0 = luminance(f3{l0.color_intensity[0], l0.color_intensity[1], l0.color_intensity[2]}) *
+                                   l0.color_intensity[3];
+                        float selPdf = 1.0f;
+                        random2D(rng);
+                        if (w0 > 0.0f) selPdf = w0 / w0;
+                        if (l0.color_intensity[3] > 0.0f) {
+                            dirColor = shadeDirectional(l0, m, sh, V, rng, dirShadow) / fmaxf(selPdf, 1e-4f);
+                            hasDir = true;
+                        }
+                    }
+                    // ---- environment MIS, ShadeFunctions.slang:135-259 ----
+                    bool emitsMat = false, emitsLight = false;
+                    int misLobe = kNoSecondaryRay;
+                    f3 matRes = splat(0.0f), matDir = splat(0.0f), lightRes = splat(0.0f), lightDir = splat(0.0f);
+                    {
+                        f2 r = random2D(rng);
+                        f3 L;
+                        float matPdf;
+                        int lobe;
+                        f3 f = sampleMaterial(m, sh, V, r, L, matPdf, lobe);
+                        misLobe = lobe;
+                        if (lobe != kTransmissionLobe) {
+                            f3 Le = evalEnvironment(env, L, false, px, py, W, H);
+                            if (!isBlack(Le) && !isBlack(f)) {
+                                float lightPdf = env.hasLightTex ? luminance(Le) / env.lightIntegral : 0.25f * kPiInv;
+                                matRes = Le * misWeight(matPdf, lightPdf) * f / matPdf;
+                                matDir = L;
+                                emitsMat = true;
+                            }
+                            float lightPdf;
+                            Le = sampleEnvironment(env, r, L, lightPdf);
+                            int evLobe;
+                            f = evalMaterialPdf(m, sh, V, L, r, matPdf, evLobe);
+                            if (!isBlack(Le) && !isBlack(f)) {
+                                lightRes = Le * misWeight(lightPdf, matPdf) * f / lightPdf;
+                                lightDir = L;
+                                emitsLight = true;
+                            }
+                        }
+                    }
+                    contrib = contrib * (m.opacity / (1.0f - P));
+                    outputAlpha = (outputAlpha == 0.0f && rayType == kTransmissionLobe) ? 0.0f : 1.0f;
+                    // ---- setupIndirectLightRay, ShadeFunctions.slang:263-310 ----
+                    {
+                        f2 r = random2D(rng);
+                        f3 L;
+                        float pdf;
+                        int lobe;
+                        f3 f = sampleMaterial(m, sh, V, r, L, pdf, lobe);
+                        if (isBlack(f)) {
+                            nextRayType = kNoSecondaryRay;
+                        } else {
+                            bool trans = lobe == kTransmissionLobe;
+                            org = (dot(V, sh.normal) < 0.0f || trans) ? sh.geomPosition : sh.position;
+                            dir = L;
+                            nextContrib = f / pdf;
+                            nextRayType = lobe;
+                        }
+                    }
+                    f3 so = dot(V, sh.normal) < 0.0f ? sh.geomPosition : sh.position;
+                    if (hasDir) direct = direct + dirColor * traceShadow<STATS>(rs, so, dirShadow, stk, pc);
+                    if (emitsMat || emitsLight) {
+                        f3 mis = splat(0.0f);
+                        if (emitsMat) mis = mis + matRes * traceShadow<STATS>(rs, so, matDir, stk, pc);
+                        if (emitsLight) mis = mis + lightRes * traceShadow<STATS>(rs, so, lightDir, stk, pc);
+                        indirect = mis;
+                        misLobeID = misLobe;
+                    }
+                    if (nextRayType == kNoSecondaryRay) pathEnded = true;
+                    isPrimary = false;
+                } else {
+                    contrib = contrib * (transparency / P);
+                    // continue straight through, as if nothing had been hit
+                    dir = -V;
+                    org = sh.geomPosition;
+                    skip = true;
+                }
+                if (!skip) {
+                    direct = direct * contrib;
+                    indirect = indirect * contrib;
+                    contrib = contrib * nextContrib;
+                    if (fp.rr_enabled && depth + 1 >= fp.rr_start_depth) {
+                        float surv = clampf(maxc(contrib), 0.05f, 1.0f);
+                        if (random2D(rng).x > surv) pathEnded = true;
+                        else contrib = contrib / surv;
+                    }
+                }
+            }
+            alpha = outputAlpha;
+            if (skip) continue;
+        }
+        // Route into the direct / diffuse / glossy outputs like MainEntryPoints.slang:160-243.
+        if (depth == 0) {
+            isPrimary = false;
+            outDirect = direct;
+            if (misLobeID == kDiffuseLobe) outDiffuse = indirect;
+            else outGlossy = indirect;
+        } else {
+            f3 radiance = direct + indirect;
+            if (rayType == kDiffuseLobe) outDiffuse = outDiffuse + radiance;
+            else outGlossy = outGlossy + radiance;
+        }
+        rayType = nextRayType;
+    }
+    f3 fin = adjustRadiance(fp.max_luminance, fp.display_errors != 0, outDirect + outDiffuse + outGlossy);
+    return make_float4(fin.x, fin.y, fin.z, alpha);
+}
+
+}  // namespace aph

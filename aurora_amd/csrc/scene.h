@@ -1,0 +1,99 @@
+// Device-resident scene layout for the gfx950 path tracer.
+//
+// HBM layout (all arrays 16-byte aligned):
+//   BLAS per mesh   BvhNode[n-1]  64 B/node, both children's boxes in the parent
+//                   Tri[n]        48 B/triangle, vertices in BVH leaf order,
+//                                 original primitive index in v0.w
+//   mesh shading    pos float[3V], nrm float[3V] (optional), idx uint[3T]
+//   TLAS            BvhNode[m-1]  over instance world boxes
+//                   InstTrav[m]   64 B: world->object 3x4 + BLAS pointers
+//                   InstShade[m]  object->world 3x4 + shading pointers
+//   materials       DevMaterial[k]
+//   environment     float4 texels, AliasEntry (16 B) per texel
+#pragma once
+
+#include "dmath.h"
+
+namespace aph {
+
+// Two-child BVH node: a/b = child 0 box (lo.xyz, ref0) / (hi.xyz, ref1), c/d = child 1 box.
+struct alignas(16) BvhNode {
+    float4 a, b, c, d;
+};
+
+// Child reference: >= 0 internal node index; leaf = kLeafBit | (count-1) << 27 | first.
+constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kLeafFirstMask = 0x07FFFFFFu;
+constexpr int32_t kExitMarker = 0x7FFFFFFF;  // stack marker: leave the BLAS, back to the TLAS
+constexpr uint32_t kMaxLeafPrims = 16;
+APH_HD int32_t makeLeaf(uint32_t first, uint32_t count)
+{
+    return (int32_t)(kLeafBit | ((count - 1u) << 27) | first);
+}
+APH_HD uint32_t leafFirst(int32_t ref) { return (uint32_t)ref & kLeafFirstMask; }
+APH_HD uint32_t leafCount(int32_t ref) { return (((uint32_t)ref >> 27) & 0xFu) + 1u; }
+
+struct alignas(16) Tri {
+    float4 v0, v1, v2;  // v0.w = original primitive index (bits)
+};
+
+struct alignas(16) InstTrav {
+    float w2o[12];
+    const BvhNode* nodes;
+    const Tri* tris;
+};
+
+struct alignas(16) InstShade {
+    float o2w[12];
+    const float* pos;
+    const float* nrm;
+    const uint32_t* idx;
+    uint32_t material;
+    uint32_t flags;
+};
+
+struct TraceScene {
+    const BvhNode* tlas;
+    const InstTrav* inst;
+    float extent;      // 3 x max |coordinate| of all instance world boxes
+    float padScale;    // relative pad of instance boxes (times condition number bound)
+    uint32_t ninst;
+    uint32_t pad_;
+};
+
+// Runtime material: the Material struct of Material.slang:20-57 (no textures).
+struct DevMaterial {
+    float base; f3 baseColor;
+    float diffuseRoughness, metalness, specular, specularRoughness;
+    f3 specularColor; float specularIOR;
+    float specularAnisotropy, specularRotation, transmission, subsurface;
+    f3 transmissionColor; float sheen;
+    f3 subsurfaceColor; float sheenRoughness;
+    f3 sheenColor; float coat;
+    f3 coatColor; float coatRoughness;
+    float coatAnisotropy, coatRotation, coatIOR, coatAffectColor;
+    float coatAffectRoughness, emission; int thinWalled; int pad0;
+    f3 emissionColor; float pad1;
+    f3 opacity; float pad2;
+};
+
+struct alignas(16) AliasEntry {
+    uint32_t alias;
+    float prob;
+    float pdf;
+    float pad;
+};
+
+struct DevEnv {
+    f3 lightTop, lightBottom, bgTop, bgBottom;
+    float lightXf[16], lightXfInv[16], bgXf[16];
+    int bgUseScreen, filter, hasLightTex, hasBgTex;
+    const float4* lightTex;
+    const AliasEntry* alias;
+    const float4* bgTex;
+    uint32_t lw, lh, bw, bh;
+    float lightIntegral;
+    float pad_;
+};
+
+}  // namespace aph

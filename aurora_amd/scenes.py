@@ -1,0 +1,351 @@
+"""Seeded synthetic scenes, environments and cameras for the path tracer.
+
+All generators are deterministic for a given seed (numpy PCG64) so that the
+GPU backend, the CPU oracle and the benchmark see identical inputs.
+
+Workloads:
+  * ``config1`` - plain path tracer, CPU-runnable size: ground quad + 16
+    icospheres (subdivision 3, 1,280 triangles each), 256x128 sky-gradient
+    lat-long environment, 512x512, 16 spp, 3 bounces.
+  * ``config2`` - the 1M-triangle, 1080p workload: 200 instances of 5
+    noise-displaced icospheres (subdivision 4 gives 1.02 M instanced
+    triangles; subdivision 5 gives 4.1 M), random rotations, materials with
+    transmission and coat, 2048x1024 HDR environment with a 1e4 sun,
+    1920x1080, 64 spp, 5 bounces.
+"""
+import math
+
+import numpy as np
+
+from . import capi
+
+# ---------------------------------------------------------------- meshes
+
+
+def icosphere(subdiv):
+    """Unit icosphere: 10*4^s+2 vertices, 20*4^s triangles, outward winding."""
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    v = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+                  [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], dtype=np.float64)
+    f = np.array([[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2],
+                  [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5],
+                  [2, 4, 11], [6, 2, 10], [8, 6, 7], [9, 8, 1]], dtype=np.int64)
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    for _ in range(subdiv):
+        e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
+        key = np.sort(e, axis=1)
+        uniq, inv = np.unique(key[:, 0] * (1 << 32) + key[:, 1], return_inverse=True)
+        a = uniq >> 32
+        b = uniq & 0xFFFFFFFF
+        mid = v[a] + v[b]
+        mid /= np.linalg.norm(mid, axis=1, keepdims=True)
+        base = len(v)
+        v = np.concatenate([v, mid])
+        nf = len(f)
+        m01 = base + inv[:nf]
+        m12 = base + inv[nf:2 * nf]
+        m20 = base + inv[2 * nf:]
+        f = np.concatenate([np.stack([f[:, 0], m01, m20], 1), np.stack([f[:, 1], m12, m01], 1),
+                            np.stack([f[:, 2], m20, m12], 1), np.stack([m01, m12, m20], 1)])
+    return v.astype(np.float32), f.astype(np.uint32)
+
+
+def vertex_normals(pos, idx):
+    p = pos.astype(np.float64)
+    fn = np.cross(p[idx[:, 1]] - p[idx[:, 0]], p[idx[:, 2]] - p[idx[:, 0]])
+    n = np.zeros_like(p)
+    for k in range(3):
+        np.add.at(n, idx[:, k], fn)
+    n /= np.maximum(np.linalg.norm(n, axis=1, keepdims=True), 1e-30)
+    return n.astype(np.float32)
+
+
+def displaced_icosphere(subdiv, rng, amplitude=0.15, octaves=6):
+    """Icosphere displaced along the normal by a seeded smooth noise (sum of sinusoids)."""
+    v, f = icosphere(subdiv)
+    p = v.astype(np.float64)
+    disp = np.zeros(len(p))
+    for o in range(octaves):
+        freq = 1.5 * (1.9 ** o)
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        disp += (amplitude / (1.6 ** o)) * np.sin(freq * (p @ d) * math.pi + rng.uniform(0, 2 * math.pi))
+    p *= (1.0 + disp)[:, None]
+    pos = p.astype(np.float32)
+    return pos, vertex_normals(pos, f), f
+
+
+def ground_quad(half=10.0):
+    pos = np.array([[-half, 0, -half], [half, 0, -half], [half, 0, half], [-half, 0, half]], dtype=np.float32)
+    nrm = np.tile(np.array([[0, 1, 0]], dtype=np.float32), (4, 1))
+    idx = np.array([[0, 2, 1], [0, 3, 2]], dtype=np.uint32)
+    return pos, nrm, idx
+
+
+def load_teapot(path):
+    d = np.load(path, allow_pickle=False)
+    return d["positions"], d["normals"], d["indices"].astype(np.uint32)
+
+
+# ---------------------------------------------------------------- transforms
+
+
+def affine(rot=None, scale=1.0, trans=(0.0, 0.0, 0.0)):
+    """Row-major 3x4 object-to-world matrix T * R * S."""
+    r = np.eye(3) if rot is None else np.asarray(rot, dtype=np.float64)
+    m = np.zeros((3, 4))
+    m[:, :3] = r * scale
+    m[:, 3] = trans
+    return m.astype(np.float32).reshape(12)
+
+
+def random_rotation(rng):
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    w, x, y, z = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+def inverse3x4(m12):
+    """Same definition as aph_inverse3x4: double cofactors, rounded to float."""
+    m = np.asarray(m12, dtype=np.float64)
+    a, b, c, d, e, f, g, h, k = m[0], m[1], m[2], m[4], m[5], m[6], m[8], m[9], m[10]
+    A = e * k - f * h
+    B = f * g - d * k
+    Cc = d * h - e * g
+    det = a * A + b * B + c * Cc
+    r = np.array([A / det, (c * h - b * k) / det, (b * f - c * e) / det,
+                  B / det, (a * k - c * g) / det, (c * d - a * f) / det,
+                  Cc / det, (b * g - a * h) / det, (a * e - b * d) / det])
+    out = np.zeros(12, dtype=np.float32)
+    t = (m[3], m[7], m[11])
+    for i in range(3):
+        out[4 * i:4 * i + 3] = r[3 * i:3 * i + 3].astype(np.float32)
+        out[4 * i + 3] = np.float32(-((r[3 * i] * t[0] + r[3 * i + 1] * t[1]) + r[3 * i + 2] * t[2]))
+    return out
+
+
+# ---------------------------------------------------------------- camera
+
+
+def look_at_frame(eye, target, up, fov_y_deg, aspect, trace_depth=5, max_luminance=1000.0):
+    """aph_frame for a glm::lookAt / glm::perspective camera (RendererBase.cpp:237-246)."""
+    eye = np.asarray(eye, dtype=np.float64)
+    fwd = np.asarray(target, dtype=np.float64) - eye
+    fwd /= np.linalg.norm(fwd)
+    s = np.cross(fwd, up)
+    s /= np.linalg.norm(s)
+    u = np.cross(s, fwd)
+    fr = capi.Frame()
+    inv = [s[0], s[1], s[2], 0.0, u[0], u[1], u[2], 0.0, -fwd[0], -fwd[1], -fwd[2], 0.0,
+           eye[0], eye[1], eye[2], 1.0]
+    for i, val in enumerate(inv):
+        fr.inv_view[i] = float(np.float32(val))
+    f = 1.0 / math.tan(math.radians(fov_y_deg) / 2.0)
+    p00, p11 = np.float32(f / aspect), np.float32(f)
+    fr.view_size[0] = float(np.float32(2.0) / p00)
+    fr.view_size[1] = float(np.float32(2.0) / p11)
+    fr.trace_depth = trace_depth
+    fr.max_luminance = max_luminance
+    fr.rr_enabled = 1
+    fr.rr_start_depth = 3
+    fr.force_opaque_shadows = 1
+    return fr
+
+
+# ---------------------------------------------------------------- environments
+
+
+def sky_gradient(w, h, rng):
+    """Seeded RGB32F sky: horizon/zenith colors blended by latitude plus a soft glow."""
+    zen = rng.uniform(0.1, 0.6, 3) * np.array([0.5, 0.7, 1.0])
+    hor = rng.uniform(0.6, 1.2, 3)
+    gnd = rng.uniform(0.05, 0.3, 3)
+    v = (np.arange(h) + 0.5) / h
+    lat = (0.5 - v) * math.pi
+    up = np.clip(np.sin(lat), 0, 1)[:, None]
+    down = np.clip(-np.sin(lat), 0, 1)[:, None]
+    row = hor[None, :] * (1 - up) + zen[None, :] * up
+    row = row * (1 - down) + gnd[None, :] * down
+    img = np.repeat(row[:, None, :], w, axis=1)
+    u = (np.arange(w) + 0.5) / w
+    glow = 0.5 * np.exp(-((u - rng.uniform(0.2, 0.8)) ** 2) / 0.01)[None, :, None] * up[:, :, None]
+    img = img + glow
+    return np.ascontiguousarray(img.astype(np.float32))
+
+
+def sun_sky(w, h, rng, sun_radiance=1.0e4, sun_radius_deg=0.5):
+    img = sky_gradient(w, h, rng)
+    u = (np.arange(w) + 0.5) / w
+    v = (np.arange(h) + 0.5) / h
+    phi = (u[None, :] + 0.25) * 2 * math.pi
+    theta = v[:, None] * math.pi
+    dirs = np.stack([np.cos(phi) * np.sin(theta), np.cos(theta) * np.ones_like(phi), np.sin(phi) * np.sin(theta)], -1)
+    el = math.radians(rng.uniform(25, 60))
+    az = rng.uniform(0, 2 * math.pi)
+    sd = np.array([math.cos(el) * math.cos(az), math.sin(el), math.cos(el) * math.sin(az)])
+    cosang = dirs @ sd
+    mask = cosang > math.cos(math.radians(sun_radius_deg))
+    if not mask.any():
+        i = np.unravel_index(np.argmax(cosang), cosang.shape)
+        mask[i] = True
+    img[mask] = np.array([1.0, 0.95, 0.85], dtype=np.float32) * sun_radiance
+    return np.ascontiguousarray(img)
+
+
+def identity4():
+    return [1.0, 0, 0, 0, 0, 1.0, 0, 0, 0, 0, 1.0, 0, 0, 0, 0, 1.0]
+
+
+# ---------------------------------------------------------------- scene container
+
+
+class Scene:
+    """Meshes, instances, materials, environment and camera of one workload."""
+
+    def __init__(self):
+        self.meshes = []        # list of (pos, nrm or None, idx)
+        self.instances = None   # INSTANCE_DTYPE array; mesh field = 0-based mesh index
+        self.materials = []     # list of capi.Material
+        self.env = None         # dict
+        self.frame = None       # capi.Frame
+        self.width = 0
+        self.height = 0
+        self.spp = 1
+        self.name = ""
+
+    @property
+    def instanced_triangles(self):
+        return int(sum(len(self.meshes[i][2]) for i in self.instances["mesh"]))
+
+    @property
+    def unique_triangles(self):
+        return int(sum(len(m[2]) for m in self.meshes))
+
+
+def _instances(list_of):
+    arr = np.zeros(len(list_of), dtype=capi.INSTANCE_DTYPE)
+    for i, (mesh, mat, o2w) in enumerate(list_of):
+        arr[i]["o2w"] = o2w
+        arr[i]["w2o"] = inverse3x4(o2w)
+        arr[i]["mesh"] = mesh
+        arr[i]["material"] = mat
+        arr[i]["mask"] = 0xFF
+    return arr
+
+
+def _material(base_color=(0.8, 0.8, 0.8), metalness=0.0, roughness=0.2, **kw):
+    m = capi.default_material()
+    for i in range(3):
+        m.base_color[i] = base_color[i]
+    m.metalness = metalness
+    m.specular_roughness = roughness
+    for k, val in kw.items():
+        if isinstance(val, (tuple, list, np.ndarray)):
+            arr = getattr(m, k)
+            for i in range(3):
+                arr[i] = val[i]
+        else:
+            setattr(m, k, val)
+    return m
+
+
+def config1(seed=1, width=512, height=512, spp=16, trace_depth=3):
+    """Ground quad + 16 icospheres (subdiv 3), 256x128 sky, 512^2 / 16 spp / 3 bounces."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sc = Scene()
+    sc.name = "config1"
+    gp, gn, gi = ground_quad(10.0)
+    sp, sf = icosphere(3)
+    sn = vertex_normals(sp, sf)
+    sc.meshes = [(gp, gn, gi), (sp, sn, sf)]
+    inst = [(0, 0, affine())]
+    sc.materials = [_material((0.6, 0.6, 0.6), 0.0, 0.5)]
+    for i in range(16):
+        c = (rng.uniform(-5, 5), rng.uniform(0.2, 2.0), rng.uniform(-5, 5))
+        r = rng.uniform(0.2, 1.0)
+        inst.append((1, i + 1, affine(scale=r, trans=c)))
+        sc.materials.append(_material(tuple(rng.uniform(0, 1, 3)), float(rng.uniform() < 0.3),
+                                      float(rng.uniform(0.05, 0.8))))
+    sc.instances = _instances(inst)
+    sky = sky_gradient(256, 128, rng)
+    sc.env = dict(light_rgb=sky, background_rgb=sky)
+    sc.width, sc.height, sc.spp = width, height, spp
+    sc.frame = look_at_frame((0.0, 6.0, 14.0), (0.0, 0.5, 0.0), (0, 1, 0), 45.0, width / height, trace_depth)
+    return sc
+
+
+def config2(seed=2, subdiv=4, width=1920, height=1080, spp=64, trace_depth=5, env_w=2048, env_h=1024):
+    """200 instances of 5 displaced icospheres, transmission/coat materials, 2048x1024 sun sky."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sc = Scene()
+    sc.name = "config2"
+    gp, gn, gi = ground_quad(40.0)
+    sc.meshes = [(gp, gn, gi)]
+    for _ in range(5):
+        p, n, f = displaced_icosphere(subdiv, rng)
+        sc.meshes.append((p, n, f))
+    inst = [(0, 0, affine())]
+    sc.materials = [_material((0.5, 0.5, 0.5), 0.0, 0.6)]
+    for i in range(200):
+        mesh = 1 + int(rng.integers(0, 5))
+        s = rng.uniform(0.4, 1.2)
+        c = (rng.uniform(-18, 18), rng.uniform(0.5, 3.0), rng.uniform(-18, 18))
+        inst.append((mesh, i + 1, affine(random_rotation(rng), s, c)))
+        sc.materials.append(_material(tuple(rng.uniform(0, 1, 3)), float(rng.uniform() < 0.3),
+                                      float(rng.uniform(0.05, 0.8)), transmission=float(rng.uniform(0, 1)),
+                                      coat=float(rng.uniform(0, 1)), coat_roughness=float(rng.uniform(0.02, 0.3))))
+    sc.instances = _instances(inst)
+    sky = sun_sky(env_w, env_h, rng)
+    sc.env = dict(light_rgb=sky, background_rgb=sky)
+    sc.width, sc.height, sc.spp = width, height, spp
+    sc.frame = look_at_frame((0.0, 14.0, 34.0), (0.0, 1.0, 0.0), (0, 1, 0), 50.0, width / height, trace_depth)
+    return sc
+
+
+def teapot_scene(path, width=128, height=128, spp=4, trace_depth=5):
+    """The reference fixture: teapot, 45 degree camera at (0,0,-5) (AuroraTestHelpers.cpp:617-622)."""
+    sc = Scene()
+    sc.name = "teapot"
+    p, n, f = load_teapot(path)
+    sc.meshes = [(p, n, f)]
+    sc.materials = [_material((0.8, 0.8, 0.8), 0.0, 0.2)]
+    sc.instances = _instances([(0, 0, affine())])
+    sc.env = dict()  # default gradient environment (EnvironmentBase.cpp:33-45)
+    sc.width, sc.height, sc.spp = width, height, spp
+    sc.frame = look_at_frame((0.0, 0.0, -5.0), (0.0, 0.0, 0.0), (0, 1, 0), 45.0, width / height, trace_depth)
+    return sc
+
+
+def environment_struct(env):
+    """capi.Environment for an env dict (keeps numpy arrays alive via the returned tuple)."""
+    e = capi.Environment()
+    defaults = dict(light_top=(1, 1, 1), light_bottom=(0, 0, 0), background_top=(0.02, 0.25, 0.60),
+                    background_bottom=(0.32, 0.79, 1.0))
+    for k, val in defaults.items():
+        val = env.get(k, val)
+        arr = getattr(e, k)
+        for i in range(3):
+            arr[i] = val[i]
+    lt = env.get("light_transform", identity4())
+    bt = env.get("background_transform", identity4())
+    for i in range(16):
+        e.light_transform[i] = lt[i]
+        e.background_transform[i] = bt[i]
+    e.background_use_screen = int(env.get("background_use_screen", 0))
+    e.filter = int(env.get("filter", 0))
+    keep = []
+    import ctypes as C
+    if env.get("light_rgb") is not None:
+        a = np.ascontiguousarray(env["light_rgb"], dtype=np.float32)
+        keep.append(a)
+        e.light_rgb = a.ctypes.data_as(C.POINTER(C.c_float))
+        e.light_height, e.light_width = a.shape[0], a.shape[1]
+    if env.get("background_rgb") is not None:
+        a = np.ascontiguousarray(env["background_rgb"], dtype=np.float32)
+        keep.append(a)
+        e.background_rgb = a.ctypes.data_as(C.POINTER(C.c_float))
+        e.background_height, e.background_width = a.shape[0], a.shape[1]
+    return e, keep

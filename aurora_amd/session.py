@@ -1,0 +1,160 @@
+"""Python host wrapper over the C-ABI: one backend context per object.
+
+This is plumbing for tests and benchmarks; every operation is a call into
+libaph.so (hand-written HIP for gfx950).  There is no CPU fallback: creating a
+Renderer without the built library or without a GPU raises.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import capi
+from . import scenes
+
+
+class AphError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"aph error {code}: {msg}")
+        self.code = code
+
+
+class Renderer:
+    """A backend context (aph_create / aph_destroy): createRenderer(Backend::HIP)."""
+
+    def __init__(self, device=0):
+        self.lib = capi.load()
+        self.ctx = C.c_void_p()
+        rc = self.lib.aph_create(device, C.byref(self.ctx))
+        if rc != capi.APH_OK:
+            raise AphError(rc, "aph_create failed (no usable HIP device?)")
+        self._keep = []
+
+    def close(self):
+        if self.ctx:
+            self.lib.aph_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != capi.APH_OK:
+            raise AphError(rc, self.lib.aph_last_error(self.ctx).decode())
+
+    # ---- scene objects
+    def mesh(self, pos, idx, nrm=None, uv=None):
+        pos = np.ascontiguousarray(pos, dtype=np.float32)
+        idx = None if idx is None else np.ascontiguousarray(idx, dtype=np.uint32)
+        nrm = None if nrm is None else np.ascontiguousarray(nrm, dtype=np.float32)
+        uv = None if uv is None else np.ascontiguousarray(uv, dtype=np.float32)
+        h = C.c_uint64()
+        nt = (len(idx.reshape(-1)) // 3) if idx is not None else len(pos.reshape(-1, 3)) // 3
+        self._check(self.lib.aph_mesh_upload(self.ctx, capi.ptr(pos), len(pos.reshape(-1, 3)), capi.ptr(idx), nt,
+                                             capi.ptr(nrm), capi.ptr(uv), C.byref(h)))
+        return h.value
+
+    def build_blas(self, mesh, builder=capi.APH_BUILDER_LBVH):
+        self._check(self.lib.aph_blas_build(self.ctx, mesh, builder))
+
+    def tlas(self, instances):
+        inst = np.ascontiguousarray(instances, dtype=capi.INSTANCE_DTYPE)
+        h = C.c_uint64()
+        self._check(self.lib.aph_tlas_build(self.ctx, capi.ptr(inst), len(inst), C.byref(h)))
+        return h.value
+
+    def materials(self, mats):
+        arr = (capi.Material * len(mats))(*mats)
+        self._check(self.lib.aph_materials_upload(self.ctx, C.cast(arr, C.c_void_p), len(mats)))
+
+    def environment(self, env):
+        e, keep = scenes.environment_struct(env)
+        self._check(self.lib.aph_environment_upload(self.ctx, C.byref(e)))
+
+    def load_scene(self, sc, builder=capi.APH_BUILDER_LBVH):
+        """Uploads a scenes.Scene; returns the TLAS handle."""
+        handles = []
+        for pos, nrm, idx in sc.meshes:
+            m = self.mesh(pos, idx, nrm)
+            self.build_blas(m, builder)
+            handles.append(m)
+        inst = sc.instances.copy()
+        inst["mesh"] = np.array([handles[i] for i in sc.instances["mesh"]], dtype=np.uint32)
+        t = self.tlas(inst)
+        self.materials(sc.materials)
+        self.environment(sc.env or {})
+        return t
+
+    # ---- ray queries
+    def trace_closest(self, tlas, rays):
+        rays = np.ascontiguousarray(rays, dtype=capi.RAY_DTYPE)
+        hits = np.zeros(len(rays), dtype=capi.HIT_DTYPE)
+        self._check(self.lib.aph_trace_closest(self.ctx, tlas, capi.ptr(rays), len(rays), capi.ptr(hits)))
+        return hits
+
+    def trace_any(self, tlas, rays):
+        rays = np.ascontiguousarray(rays, dtype=capi.RAY_DTYPE)
+        occ = np.zeros(len(rays), dtype=np.uint8)
+        self._check(self.lib.aph_trace_any(self.ctx, tlas, capi.ptr(rays), len(rays), capi.ptr(occ)))
+        return occ
+
+    # ---- rendering
+    def target(self, w, h):
+        t = C.c_uint64()
+        self._check(self.lib.aph_target_create(self.ctx, w, h, C.byref(t)))
+        return t.value
+
+    def render(self, requests, mode=capi.APH_MODE_MEGAKERNEL):
+        arr = (capi.Request * len(requests))(*requests)
+        self._check(self.lib.aph_render(self.ctx, C.cast(arr, C.c_void_p), len(requests), mode))
+
+    def request(self, target, tlas, frame, sample_start, sample_count, seed_offset=0):
+        r = capi.Request()
+        r.target = target
+        r.tlas = tlas
+        r.frame = frame
+        r.sample_start = sample_start
+        r.sample_count = sample_count
+        r.seed_offset = seed_offset
+        return r
+
+    def wait(self):
+        self._check(self.lib.aph_wait(self.ctx))
+
+    def readback(self, target, w, h):
+        out = np.zeros((h, w, 4), dtype=np.float32)
+        self._check(self.lib.aph_readback(self.ctx, target, capi.ptr(out), out.nbytes))
+        return out
+
+    def post(self, target, w, h, brightness=(1.0, 1.0, 1.0), tone_map=False, gamma=True):
+        out = np.zeros((h, w, 4), dtype=np.float32)
+        b = (C.c_float * 3)(*brightness)
+        self._check(self.lib.aph_post(self.ctx, target, C.cast(b, C.c_void_p), int(tone_map), int(gamma),
+                                      capi.ptr(out), out.nbytes))
+        return out
+
+    # ---- instrumentation
+    def set_stats_level(self, level):
+        self._check(self.lib.aph_set_stats_level(self.ctx, level))
+
+    def stats(self):
+        s = capi.Stats()
+        self._check(self.lib.aph_stats_get(self.ctx, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in capi.Stats._fields_}
+
+    def reset_stats(self):
+        self._check(self.lib.aph_stats_reset(self.ctx))
+
+    def enable_timing(self, on=True):
+        self._check(self.lib.aph_timing_enable(self.ctx, int(on)))
+
+    def timing(self, kernel):
+        ms = C.c_double()
+        n = C.c_uint64()
+        self._check(self.lib.aph_timing_get(self.ctx, kernel.encode(), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def reset_timing(self):
+        self._check(self.lib.aph_timing_reset(self.ctx))

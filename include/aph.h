@@ -1,0 +1,257 @@
+/*
+ * aph.h - C-ABI of the MI355X (gfx950) path-tracing backend for Aurora.
+ *
+ * This is the thin boundary between Aurora-side C++ host code (a HIP renderer
+ * type behind IRenderer/IScene) and the hand-written HIP kernels in
+ * libaph.so.  Plain C types only: pointers, sizes, POD structs, int status.
+ * Reference citations are relative to the Aurora source tree.
+ *
+ * Conventions
+ *   - Every call returns APH_OK (0) or a negative APH_E_* code; the message is
+ *     available from aph_last_error(ctx).  No C++ exception crosses the ABI.
+ *     (Aurora reports the same conditions through AU_ERROR / AU_FAIL,
+ *     Libraries/Foundation/API/Aurora/Foundation/Log.h:79-113.)
+ *   - Host arrays passed in are caller-owned and copied before the call
+ *     returns (GeometryBase.cpp:94-112 copies client geometry the same way).
+ *   - Device objects (meshes, BVHs, targets) are owned by the context and named
+ *     by 64-bit handles; 0 is never a valid handle.
+ *   - Calls on one context are externally serialized (Aurora has no internal
+ *     threading contract, Aurora.h:1068-1227); separate contexts may be used
+ *     from separate threads.
+ *   - Work is enqueued on the context's stream; aph_wait() blocks until it
+ *     drains.  Entry points taking *host* output pointers are synchronous.
+ */
+#ifndef APH_H
+#define APH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define APH_VERSION 1
+
+enum {
+    APH_OK = 0,
+    APH_E_INVALID = -1,   /* bad argument or handle */
+    APH_E_DEVICE = -2,    /* HIP runtime error */
+    APH_E_MEMORY = -3,    /* allocation failed / over budget */
+    APH_E_STATE = -4,     /* call order violated (e.g. BLAS not built) */
+    APH_E_UNSUPPORTED = -5,
+    APH_E_OVERFLOW = -6   /* traversal stack or queue capacity exceeded */
+};
+
+enum { APH_BUILDER_LBVH = 0, APH_BUILDER_SAH = 1 };
+
+typedef struct aph_ctx* aph_ctx_t;
+
+/* Ray and hit records, 32 B / 20 B.  A miss has inst == prim == 0xFFFFFFFF and
+ * t == +inf.  (u, v) are the barycentric weights of vertices 1 and 2, the same
+ * convention as BuiltInTriangleIntersectionAttributes (Geometry.slang:115-118). */
+typedef struct { float org[3]; float tmin; float dir[3]; float tmax; } aph_ray;
+typedef struct { float t; uint32_t inst; uint32_t prim; float u; float v; } aph_hit;
+
+/* Instance record, 112 B.  o2w is the row-major 3x4 object-to-world transform,
+ * the layout of the HGI instance transform (HGIScene.h:44-48).  w2o may be left
+ * all zero: the library then fills it with aph_inverse3x4(o2w). */
+typedef struct {
+    float o2w[12];
+    float w2o[12];
+    uint32_t mesh;      /* 1-based mesh handle (low 32 bits) */
+    uint32_t material;  /* index into the table given to aph_materials_upload */
+    uint32_t mask;      /* reserved, 0xFF */
+    uint32_t flags;     /* reserved, 0 */
+} aph_instance;
+
+/* Standard Surface constants, 400 B, byte-for-byte the MaterialConstants layout
+ * of Shaders/DefaultMaterialUniformBuffer.slang:3-72 (offsets in comments). */
+typedef struct {
+    float base;                 /*   0 */
+    float base_color[3];        /*   4 */
+    float diffuse_roughness;    /*  16 */
+    float metalness;            /*  20 */
+    float specular;             /*  24 */
+    int32_t _pad0;              /*  28 */
+    float specular_color[3];    /*  32 */
+    float specular_roughness;   /*  44 */
+    float specular_ior;         /*  48 */
+    float specular_anisotropy;  /*  52 */
+    float specular_rotation;    /*  56 */
+    float transmission;         /*  60 */
+    float transmission_color[3];/*  64 */
+    float subsurface;           /*  76 */
+    float subsurface_color[3];  /*  80 */
+    int32_t _pad1;              /*  92 */
+    float subsurface_radius[3]; /*  96 */
+    float subsurface_scale;     /* 108 */
+    float subsurface_anisotropy;/* 112 */
+    float sheen;                /* 116 */
+    int32_t _pad2, _pad3;       /* 120 */
+    float sheen_color[3];       /* 128 */
+    float sheen_roughness;      /* 140 */
+    float coat;                 /* 144 */
+    float coat_color[3];        /* 148 */
+    float coat_roughness;       /* 160 */
+    float coat_anisotropy;      /* 164 */
+    float coat_rotation;        /* 168 */
+    float coat_ior;             /* 172 */
+    float coat_affect_color;    /* 176 */
+    float coat_affect_roughness;/* 180 */
+    float emission;             /* 184 */
+    int32_t _pad4;              /* 188 */
+    float emission_color[3];    /* 192 */
+    int32_t _pad5;              /* 204 */
+    float opacity[3];           /* 208 */
+    int32_t thin_walled;        /* 220 */
+    int32_t texture_block[44];  /* 224..399: has_*_image flags and UV transforms */
+} aph_material;
+
+/* Environment description, the EnvironmentData of EnvironmentBase.h:33-55 plus
+ * image pointers.  Transforms are row-major 4x4 applied as M*(v,0). */
+typedef struct {
+    float light_top[3];
+    float light_bottom[3];
+    float background_top[3];
+    float background_bottom[3];
+    float light_transform[16];
+    float background_transform[16];
+    int32_t background_use_screen;
+    int32_t filter;             /* 0 nearest (HGI default sampler), 1 bilinear */
+    const float* light_rgb;     /* RGB32F lat-long image or NULL (gradient) */
+    uint32_t light_width, light_height;
+    const float* background_rgb;/* RGB32F or NULL (gradient) */
+    uint32_t background_width, background_height;
+} aph_environment;
+
+typedef struct {
+    float color_intensity[4];   /* rgb, intensity */
+    float direction[3];         /* toward the light */
+    float cos_radius;
+} aph_distant_light;            /* DistantLight, Frame.slang:33-43 */
+
+/* Per-request camera and options: the fields of FrameData (Frame.slang:61-157,
+ * RendererBase.cpp:225-295) that the path reads. */
+typedef struct {
+    float inv_view[16];         /* rows: right, up, back, eye = transpose(inverse(view)) */
+    float view_size[2];         /* (2/P[0][0], 2/P[1][1]) */
+    float jitter[2];
+    int32_t is_ortho;
+    float focal_distance;
+    float lens_radius;
+    int32_t trace_depth;        /* clamped to [1,10] like RendererBase.cpp:265 */
+    float max_luminance;
+    int32_t rr_enabled;
+    int32_t rr_start_depth;
+    int32_t force_opaque_shadows;
+    int32_t display_errors;
+    int32_t temporal_jitter;
+    int32_t light_count;
+    aph_distant_light lights[4];
+} aph_frame;
+
+/* One render request: accumulate samples [sample_start, sample_start+sample_count)
+ * of one camera into one target (IRenderer::render(sampleStart, sampleCount),
+ * Aurora.h:1196). */
+typedef struct {
+    uint64_t target;
+    uint64_t tlas;
+    aph_frame frame;
+    uint32_t sample_start;
+    uint32_t sample_count;
+    uint32_t seed_offset;       /* SampleData.seedOffset (Frame.slang:18-25), 0 normally */
+    uint32_t _pad;
+} aph_request;
+
+enum { APH_MODE_MEGAKERNEL = 0, APH_MODE_WAVEFRONT = 1 };
+
+typedef struct {
+    uint64_t extension_rays;    /* closest-hit queries */
+    uint64_t shadow_rays;       /* any-hit queries */
+    uint64_t node_visits;       /* BVH nodes fetched (stats mode only) */
+    uint64_t tri_tests;         /* triangles tested (stats mode only) */
+    uint64_t inst_visits;       /* instance transforms applied (stats mode only) */
+    uint64_t paths;             /* pixel-samples started */
+} aph_stats;
+
+/* ---- context: createRenderer(Backend, taskCount), Aurora.h:1233-1234;
+ *      Aurora.cpp:31-69 returns nullptr when the backend is not valid ---- */
+int aph_create(int device, aph_ctx_t* out);
+void aph_destroy(aph_ctx_t ctx);
+const char* aph_last_error(aph_ctx_t ctx);
+int aph_version(void);
+/* The HIP stream (hipStream_t) all work of the context is enqueued on. */
+int aph_get_stream(aph_ctx_t ctx, void** stream);
+
+/* ---- geometry: IRenderer::createGeometryPointer + GeometryBase copy
+ *      (GeometryBase.cpp:22-125); positions required, normals/uv optional ---- */
+int aph_mesh_upload(aph_ctx_t ctx, const float* pos, uint32_t nv, const uint32_t* idx, uint32_t nt,
+                    const float* nrm, const float* uv, uint64_t* mesh);
+/* BLAS build, replaces the driver build of HGIGeometry.cpp:127-160 /
+ * PTGeometry.cpp:155-223. */
+int aph_blas_build(aph_ctx_t ctx, uint64_t mesh, int builder);
+/* TLAS build over instances, replaces HGIScene::rebuildAccelerationStructure
+ * (HGIScene.cpp:377-449) / PTScene::buildTLAS (PTScene.cpp:1460-1552).
+ * Instance IDs are the indices in inst[] (active order, HGIScene.cpp:392). */
+int aph_tlas_build(aph_ctx_t ctx, const aph_instance* inst, uint32_t n, uint64_t* tlas);
+/* Row-major 3x4 affine inverse in double, rounded to float (the w2o convention). */
+int aph_inverse3x4(const float m[12], float out[12]);
+
+/* ---- materials: MaterialBase::updateBuiltInMaterial (MaterialBase.cpp:193-214),
+ *      uploaded per change like HGIMaterial.cpp:39-59 ---- */
+int aph_materials_upload(aph_ctx_t ctx, const aph_material* mats, uint32_t n);
+void aph_material_defaults(aph_material* m);
+
+/* ---- environment: EnvironmentBase::updateGPUStruct (EnvironmentBase.cpp:52-71)
+ *      + AliasMap::build (AliasMap.cpp:31-166) ---- */
+int aph_environment_upload(aph_ctx_t ctx, const aph_environment* env);
+/* Alias table + luminance integral for an RGB32F lat-long image (host build,
+ * the AliasMap::build algorithm); entries are 16 B {alias, prob, pdf, pad}. */
+int aph_alias_build(const float* rgb, uint32_t w, uint32_t h, void* entries, float* integral);
+
+/* ---- ray queries: TraceRay in traceInstanceRay (RayTrace.slang:185-210) and
+ *      traceShadowRay (RayTrace.slang:221-270, forced opaque) ---- */
+int aph_trace_closest(aph_ctx_t ctx, uint64_t tlas, const aph_ray* rays, uint32_t n, aph_hit* hits);
+int aph_trace_any(aph_ctx_t ctx, uint64_t tlas, const aph_ray* rays, uint32_t n, uint8_t* occluded);
+/* Same queries on device-resident buffers, asynchronous on the context stream. */
+int aph_trace_closest_device(aph_ctx_t ctx, uint64_t tlas, const aph_ray* d_rays, uint32_t n, aph_hit* d_hits);
+int aph_trace_any_device(aph_ctx_t ctx, uint64_t tlas, const aph_ray* d_rays, uint32_t n, uint8_t* d_occluded);
+
+/* ---- render targets: IRenderer::createRenderBuffer (Aurora.h:1068-1227); the
+ *      accumulation texture of HGIRenderer.cpp:155-165 (RGBA32F) ---- */
+int aph_target_create(aph_ctx_t ctx, uint32_t width, uint32_t height, uint64_t* target);
+int aph_target_destroy(aph_ctx_t ctx, uint64_t target);
+/* Device pointer of a target's RGBA32F accumulation buffer. */
+int aph_target_device_ptr(aph_ctx_t ctx, uint64_t target, void** ptr);
+
+/* ---- render: IRenderer::render(sampleStart, sampleCount) (Aurora.h:1196;
+ *      HGIRenderer::render, HGIRenderer.cpp:477-589).  Asynchronous.
+ *      mode APH_MODE_MEGAKERNEL: one launch per request and sample, the
+ *      reference's one-dispatch-per-sample structure;
+ *      APH_MODE_WAVEFRONT: all requests and samples share one wavefront
+ *      pipeline (raygen, extend, shade, shadow, accumulate). ---- */
+int aph_render(aph_ctx_t ctx, const aph_request* reqs, uint32_t nreq, int mode);
+/* IRenderer::waitForTask (HGIRenderer.cpp:591-594 is a no-op because HGI is synchronous). */
+int aph_wait(aph_ctx_t ctx);
+/* IRenderBuffer::data (HGIRenderBuffer.cpp:87-136): blocking copy of the
+ * RGBA32F accumulation buffer. */
+int aph_readback(aph_ctx_t ctx, uint64_t target, void* dst, size_t bytes);
+/* PostProcessing.glsl:37-64 on a target into RGBA32F host memory. */
+int aph_post(aph_ctx_t ctx, uint64_t target, const float brightness[3], int tone_map, int gamma,
+             float* dst, size_t bytes);
+
+/* ---- instrumentation ---- */
+/* stats_level 0: ray counts only; 1: also node/triangle/instance visit counts. */
+int aph_set_stats_level(aph_ctx_t ctx, int level);
+int aph_stats_get(aph_ctx_t ctx, aph_stats* out);   /* synchronizes */
+int aph_stats_reset(aph_ctx_t ctx);
+/* Per-kernel timing with HIP events on the context stream. */
+int aph_timing_enable(aph_ctx_t ctx, int enable);
+int aph_timing_get(aph_ctx_t ctx, const char* kernel, double* total_ms, uint64_t* launches);
+int aph_timing_reset(aph_ctx_t ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

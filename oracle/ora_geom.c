@@ -1,0 +1,466 @@
+/* Oracle: scene, FP64 watertight intersection, CPU BVH (test infrastructure only).
+ *
+ * The reference delegates traversal and intersection to the ray-tracing driver
+ * (Shaders/RayTrace.slang:185-210,221-270 call TraceRay; BLAS/TLAS are built by
+ * HGI/HGIGeometry.cpp:127-160 and HGI/HGIScene.cpp:377-420).  The exact hit
+ * semantics are defined here instead and shared with the GPU path:
+ *
+ *   - world ray (float) -> object ray: o' = W2O*(o,1), d' = W2O*(d,0) evaluated
+ *     in double (W2O = ora_inverse3x4(O2W) as float, products summed left to
+ *     right) and rounded to float; every test of that instance uses this float
+ *     object ray (exactly representable in double);
+ *   - watertight test of Woop, Benthin & Wald (JCGT 2(1), 2013) evaluated in
+ *     double with no FMA, edges accepted when U,V,W share a sign (zeros allowed),
+ *     det == 0 rejected, t = T/det accepted when tmin <= t <= tmax;
+ *   - closest hit = lexicographic minimum of (t, instance, primitive);
+ *   - barycentrics (u, v) = (V/det, W/det) are the weights of vertices 1 and 2.
+ */
+#include "ora.h"
+#include "ora_geom.h"
+
+#include <float.h>
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ---------------- watertight ray / triangle (double) ---------------- */
+
+void ora_wray_setup(ora_wray* r, const double o[3], const double d[3], double tmin, double tmax)
+{
+    for (int i = 0; i < 3; i++) { r->o[i] = o[i]; r->d[i] = d[i]; }
+    r->tmin = tmin;
+    r->tmax = tmax;
+    double ax = fabs(d[0]), ay = fabs(d[1]), az = fabs(d[2]);
+    int kz;
+    if (ax >= ay && ax >= az) kz = 0;
+    else if (ay >= az) kz = 1;
+    else kz = 2;
+    int kx = kz + 1; if (kx == 3) kx = 0;
+    int ky = kx + 1; if (ky == 3) ky = 0;
+    if (d[kz] < 0.0) { int t = kx; kx = ky; ky = t; }
+    r->kx = kx; r->ky = ky; r->kz = kz;
+    r->Sx = d[kx] / d[kz];
+    r->Sy = d[ky] / d[kz];
+    r->Sz = 1.0 / d[kz];
+    for (int i = 0; i < 3; i++) {
+        r->dz[i] = d[i] == 0.0;
+        r->inv[i] = r->dz[i] ? 0.0 : 1.0 / d[i];
+    }
+}
+
+int ora_wray_triangle(const ora_wray* r, const float* p0, const float* p1, const float* p2,
+                      double* t_out, double* u_out, double* v_out)
+{
+    int kx = r->kx, ky = r->ky, kz = r->kz;
+    double A[3], B[3], C[3];
+    for (int i = 0; i < 3; i++) {
+        A[i] = (double)p0[i] - r->o[i];
+        B[i] = (double)p1[i] - r->o[i];
+        C[i] = (double)p2[i] - r->o[i];
+    }
+    double Ax = A[kx] - r->Sx * A[kz], Ay = A[ky] - r->Sy * A[kz];
+    double Bx = B[kx] - r->Sx * B[kz], By = B[ky] - r->Sy * B[kz];
+    double Cx = C[kx] - r->Sx * C[kz], Cy = C[ky] - r->Sy * C[kz];
+    double U = Cx * By - Cy * Bx;
+    double V = Ax * Cy - Ay * Cx;
+    double W = Bx * Ay - By * Ax;
+    if ((U < 0.0 || V < 0.0 || W < 0.0) && (U > 0.0 || V > 0.0 || W > 0.0)) return 0;
+    double det = U + V + W;
+    if (det == 0.0) return 0;
+    double Az = r->Sz * A[kz], Bz = r->Sz * B[kz], Cz = r->Sz * C[kz];
+    double T = U * Az + V * Bz + W * Cz;
+    double t = T / det;
+    if (!(t >= r->tmin && t <= r->tmax)) return 0;
+    *t_out = t;
+    *u_out = V / det;
+    *v_out = W / det;
+    return 1;
+}
+
+int ora_intersect_triangle(const float org[3], const float dir[3], float tmin, float tmax,
+                           const float v0[3], const float v1[3], const float v2[3],
+                           double* t, double* u, double* v)
+{
+    double o[3] = { org[0], org[1], org[2] }, d[3] = { dir[0], dir[1], dir[2] };
+    ora_wray r;
+    ora_wray_setup(&r, o, d, (double)tmin, (double)tmax);
+    return ora_wray_triangle(&r, v0, v1, v2, t, u, v);
+}
+
+/* world -> object ray: double evaluation, fixed summation order, rounded to float */
+void ora_object_ray(const float* w2o, const float org[3], const float dir[3], double o[3], double d[3])
+{
+    double ox = org[0], oy = org[1], oz = org[2];
+    double dx = dir[0], dy = dir[1], dz = dir[2];
+    for (int i = 0; i < 3; i++) {
+        const float* m = w2o + 4 * i;
+        o[i] = (double)(float)((((double)m[0] * ox + (double)m[1] * oy) + (double)m[2] * oz) + (double)m[3]);
+        d[i] = (double)(float)(((double)m[0] * dx + (double)m[1] * dy) + (double)m[2] * dz);
+    }
+}
+
+/* ---------------- CPU BVH (top-down median split) ---------------- */
+
+typedef struct { float lo[3], hi[3]; float c[3]; uint32_t id; } prim_ref;
+
+static int g_axis;
+static int cmp_axis(const void* a, const void* b)
+{
+    float ca = ((const prim_ref*)a)->c[g_axis], cb = ((const prim_ref*)b)->c[g_axis];
+    if (ca < cb) return -1;
+    if (ca > cb) return 1;
+    uint32_t ia = ((const prim_ref*)a)->id, ib = ((const prim_ref*)b)->id;
+    return ia < ib ? -1 : (ia > ib ? 1 : 0);
+}
+
+static uint32_t build_rec(ora_bvh* b, prim_ref* refs, uint32_t first, uint32_t count)
+{
+    uint32_t ni = b->nnodes++;
+    ora_bvh_node* n = &b->nodes[ni];
+    float lo[3] = { FLT_MAX, FLT_MAX, FLT_MAX }, hi[3] = { -FLT_MAX, -FLT_MAX, -FLT_MAX };
+    float clo[3] = { FLT_MAX, FLT_MAX, FLT_MAX }, chi[3] = { -FLT_MAX, -FLT_MAX, -FLT_MAX };
+    for (uint32_t i = first; i < first + count; i++) {
+        for (int k = 0; k < 3; k++) {
+            if (refs[i].lo[k] < lo[k]) lo[k] = refs[i].lo[k];
+            if (refs[i].hi[k] > hi[k]) hi[k] = refs[i].hi[k];
+            if (refs[i].c[k] < clo[k]) clo[k] = refs[i].c[k];
+            if (refs[i].c[k] > chi[k]) chi[k] = refs[i].c[k];
+        }
+    }
+    for (int k = 0; k < 3; k++) {
+        n->lo[k] = lo[k];
+        n->hi[k] = hi[k];
+    }
+    if (count <= 4) {
+        n->leaf = 1;
+        n->first = first;
+        n->count = count;
+        return ni;
+    }
+    int axis = 0;
+    float ext = chi[0] - clo[0];
+    if (chi[1] - clo[1] > ext) { axis = 1; ext = chi[1] - clo[1]; }
+    if (chi[2] - clo[2] > ext) { axis = 2; }
+    g_axis = axis;
+    qsort(refs + first, count, sizeof(prim_ref), cmp_axis);
+    uint32_t half = count / 2;
+    n->leaf = 0;
+    uint32_t l = build_rec(b, refs, first, half);
+    uint32_t r = build_rec(b, refs, first + half, count - half);
+    n = &b->nodes[ni];
+    n->left = l;
+    n->right = r;
+    return ni;
+}
+
+static void bvh_build(ora_bvh* b, prim_ref* refs, uint32_t count)
+{
+    b->nodes = (ora_bvh_node*)calloc(2 * (size_t)count + 1, sizeof(ora_bvh_node));
+    b->nnodes = 0;
+    b->prims = (uint32_t*)malloc(sizeof(uint32_t) * (count ? count : 1));
+    if (count == 0) return;
+    build_rec(b, refs, 0, count);
+    for (uint32_t i = 0; i < count; i++) b->prims[i] = refs[i].id;
+}
+
+ora_scene* ora_scene_create(const ora_mesh* meshes, uint32_t nmesh, const ora_instance* inst, uint32_t ninst)
+{
+    ora_scene* s = (ora_scene*)calloc(1, sizeof(ora_scene));
+    s->nmesh = nmesh;
+    s->ninst = ninst;
+    s->meshes = (ora_mesh*)malloc(sizeof(ora_mesh) * (nmesh ? nmesh : 1));
+    s->inst = (ora_instance*)malloc(sizeof(ora_instance) * (ninst ? ninst : 1));
+    s->blas = (ora_bvh*)calloc(nmesh ? nmesh : 1, sizeof(ora_bvh));
+    memcpy(s->meshes, meshes, sizeof(ora_mesh) * nmesh);
+    memcpy(s->inst, inst, sizeof(ora_instance) * ninst);
+    for (uint32_t m = 0; m < nmesh; m++) {
+        const ora_mesh* me = &meshes[m];
+        prim_ref* refs = (prim_ref*)malloc(sizeof(prim_ref) * (me->nt ? me->nt : 1));
+        for (uint32_t t = 0; t < me->nt; t++) {
+            prim_ref* r = &refs[t];
+            r->id = t;
+            for (int k = 0; k < 3; k++) { r->lo[k] = FLT_MAX; r->hi[k] = -FLT_MAX; }
+            for (int v = 0; v < 3; v++) {
+                const float* p = me->pos + 3 * (size_t)me->idx[3 * (size_t)t + v];
+                for (int k = 0; k < 3; k++) {
+                    if (p[k] < r->lo[k]) r->lo[k] = p[k];
+                    if (p[k] > r->hi[k]) r->hi[k] = p[k];
+                }
+            }
+            for (int k = 0; k < 3; k++) r->c[k] = 0.5f * (r->lo[k] + r->hi[k]);
+        }
+        bvh_build(&s->blas[m], refs, me->nt);
+        free(refs);
+    }
+    /* instance-level BVH over world boxes of the transformed BLAS roots */
+    prim_ref* refs = (prim_ref*)malloc(sizeof(prim_ref) * (ninst ? ninst : 1));
+    uint32_t nref = 0;
+    for (uint32_t i = 0; i < ninst; i++) {
+        const ora_instance* in = &inst[i];
+        const ora_bvh* b = &s->blas[in->mesh];
+        if (b->nnodes == 0) continue;
+        prim_ref* r = &refs[nref++];
+        r->id = i;
+        for (int k = 0; k < 3; k++) { r->lo[k] = FLT_MAX; r->hi[k] = -FLT_MAX; }
+        const ora_bvh_node* root = &b->nodes[0];
+        for (int c = 0; c < 8; c++) {
+            double p[3] = { (c & 1) ? root->hi[0] : root->lo[0], (c & 2) ? root->hi[1] : root->lo[1],
+                            (c & 4) ? root->hi[2] : root->lo[2] };
+            for (int k = 0; k < 3; k++) {
+                const float* m = in->o2w + 4 * k;
+                double w = m[0] * p[0] + m[1] * p[1] + m[2] * p[2] + m[3];
+                float wl = (float)w, wh = (float)w;
+                wl = nextafterf(wl, -FLT_MAX);
+                wh = nextafterf(wh, FLT_MAX);
+                if (wl < r->lo[k]) r->lo[k] = wl;
+                if (wh > r->hi[k]) r->hi[k] = wh;
+            }
+        }
+        for (int k = 0; k < 3; k++) r->c[k] = 0.5f * (r->lo[k] + r->hi[k]);
+    }
+    bvh_build(&s->tlas, refs, nref);
+    s->extent = 0.0;
+    for (uint32_t i = 0; i < nref; i++)
+        for (int k = 0; k < 3; k++) {
+            if (fabs(refs[i].lo[k]) > s->extent) s->extent = fabs(refs[i].lo[k]);
+            if (fabs(refs[i].hi[k]) > s->extent) s->extent = fabs(refs[i].hi[k]);
+        }
+    s->extent *= 3.0;
+    free(refs);
+    return s;
+}
+
+void ora_scene_destroy(ora_scene* s)
+{
+    if (!s) return;
+    for (uint32_t m = 0; m < s->nmesh; m++) {
+        free(s->blas[m].nodes);
+        free(s->blas[m].prims);
+    }
+    free(s->tlas.nodes);
+    free(s->tlas.prims);
+    free(s->blas);
+    free(s->meshes);
+    free(s->inst);
+    free(s);
+}
+
+/* Slab test in double on the (float-valued) ray.  pad widens the box by an
+ * absolute amount; the far distance is widened by a relative 1e-12, far above
+ * the double rounding of the slab distances. */
+static int box_hit(const ora_wray* r, const float lo[3], const float hi[3], double pad, double tmin, double tmax)
+{
+    for (int k = 0; k < 3; k++) {
+        double l = (double)lo[k] - pad, h = (double)hi[k] + pad;
+        if (r->dz[k]) {
+            if (r->o[k] < l || r->o[k] > h) return 0;
+            continue;
+        }
+        double t0 = (l - r->o[k]) * r->inv[k];
+        double t1 = (h - r->o[k]) * r->inv[k];
+        if (t0 > t1) { double t = t0; t0 = t1; t1 = t; }
+        if (t0 > tmin) tmin = t0;
+        if (t1 < tmax) tmax = t1;
+    }
+    return tmin <= tmax * (1.0 + 1e-12) + 1e-300;
+}
+
+static inline int better(double t, uint32_t inst, uint32_t prim, double bt, uint32_t binst, uint32_t bprim)
+{
+    if (t < bt) return 1;
+    if (t > bt) return 0;
+    if (inst != binst) return inst < binst;
+    return prim < bprim;
+}
+
+typedef struct {
+    double t, u, v;
+    uint32_t inst, prim;
+    int found;
+} best_hit;
+
+/* Closest (any_hit = 0) or first found (any_hit = 1) over one instance's BLAS. */
+static int blas_trace(const ora_scene* s, uint32_t ii, const ora_ray* ray, int any_hit, best_hit* best)
+{
+    const ora_instance* in = &s->inst[ii];
+    const ora_mesh* me = &s->meshes[in->mesh];
+    const ora_bvh* b = &s->blas[in->mesh];
+    if (b->nnodes == 0) return 0;
+    double o[3], d[3];
+    ora_object_ray(in->w2o, ray->org, ray->dir, o, d);
+    ora_wray wr;
+    ora_wray_setup(&wr, o, d, (double)ray->tmin, (double)ray->tmax);
+    uint32_t stack[128];
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp) {
+        const ora_bvh_node* n = &b->nodes[stack[--sp]];
+        double tlim = best->found ? best->t : wr.tmax;
+        if (!box_hit(&wr, n->lo, n->hi, 0.0, wr.tmin, tlim)) continue;
+        if (n->leaf) {
+            for (uint32_t i = n->first; i < n->first + n->count; i++) {
+                uint32_t prim = b->prims[i];
+                const uint32_t* tri = me->idx + 3 * (size_t)prim;
+                double t, u, v;
+                if (!ora_wray_triangle(&wr, me->pos + 3 * (size_t)tri[0], me->pos + 3 * (size_t)tri[1],
+                                       me->pos + 3 * (size_t)tri[2], &t, &u, &v))
+                    continue;
+                if (!best->found || better(t, ii, prim, best->t, best->inst, best->prim)) {
+                    best->found = 1; best->t = t; best->u = u; best->v = v; best->inst = ii; best->prim = prim;
+                    if (any_hit) return 1;
+                }
+            }
+        } else {
+            stack[sp++] = n->left;
+            stack[sp++] = n->right;
+        }
+    }
+    return best->found;
+}
+
+static void trace_one(const ora_scene* s, const ora_ray* ray, int any_hit, best_hit* best)
+{
+    best->found = 0;
+    if (s->tlas.nnodes == 0) return;
+    double o[3] = { ray->org[0], ray->org[1], ray->org[2] }, d[3] = { ray->dir[0], ray->dir[1], ray->dir[2] };
+    ora_wray wr;
+    ora_wray_setup(&wr, o, d, (double)ray->tmin, (double)ray->tmax);
+    /* Instance boxes are world boxes of the transformed BLAS roots, while the
+     * triangle tests run on the rounded object-space ray: widen them by a pad
+     * that bounds that rounding for any hit inside the scene. */
+    double on = fabs(o[0]) + fabs(o[1]) + fabs(o[2]);
+    double pad = 1e-4 * (on + s->extent);
+    uint32_t stack[128];
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp) {
+        const ora_bvh_node* n = &s->tlas.nodes[stack[--sp]];
+        double tlim = best->found ? best->t : wr.tmax;
+        if (!box_hit(&wr, n->lo, n->hi, pad, wr.tmin, tlim * (1.0 + 1e-6))) continue;
+        if (n->leaf) {
+            for (uint32_t i = n->first; i < n->first + n->count; i++) {
+                if (blas_trace(s, s->tlas.prims[i], ray, any_hit, best) && any_hit) return;
+            }
+        } else {
+            stack[sp++] = n->left;
+            stack[sp++] = n->right;
+        }
+    }
+}
+
+void ora_trace_closest_one(const ora_scene* s, const ora_ray* ray, ora_hit* hit)
+{
+    best_hit b;
+    trace_one(s, ray, 0, &b);
+    if (b.found) {
+        hit->t = (float)b.t; hit->inst = b.inst; hit->prim = b.prim; hit->u = (float)b.u; hit->v = (float)b.v;
+    } else {
+        hit->t = INFINITY; hit->inst = 0xFFFFFFFFu; hit->prim = 0xFFFFFFFFu; hit->u = 0.0f; hit->v = 0.0f;
+    }
+}
+
+int ora_trace_any_one(const ora_scene* s, const ora_ray* ray)
+{
+    best_hit b;
+    trace_one(s, ray, 1, &b);
+    return b.found;
+}
+
+/* ---------------- threaded batch entry points ---------------- */
+
+typedef struct {
+    const ora_scene* s; const ora_ray* rays; uint32_t n; ora_hit* hits; uint8_t* occ;
+    int tid, nthreads;
+} trace_job;
+
+static void* trace_worker(void* p)
+{
+    trace_job* j = (trace_job*)p;
+    for (uint32_t i = (uint32_t)j->tid; i < j->n; i += (uint32_t)j->nthreads) {
+        if (j->hits) ora_trace_closest_one(j->s, &j->rays[i], &j->hits[i]);
+        else j->occ[i] = (uint8_t)ora_trace_any_one(j->s, &j->rays[i]);
+    }
+    return NULL;
+}
+
+static void run_trace(const ora_scene* s, const ora_ray* rays, uint32_t n, ora_hit* hits, uint8_t* occ, int nthreads)
+{
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    pthread_t th[256];
+    trace_job jobs[256];
+    for (int t = 0; t < nthreads; t++) {
+        jobs[t] = (trace_job){ s, rays, n, hits, occ, t, nthreads };
+        pthread_create(&th[t], NULL, trace_worker, &jobs[t]);
+    }
+    for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+}
+
+void ora_trace_closest(const ora_scene* s, const ora_ray* rays, uint32_t n, ora_hit* hits, int nthreads)
+{
+    run_trace(s, rays, n, hits, NULL, nthreads);
+}
+
+void ora_trace_any(const ora_scene* s, const ora_ray* rays, uint32_t n, uint8_t* occ, int nthreads)
+{
+    run_trace(s, rays, n, NULL, occ, nthreads);
+}
+
+/* Brute force: every instance x every triangle, no BVH. */
+void ora_trace_closest_brute(const ora_scene* s, const ora_ray* rays, uint32_t n, ora_hit* hits)
+{
+    for (uint32_t r = 0; r < n; r++) {
+        best_hit b;
+        b.found = 0;
+        for (uint32_t ii = 0; ii < s->ninst; ii++) {
+            const ora_instance* in = &s->inst[ii];
+            const ora_mesh* me = &s->meshes[in->mesh];
+            double o[3], d[3];
+            ora_object_ray(in->w2o, rays[r].org, rays[r].dir, o, d);
+            ora_wray wr;
+            ora_wray_setup(&wr, o, d, (double)rays[r].tmin, (double)rays[r].tmax);
+            for (uint32_t t = 0; t < me->nt; t++) {
+                const uint32_t* tri = me->idx + 3 * (size_t)t;
+                double tt, u, v;
+                if (!ora_wray_triangle(&wr, me->pos + 3 * (size_t)tri[0], me->pos + 3 * (size_t)tri[1],
+                                       me->pos + 3 * (size_t)tri[2], &tt, &u, &v))
+                    continue;
+                if (!b.found || better(tt, ii, t, b.t, b.inst, b.prim)) {
+                    b.found = 1; b.t = tt; b.u = u; b.v = v; b.inst = ii; b.prim = t;
+                }
+            }
+        }
+        if (b.found) {
+            hits[r].t = (float)b.t; hits[r].inst = b.inst; hits[r].prim = b.prim;
+            hits[r].u = (float)b.u; hits[r].v = (float)b.v;
+        } else {
+            hits[r].t = INFINITY; hits[r].inst = 0xFFFFFFFFu; hits[r].prim = 0xFFFFFFFFu;
+            hits[r].u = 0.0f; hits[r].v = 0.0f;
+        }
+    }
+}
+
+void ora_trace_any_brute(const ora_scene* s, const ora_ray* rays, uint32_t n, uint8_t* occ)
+{
+    for (uint32_t r = 0; r < n; r++) {
+        occ[r] = 0;
+        for (uint32_t ii = 0; ii < s->ninst && !occ[r]; ii++) {
+            const ora_instance* in = &s->inst[ii];
+            const ora_mesh* me = &s->meshes[in->mesh];
+            double o[3], d[3];
+            ora_object_ray(in->w2o, rays[r].org, rays[r].dir, o, d);
+            ora_wray wr;
+            ora_wray_setup(&wr, o, d, (double)rays[r].tmin, (double)rays[r].tmax);
+            for (uint32_t t = 0; t < me->nt; t++) {
+                const uint32_t* tri = me->idx + 3 * (size_t)t;
+                double tt, u, v;
+                if (ora_wray_triangle(&wr, me->pos + 3 * (size_t)tri[0], me->pos + 3 * (size_t)tri[1],
+                                      me->pos + 3 * (size_t)tri[2], &tt, &u, &v)) {
+                    occ[r] = 1;
+                    break;
+                }
+            }
+        }
+    }
+}

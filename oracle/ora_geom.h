@@ -1,0 +1,44 @@
+/* Oracle-internal scene and ray types (test infrastructure only). */
+#ifndef ORA_GEOM_H
+#define ORA_GEOM_H
+
+#include "ora.h"
+
+typedef struct {
+    double o[3], d[3], inv[3];
+    int dz[3];
+    int kx, ky, kz;
+    double Sx, Sy, Sz;
+    double tmin, tmax;
+} ora_wray;
+
+typedef struct {
+    float lo[3], hi[3];
+    int leaf;
+    uint32_t left, right, first, count;
+} ora_bvh_node;
+
+typedef struct {
+    ora_bvh_node* nodes;
+    uint32_t nnodes;
+    uint32_t* prims;
+} ora_bvh;
+
+struct ora_scene {
+    ora_mesh* meshes;
+    uint32_t nmesh;
+    ora_instance* inst;
+    uint32_t ninst;
+    ora_bvh* blas;
+    ora_bvh tlas;
+    double extent;   /* 3 x max |coordinate| of the instance world boxes */
+};
+
+void ora_wray_setup(ora_wray* r, const double o[3], const double d[3], double tmin, double tmax);
+int ora_wray_triangle(const ora_wray* r, const float* p0, const float* p1, const float* p2,
+                      double* t, double* u, double* v);
+void ora_object_ray(const float* w2o, const float org[3], const float dir[3], double o[3], double d[3]);
+void ora_trace_closest_one(const ora_scene* s, const ora_ray* ray, ora_hit* hit);
+int ora_trace_any_one(const ora_scene* s, const ora_ray* ray);
+
+#endif

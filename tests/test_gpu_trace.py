@@ -1,0 +1,200 @@
+"""GPU ray-query parity: libaph.so traversal vs the FP64 oracle.
+
+Bar: closest-hit instance/primitive IDs, t and barycentrics are bit-exact
+(the oracle and the kernels evaluate the same IEEE double formula); any-hit
+booleans are exact.  Hit distances therefore also agree with the FP64 oracle
+far inside the 1e-6 relative tolerance of the specification.
+"""
+import numpy as np
+import pytest
+
+import pyoracle
+from aurora_amd import capi, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def random_rays(rng, n, lo, hi, tmin=0.0):
+    rays = np.zeros(n, dtype=capi.RAY_DTYPE)
+    rays["org"] = rng.uniform(lo, hi, (n, 3))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays["dir"] = d
+    rays["tmin"] = tmin
+    rays["tmax"] = np.inf
+    return rays
+
+
+def targeted_rays(rng, sc, n, tmin=0.0):
+    """Rays aimed at random points of random instances (high hit rate, grazing cases)."""
+    rays = np.zeros(n, dtype=capi.RAY_DTYPE)
+    for k in range(n):
+        ii = rng.integers(0, len(sc.instances))
+        inst = sc.instances[ii]
+        pos, _, idx = sc.meshes[inst["mesh"]]
+        tri = idx[rng.integers(0, len(idx))]
+        b = rng.dirichlet([1, 1, 1])
+        p = (pos[tri] * b[:, None]).sum(0)
+        m = inst["o2w"].reshape(3, 4).astype(np.float64)
+        w = m[:, :3] @ p + m[:, 3]
+        o = w + rng.normal(size=3) * rng.uniform(0.5, 20)
+        d = w - o
+        d /= np.linalg.norm(d)
+        rays[k]["org"] = o
+        rays[k]["dir"] = d
+    rays["tmin"] = tmin
+    rays["tmax"] = np.inf
+    return rays
+
+
+def camera_rays(sc, step=4):
+    fr = sc.frame
+    right = np.array(fr.inv_view[0:3])
+    up = np.array(fr.inv_view[4:7])
+    front = -np.array(fr.inv_view[8:11])
+    eye = np.array(fr.inv_view[12:15])
+    ys, xs = np.mgrid[0:sc.height:step, 0:sc.width:step]
+    nx = ((xs + 0.5) / sc.width) * 2 - 1
+    ny = -(((ys + 0.5) / sc.height) * 2 - 1)
+    d = front[None, None, :] + right * (fr.view_size[0] * 0.5 * nx)[..., None] + up * (fr.view_size[1] * 0.5 * ny)[..., None]
+    d /= np.linalg.norm(d, axis=-1, keepdims=True)
+    rays = np.zeros(d.shape[0] * d.shape[1], dtype=capi.RAY_DTYPE)
+    rays["org"] = eye
+    rays["dir"] = d.reshape(-1, 3)
+    rays["tmax"] = np.inf
+    return rays
+
+
+def assert_hits_equal(gpu, ref):
+    assert gpu.dtype == ref.dtype
+    g = gpu.view(np.uint8).reshape(len(gpu), -1)
+    r = ref.view(np.uint8).reshape(len(ref), -1)
+    diff = np.nonzero((g != r).any(axis=1))[0]
+    if len(diff):
+        i = diff[0]
+        raise AssertionError(f"{len(diff)} / {len(gpu)} hits differ; first ray {i}: gpu={gpu[i]} oracle={ref[i]}")
+
+
+def _scenes(teapot_path):
+    return [
+        ("teapot", scenes.teapot_scene(teapot_path)),
+        ("config1", scenes.config1()),
+        ("config2_small", scenes.config2(subdiv=2, env_w=64, env_h=32)),
+    ]
+
+
+@pytest.mark.parametrize("which", [0, 1, 2])
+def test_closest_and_any_match_oracle(renderer, teapot_path, which):
+    name, sc = _scenes(teapot_path)[which]
+    rng = np.random.default_rng(100 + which)
+    tlas = renderer.load_scene(sc)
+    ora = pyoracle.OracleScene(sc)
+    ext = 15.0 if name != "teapot" else 3.0
+    sets = [random_rays(rng, 20000, -ext, ext), targeted_rays(rng, sc, 3000), targeted_rays(rng, sc, 2000, 0.01),
+            camera_rays(sc) if sc.width else random_rays(rng, 10, -1, 1)]
+    for rays in sets:
+        g = renderer.trace_closest(tlas, rays)
+        o = ora.trace_closest(rays)
+        assert (o["inst"] != 0xFFFFFFFF).sum() > 0
+        assert_hits_equal(g, o)
+        ga = renderer.trace_any(tlas, rays)
+        oa = ora.trace_any(rays)
+        np.testing.assert_array_equal(ga, oa)
+
+
+def test_oracle_bvh_equals_brute_force_on_gpu_rays(renderer, teapot_path):
+    sc = scenes.teapot_scene(teapot_path)
+    rng = np.random.default_rng(7)
+    rays = targeted_rays(rng, sc, 500)
+    tlas = renderer.load_scene(sc)
+    ora = pyoracle.OracleScene(sc)
+    assert_hits_equal(renderer.trace_closest(tlas, rays), ora.trace_closest(rays, brute=True))
+
+
+def test_shared_edge_and_vertex_rays_are_watertight(renderer):
+    """A fan of triangles around a vertex: rays through shared edges and the shared
+    vertex must hit exactly one triangle each (never fall through)."""
+    n = 16
+    ang = np.linspace(0, 2 * np.pi, n, endpoint=False)
+    pos = np.concatenate([[[0, 0, 0]], np.stack([np.cos(ang), np.sin(ang), np.zeros(n)], 1)]).astype(np.float32)
+    idx = np.array([[0, 1 + i, 1 + (i + 1) % n] for i in range(n)], dtype=np.uint32)
+    sc = scenes.Scene()
+    sc.meshes = [(pos, None, idx)]
+    sc.instances = scenes._instances([(0, 0, scenes.affine())])
+    sc.materials = [capi.default_material()]
+    sc.env = {}
+    tlas = renderer.load_scene(sc)
+    ora = pyoracle.OracleScene(sc)
+    pts = [np.zeros(3)] + [0.5 * np.array([np.cos(a), np.sin(a), 0]) for a in ang]
+    rays = np.zeros(len(pts) * 8, dtype=capi.RAY_DTYPE)
+    rng = np.random.default_rng(3)
+    k = 0
+    for p in pts:
+        for _ in range(8):
+            d = np.array([rng.normal() * 0.3, rng.normal() * 0.3, 1.0])
+            d /= np.linalg.norm(d)
+            rays[k]["org"] = (p - 2.0 * d).astype(np.float32)
+            rays[k]["dir"] = d
+            rays[k]["tmax"] = np.inf
+            k += 1
+    g = renderer.trace_closest(tlas, rays)
+    assert (g["inst"] == 0).all(), "a ray through a shared edge/vertex fell through"
+    assert_hits_equal(g, ora.trace_closest(rays))
+
+
+def test_degenerate_inputs(renderer):
+    """Zero-direction components, rays in the plane of an axis-aligned quad, tmin/tmax limits,
+    an empty mesh and an instance-less TLAS."""
+    gp, gn, gi = scenes.ground_quad(1.0)
+    sc = scenes.Scene()
+    empty_pos = np.zeros((3, 3), dtype=np.float32)
+    sc.meshes = [(gp, gn, gi), (empty_pos, None, np.zeros((0, 3), dtype=np.uint32))]
+    sc.instances = scenes._instances([(0, 0, scenes.affine()), (1, 0, scenes.affine())])
+    sc.materials = [capi.default_material()]
+    sc.env = {}
+    tlas = renderer.load_scene(sc)
+    ora = pyoracle.OracleScene(sc)
+    rays = np.zeros(8, dtype=capi.RAY_DTYPE)
+    rays["tmax"] = np.inf
+    rays[0]["org"], rays[0]["dir"] = (0.2, 1.0, 0.3), (0.0, -1.0, 0.0)      # axis-aligned down
+    rays[1]["org"], rays[1]["dir"] = (-2.0, 0.0, 0.0), (1.0, 0.0, 0.0)      # in the plane of the quad
+    rays[2]["org"], rays[2]["dir"] = (1.0, 1.0, 1.0), (0.0, -1.0, 0.0)      # exactly through a corner
+    rays[3]["org"], rays[3]["dir"] = (0.0, 1.0, 0.0), (0.0, -1.0, 0.0)      # through the shared diagonal
+    rays[4]["org"], rays[4]["dir"] = (0.2, 1.0, 0.3), (0.0, -1.0, 0.0)
+    rays[4]["tmax"] = 0.5                                                     # tmax before the hit
+    rays[5]["org"], rays[5]["dir"] = (0.2, 1.0, 0.3), (0.0, -1.0, 0.0)
+    rays[5]["tmin"] = 1.5                                                     # tmin after the hit
+    rays[6]["org"], rays[6]["dir"] = (0.2, 1.0, 0.3), (0.0, 1.0, 0.0)       # pointing away
+    rays[7]["org"], rays[7]["dir"] = (0.2, 1.0, 0.3), (0.0, -1.0, 0.0)
+    rays[7]["tmax"] = 1.0                                                     # t == tmax exactly
+    g = renderer.trace_closest(tlas, rays)
+    o = ora.trace_closest(rays)
+    assert_hits_equal(g, o)
+    assert g[0]["inst"] == 0 and g[0]["t"] == 1.0
+    assert g[4]["inst"] == 0xFFFFFFFF and g[5]["inst"] == 0xFFFFFFFF and g[6]["inst"] == 0xFFFFFFFF
+    assert g[7]["inst"] == 0 and g[7]["t"] == 1.0
+    np.testing.assert_array_equal(renderer.trace_any(tlas, rays), ora.trace_any(rays))
+    # TLAS with no instances: everything misses
+    t0 = renderer.tlas(np.zeros(0, dtype=capi.INSTANCE_DTYPE))
+    assert (renderer.trace_closest(t0, rays)["inst"] == 0xFFFFFFFF).all()
+    assert (renderer.trace_any(t0, rays) == 0).all()
+
+
+def test_transformed_instances_scaled_rotated(renderer):
+    """Non-uniform scale + rotation + translation; instance IDs and t must match the oracle."""
+    rng = np.random.default_rng(11)
+    p, f = scenes.icosphere(2)
+    sc = scenes.Scene()
+    sc.meshes = [(p, scenes.vertex_normals(p, f), f)]
+    inst = []
+    for i in range(50):
+        r = scenes.random_rotation(rng) @ np.diag(rng.uniform(0.3, 2.0, 3))
+        inst.append((0, 0, scenes.affine(r, 1.0, rng.uniform(-10, 10, 3))))
+    sc.instances = scenes._instances(inst)
+    sc.materials = [capi.default_material()]
+    sc.env = {}
+    tlas = renderer.load_scene(sc)
+    ora = pyoracle.OracleScene(sc)
+    for rays in (random_rays(rng, 20000, -12, 12), targeted_rays(rng, sc, 3000)):
+        assert_hits_equal(renderer.trace_closest(tlas, rays), ora.trace_closest(rays))
+        np.testing.assert_array_equal(renderer.trace_any(tlas, rays), ora.trace_any(rays))
