@@ -1,0 +1,190 @@
+"""Pins the oracle's Standard Surface BSDF (StandardSurfaceBSDF.slang) with
+closed forms written independently from the published lobe models:
+Lambert, qualitative Oren-Nayar (1994, angle form), GGX (Walter 2007) with the
+height-correlated Smith G2 (Heitz 2014 eq. 99), Schlick Fresnel, Turquin
+energy compensation over the Karis curve-fit albedo, the conductor F0 = r
+property of the Gulbrandsen artistic IOR mapping; the sheen LUT at grid points
+(data of StandardSurfaceBSDF.slang:308-337); and, for the metal lobe, Monte
+Carlo consistency between VNDF sampling (Heitz 2018) and evaluation.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+import pyoracle
+from aurora_amd import capi
+
+FRAME = pyoracle.Frame3((0, 0, 1), (1, 0, 0), (0, 1, 0))
+
+
+def mat(**kw):
+    m = capi.default_material()
+    for k, v in kw.items():
+        if isinstance(v, (tuple, list)):
+            arr = getattr(m, k)
+            for i in range(3):
+                arr[i] = v[i]
+        else:
+            setattr(m, k, v)
+    return m
+
+
+def evaluate(m, V, L):
+    out = np.zeros(3, dtype=np.float32)
+    pyoracle.load().ora_bsdf_evaluate(C.byref(m), C.byref(FRAME), pyoracle._p(pyoracle.f32(*V)),
+                                      pyoracle._p(pyoracle.f32(*L)), pyoracle._p(out))
+    return out.astype(np.float64)
+
+
+def sample(m, V, r):
+    out = np.zeros(3, dtype=np.float32)
+    L = np.zeros(3, dtype=np.float32)
+    pdf = C.c_float()
+    lobe = C.c_int()
+    pyoracle.load().ora_bsdf_sample(C.byref(m), C.byref(FRAME), pyoracle._p(pyoracle.f32(*V)),
+                                    pyoracle._p(pyoracle.f32(*r)), pyoracle._p(out), pyoracle._p(L), C.byref(pdf),
+                                    C.byref(lobe))
+    return out.astype(np.float64), L.astype(np.float64), pdf.value, lobe.value
+
+
+def eval_pdf(m, V, L, r):
+    out = np.zeros(3, dtype=np.float32)
+    pdf = C.c_float()
+    lobe = C.c_int()
+    pyoracle.load().ora_bsdf_evaluate_pdf(C.byref(m), C.byref(FRAME), pyoracle._p(pyoracle.f32(*V)),
+                                          pyoracle._p(pyoracle.f32(*L)), pyoracle._p(pyoracle.f32(*r)),
+                                          pyoracle._p(out), C.byref(pdf), C.byref(lobe))
+    return out.astype(np.float64), pdf.value, lobe.value
+
+
+def hemi_dirs(rng, n, min_z=0.05):
+    d = rng.normal(size=(n, 3))
+    d[:, 2] = np.abs(d[:, 2])
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return d[d[:, 2] > min_z]
+
+
+# independent closed forms ---------------------------------------------------
+
+def ggx_D(nh, a):
+    return a * a / (math.pi * (nh * nh * (a * a - 1) + 1) ** 2)
+
+
+def smith_lambda(cos, a):
+    tan2 = (1 - cos * cos) / (cos * cos)
+    return (-1 + math.sqrt(1 + a * a * tan2)) / 2
+
+
+def schlick(cos, f0):
+    return f0 + (1 - f0) * (1 - cos) ** 5
+
+
+def karis_albedo(nv, rough, f0, f90):
+    c0 = np.array([-1, -0.0275, -0.572, 0.022])
+    c1 = np.array([1, 0.0425, 1.04, -0.04])
+    r = rough * c0 + c1
+    a004 = min(r[0] * r[0], 2 ** (-9.28 * nv)) * r[0] + r[1]
+    return f0 * (-1.04 * a004 + r[2]) + f90 * (1.04 * a004 + r[3])
+
+
+def test_lambert_closed_form():
+    rng = np.random.default_rng(1)
+    c = (0.2, 0.5, 0.9)
+    m = mat(base=1.0, base_color=c, specular=0.0)
+    for V in hemi_dirs(rng, 30):
+        for L in hemi_dirs(rng, 5):
+            np.testing.assert_allclose(evaluate(m, V, L), np.array(c) * L[2] / math.pi, rtol=2e-6)
+    # opposite hemispheres: no reflection
+    assert (evaluate(m, (0, 0, 1), (0, 0.6, -0.8)) == 0).all()
+
+
+def test_oren_nayar_angle_form():
+    rng = np.random.default_rng(2)
+    rough = 0.4
+    m = mat(base=1.0, base_color=(1, 1, 1), specular=0.0, diffuse_roughness=rough)
+    sigma = rough * math.pi
+    s2 = sigma * sigma
+    A = 1 - 0.5 * s2 / (s2 + 0.33)
+    B = 0.45 * s2 / (s2 + 0.09)
+    for V in hemi_dirs(rng, 20):
+        for L in hemi_dirs(rng, 5):
+            ti, tr = math.acos(L[2]), math.acos(V[2])
+            dphi = math.atan2(L[1], L[0]) - math.atan2(V[1], V[0])
+            alpha, beta = max(ti, tr), min(ti, tr)
+            on = A + B * max(0.0, math.cos(dphi)) * math.sin(alpha) * math.tan(beta)
+            np.testing.assert_allclose(evaluate(m, V, L), L[2] / math.pi * on, rtol=1e-4)
+
+
+def test_dielectric_specular_closed_form():
+    rng = np.random.default_rng(3)
+    for rough in (0.15, 0.4, 0.8):
+        m = mat(base=0.0, specular=1.0, specular_roughness=rough, specular_ior=1.5)
+        a = rough * rough
+        f0 = ((1.5 - 1) / (1.5 + 1)) ** 2
+        for V in hemi_dirs(rng, 25, 0.1):
+            for L in hemi_dirs(rng, 4, 0.1):
+                H = (V + L) / np.linalg.norm(V + L)
+                F = schlick(V @ H, f0)
+                G = 1 / (1 + smith_lambda(V[2], a) + smith_lambda(L[2], a))
+                Ess = karis_albedo(V[2], a, 1, 1)
+                comp = 1 + F * (1 - Ess) / Ess
+                exp = ggx_D(H[2], a) * F * G * comp / (4 * V[2])
+                np.testing.assert_allclose(evaluate(m, V, L), exp, rtol=3e-4)
+
+
+def test_conductor_normal_incidence_reflectivity():
+    """Gulbrandsen mapping: the conductor Fresnel at normal incidence equals the reflectivity."""
+    for r in (0.1, 0.5, 0.9):
+        m = mat(base=1.0, base_color=(r, r, r), metalness=1.0, specular=1.0, specular_color=(1, 1, 1),
+                specular_roughness=0.3)
+        a = 0.09
+        Ess = karis_albedo(1.0, a, 1, 1)
+        V = L = np.array([0.0, 0.0, 1.0])
+        comp = 1 + r * (1 - Ess) / Ess
+        exp = ggx_D(1.0, a) * r * comp * (1 / (1 + 0 + 0)) / 4
+        np.testing.assert_allclose(evaluate(m, V, L), exp, rtol=2e-4)
+
+
+@pytest.mark.parametrize("k,j", [(3, 0), (5, 2), (0, 6), (15, 15), (8, 9)])
+def test_sheen_lut_grid_values(k, j):
+    table = {(3, 0): 0.391714, (5, 2): 0.322736, (0, 6): 0.695353, (15, 15): 0.147049, (8, 9): 0.259885}
+    v = pyoracle.load().ora_sheen_albedo(np.float32(k / 15), np.float32(j / 15))
+    assert v == pytest.approx(table[(k, j)], rel=1e-5)
+
+
+def test_metal_sampling_is_consistent_with_evaluation():
+    """E[f/pdf] under VNDF sampling equals the uniform-sphere integral of the evaluated BSDF."""
+    rng = np.random.default_rng(4)
+    m = mat(base=1.0, base_color=(0.9, 0.7, 0.4), metalness=1.0, specular_roughness=0.5)
+    V = np.array([0.3, 0.1, 0.948683])
+    V /= np.linalg.norm(V)
+    n = 20000
+    est = np.zeros(3)
+    for r in rng.uniform(0, 1, (n, 2)):
+        f, L, pdf, lobe = sample(m, V, r)
+        assert lobe == 3
+        if pdf > 0 and (f > 0).any():
+            est += f / pdf
+            if rng.uniform() < 0.02:  # the pdf of the sampled direction, recomputed by evaluateMaterialAndPDF
+                _, pdf2, lobe2 = eval_pdf(m, V, L, r)
+                assert lobe2 == 3 and pdf2 == pytest.approx(pdf, rel=1e-3)
+    est /= n
+    ref = np.zeros(3)
+    dirs = rng.normal(size=(4 * n, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    for L in dirs:
+        if L[2] > 0:
+            ref += evaluate(m, V, L)
+    ref *= 4 * math.pi / len(dirs)
+    np.testing.assert_allclose(est, ref, rtol=0.04)
+
+
+def test_lambert_white_furnace():
+    rng = np.random.default_rng(5)
+    m = mat(base=1.0, base_color=(1, 1, 1), specular=0.0)
+    dirs = rng.normal(size=(40000, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    tot = sum(evaluate(m, (0.2, 0.0, 0.9797959), L)[0] for L in dirs if L[2] > 0)
+    assert tot * 4 * math.pi / len(dirs) == pytest.approx(1.0, rel=0.02)

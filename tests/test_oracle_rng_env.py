@@ -1,0 +1,209 @@
+"""Pins the oracle's RNG, transforms, environment mapping and alias map.
+
+RNG (Random.slang:96-146): pcg2DNext2D assigns the returned mix of the *old*
+state to rng.state, overwriting the LCG advance pcg2D made through its inout
+parameter (the copy-out happens before the assignment).  So one draw is the
+Jarzynski & Olano pcg2d (JCGT 9(3), 2020) output for the state whose LCG image
+is the current state.  The test implements the published pcg2d (LCG then two
+mixing rounds) independently and checks the oracle against it through the
+inverse LCG.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+import pyoracle
+from aurora_amd import capi, scenes
+
+M32 = np.uint64(0xFFFFFFFF)
+A_LCG = 1664525
+C_LCG = 1013904223
+A_INV = pow(A_LCG, -1, 1 << 32)
+
+
+def jcgt_pcg2d(vx, vy):
+    """pcg2d as published (JCGT 9(3) listing): v = v*1664525 + 1013904223, then mixing."""
+    vx = (vx * A_LCG + C_LCG) & 0xFFFFFFFF
+    vy = (vy * A_LCG + C_LCG) & 0xFFFFFFFF
+    for _ in range(2):
+        vx = (vx + vy * A_LCG) & 0xFFFFFFFF
+        vy = (vy + vx * A_LCG) & 0xFFFFFFFF
+        vx ^= vx >> 16
+        vy ^= vy >> 16
+    return vx, vy
+
+
+def oracle_stream(sx, sy, n):
+    out = np.zeros(2 * n, dtype=np.float32)
+    pyoracle.load().ora_random_stream(sx, sy, n, pyoracle._p(out))
+    return out.reshape(n, 2)
+
+
+def oracle_state_after(sx, sy, n):
+    st = np.array([sx, sy], dtype=np.uint32)
+    for _ in range(n):
+        pyoracle.load().ora_pcg2d(pyoracle._p(st))
+    return int(st[0]), int(st[1])
+
+
+def test_pcg2d_matches_published_hash_through_inverse_lcg():
+    rng = np.random.default_rng(0)
+    for sx, sy in [(0, 0), (1, 0), (123456, 7), (0xFFFFFFFF, 0xFFFFFFFF)] + [tuple(map(int, rng.integers(0, 2**32, 2))) for _ in range(50)]:
+        # the LCG pre-image of (sx, sy)
+        px = ((sx - C_LCG) * A_INV) & 0xFFFFFFFF
+        py = ((sy - C_LCG) * A_INV) & 0xFFFFFFFF
+        assert jcgt_pcg2d(px, py) == oracle_state_after(sx, sy, 1)
+
+
+def test_pcg2d_init_is_pixel_index_and_sample_index():
+    st = np.zeros(2, dtype=np.uint32)
+    pyoracle.load().ora_pcg2d_init(17, 512, 3, 9, pyoracle._p(st))
+    assert tuple(st) == (9 * 512 + 3, 17)
+
+
+def test_uniform_float_conversion_reaches_one():
+    """u32 / float(0xFFFFFFFF): both round to float, so u >= 2^32 - 128 gives exactly 1.0."""
+    found = None
+    xs = np.arange(0, 1 << 22, dtype=np.uint64)
+    for base in range(0, 64):
+        vx = (xs + (base << 22)) & M32
+        vy = np.zeros_like(vx)
+        for _ in range(2):
+            vx = (vx + vy * A_LCG) & M32
+            vy = (vy + vx * A_LCG) & M32
+            vx ^= vx >> np.uint64(16)
+            vy ^= vy >> np.uint64(16)
+        hit = np.nonzero(vx >= 0xFFFFFF80)[0]
+        if len(hit):
+            found = int(xs[hit[0]] + (base << 22))
+            break
+    assert found is not None
+    r = oracle_stream(found, 0, 1)[0]
+    assert r[0] == 1.0
+
+
+def test_uniformity_chi2():
+    draws = oracle_stream(5, 77, 200000).reshape(-1)
+    hist, _ = np.histogram(draws, bins=64, range=(0, 1))
+    exp = len(draws) / 64
+    chi2 = ((hist - exp) ** 2 / exp).sum()
+    assert chi2 < 120  # 63 dof, p ~ 1e-5
+
+
+def test_inverse3x4_definitions_agree_bitwise():
+    lib = capi.load()
+    rng = np.random.default_rng(1)
+    for _ in range(200):
+        r = scenes.random_rotation(rng) @ np.diag(rng.uniform(0.1, 3, 3))
+        m = scenes.affine(r, 1.0, rng.uniform(-50, 50, 3))
+        a = np.zeros(12, dtype=np.float32)
+        b = np.zeros(12, dtype=np.float32)
+        assert lib.aph_inverse3x4(capi.ptr(m), capi.ptr(a)) == 0
+        assert pyoracle.load().ora_inverse3x4(pyoracle._p(m), pyoracle._p(b)) == 0
+        np.testing.assert_array_equal(a, b)
+        np.testing.assert_array_equal(a, scenes.inverse3x4(m))
+        full = np.eye(4)
+        full[:3] = m.reshape(3, 4)
+        inv = np.eye(4)
+        inv[:3] = a.reshape(3, 4)
+        np.testing.assert_allclose(full @ inv, np.eye(4), atol=1e-5)
+
+
+# ---------------------------------------------------------------- environment
+
+def test_latlong_closed_forms_and_round_trip():
+    lib = pyoracle.load()
+    uv = np.zeros(2, dtype=np.float32)
+    # at the poles atan2(+0, -0) = pi puts u at 1.0
+    for d, exp in [((0, 0, -1), (0.5, 0.5)), ((0, 1, 0), (1.0, 0.0)), ((0, -1, 0), (1.0, 1.0)),
+                   ((1, 0, 0), (0.75, 0.5)), ((-1, 0, 0), (0.25, 0.5)), ((0, 0, 1), (1.0, 0.5))]:
+        lib.ora_dir_to_latlong(pyoracle._p(pyoracle.f32(*d)), pyoracle._p(uv))
+        np.testing.assert_allclose(uv, exp, atol=1e-6)
+    rng = np.random.default_rng(2)
+    d2 = np.zeros(3, dtype=np.float32)
+    for _ in range(500):
+        u = rng.uniform(0.01, 0.99, 2).astype(np.float32)
+        lib.ora_latlong_to_dir(pyoracle._p(u), pyoracle._p(d2))
+        lib.ora_dir_to_latlong(pyoracle._p(d2), pyoracle._p(uv))
+        np.testing.assert_allclose(uv, u, atol=2e-5)
+
+
+def test_gradient_environment_is_lerp_on_v():
+    env, keep = pyoracle.make_env(dict(light_top=(1.0, 0.5, 0.25), light_bottom=(0.0, 0.1, 0.2)))
+    out = np.zeros(3, dtype=np.float32)
+    for d in [(0, 1, 0), (0, -1, 0), (0.3, 0.2, -0.9)]:
+        d = np.array(d, dtype=np.float32) / np.linalg.norm(d)
+        pyoracle.load().ora_env_evaluate(C.byref(env), pyoracle._p(d.astype(np.float32)), 0, 0, 0, 1, 1,
+                                         pyoracle._p(out))
+        v = -math.asin(d[1]) / math.pi + 0.5
+        exp = np.array([1.0, 0.5, 0.25]) + (np.array([0.0, 0.1, 0.2]) - np.array([1.0, 0.5, 0.25])) * v
+        np.testing.assert_allclose(out, exp, atol=1e-6)
+
+
+def effective_probabilities(al):
+    n = len(al)
+    p = np.minimum(al["prob"].astype(np.float64), 1.0)
+    eff = p.copy()
+    np.add.at(eff, al["alias"], 1.0 - p)
+    return eff / n
+
+
+def test_alias_map_closed_forms():
+    w, h = 64, 32
+    img = np.full((h, w, 3), 2.0, dtype=np.float32)
+    al, integral = pyoracle.alias_build(img)
+    lum = 2.0 * (0.2125 + 0.7154 + 0.0721)
+    assert integral == pytest.approx(4 * math.pi * lum, rel=5e-5)  # sum of solid angles = 4 pi (float sums)
+    np.testing.assert_allclose(al["pdf"], 1.0 / (4 * math.pi), rtol=5e-5)  # pdf = lum / integral
+    np.testing.assert_allclose(al["pdf"], lum / integral, rtol=1e-6)
+
+
+def test_alias_map_distribution_and_host_build_agree():
+    rng = np.random.default_rng(4)
+    img = scenes.sun_sky(128, 64, rng, sun_radiance=500.0, sun_radius_deg=5.0)
+    al, integral = pyoracle.alias_build(img)
+    # the alias table reproduces luminance x solid angle / integral
+    h, w = img.shape[:2]
+    lat = math.pi / 2 - np.arange(h) * (math.pi / h)
+    sa = (np.sin(lat) - np.sin(lat - math.pi / h)) * 2 * math.pi / w
+    lum = img @ np.array([0.2125, 0.7154, 0.0721])
+    target = (lum * sa[:, None]).ravel()
+    target /= target.sum()
+    np.testing.assert_allclose(effective_probabilities(al), target, rtol=2e-3, atol=1e-7)
+    # the library's host build is the same algorithm, bit for bit
+    lib = capi.load()
+    mine = np.zeros(w * h, dtype=capi.ALIAS_DTYPE)
+    integ = C.c_float()
+    assert lib.aph_alias_build(capi.ptr(np.ascontiguousarray(img)), w, h, capi.ptr(mine), C.byref(integ)) == 0
+    assert integ.value == integral
+    np.testing.assert_array_equal(mine.view(np.uint32), al.view(np.uint32))
+
+
+def test_environment_sampling_chi2():
+    rng = np.random.default_rng(6)
+    img = scenes.sky_gradient(16, 8, rng) + rng.uniform(0, 1, (8, 16, 3)).astype(np.float32)
+    img = np.ascontiguousarray(img.astype(np.float32))
+    env, keep = pyoracle.make_env(dict(light_rgb=img))
+    al, _ = pyoracle.alias_build(img)
+    target = effective_probabilities(al)
+    lib = pyoracle.load()
+    n = 40000
+    rnd = rng.uniform(0, 1, (n, 2)).astype(np.float32)
+    L = np.zeros(3, dtype=np.float32)
+    pdf = C.c_float()
+    rad = np.zeros(3, dtype=np.float32)
+    counts = np.zeros(16 * 8)
+    uv = np.zeros(2, dtype=np.float32)
+    for i in range(n):
+        lib.ora_env_sample(C.byref(env), pyoracle._p(rnd[i]), pyoracle._p(L), C.byref(pdf), pyoracle._p(rad))
+        lib.ora_dir_to_latlong(pyoracle._p(L), pyoracle._p(uv))
+        x = min(int(uv[0] * 16), 15)
+        y = min(int(uv[1] * 8), 7)
+        counts[y * 16 + x] += 1
+        if i < 200:
+            np.testing.assert_allclose(rad, img[y, x], rtol=1e-6)
+    exp = target * n
+    chi2 = ((counts - exp) ** 2 / exp).sum()
+    assert chi2 < 200  # 127 dof
