@@ -209,6 +209,11 @@ __device__ bool traceRay(const TraceScene& sc, f3 org, f3 dir, float tmin, float
     best.inst = best.prim = 0xFFFFFFFFu;
     bool found = false;
     if (sc.ninst == 0) return false;
+    // A ray with a non-finite component misses: every triangle test rejects it, but the
+    // NaN-ignoring slab test would accept every box.
+    if (!(isfinite(org.x) && isfinite(org.y) && isfinite(org.z) && isfinite(dir.x) && isfinite(dir.y) &&
+          isfinite(dir.z)))
+        return false;
     const BoxRay wr = makeBoxRay(org, dir);
     const float wpad = sc.padScale * (fabsf(org.x) + fabsf(org.y) + fabsf(org.z) + sc.extent);
     BoxRay br = wr;

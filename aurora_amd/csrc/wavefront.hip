@@ -1,13 +1,722 @@
-// Wavefront path-tracing pipeline for gfx950.
+// Wavefront path tracing for gfx950.
+//
+// The reference runs one megakernel thread per pixel-sample (RayGenShader,
+// Shaders/MainEntryPoints.slang:70-334, dispatched W x H per sample by
+// HGIRenderer.cpp:551).  Here the same path is split into stages over SoA
+// queues shared by every request and sample of a chunk:
+//
+//   gen      camera rays for all (request, sample, pixel) paths of the chunk
+//   extend   closest-hit traversal of the live ray queue          (lean, high occupancy)
+//   shade    material, opacity coin, lights, environment MIS, next ray, roulette;
+//            appends shadow rays and continuing paths with wave-ballot compaction
+//   shadow   any-hit traversal of the shadow queue
+//   resolve  adds shadow-weighted lighting in the reference's order, routes it
+//            to the direct/diffuse/glossy outputs, applies the terminal bounce
+//   accum    per pixel, folds the chunk's samples into the target in sample order
+//
+// Arithmetic per path is the megakernel's (pathtrace.h helpers, same operation
+// order), so a chunk's result does not depend on what else shares the queues.
 #include "wavefront.h"
 
+#include <algorithm>
+#include <cstring>
+
 namespace aph {
+
+namespace {
+
+constexpr int kWB = 128;
+constexpr int kSlots = 6;           // shadow slots per path: 0..3 distant lights, 4 MIS material, 5 MIS light
+constexpr int kSlotMisMat = 4, kSlotMisLight = 5;
+
+// Event of the last traced segment.
+constexpr uint32_t kEvMiss = 0, kEvHit = 1, kEvSkip = 2;
+
+// Packed per-path flags.
+struct Flags {
+    uint32_t v;
+    APH_HD uint32_t get(int sh, int bits) const { return (v >> sh) & ((1u << bits) - 1u); }
+    APH_HD void set(int sh, int bits, uint32_t x) { v = (v & ~(((1u << bits) - 1u) << sh)) | ((x & ((1u << bits) - 1u)) << sh); }
+    APH_HD uint32_t rayType() const { return get(0, 4); }
+    APH_HD void rayType(uint32_t x) { set(0, 4, x); }
+    APH_HD uint32_t nextRayType() const { return get(4, 4); }
+    APH_HD void nextRayType(uint32_t x) { set(4, 4, x); }
+    APH_HD uint32_t misLobe() const { return get(8, 4); }
+    APH_HD void misLobe(uint32_t x) { set(8, 4, x); }
+    APH_HD uint32_t depth() const { return get(12, 4); }
+    APH_HD void depth(uint32_t x) { set(12, 4, x); }
+    APH_HD uint32_t event() const { return get(16, 2); }
+    APH_HD void event(uint32_t x) { set(16, 2, x); }
+    APH_HD bool primary() const { return get(18, 1); }
+    APH_HD void primary(bool x) { set(18, 1, x); }
+    APH_HD bool ended() const { return get(19, 1); }
+    APH_HD void ended(bool x) { set(19, 1, x); }
+    APH_HD bool terminal() const { return get(20, 1); }
+    APH_HD void terminal(bool x) { set(20, 1, x); }
+    APH_HD float alpha() const { return get(21, 1) ? 1.0f : 0.0f; }
+    APH_HD void alpha(float a) { set(21, 1, a != 0.0f); }
+    APH_HD bool inside() const { return get(22, 1); }
+    APH_HD void inside(bool x) { set(22, 1, x); }
+};
+
+struct Paths {
+    uint32_t* pix;
+    uint32_t* seg;
+    uint2* rng;
+    float4* contrib;
+    uint32_t* flags;
+    float4* outDirect;
+    float4* outDiffuse;
+    float4* outGlossy;
+    float4* lastIndirect;
+    float4* pendDirect;    // emission (hit) or background (miss) of the last segment
+    float4* pendContrib;   // contribution that scales this segment's direct / indirect
+    float4* nextDir;       // direction of the continuing / terminal ray
+    float4* shColor;       // [C * kSlots]
+    uint8_t* shMask;
+    uint8_t* vis;          // [C * kSlots]
+    float4* sampleOut;
+};
+
+struct Queues {
+    float4* rayA[2];       // org.xyz, tmin
+    float4* rayB[2];       // dir.xyz, path id bits
+    float4* hitA;          // t, inst, prim, u
+    float* hitV;
+    float4* shA;           // org.xyz, tmin
+    float4* shB;           // dir.xyz, (path * kSlots + slot) bits
+    uint32_t* counts;      // [0],[1] ray queues, [2] shadow queue
+};
+
+struct SegInfo {
+    uint32_t req, sample, pathOffset, count;
+};
+
+struct ChunkReq {
+    uint32_t req, firstSeg, nSeg, pixelBase;
+};
+
+struct DevReq {
+    RenderScene rs;
+    aph_frame fp;
+    uint32_t W, H, seedOffset, pad;
+    float4* accum;
+};
+
+__device__ __forceinline__ uint32_t findSeg(const SegInfo* segs, uint32_t nseg, uint32_t p)
+{
+    uint32_t lo = 0, hi = nseg;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (segs[mid].pathOffset <= p) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint32_t waveAppend(bool keep, uint32_t* counter)
+{
+    uint64_t active = __ballot(1);
+    uint64_t mask = __ballot(keep);
+    int lane = __lane_id();
+    int leader = __ffsll((unsigned long long)active) - 1;
+    uint32_t base = 0;
+    if (lane == leader && mask) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader, 64);
+    return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+}
+
+__device__ __forceinline__ uint32_t waveSum(uint32_t v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ void addCounter(unsigned long long* c, uint32_t v)
+{
+    uint32_t s = waveSum(v);
+    if ((threadIdx.x & 63) == 0 && s) atomicAdd(c, (unsigned long long)s);
+}
+
+__device__ __forceinline__ f3 xyz(float4 v) { return f3{v.x, v.y, v.z}; }
+__device__ __forceinline__ float4 f4(f3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
+
+__global__ void kReset(uint32_t* counts, uint32_t q0)
+{
+    if (threadIdx.x == 0) {
+        counts[0] = q0;
+        counts[1] = 0;
+        counts[2] = 0;
+    }
+}
+
+__global__ void kResetNext(uint32_t* counts, int next)
+{
+    if (threadIdx.x == 0) {
+        counts[next] = 0;
+        counts[2] = 0;
+    }
+}
+
+// ---- gen: RayGenShader prologue (MainEntryPoints.slang:83-103) ----
+__global__ __launch_bounds__(kWB) void kGen(const DevReq* reqs, const SegInfo* segs, uint32_t nseg, uint32_t total,
+                                           Paths P, Queues Q, DevCounters* ctr)
+{
+    uint32_t started = 0;
+    for (uint32_t p = blockIdx.x * kWB + threadIdx.x; p < total; p += gridDim.x * kWB) {
+        uint32_t s = findSeg(segs, nseg, p);
+        SegInfo sg = segs[s];
+        const DevReq& rq = reqs[sg.req];
+        uint32_t pix = p - sg.pathOffset;
+        uint32_t px = pix % rq.W, py = pix / rq.W;
+        Rng rng = rngInit(sg.sample + rq.seedOffset, rq.W, px, py);
+        f3 org, dir;
+        cameraRay(rq.fp, px, py, rq.W, rq.H, rng, org, dir);
+        P.pix[p] = pix;
+        P.seg[p] = s;
+        P.rng[p] = make_uint2(rng.x, rng.y);
+        P.contrib[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+        Flags f{0};
+        f.rayType(kCameraRay);
+        f.nextRayType(kCameraRay);
+        f.misLobe(kNoSecondaryRay);
+        f.primary(true);
+        P.flags[p] = f.v;
+        float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        P.outDirect[p] = z;
+        P.outDiffuse[p] = z;
+        P.outGlossy[p] = z;
+        P.lastIndirect[p] = z;
+        Q.rayA[0][p] = f4(org, 0.0f);
+        Q.rayB[0][p] = f4(dir, __uint_as_float(p));
+        started++;
+    }
+    addCounter(&ctr->paths, started);
+}
+
+// ---- extend: closest hit of the live queue ----
+template <bool STATS>
+__global__ __launch_bounds__(kWB) void kExtend(const DevReq* reqs, const SegInfo* segs, Paths P, Queues Q, int cur,
+                                              DevCounters* ctr)
+{
+    __shared__ uint32_t sStack[kLdsStack * kWB];
+    ShortStack stk;
+    stk.lds = sStack + threadIdx.x;
+    stk.stride = kWB;
+    stk.sp = 0;
+    TraceStats ts = {};
+    uint32_t n = Q.counts[cur], done = 0;
+    for (uint32_t i = blockIdx.x * kWB + threadIdx.x; i < n; i += gridDim.x * kWB) {
+        float4 a = Q.rayA[cur][i], b = Q.rayB[cur][i];
+        uint32_t p = __float_as_uint(b.w);
+        const TraceScene& sc = reqs[segs[P.seg[p]].req].rs.sc;
+        HitRec h;
+        bool found = traceRay<false, STATS>(sc, xyz(a), xyz(b), a.w, INFINITY, stk, h, ts);
+        if (found) {
+            Q.hitA[i] = make_float4((float)h.t, __uint_as_float(h.inst), __uint_as_float(h.prim), (float)h.u);
+            Q.hitV[i] = (float)h.v;
+        } else {
+            Q.hitA[i] = make_float4(INFINITY, __uint_as_float(0xFFFFFFFFu), __uint_as_float(0xFFFFFFFFu), 0.0f);
+            Q.hitV[i] = 0.0f;
+        }
+        done++;
+    }
+    addCounter(&ctr->ext, done);
+    if (STATS) {
+        addCounter(&ctr->nodes, ts.nodes);
+        addCounter(&ctr->tris, ts.tris);
+        addCounter(&ctr->insts, ts.insts);
+    }
+    addCounter(&ctr->overflow, ts.overflow);
+}
+
+__device__ __forceinline__ void emitShadow(Queues& Q, bool emit, f3 org, f3 dir, uint32_t slotId)
+{
+    uint32_t at = waveAppend(emit, &Q.counts[2]);
+    if (emit) {
+        Q.shA[at] = f4(org, kRayTMin);
+        Q.shB[at] = f4(dir, __uint_as_float(slotId));
+    }
+}
+
+// ---- shade: closest-hit / miss programs without the shadow traces ----
+__global__ __launch_bounds__(kWB) void kShade(const DevReq* reqs, const SegInfo* segs, Paths P, Queues Q, int cur)
+{
+    const int nxt = cur ^ 1;
+    uint32_t n = Q.counts[cur];
+    // Every lane of a wave runs the same number of iterations so that the
+    // ballot-based appends see whole waves.
+    uint32_t stride = gridDim.x * kWB;
+    uint32_t iters = (n + stride - 1) / stride;
+    for (uint32_t it = 0; it < iters; it++) {
+        uint32_t i = it * stride + blockIdx.x * kWB + threadIdx.x;
+        bool live = i < n;
+        bool contPath = false;
+        f3 nOrg = splat(0.0f), nDir = splat(0.0f);
+        float nTmin = kRayTMin;
+        uint32_t p = 0;
+        bool sh[kSlots] = {false, false, false, false, false, false};
+        f3 shO[kSlots], shD[kSlots];
+        if (live) {
+            float4 ra = Q.rayA[cur][i], rb = Q.rayB[cur][i];
+            p = __float_as_uint(rb.w);
+            const DevReq& rq = reqs[segs[P.seg[p]].req];
+            const aph_frame& fp = rq.fp;
+            const DevEnv& env = rq.rs.env;
+            uint32_t pix = P.pix[p], px = pix % rq.W, py = pix / rq.W;
+            Flags f{P.flags[p]};
+            f3 dir = xyz(rb);
+            f3 contrib = xyz(P.contrib[p]);
+            uint2 rs = P.rng[p];
+            Rng rng{rs.x, rs.y};
+            float4 ha = Q.hitA[i];
+            uint32_t inst = __float_as_uint(ha.y);
+            uint32_t depth = f.depth();
+            uint8_t mask = 0;
+            if (inst == 0xFFFFFFFFu) {
+                bool bg = f.primary() || f.rayType() == (uint32_t)kTransmissionLobe ||
+                          f.rayType() == (uint32_t)kGroundPlaneReflectionRay;
+                f3 d = bg ? evalEnvironment(env, dir, true, px, py, rq.W, rq.H) * contrib : splat(0.0f);
+                P.pendDirect[p] = f4(d, 0.0f);
+                f.event(kEvMiss);
+                f.ended(true);
+            } else {
+                f3 V = -dir;
+                const InstShade& ins = rq.rs.ishade[inst];
+                Shading shd;
+                computeShading(ins, __float_as_uint(ha.z), ha.w, Q.hitV[i], shd);
+                uint32_t mi = ins.material < rq.rs.nmat ? ins.material : 0u;
+                const DevMaterial m = rq.rs.mats[mi];
+                f3 transparency = splat(1.0f) - m.opacity;
+                float Pt = luminance(transparency);
+                bool hitLayer = random2D(rng).x >= Pt;
+                if (!hitLayer) {
+                    contrib = contrib * (transparency / Pt);
+                    nOrg = shd.geomPosition;
+                    nDir = dir;
+                    f.event(kEvSkip);
+                    contPath = true;
+                } else {
+                    f3 direct = m.emissionColor * m.emission * lerp(splat(1.0f), m.coatColor, m.coat);
+                    f3 so = dot(V, shd.normal) < 0.0f ? shd.geomPosition : shd.position;
+                    f.primary(false);
+                    if (fp.light_count > 1) {
+                        for (int li = 0; li < fp.light_count; li++) {
+                            if (fp.lights[li].color_intensity[3] <= 0.0f) continue;
+                            f3 sd;
+                            f3 lr = shadeDirectional(fp.lights[li], m, shd, V, rng, sd);
+                            if (isBlack(lr)) continue;
+                            P.shColor[(size_t)p * kSlots + li] = f4(lr, 0.0f);
+                            sh[li] = true;
+                            shO[li] = so;
+                            shD[li] = sd;
+                        }
+                    } else if (fp.light_count > 0) {
+                        const aph_distant_light& l0 = fp.lights[0];
+                        float w0 = luminance(f3{l0.color_intensity[0], l0.color_intensity[1], l0.color_intensity[2]}) *
+                                   l0.color_intensity[3];
+                        float selPdf = 1.0f;
+                        random2D(rng);
+                        if (w0 > 0.0f) selPdf = w0 / w0;
+                        if (l0.color_intensity[3] > 0.0f) {
+                            f3 sd;
+                            f3 dc = shadeDirectional(l0, m, shd, V, rng, sd) / fmaxf(selPdf, 1e-4f);
+                            P.shColor[(size_t)p * kSlots] = f4(dc, 0.0f);
+                            sh[0] = true;
+                            shO[0] = so;
+                            shD[0] = sd;
+                        }
+                    }
+                    // environment MIS (ShadeFunctions.slang:135-259)
+                    {
+                        f2 r = random2D(rng);
+                        f3 L;
+                        float matPdf;
+                        int lobe;
+                        f3 fs = sampleMaterial(m, shd, V, r, L, matPdf, lobe);
+                        if (lobe != kTransmissionLobe) {
+                            f3 Le = evalEnvironment(env, L, false, px, py, rq.W, rq.H);
+                            if (!isBlack(Le) && !isBlack(fs)) {
+                                float lightPdf = env.hasLightTex ? luminance(Le) / env.lightIntegral : 0.25f * kPiInv;
+                                P.shColor[(size_t)p * kSlots + kSlotMisMat] =
+                                    f4(Le * misWeight(matPdf, lightPdf) * fs / matPdf, 0.0f);
+                                sh[kSlotMisMat] = true;
+                                shO[kSlotMisMat] = so;
+                                shD[kSlotMisMat] = L;
+                            }
+                            float lightPdf;
+                            Le = sampleEnvironment(env, r, L, lightPdf);
+                            int evLobe;
+                            fs = evalMaterialPdf(m, shd, V, L, r, matPdf, evLobe);
+                            if (!isBlack(Le) && !isBlack(fs)) {
+                                P.shColor[(size_t)p * kSlots + kSlotMisLight] =
+                                    f4(Le * misWeight(lightPdf, matPdf) * fs / lightPdf, 0.0f);
+                                sh[kSlotMisLight] = true;
+                                shO[kSlotMisLight] = so;
+                                shD[kSlotMisLight] = L;
+                            }
+                        }
+                        if (sh[kSlotMisMat] || sh[kSlotMisLight]) f.misLobe((uint32_t)lobe);
+                    }
+                    contrib = contrib * (m.opacity / (1.0f - Pt));
+                    f.alpha((f.alpha() == 0.0f && f.rayType() == (uint32_t)kTransmissionLobe) ? 0.0f : 1.0f);
+                    f3 nextContrib = splat(1.0f);
+                    uint32_t nextType;
+                    {
+                        f2 r = random2D(rng);
+                        f3 L;
+                        float pdf;
+                        int lobe;
+                        f3 fs = sampleMaterial(m, shd, V, r, L, pdf, lobe);
+                        if (isBlack(fs)) {
+                            nextType = kNoSecondaryRay;
+                        } else {
+                            bool trans = lobe == kTransmissionLobe;
+                            nOrg = (dot(V, shd.normal) < 0.0f || trans) ? shd.geomPosition : shd.position;
+                            nDir = L;
+                            nextContrib = fs / pdf;
+                            nextType = (uint32_t)lobe;
+                        }
+                    }
+                    bool ended = nextType == (uint32_t)kNoSecondaryRay;
+                    if (nextType == (uint32_t)kTransmissionLobe) f.inside(!f.inside());
+                    P.pendDirect[p] = f4(direct, 0.0f);
+                    P.pendContrib[p] = f4(contrib, 0.0f);
+                    contrib = contrib * nextContrib;
+                    if (fp.rr_enabled && (int)depth + 1 >= fp.rr_start_depth) {
+                        float surv = clampf(maxc(contrib), 0.05f, 1.0f);
+                        if (random2D(rng).x > surv) ended = true;
+                        else contrib = contrib / surv;
+                    }
+                    f.nextRayType(nextType);
+                    f.event(kEvHit);
+                    f.ended(ended);
+                    contPath = !ended;
+                    for (int k = 0; k < kSlots; k++) mask |= sh[k] ? (uint8_t)(1u << k) : (uint8_t)0;
+                }
+            }
+            f.terminal(false);
+            if (contPath) {
+                uint32_t nd = depth + 1;
+                if ((int)nd >= fp.trace_depth) {
+                    f.terminal(true);
+                    contPath = false;
+                    P.nextDir[p] = f4(nDir, 0.0f);
+                }
+            }
+            P.shMask[p] = mask;
+            P.flags[p] = f.v;
+            P.contrib[p] = f4(contrib, 0.0f);
+            P.rng[p] = make_uint2(rng.x, rng.y);
+        }
+        // wave-uniform appends: next ray queue, then one shadow slot at a time
+        uint32_t at = waveAppend(contPath, &Q.counts[nxt]);
+        if (contPath) {
+            Q.rayA[nxt][at] = f4(nOrg, nTmin);
+            Q.rayB[nxt][at] = f4(nDir, __uint_as_float(p));
+        }
+#pragma unroll
+        for (int k = 0; k < kSlots; k++) emitShadow(Q, sh[k], shO[k], shD[k], p * kSlots + (uint32_t)k);
+    }
+}
+
+// ---- shadow: any-hit of the shadow queue (traceShadowRay, forced opaque) ----
+template <bool STATS>
+__global__ __launch_bounds__(kWB) void kShadow(const DevReq* reqs, const SegInfo* segs, Paths P, Queues Q,
+                                              DevCounters* ctr)
+{
+    __shared__ uint32_t sStack[kLdsStack * kWB];
+    ShortStack stk;
+    stk.lds = sStack + threadIdx.x;
+    stk.stride = kWB;
+    stk.sp = 0;
+    TraceStats ts = {};
+    uint32_t n = Q.counts[2], done = 0;
+    for (uint32_t i = blockIdx.x * kWB + threadIdx.x; i < n; i += gridDim.x * kWB) {
+        float4 a = Q.shA[i], b = Q.shB[i];
+        uint32_t slot = __float_as_uint(b.w);
+        uint32_t p = slot / kSlots;
+        const TraceScene& sc = reqs[segs[P.seg[p]].req].rs.sc;
+        HitRec h;
+        bool occ = traceRay<true, STATS>(sc, xyz(a), xyz(b), a.w, INFINITY, stk, h, ts);
+        P.vis[slot] = occ ? 0 : 1;
+        done++;
+    }
+    addCounter(&ctr->shadow, done);
+    if (STATS) {
+        addCounter(&ctr->nodes, ts.nodes);
+        addCounter(&ctr->tris, ts.tris);
+        addCounter(&ctr->insts, ts.insts);
+    }
+    addCounter(&ctr->overflow, ts.overflow);
+}
+
+__device__ __forceinline__ void finalizePath(const Paths& P, uint32_t p, const aph_frame& fp, Flags f)
+{
+    f3 fin = adjustRadiance(fp.max_luminance, fp.display_errors != 0,
+                            xyz(P.outDirect[p]) + xyz(P.outDiffuse[p]) + xyz(P.outGlossy[p]));
+    P.sampleOut[p] = f4(fin, f.alpha());
+}
+
+// ---- resolve: the accumulation part of the RayGenShader loop (MainEntryPoints.slang:124-247) ----
+__global__ __launch_bounds__(kWB) void kResolve(const DevReq* reqs, const SegInfo* segs, Paths P, Queues Q, int cur)
+{
+    uint32_t n = Q.counts[cur];
+    for (uint32_t i = blockIdx.x * kWB + threadIdx.x; i < n; i += gridDim.x * kWB) {
+        uint32_t p = __float_as_uint(Q.rayB[cur][i].w);
+        const DevReq& rq = reqs[segs[P.seg[p]].req];
+        const aph_frame& fp = rq.fp;
+        Flags f{P.flags[p]};
+        uint32_t depth = f.depth();
+        uint32_t ev = f.event();
+        if (ev != kEvSkip) {
+            f3 direct = xyz(P.pendDirect[p]);
+            f3 indirect = splat(0.0f);
+            if (ev == kEvHit) {
+                uint8_t mask = P.shMask[p];
+                const size_t b = (size_t)p * kSlots;
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    if (mask & (1u << k)) direct = direct + xyz(P.shColor[b + k]) * (P.vis[b + k] ? 1.0f : 0.0f);
+                if (mask & ((1u << kSlotMisMat) | (1u << kSlotMisLight))) {
+                    f3 mis = splat(0.0f);
+                    if (mask & (1u << kSlotMisMat))
+                        mis = mis + xyz(P.shColor[b + kSlotMisMat]) * (P.vis[b + kSlotMisMat] ? 1.0f : 0.0f);
+                    if (mask & (1u << kSlotMisLight))
+                        mis = mis + xyz(P.shColor[b + kSlotMisLight]) * (P.vis[b + kSlotMisLight] ? 1.0f : 0.0f);
+                    indirect = mis;
+                }
+                f3 c = xyz(P.pendContrib[p]);
+                direct = direct * c;
+                indirect = indirect * c;
+            }
+            if (depth == 0) {
+                f.primary(false);
+                P.outDirect[p] = f4(direct, 0.0f);
+                if (f.misLobe() == (uint32_t)kDiffuseLobe) P.outDiffuse[p] = f4(indirect, 0.0f);
+                else P.outGlossy[p] = f4(indirect, 0.0f);
+            } else {
+                f3 radiance = direct + indirect;
+                if (f.rayType() == (uint32_t)kDiffuseLobe) P.outDiffuse[p] = f4(xyz(P.outDiffuse[p]) + radiance, 0.0f);
+                else P.outGlossy[p] = f4(xyz(P.outGlossy[p]) + radiance, 0.0f);
+            }
+            if (ev == kEvHit) f.rayType(f.nextRayType());
+            P.lastIndirect[p] = f4(indirect, 0.0f);
+        } else {
+            P.lastIndirect[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+        bool done = f.ended();
+        if (!done && f.terminal()) {
+            // rayDepth == traceDepth: unoccluded environment in the next direction plus the
+            // previous segment's indirect, which the reference does not clear (:127-138, :198)
+            uint32_t pix = P.pix[p];
+            bool bg = f.primary() || f.rayType() == (uint32_t)kTransmissionLobe ||
+                      f.rayType() == (uint32_t)kGroundPlaneReflectionRay;
+            f3 dterm = evalEnvironment(rq.rs.env, xyz(P.nextDir[p]), bg, pix % rq.W, pix / rq.W, rq.W, rq.H) *
+                       xyz(P.contrib[p]);
+            f3 radiance = dterm + xyz(P.lastIndirect[p]);
+            if (f.rayType() == (uint32_t)kDiffuseLobe) P.outDiffuse[p] = f4(xyz(P.outDiffuse[p]) + radiance, 0.0f);
+            else P.outGlossy[p] = f4(xyz(P.outGlossy[p]) + radiance, 0.0f);
+            done = true;
+        }
+        f.depth(depth + 1);
+        P.flags[p] = f.v;
+        if (done) finalizePath(P, p, fp, f);
+    }
+}
+
+// ---- accum: running mean in sample order (Accumulation.glsl:22-30) ----
+__global__ void kAccum(const DevReq* reqs, const SegInfo* segs, const ChunkReq* creqs, uint32_t ncreq,
+                       uint32_t totalPix, Paths P)
+{
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < totalPix; t += gridDim.x * blockDim.x) {
+        uint32_t lo = 0, hi = ncreq;
+        while (hi - lo > 1) {
+            uint32_t mid = (lo + hi) >> 1;
+            if (creqs[mid].pixelBase <= t) lo = mid;
+            else hi = mid;
+        }
+        ChunkReq cr = creqs[lo];
+        const DevReq& rq = reqs[cr.req];
+        uint32_t pix = t - cr.pixelBase;
+        float4 acc = rq.accum[pix];
+        for (uint32_t s = cr.firstSeg; s < cr.firstSeg + cr.nSeg; s++) {
+            SegInfo sg = segs[s];
+            float4 r = P.sampleOut[sg.pathOffset + pix];
+            if (sg.sample > 0) {
+                float tt = 1.0f / (float)(sg.sample + 1);
+                float u = 1.0f - tt;
+                acc = make_float4(acc.x * u + r.x * tt, acc.y * u + r.y * tt, acc.z * u + r.z * tt,
+                                  acc.w * u + r.w * tt);
+            } else {
+                acc = r;
+            }
+        }
+        rq.accum[pix] = acc;
+    }
+}
+
+size_t alignUp(size_t x) { return (x + 255) & ~(size_t)255; }
+
+}  // namespace
 
 hipError_t wavefrontRender(WavefrontState& st, const WavefrontRequest* reqs, uint32_t nreq, DevCounters* ctr,
                            bool stats, hipStream_t stream, std::vector<PendingEvent>* timing)
 {
-    (void)st; (void)reqs; (void)nreq; (void)ctr; (void)stats; (void)stream; (void)timing;
-    return hipErrorNotSupported;
+    if (nreq == 0) return hipSuccess;
+    uint32_t maxPix = 0;
+    for (uint32_t i = 0; i < nreq; i++) maxPix = std::max(maxPix, reqs[i].width * reqs[i].height);
+    // Chunk capacity: at least one full frame of the largest request, ~4M paths otherwise.
+    uint32_t cap = std::max<uint32_t>(maxPix, 4u << 20);
+    size_t perPath = 4 + 4 + 8 + 16 + 4 + 16 * 4 + 16 * 3 + 16 * kSlots + 1 + kSlots + 16 +  // path state
+                     16 * 4 + 16 + 4 + 32 * kSlots;                                              // queues
+    const size_t kTableBytes = 1u << 20;  // segment + chunk-request tables of one chunk
+    const uint32_t kMaxSegs = (uint32_t)(kTableBytes / 2 / sizeof(uint32_t) / 4);
+    size_t need = alignUp((size_t)cap * perPath) + 64 * 1024 + alignUp(sizeof(DevReq) * nreq) + kTableBytes;
+    if (st.bytes < need) {
+        if (st.buf) hipFree(st.buf);
+        st.buf = nullptr;
+        st.bytes = 0;
+        hipError_t e = hipMalloc(&st.buf, need);
+        if (e != hipSuccess) return e;
+        st.bytes = need;
+    }
+    st.capacity = cap;
+    char* base = (char*)st.buf;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        void* p = base + off;
+        off += alignUp(bytes);
+        return p;
+    };
+    Paths P;
+    P.pix = (uint32_t*)take((size_t)cap * 4);
+    P.seg = (uint32_t*)take((size_t)cap * 4);
+    P.rng = (uint2*)take((size_t)cap * 8);
+    P.contrib = (float4*)take((size_t)cap * 16);
+    P.flags = (uint32_t*)take((size_t)cap * 4);
+    P.outDirect = (float4*)take((size_t)cap * 16);
+    P.outDiffuse = (float4*)take((size_t)cap * 16);
+    P.outGlossy = (float4*)take((size_t)cap * 16);
+    P.lastIndirect = (float4*)take((size_t)cap * 16);
+    P.pendDirect = (float4*)take((size_t)cap * 16);
+    P.pendContrib = (float4*)take((size_t)cap * 16);
+    P.nextDir = (float4*)take((size_t)cap * 16);
+    P.shColor = (float4*)take((size_t)cap * kSlots * 16);
+    P.shMask = (uint8_t*)take((size_t)cap);
+    P.vis = (uint8_t*)take((size_t)cap * kSlots);
+    P.sampleOut = (float4*)take((size_t)cap * 16);
+    Queues Q;
+    for (int k = 0; k < 2; k++) {
+        Q.rayA[k] = (float4*)take((size_t)cap * 16);
+        Q.rayB[k] = (float4*)take((size_t)cap * 16);
+    }
+    Q.hitA = (float4*)take((size_t)cap * 16);
+    Q.hitV = (float*)take((size_t)cap * 4);
+    Q.shA = (float4*)take((size_t)cap * kSlots * 16);
+    Q.shB = (float4*)take((size_t)cap * kSlots * 16);
+    Q.counts = (uint32_t*)take(64);
+    DevReq* dreq = (DevReq*)take(sizeof(DevReq) * nreq);
+    char* tbl = (char*)take(kTableBytes);
+    if (off > need) return hipErrorOutOfMemory;
+
+    // request table
+    std::vector<DevReq> hreq(nreq);
+    uint32_t maxDepth = 1;
+    for (uint32_t i = 0; i < nreq; i++) {
+        std::memset(&hreq[i], 0, sizeof(DevReq));
+        hreq[i].rs = reqs[i].rs;
+        hreq[i].fp = reqs[i].frame;
+        hreq[i].W = reqs[i].width;
+        hreq[i].H = reqs[i].height;
+        hreq[i].seedOffset = reqs[i].seedOffset;
+        hreq[i].accum = reqs[i].accum;
+        maxDepth = std::max<uint32_t>(maxDepth, (uint32_t)reqs[i].frame.trace_depth);
+    }
+    hipError_t e = hipMemcpyAsync(dreq, hreq.data(), sizeof(DevReq) * nreq, hipMemcpyHostToDevice, stream);
+    if (e != hipSuccess) return e;
+
+    // Segments: (request, sample) pairs in request-major, sample-ascending order, cut into chunks.
+    struct PendingSeg {
+        uint32_t req, sample, count;
+    };
+    std::vector<PendingSeg> all;
+    for (uint32_t i = 0; i < nreq; i++)
+        for (uint32_t s = 0; s < reqs[i].sampleCount; s++)
+            all.push_back(PendingSeg{i, reqs[i].sampleStart + s, reqs[i].width * reqs[i].height});
+    int dev = 0, cus = 256;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const unsigned traceGrid = (unsigned)cus * 16, shadeGrid = (unsigned)cus * 8, lightGrid = (unsigned)cus * 8;
+    size_t k = 0;
+    while (k < all.size()) {
+        std::vector<SegInfo> segs;
+        std::vector<ChunkReq> creqs;
+        uint32_t total = 0, totalPix = 0;
+        while (k < all.size() && total + all[k].count <= cap && segs.size() < kMaxSegs) {
+            const PendingSeg& s = all[k];
+            if (creqs.empty() || creqs.back().req != s.req) {
+                creqs.push_back(ChunkReq{s.req, (uint32_t)segs.size(), 0, totalPix});
+                totalPix += s.count;
+            }
+            creqs.back().nSeg++;
+            segs.push_back(SegInfo{s.req, s.sample, total, s.count});
+            total += s.count;
+            k++;
+        }
+        if (segs.empty()) return hipErrorInvalidValue;
+        size_t sb = sizeof(SegInfo) * segs.size(), cb = sizeof(ChunkReq) * creqs.size();
+        SegInfo* gseg = (SegInfo*)tbl;
+        ChunkReq* gcr = (ChunkReq*)(tbl + kTableBytes / 2);
+        if ((e = hipMemcpyAsync(gseg, segs.data(), sb, hipMemcpyHostToDevice, stream)) != hipSuccess) return e;
+        if ((e = hipMemcpyAsync(gcr, creqs.data(), cb, hipMemcpyHostToDevice, stream)) != hipSuccess) return e;
+        uint32_t nseg = (uint32_t)segs.size();
+        {
+            EventScope ev(timing, "wf_gen", stream);
+            kReset<<<1, 64, 0, stream>>>(Q.counts, total);
+            unsigned g = std::min<unsigned>((total + kWB - 1) / kWB, (unsigned)cus * 16);
+            kGen<<<g, kWB, 0, stream>>>(dreq, gseg, nseg, total, P, Q, ctr);
+        }
+        int cur = 0;
+        unsigned tg = std::min<unsigned>((total + kWB - 1) / kWB, traceGrid);
+        unsigned sg = std::min<unsigned>((total + kWB - 1) / kWB, shadeGrid);
+        unsigned lg = std::min<unsigned>((total + kWB - 1) / kWB, lightGrid);
+        unsigned shg = std::min<unsigned>((total * 2 + kWB - 1) / kWB, traceGrid);
+        for (uint32_t d = 0; d < maxDepth; d++) {
+            kResetNext<<<1, 64, 0, stream>>>(Q.counts, cur ^ 1);
+            {
+                EventScope ev(timing, "wf_extend", stream);
+                if (stats) kExtend<true><<<tg, kWB, 0, stream>>>(dreq, gseg, P, Q, cur, ctr);
+                else kExtend<false><<<tg, kWB, 0, stream>>>(dreq, gseg, P, Q, cur, ctr);
+            }
+            {
+                EventScope ev(timing, "wf_shade", stream);
+                kShade<<<sg, kWB, 0, stream>>>(dreq, gseg, P, Q, cur);
+            }
+            {
+                EventScope ev(timing, "wf_shadow", stream);
+                if (stats) kShadow<true><<<shg, kWB, 0, stream>>>(dreq, gseg, P, Q, ctr);
+                else kShadow<false><<<shg, kWB, 0, stream>>>(dreq, gseg, P, Q, ctr);
+            }
+            {
+                EventScope ev(timing, "wf_resolve", stream);
+                kResolve<<<lg, kWB, 0, stream>>>(dreq, gseg, P, Q, cur);
+            }
+            cur ^= 1;
+        }
+        {
+            EventScope ev(timing, "wf_accum", stream);
+            unsigned g = std::min<unsigned>((totalPix + 255) / 256, (unsigned)cus * 8);
+            kAccum<<<g, 256, 0, stream>>>(dreq, gseg, gcr, (uint32_t)creqs.size(), totalPix, P);
+        }
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        // the tables are overwritten by the next chunk
+        if (k < all.size()) {
+            if ((e = hipStreamSynchronize(stream)) != hipSuccess) return e;
+        }
+    }
+    return hipSuccess;
 }
 
 void wavefrontRelease(WavefrontState& st)

@@ -13,6 +13,7 @@
  *     double with no FMA, edges accepted when U,V,W share a sign (zeros allowed),
  *     det == 0 rejected, t = T/det accepted when tmin <= t <= tmax;
  *   - closest hit = lexicographic minimum of (t, instance, primitive);
+ *   - a ray with a non-finite origin or direction component misses;
  *   - barycentrics (u, v) = (V/det, W/det) are the weights of vertices 1 and 2.
  */
 #include "ora.h"
@@ -323,6 +324,9 @@ static void trace_one(const ora_scene* s, const ora_ray* ray, int any_hit, best_
 {
     best->found = 0;
     if (s->tlas.nnodes == 0) return;
+    /* non-finite rays miss (every triangle test would reject them) */
+    for (int k = 0; k < 3; k++)
+        if (!isfinite(ray->org[k]) || !isfinite(ray->dir[k])) return;
     double o[3] = { ray->org[0], ray->org[1], ray->org[2] }, d[3] = { ray->dir[0], ray->dir[1], ray->dir[2] };
     ora_wray wr;
     ora_wray_setup(&wr, o, d, (double)ray->tmin, (double)ray->tmax);

@@ -104,3 +104,47 @@ def test_post_processing_matches_oracle(renderer, teapot_path):
         b = (C.c_float * 3)(1.2, 1.0, 0.9)
         pyoracle.load().ora_post(pyoracle._p(acc), 32 * 32, C.cast(b, C.c_void_p), int(tone), 1, pyoracle._p(ref))
         np.testing.assert_allclose(out, ref, rtol=1e-5, atol=1e-6)
+
+
+# ---------------------------------------------------------------- wavefront pipeline
+
+def test_wavefront_matches_oracle_and_megakernel(renderer, teapot_path):
+    for sc, spp in ((scenes.teapot_scene(teapot_path, 64, 64), 4),
+                    (scenes.config2(subdiv=2, width=96, height=64, trace_depth=5, env_w=256, env_h=128), 2)):
+        gpu_wf, cpu = render_both(renderer, sc, spp, mode=capi.APH_MODE_WAVEFRONT)
+        compare(gpu_wf, cpu, min_frac=0.98)
+        gpu_mk, _ = render_both(renderer, sc, spp, mode=capi.APH_MODE_MEGAKERNEL)
+        same = np.isclose(gpu_wf, gpu_mk, rtol=1e-5, atol=1e-6).all(axis=-1).mean()
+        assert same >= 0.995, same
+
+
+def test_wavefront_request_alone_equals_request_in_a_batch(renderer, teapot_path):
+    """Serving determinism: a request's image is bit-identical whether it runs alone or
+    shares the wavefront queues with other requests (different scenes, sizes, spp)."""
+    a = scenes.teapot_scene(teapot_path, 40, 32)
+    b = scenes.config1(width=56, height=48, trace_depth=3)
+    ta = renderer.load_scene(a)
+    tb = renderer.load_scene(b)
+    alone = renderer.target(40, 32)
+    renderer.render([renderer.request(alone, ta, a.frame, 0, 3)], capi.APH_MODE_WAVEFRONT)
+    mixa = renderer.target(40, 32)
+    mixb = renderer.target(56, 48)
+    renderer.render([renderer.request(mixb, tb, b.frame, 0, 5), renderer.request(mixa, ta, a.frame, 0, 3)],
+                    capi.APH_MODE_WAVEFRONT)
+    renderer.wait()
+    np.testing.assert_array_equal(renderer.readback(alone, 40, 32), renderer.readback(mixa, 40, 32))
+    solo_b = renderer.target(56, 48)
+    renderer.render([renderer.request(solo_b, tb, b.frame, 0, 5)], capi.APH_MODE_WAVEFRONT)
+    np.testing.assert_array_equal(renderer.readback(solo_b, 56, 48), renderer.readback(mixb, 56, 48))
+
+
+def test_wavefront_progressive_accumulation(renderer, teapot_path):
+    sc = scenes.teapot_scene(teapot_path, 48, 40)
+    tlas = renderer.load_scene(sc)
+    a = renderer.target(48, 40)
+    b = renderer.target(48, 40)
+    renderer.render([renderer.request(a, tlas, sc.frame, 0, 5)], capi.APH_MODE_WAVEFRONT)
+    renderer.render([renderer.request(b, tlas, sc.frame, 0, 2)], capi.APH_MODE_WAVEFRONT)
+    renderer.render([renderer.request(b, tlas, sc.frame, 2, 3)], capi.APH_MODE_WAVEFRONT)
+    renderer.wait()
+    np.testing.assert_array_equal(renderer.readback(a, 48, 40), renderer.readback(b, 48, 40))
