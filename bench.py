@@ -85,7 +85,20 @@ def main():
     ap.add_argument("--subdiv", type=int, default=4, help="4: 1.02M instanced tris; 5: 4.1M")
     ap.add_argument("--mode", choices=["megakernel", "wavefront"], default="wavefront")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device-to-device copy (calibrates FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        hip = C.CDLL("libamdhip64.so")
+        n = 1 << 30
+        a, b = C.c_void_p(), C.c_void_p()
+        assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+        hip.hipMemset(a, 1, C.c_size_t(n))
+        for _ in range(3):
+            assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
+        hip.hipDeviceSynchronize()
+        print(json.dumps({"copy_bytes": n, "copies": 3}))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
               "the path shards by sample range across GPUs, see DESIGN.md.", file=sys.stderr)
