@@ -100,8 +100,13 @@ __device__ __forceinline__ f3 texel(const float4* tex, uint32_t w, int x, int y)
     return f3{t.x, t.y, t.z};
 }
 // Repeat addressing; filter 0 = nearest (HGI default sampler), 1 = bilinear.
+// A non-finite coordinate reads as 0 (a NaN direction comes from sampleHemisphere
+// normalizing n + d with d = -n when a draw is exactly 1.0); without this the
+// float-to-int conversion below would be undefined.
 __device__ __forceinline__ f3 texLookup(const float4* tex, uint32_t w, uint32_t h, f2 uv, int filter)
 {
+    if (!isfinite(uv.x)) uv.x = 0.0f;
+    if (!isfinite(uv.y)) uv.y = 0.0f;
     if (filter == 0) {
         float fu = uv.x - floorf(uv.x), fv = uv.y - floorf(uv.y);
         int ix = (int)(fu * (float)w), iy = (int)(fv * (float)h);
@@ -354,7 +359,11 @@ __device__ __constant__ const float kSheenLut[256] = {1.6177f,
 __device__ __forceinline__ float sheenAlbedo(float cosTheta, float roughness)  // :341-362
 {
     float x = cosTheta * 15.0f, y = roughness * 15.0f;
-    int ix = (int)x, iy = (int)y;
+    // NaN reads row/column 0 and indices stay inside the table; the callers'
+    // finite arguments lie in [0, 1] and are unaffected.
+    if (!(x >= 0.0f)) x = 0.0f;
+    if (!(y >= 0.0f)) y = 0.0f;
+    int ix = min((int)fminf(x, 15.0f), 15), iy = min((int)fminf(y, 15.0f), 15);
     int ix2 = min(max(ix + 1, 0), 15), iy2 = min(max(iy + 1, 0), 15);
     float fx = x - (float)ix, fy = y - (float)iy;
     float v1 = lerp(kSheenLut[iy * 16 + ix], kSheenLut[iy * 16 + ix2], fx);

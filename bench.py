@@ -8,15 +8,23 @@ Prints one JSON line.  value = Mrays/s over the timed steps, where a ray is a
 BVH query actually executed (extension + shadow rays, counted by the kernels).
 The roofline entry prices the dominant kernel by its algorithmic bytes (BVH
 nodes, triangles, instance records, ray and hit records, accumulation) per
-launch over its HIP-event launch time.  The CPU baseline is the oracle's
-restatement of the reference path tracer on the host cores, on a bounded
-pixel sample of the same frame.
+launch over its HIP-event launch time.  Its HBM traffic comes from two
+rocprofv3 counter passes (FETCH_SIZE, then WRITE_SIZE) over one frame, run as
+child processes before this process touches the GPU; each counter is scaled by
+what it reads for 1 GiB device-to-device copies made in the same pass.  The
+CPU baseline is the oracle's restatement of the reference path tracer on the
+host cores, on a bounded pixel sample of the same frame.
 """
 import argparse
 import ctypes as C
+import glob
 import json
 import os
+import shutil
+import sqlite3
+import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -38,6 +46,72 @@ def log(msg):
 def device_sync():
     hip = C.CDLL("libamdhip64.so")
     hip.hipDeviceSynchronize()
+
+
+KERNEL_SYMBOL = {"render_megakernel": "kRenderMega<false>", "wf_extend": "kExtend<false>"}
+COPY_BYTES, COPIES = 1 << 30, 3
+
+
+def copy_calibration():
+    """Three 1 GiB device-to-device copies: the plain copy kernel whose counter
+    readings calibrate FETCH_SIZE and WRITE_SIZE."""
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(COPY_BYTES)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(COPY_BYTES)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(COPY_BYTES)) == 0
+    for _ in range(COPIES):
+        assert hip.hipMemcpy(b, a, C.c_size_t(COPY_BYTES), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
+def pmc_dispatches(db_dir, counter):
+    """(kernel name, counter value in KiB) per dispatch from a rocprofv3 database."""
+    rows = []
+    for db in glob.glob(os.path.join(db_dir, "**", "*.db"), recursive=True):
+        con = sqlite3.connect(db)
+        rows += con.execute("select kernel_name, value from counters_collection where counter_name = ?",
+                            (counter,)).fetchall()
+        con.close()
+    return rows
+
+
+def measure_traffic(args, kname):
+    """HBM bytes per launch of the dominant kernel: FETCH_SIZE and WRITE_SIZE per
+    dispatch (two rocprofv3 passes over one frame), each scaled by actual bytes /
+    counter reading of the calibration copies in the same pass."""
+    prof = shutil.which("rocprofv3")
+    if prof is None:
+        return None, "rocprofv3 not found"
+    sym = KERNEL_SYMBOL[kname]
+    res = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="aph_pmc_", dir="/tmp")
+        cmd = ["timeout", "-s", "KILL", "240", prof, "--pmc", counter, "-d", d, "-o", "pmc", "--",
+               sys.executable, "-u", os.path.abspath(__file__), "--pmc-child", "--spp", str(args.spp),
+               "--subdiv", str(args.subdiv), "--mode", args.mode]
+        log(f"pmc pass {counter}: {' '.join(cmd)}")
+        env = dict(os.environ, TMPDIR="/tmp")
+        p = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        rows = pmc_dispatches(d, counter)
+        shutil.rmtree(d, ignore_errors=True)
+        if p.returncode != 0 or not rows:
+            log(p.stdout[-2000:])
+            return None, f"{counter} pass failed (rc {p.returncode})"
+        kern = [v for n, v in rows if sym in n]
+        copies = sorted((v for n, v in rows if "copyBuffer" in n), reverse=True)[:COPIES]
+        if not kern or len(copies) < COPIES:
+            return None, f"{counter} pass: {len(kern)} {sym} dispatches, {len(copies)} copies"
+        factor = COPY_BYTES * COPIES / (sum(copies) * 1024.0)
+        res[counter] = (sum(kern) / len(kern), factor, len(kern))
+        log(f"{counter}: {res[counter][0]:.1f} KiB per {sym} dispatch over {len(kern)}, copy factor {factor:.4f}")
+    f_kib, f_fac, nd = res["FETCH_SIZE"]
+    w_kib, w_fac, _ = res["WRITE_SIZE"]
+    detail = {"fetch_kib_per_dispatch": round(f_kib, 1), "fetch_factor": round(f_fac, 4),
+              "write_kib_per_dispatch": round(w_kib, 1), "write_factor": round(w_fac, 4), "dispatches": nd}
+    return int((f_kib * f_fac + w_kib * w_fac) * 1024), detail
 
 
 def cpu_baseline(sc, budget_s=12.0):
@@ -76,6 +150,21 @@ def cpu_baseline(sc, budget_s=12.0):
     }
 
 
+def cpu_baseline_child(args):
+    """Runs cpu_baseline in a child process that never touches the GPU, so that a
+    failure there costs the baseline and not the measured line."""
+    cmd = [sys.executable, "-X", "faulthandler", "-u", os.path.abspath(__file__), "--cpu-baseline-child",
+           "--spp", str(args.spp), "--subdiv", str(args.subdiv)]
+    try:
+        p = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=300)
+    except subprocess.TimeoutExpired:
+        return {"value": None, "error": "CPU baseline child timed out"}
+    if p.returncode != 0:
+        log(p.stderr[-3000:])
+        return {"value": None, "error": f"CPU baseline child failed (rc {p.returncode})"}
+    return json.loads(p.stdout.strip().splitlines()[-1])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,20 +174,12 @@ def main():
     ap.add_argument("--subdiv", type=int, default=4, help="4: 1.02M instanced tris; 5: 4.1M")
     ap.add_argument("--mode", choices=["megakernel", "wavefront"], default="wavefront")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--copy-calibration", action="store_true",
-                    help="only run a 1 GiB device-to-device copy (calibrates FETCH_SIZE/WRITE_SIZE counters)")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 counter passes")
+    ap.add_argument("--cpu-baseline-child", action="store_true",
+                    help="internal: the oracle's CPU baseline alone (never touches the GPU)")
+    ap.add_argument("--pmc-child", action="store_true",
+                    help="internal: calibration copies + one frame, run under rocprofv3 --pmc")
     args = ap.parse_args()
-    if args.copy_calibration:
-        hip = C.CDLL("libamdhip64.so")
-        n = 1 << 30
-        a, b = C.c_void_p(), C.c_void_p()
-        assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
-        hip.hipMemset(a, 1, C.c_size_t(n))
-        for _ in range(3):
-            assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
-        hip.hipDeviceSynchronize()
-        print(json.dumps({"copy_bytes": n, "copies": 3}))
-        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
               "the path shards by sample range across GPUs, see DESIGN.md.", file=sys.stderr)
@@ -107,6 +188,19 @@ def main():
     sc = scenes.config2(subdiv=args.subdiv, spp=args.spp)
     mode = capi.APH_MODE_MEGAKERNEL if args.mode == "megakernel" else capi.APH_MODE_WAVEFRONT
     kname = "render_megakernel" if args.mode == "megakernel" else "wf_extend"
+    if args.cpu_baseline_child:
+        print(json.dumps(cpu_baseline(sc)), flush=True)
+        return
+    if args.pmc_child:
+        copy_calibration()
+        r = Renderer(0)
+        tgt = r.target(sc.width, sc.height)
+        r.render([r.request(tgt, r.load_scene(sc), sc.frame, 0, sc.spp)], mode)
+        r.wait()
+        r.close()
+        print("pmc child done", flush=True)
+        return
+    traffic, traffic_detail = (None, "skipped (--no-traffic)") if args.no_traffic else measure_traffic(args, kname)
     log(f"scene: {sc.instanced_triangles} instanced / {sc.unique_triangles} unique triangles, "
         f"{len(sc.instances)} instances, {sc.width}x{sc.height} x {sc.spp} spp, mode {args.mode}")
     r = Renderer(0)
@@ -179,7 +273,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_detail": traffic_detail,
             "bytes_per_launch": int(bytes_per_launch),
             "avg_launch_ms": round(avg_launch_ms, 3),
         },
@@ -195,11 +290,11 @@ def main():
         out["config"]["plain_megakernel_mrays_s"] = round(
             (pst["extension_rays"] + pst["shadow_rays"]) / t_plain / 1e6, 2)
         log(f"plain megakernel frame: {t_plain * 1e3:.1f} ms")
-    if not args.no_cpu_baseline:
-        log("cpu baseline (oracle on host cores)...")
-        out["cpu_baseline"] = cpu_baseline(sc)
-    print(json.dumps(out), flush=True)
     r.close()
+    if not args.no_cpu_baseline:
+        log("cpu baseline (oracle on host cores, child process)...")
+        out["cpu_baseline"] = cpu_baseline_child(args)
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

@@ -395,7 +395,11 @@ float ora_sheen_albedo(float cosTheta, float roughness)
 {
     float x = cosTheta * 15.0f;
     float y = roughness * 15.0f;
-    int ix = (int)x, iy = (int)y;
+    /* NaN reads row/column 0 and indices stay inside the table (the GPU path does
+     * the same); finite arguments in [0, 1] are unaffected. */
+    if (!(x >= 0.0f)) x = 0.0f;
+    if (!(y >= 0.0f)) y = 0.0f;
+    int ix = (int)fminf(x, 15.0f), iy = (int)fminf(y, 15.0f);
     int ix2 = ix + 1 < 0 ? 0 : (ix + 1 > 15 ? 15 : ix + 1);
     int iy2 = iy + 1 < 0 ? 0 : (iy + 1 > 15 ? 15 : iy + 1);
     float fx = x - (float)ix, fy = y - (float)iy;

@@ -103,9 +103,16 @@ void ora_latlong_to_dir(const float uv[2], float dir[3])
     dir[2] = sp * st;
 }
 
-/* Texture lookup with repeat addressing.  filter 0 = nearest, 1 = bilinear. */
+/* Texture lookup with repeat addressing.  filter 0 = nearest, 1 = bilinear.
+ * A non-finite coordinate reads as 0: a NaN direction reaches the lookup when
+ * sampleHemisphere (Sampling.slang:39-52) normalizes n + d with d = -n, which the
+ * u32-to-float mapping makes possible (a draw of exactly 1.0).  Shader samplers
+ * leave NaN coordinates implementation-defined; gfx950's float-to-int
+ * conversion gives 0, and both paths state that explicitly. */
 static v3 tex_lookup(const float* rgb, uint32_t w, uint32_t h, float u, float v, int filter)
 {
+    if (!isfinite(u)) u = 0.0f;
+    if (!isfinite(v)) v = 0.0f;
     if (filter == 0) {
         float fu = u - floorf(u), fv = v - floorf(v);
         int ix = (int)(fu * (float)w), iy = (int)(fv * (float)h);

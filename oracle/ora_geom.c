@@ -394,11 +394,18 @@ static void run_trace(const ora_scene* s, const ora_ray* rays, uint32_t n, ora_h
     if (nthreads > 256) nthreads = 256;
     pthread_t th[256];
     trace_job jobs[256];
+    int started[256];
+    pthread_attr_t attr;
+    pthread_attr_init(&attr);
+    pthread_attr_setstacksize(&attr, (size_t)16 << 20);
     for (int t = 0; t < nthreads; t++) {
         jobs[t] = (trace_job){ s, rays, n, hits, occ, t, nthreads };
-        pthread_create(&th[t], NULL, trace_worker, &jobs[t]);
+        started[t] = pthread_create(&th[t], &attr, trace_worker, &jobs[t]) == 0;
+        if (!started[t]) trace_worker(&jobs[t]); /* no thread available: run this share inline */
     }
-    for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+    pthread_attr_destroy(&attr);
+    for (int t = 0; t < nthreads; t++)
+        if (started[t]) pthread_join(th[t], NULL);
 }
 
 void ora_trace_closest(const ora_scene* s, const ora_ray* rays, uint32_t n, ora_hit* hits, int nthreads)

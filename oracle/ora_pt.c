@@ -469,15 +469,22 @@ void ora_render(const ora_scene* s, const ora_material_constants* mats, uint32_t
     if (nthreads < 1) nthreads = 1;
     if (nthreads > 256) nthreads = 256;
     pthread_t th[256];
+    int started[256];
     render_job* jobs = (render_job*)calloc((size_t)nthreads, sizeof(render_job));
+    /* explicit stack size: the worker must not depend on the process's RLIMIT_STACK */
+    pthread_attr_t attr;
+    pthread_attr_init(&attr);
+    pthread_attr_setstacksize(&attr, (size_t)16 << 20);
     for (int t = 0; t < nthreads; t++) {
         jobs[t] = (render_job){ s, mats, nmat, env, fp, W, H, start, count, pixels, npix, accum, { 0, 0 }, t,
                                 nthreads };
-        pthread_create(&th[t], NULL, render_worker, &jobs[t]);
+        started[t] = pthread_create(&th[t], &attr, render_worker, &jobs[t]) == 0;
+        if (!started[t]) render_worker(&jobs[t]); /* no thread available: run this share inline */
     }
+    pthread_attr_destroy(&attr);
     ora_counters tot = { 0, 0 };
     for (int t = 0; t < nthreads; t++) {
-        pthread_join(th[t], NULL);
+        if (started[t]) pthread_join(th[t], NULL);
         tot.extension_rays += jobs[t].cnt.extension_rays;
         tot.shadow_rays += jobs[t].cnt.shadow_rays;
     }

@@ -148,3 +148,25 @@ def test_wavefront_progressive_accumulation(renderer, teapot_path):
     renderer.render([renderer.request(b, tlas, sc.frame, 2, 3)], capi.APH_MODE_WAVEFRONT)
     renderer.wait()
     np.testing.assert_array_equal(renderer.readback(a, 48, 40), renderer.readback(b, 48, 40))
+
+
+def test_nan_sample_direction_pixels_match_oracle(renderer):
+    """config2 at 1920x1080: pixels (104, 906) and (823, 480) each draw a sample of
+    exactly 1.0 within the first 24 sample indices, so sampleHemisphere returns a NaN
+    direction that reaches the environment lookup (defined to read texel (0, 0)) and the
+    NaN contribution is cleared by adjustRadiance.  Both paths must agree there."""
+    sc = scenes.config2(subdiv=4)
+    tlas = renderer.load_scene(sc)
+    w, h, spp = sc.width, sc.height, 24
+    pix = np.array([906 * w + 104, 480 * w + 823, 0, w * h - 1, 540 * w + 960], dtype=np.uint32)
+    ora = pyoracle.OracleScene(sc)
+    cpu, _ = ora.render(sc.frame, w, h, 0, spp, pixels=pix)
+    for mode in (capi.APH_MODE_WAVEFRONT, capi.APH_MODE_MEGAKERNEL):
+        tgt = renderer.target(w, h)
+        renderer.render([renderer.request(tgt, tlas, sc.frame, 0, spp)], mode)
+        renderer.wait()
+        gpu = renderer.readback(tgt, w, h)
+        g = gpu.reshape(-1, 4)[pix]
+        c = cpu.reshape(-1, 4)[pix]
+        assert np.isfinite(g).all()
+        np.testing.assert_allclose(g, c, rtol=2e-3, atol=2e-4)

@@ -188,3 +188,11 @@ def test_lambert_white_furnace():
     dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
     tot = sum(evaluate(m, (0.2, 0.0, 0.9797959), L)[0] for L in dirs if L[2] > 0)
     assert tot * 4 * math.pi / len(dirs) == pytest.approx(1.0, rel=0.02)
+
+
+def test_sheen_lut_nan_reads_row_zero():
+    lib = pyoracle.load()
+    assert lib.ora_sheen_albedo(np.float32(np.nan), np.float32(0.0)) == lib.ora_sheen_albedo(np.float32(0.0),
+                                                                                               np.float32(0.0))
+    assert lib.ora_sheen_albedo(np.float32(1.0), np.float32(np.nan)) == lib.ora_sheen_albedo(np.float32(1.0),
+                                                                                               np.float32(0.0))

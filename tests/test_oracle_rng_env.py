@@ -207,3 +207,22 @@ def test_environment_sampling_chi2():
     exp = target * n
     chi2 = ((counts - exp) ** 2 / exp).sum()
     assert chi2 < 200  # 127 dof
+
+
+def test_nan_direction_reads_texel_zero():
+    """sampleHemisphere normalizes n + d; a draw of exactly 1.0 gives d = -n and a NaN
+    direction, which then reaches the environment lookup.  Non-finite texture
+    coordinates read as 0 (texel (0, 0) with nearest filtering), never out of bounds."""
+    rng = np.random.default_rng(9)
+    rgb = rng.uniform(0.5, 2.0, (32, 64, 3)).astype(np.float32)
+    out = np.zeros(3, dtype=np.float32)
+    for filt in (0, 1):
+        env, keep = pyoracle.make_env(dict(light_rgb=rgb, filter=filt))
+        nan_dir = np.full(3, np.nan, dtype=np.float32)
+        pyoracle.load().ora_env_evaluate(C.byref(env), pyoracle._p(nan_dir), 0, 0, 0, 1, 1, pyoracle._p(out))
+        if filt == 0:
+            np.testing.assert_array_equal(out, rgb[0, 0])
+        else:
+            # u = v = 0 sits on the texel corner: the average of the four wrapped corners
+            exp = (rgb[0, 0] + rgb[0, -1] + rgb[-1, 0] + rgb[-1, -1]) / 4
+            np.testing.assert_allclose(out, exp, rtol=1e-6)
