@@ -842,6 +842,9 @@ int aph_stats_get(aph_ctx_t c, aph_stats* out)
     out->tri_tests = k.tris;
     out->inst_visits = k.insts;
     out->paths = k.paths;
+    out->shadow_node_visits = k.snodes;
+    out->shadow_tri_tests = k.stris;
+    out->shadow_inst_visits = k.sinsts;
     return APH_OK;
 }
 

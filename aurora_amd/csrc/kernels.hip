@@ -21,13 +21,16 @@ __device__ __forceinline__ uint32_t waveSum(uint32_t v)
 __device__ __forceinline__ void flushCounters(const PathCounters& pc, uint32_t paths, bool stats, DevCounters* out)
 {
     uint32_t ext = waveSum(pc.ext), sh = waveSum(pc.shadow), np = waveSum(paths);
-    uint32_t nodes = 0, tris = 0, insts = 0;
+    uint32_t nodes = 0, tris = 0, insts = 0, snodes = 0, stris = 0, sinsts = 0;
     if (stats) {
-        nodes = waveSum(pc.ts.nodes);
-        tris = waveSum(pc.ts.tris);
-        insts = waveSum(pc.ts.insts);
+        snodes = waveSum(pc.tsS.nodes);
+        stris = waveSum(pc.tsS.tris);
+        sinsts = waveSum(pc.tsS.insts);
+        nodes = waveSum(pc.ts.nodes) + snodes;
+        tris = waveSum(pc.ts.tris) + stris;
+        insts = waveSum(pc.ts.insts) + sinsts;
     }
-    uint32_t ovf = waveSum(pc.ts.overflow);
+    uint32_t ovf = waveSum(pc.ts.overflow | pc.tsS.overflow);
     if ((threadIdx.x & 63) == 0) {
         if (ext) atomicAdd(&out->ext, (unsigned long long)ext);
         if (sh) atomicAdd(&out->shadow, (unsigned long long)sh);
@@ -35,6 +38,9 @@ __device__ __forceinline__ void flushCounters(const PathCounters& pc, uint32_t p
         if (nodes) atomicAdd(&out->nodes, (unsigned long long)nodes);
         if (tris) atomicAdd(&out->tris, (unsigned long long)tris);
         if (insts) atomicAdd(&out->insts, (unsigned long long)insts);
+        if (snodes) atomicAdd(&out->snodes, (unsigned long long)snodes);
+        if (stris) atomicAdd(&out->stris, (unsigned long long)stris);
+        if (sinsts) atomicAdd(&out->sinsts, (unsigned long long)sinsts);
         if (ovf) atomicAdd(&out->overflow, (unsigned long long)ovf);
     }
 }
@@ -91,7 +97,7 @@ __global__ __launch_bounds__(kBlock) void kTrace(TraceScene sc, const aph_ray* r
         aph_ray r = rays[i];
         HitRec h;
         bool found = traceRay<ANY, STATS>(sc, f3{r.org[0], r.org[1], r.org[2]}, f3{r.dir[0], r.dir[1], r.dir[2]},
-                                          r.tmin, r.tmax, stk, h, pc.ts);
+                                          r.tmin, r.tmax, stk, h, ANY ? pc.tsS : pc.ts);
         if (ANY) {
             occ[i] = found ? 1 : 0;
             pc.shadow = 1;

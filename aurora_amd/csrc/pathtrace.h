@@ -22,7 +22,8 @@ struct RenderScene {
 
 struct PathCounters {
     uint32_t ext, shadow;
-    TraceStats ts;
+    TraceStats ts;   // closest-hit queries
+    TraceStats tsS;  // any-hit (shadow) queries
 };
 
 template <bool STATS>
@@ -30,7 +31,7 @@ __device__ __forceinline__ float traceShadow(const RenderScene& rs, f3 org, f3 d
 {
     HitRec h;
     pc.shadow++;
-    bool occ = traceRay<true, STATS>(rs.sc, org, dir, kRayTMin, INFINITY, stk, h, pc.ts);
+    bool occ = traceRay<true, STATS>(rs.sc, org, dir, kRayTMin, INFINITY, stk, h, pc.tsS);
     return occ ? 0.0f : 1.0f;
 }
 

@@ -448,6 +448,9 @@ __global__ __launch_bounds__(kWB) void kShadow(const DevReq* reqs, const SegInfo
         addCounter(&ctr->nodes, ts.nodes);
         addCounter(&ctr->tris, ts.tris);
         addCounter(&ctr->insts, ts.insts);
+        addCounter(&ctr->snodes, ts.nodes);
+        addCounter(&ctr->stris, ts.tris);
+        addCounter(&ctr->sinsts, ts.insts);
     }
     addCounter(&ctr->overflow, ts.overflow);
 }

@@ -173,6 +173,9 @@ typedef struct {
     uint64_t tri_tests;         /* triangles tested (stats mode only) */
     uint64_t inst_visits;       /* instance transforms applied (stats mode only) */
     uint64_t paths;             /* pixel-samples started */
+    uint64_t shadow_node_visits; /* the any-hit queries' share of node_visits */
+    uint64_t shadow_tri_tests;   /* ... of tri_tests */
+    uint64_t shadow_inst_visits; /* ... of inst_visits */
 } aph_stats;
 
 /* ---- context: createRenderer(Backend, taskCount), Aurora.h:1233-1234;
