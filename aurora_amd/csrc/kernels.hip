@@ -9,6 +9,7 @@ namespace aph {
 namespace {
 
 constexpr int kBlock = 128;  // 2 waves of 64; 16x8 pixel tiles for the megakernel
+static_assert(kBlock == kStackLanes, "the traversal stack stride is the block size");
 
 __device__ __forceinline__ uint32_t waveSum(uint32_t v)
 {
@@ -55,8 +56,6 @@ __global__ __launch_bounds__(kBlock) void kRenderMega(RenderScene rs, aph_frame 
     __shared__ uint32_t sStack[kLdsStack * kBlock];
     ShortStack stk;
     stk.lds = sStack + threadIdx.x;
-    stk.stride = kBlock;
-    stk.sp = 0;
     PathCounters pc = {};
     // 16x8 tile per block, row-major tiles
     uint32_t tilesX = (W + 15) / 16;
@@ -89,8 +88,6 @@ __global__ __launch_bounds__(kBlock) void kTrace(TraceScene sc, const aph_ray* r
     __shared__ uint32_t sStack[kLdsStack * kBlock];
     ShortStack stk;
     stk.lds = sStack + threadIdx.x;
-    stk.stride = kBlock;
-    stk.sp = 0;
     PathCounters pc = {};
     uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i < n) {

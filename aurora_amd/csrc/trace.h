@@ -13,8 +13,9 @@
 //   * closest hit = lexicographic minimum of (t, instance, primitive).
 // Boxes only cull, conservatively: BLAS boxes are tight float boxes tested with
 // the robust slab test of Ize (JCGT 2(2), 2013): sign-selected planes,
-// NaN-ignoring min/max, far distance widened by (1 + 2*gamma3); instance boxes
-// are widened by a pad that bounds the world/object rounding mismatch.
+// NaN-ignoring min/max, far distance widened to cover the rounding of the
+// distances; instance boxes are widened by a pad that bounds the world/object
+// rounding mismatch.
 #pragma once
 
 #include "scene.h"
@@ -24,7 +25,12 @@ namespace aph {
 constexpr int kLdsStack = 16;    // stack slots per lane held in LDS
 constexpr int kSpillStack = 48;  // deeper slots spill to private (scratch) memory
 constexpr int kStackCap = kLdsStack + kSpillStack;
-constexpr float kRobustFar = 1.0000004f;   // 1 + 2*gamma3, gamma3 = 3u/(1-3u), u = 2^-24
+// Far-distance widening of the slab test.  Each distance (b - o) * inv carries a
+// subtraction and a product rounding (u each, u = 2^-24) and the reciprocal's
+// error (v_rcp_f32: 1 ulp <= 2u), so |relative error| <= gamma4; near and far,
+// plus the rounding of this product, need 1 + 2*gamma5 = 1 + 5.96e-7.  Taken
+// with margin:
+constexpr float kRobustFar = 1.0000008f;
 constexpr float kTRoundUp = 1.0000002f;    // rounds a float-converted double t up
 constexpr uint32_t kMaxSteps = 1u << 22;   // traversal step cap: a corrupt BVH cannot hang a wave
 
@@ -38,31 +44,15 @@ struct TraceStats {
     uint32_t nodes, tris, insts, overflow;
 };
 
-// Per-lane stack: the newest kLdsStack entries in LDS (slot-major, so the 64
-// lanes of a wave hit 64 consecutive dwords), older entries in private memory.
+// Per-lane traversal stack: a ring of the newest kLdsStack entries in LDS,
+// slot-major with a fixed stride of kStackLanes (the block size of every kernel
+// that traces), so the 64 lanes of a wave touch 64 consecutive dwords; older
+// entries move to a private spill array.  The caller hands in this lane's LDS
+// column (its __shared__ array + threadIdx.x); the stack pointer and the spill
+// array are locals of the traversal, so that only spills touch scratch memory.
+constexpr int kStackLanes = 128;
 struct ShortStack {
     uint32_t* lds;
-    int stride;
-    int sp;
-    uint32_t spill[kSpillStack];
-
-    __device__ __forceinline__ bool push(int32_t v)
-    {
-        if (sp >= kStackCap) return false;
-        uint32_t* slot = lds + (sp & (kLdsStack - 1)) * stride;
-        if (sp >= kLdsStack) spill[sp - kLdsStack] = *slot;
-        *slot = (uint32_t)v;
-        sp++;
-        return true;
-    }
-    __device__ __forceinline__ int32_t pop()
-    {
-        sp--;
-        uint32_t* slot = lds + (sp & (kLdsStack - 1)) * stride;
-        int32_t v = (int32_t)*slot;
-        if (sp >= kLdsStack) *slot = spill[sp - kLdsStack];
-        return v;
-    }
 };
 
 // Loads through the global address space: the BVH pointers come out of device
@@ -94,29 +84,50 @@ __device__ __forceinline__ InstTrav gloadInst(const InstTrav* p)
     return t;
 }
 
+// Box-test ray.  Instance boxes are widened by a pad; instead of padding every
+// box, the pad moves the origin once per ray: (lo - pad) - o is computed as
+// lo - oLo with oLo = o + pad', hi + pad - o as hi - oHi with oHi = o - pad',
+// where pad' exceeds pad by more than the rounding of o +- pad'.  Inside a BLAS
+// both origins are o.  The reciprocal direction comes from v_rcp_f32 (1 ulp).
 struct BoxRay {
-    float ox, oy, oz, ix, iy, iz;  // the sign of ix/iy/iz selects the near plane
+    float lx, ly, lz;  // origin for the low planes
+    float hx, hy, hz;  // origin for the high planes
+    float ix, iy, iz;  // reciprocal direction; its sign selects the near plane
 };
 
-APH_HD BoxRay makeBoxRay(f3 o, f3 d)
+__device__ __forceinline__ BoxRay makeBoxRay(f3 o, f3 d)
 {
     BoxRay r;
-    r.ox = o.x; r.oy = o.y; r.oz = o.z;
-    r.ix = 1.0f / d.x; r.iy = 1.0f / d.y; r.iz = 1.0f / d.z;  // +-0 -> +-inf
+    r.lx = r.hx = o.x; r.ly = r.hy = o.y; r.lz = r.hz = o.z;
+    r.ix = __builtin_amdgcn_rcpf(d.x);  // +-0 -> +-inf
+    r.iy = __builtin_amdgcn_rcpf(d.y);
+    r.iz = __builtin_amdgcn_rcpf(d.z);
+    return r;
+}
+
+// fl(o + p) >= o + pad holds for p >= (pad + u|o|) / (1 - u), u = 2^-24; the
+// margins below (2^-20 relative on the pad, about 4u on |o|) cover that and the
+// rounding of p itself.
+__device__ __forceinline__ BoxRay makeBoxRayPadded(f3 o, f3 d, float pad)
+{
+    BoxRay r = makeBoxRay(o, d);
+    float px = pad * 1.000001f + fabsf(o.x) * 2.4e-7f;
+    float py = pad * 1.000001f + fabsf(o.y) * 2.4e-7f;
+    float pz = pad * 1.000001f + fabsf(o.z) * 2.4e-7f;
+    r.lx = o.x + px; r.hx = o.x - px;
+    r.ly = o.y + py; r.hy = o.y - py;
+    r.lz = o.z + pz; r.hz = o.z - pz;
     return r;
 }
 
 // Robust slab test.  A zero direction component gives +-inf distances, or NaN
 // when the origin lies on the plane; fmaxf/fminf ignore NaN, which treats that
 // plane as not limiting - the correct answer for a ray inside the slab.
-__device__ __forceinline__ bool slab(const BoxRay& r, float4 lo, float4 hi, float pad, float tmin, float tmax,
-                                     float& tnear)
+__device__ __forceinline__ bool slab(const BoxRay& r, float4 lo, float4 hi, float tmin, float tmax, float& tnear)
 {
-    float lx = lo.x - pad, ly = lo.y - pad, lz = lo.z - pad;
-    float hx = hi.x + pad, hy = hi.y + pad, hz = hi.z + pad;
-    float tlx = (lx - r.ox) * r.ix, thx = (hx - r.ox) * r.ix;
-    float tly = (ly - r.oy) * r.iy, thy = (hy - r.oy) * r.iy;
-    float tlz = (lz - r.oz) * r.iz, thz = (hz - r.oz) * r.iz;
+    float tlx = (lo.x - r.lx) * r.ix, thx = (hi.x - r.hx) * r.ix;
+    float tly = (lo.y - r.ly) * r.iy, thy = (hi.y - r.hy) * r.iy;
+    float tlz = (lo.z - r.lz) * r.iz, thz = (hi.z - r.hz) * r.iz;
     bool sx = r.ix >= 0.0f, sy = r.iy >= 0.0f, sz = r.iz >= 0.0f;
     float nx = sx ? tlx : thx, fx = sx ? thx : tlx;
     float ny = sy ? tly : thy, fy = sy ? thy : tly;
@@ -193,8 +204,18 @@ __device__ __forceinline__ Sheared shear(const WRay& w, float4 v)
     return s;
 }
 
-__device__ __forceinline__ bool triWatertight(const WRay& w, const Tri& tr, double tmin, double tmax, double& t,
-                                              float& u, float& v)
+// Result of an accepted triangle test; u and v are divided out only for a hit
+// that becomes the closest one.
+struct TriHit {
+    double t, det, V, W;
+};
+
+// Watertight test.  Accepts when U, V, W share a sign (zeros allowed), det != 0
+// and tmin <= t <= tmax for t = T / det.  Before dividing, |T| is compared with
+// tmin |det| and tmax |det| widened by 2^-46: outside that band the quotient
+// (which equals |T| / |det| exactly) is certainly out of range, so only
+// triangles that may pass pay for the division.
+__device__ __forceinline__ bool triWatertight(const WRay& w, const Tri& tr, double tmin, double tmax, TriHit& h)
 {
 #pragma clang fp contract(off)
     Sheared A = shear(w, tr.v0), B = shear(w, tr.v1), C = shear(w, tr.v2);
@@ -205,11 +226,11 @@ __device__ __forceinline__ bool triWatertight(const WRay& w, const Tri& tr, doub
     double det = (U + V) + W;
     if (det == 0.0) return false;
     double T = ((U * (w.Sz * A.z)) + (V * (w.Sz * B.z))) + (W * (w.Sz * C.z));
+    double adet = fabs(det), aT = det < 0.0 ? -T : T;
+    if (aT > (tmax * 1.0000000000000142) * adet || aT < (tmin * 0.9999999999999858) * adet) return false;
     double tt = T / det;
     if (!(tt >= tmin && tt <= tmax)) return false;
-    t = tt;
-    u = (float)(V / det);
-    v = (float)(W / det);
+    h.t = tt; h.det = det; h.V = V; h.W = W;
     return true;
 }
 
@@ -221,9 +242,23 @@ APH_HD bool hitBetter(double t, uint32_t inst, uint32_t prim, const HitRec& b)
     return prim < b.prim;
 }
 
+// Traversal state values besides node indices (>= 0) and leaves (< 0).
+constexpr int32_t kDone = 0x7FFFFFFE;  // stack exhausted: the ray is finished
+APH_HD bool isInner(int32_t r) { return (uint32_t)r < (uint32_t)kDone; }
+
 // Traces one ray.  ANY: stop at the first accepted hit (shadow rays).
+//
+// While-while traversal (Aila & Laine, HPG 2009) for 64-wide waves: the inner
+// loop steps through internal nodes; the first leaf a lane meets is parked and
+// the lane continues with its next stack entry, and the loop ends once every
+// lane still in it holds a parked leaf.  Leaves - instance entries in the TLAS,
+// triangle ranges in a BLAS - are then processed with the wave together.
+// Entering an instance pushes the pending TLAS entry and an exit marker; popping
+// the marker restores the world-space ray.  Traversal order never changes the
+// result: boxes only cull conservatively and the closest hit is a lexicographic
+// minimum.
 template <bool ANY, bool STATS>
-__device__ bool traceRay(const TraceScene& sc, f3 org, f3 dir, float tmin, float tmax, ShortStack& stk,
+__device__ bool traceRay(const TraceScene& sc, f3 org, f3 dir, float tmin, float tmax, ShortStack stk,
                          HitRec& best, TraceStats& st)
 {
     best.t = (double)tmax;
@@ -237,8 +272,7 @@ __device__ bool traceRay(const TraceScene& sc, f3 org, f3 dir, float tmin, float
           isfinite(dir.z)))
         return false;
     const float wpad = sc.padScale * (fabsf(org.x) + fabsf(org.y) + fabsf(org.z) + sc.extent);
-    BoxRay br = makeBoxRay(org, dir);
-    float pad = wpad;
+    BoxRay br = makeBoxRayPadded(org, dir, wpad);
     const BvhNode* nodes = sc.tlas;
     const Tri* tris = nullptr;
     uint32_t curInst = 0xFFFFFFFFu;  // != 0xFFFFFFFF while inside a BLAS
@@ -246,71 +280,105 @@ __device__ bool traceRay(const TraceScene& sc, f3 org, f3 dir, float tmin, float
     w.ox = w.oy = w.oz = 0.0f;
     w.Sx = w.Sy = w.Sz = 0.0;
     w.perm = 0;
-    const double dtmin = (double)tmin, dtmax = (double)tmax;
-    stk.sp = 0;
-    int32_t cur = 0;
+    const double dtmin = (double)tmin;
+    float tlim = tmax;  // box cull distance: the closest hit's t rounded up to float
+    uint32_t* const lds = stk.lds;
+    int sp = 0;
+    uint32_t spill[kSpillStack];
+    auto push = [&](int32_t v) -> bool {
+        if (sp >= kStackCap) return false;
+        uint32_t* slot = lds + (sp & (kLdsStack - 1)) * kStackLanes;
+        if (sp >= kLdsStack) spill[sp - kLdsStack] = *slot;
+        *slot = (uint32_t)v;
+        sp++;
+        return true;
+    };
+    // next stack entry, or kDone when the stack is empty
+    auto pop = [&]() -> int32_t {
+        if (sp == 0) return kDone;
+        sp--;
+        uint32_t* slot = lds + (sp & (kLdsStack - 1)) * kStackLanes;
+        int32_t v = (int32_t)*slot;
+        if (sp >= kLdsStack) *slot = spill[sp - kLdsStack];
+        return v;
+    };
+    int32_t cur = 0, leaf = 0;
     uint32_t steps = 0;
     while (true) {
-        if (++steps > kMaxSteps) { st.overflow = 1; break; }
-        if (cur >= 0) {
+        while (isInner(cur)) {
+            if (++steps > kMaxSteps) { st.overflow = 1; return found; }
             const float4* np = reinterpret_cast<const float4*>(nodes + cur);
             float4 a = gload4(np), b = gload4(np + 1), c = gload4(np + 2), d = gload4(np + 3);
             if (STATS) st.nodes++;
-            float tlim = found ? (float)best.t * kTRoundUp : tmax;
             float t0, t1;
-            bool h0 = slab(br, a, b, pad, tmin, tlim, t0);
-            bool h1 = slab(br, c, d, pad, tmin, tlim, t1);
+            bool h0 = slab(br, a, b, tmin, tlim, t0);
+            bool h1 = slab(br, c, d, tmin, tlim, t1);
             int32_t c0 = __float_as_int(a.w), c1 = __float_as_int(b.w);
             if (h0 && h1) {
                 bool swap = t1 < t0;
-                if (!stk.push(swap ? c0 : c1)) { st.overflow = 1; break; }
+                if (!push(swap ? c0 : c1)) { st.overflow = 1; return found; }
                 cur = swap ? c1 : c0;
-                continue;
+            } else if (h0 || h1) {
+                cur = h0 ? c0 : c1;
+            } else {
+                cur = pop();
             }
-            if (h0) { cur = c0; continue; }
-            if (h1) { cur = c1; continue; }
-        } else if (curInst == 0xFFFFFFFFu) {
-            // TLAS leaf: one instance.  Enter its BLAS in object space.
-            uint32_t ii = leafFirst(cur);
-            const InstTrav it = gloadInst(sc.inst + ii);
-            f3 oo, od;
-            objectRay(it.w2o, org, dir, oo, od);
-            w = setupWRay(oo, od);
-            br = makeBoxRay(oo, od);
-            pad = 0.0f;
-            nodes = it.nodes;
-            tris = it.tris;
-            curInst = ii;
-            if (STATS) st.insts++;
-            if (!stk.push(kExitMarker)) { st.overflow = 1; break; }
-            cur = 0;
-            continue;
-        } else {
-            uint32_t first = leafFirst(cur), cnt = leafCount(cur);
-            for (uint32_t i = first; i < first + cnt; i++) {
-                Tri tr = gloadTri(tris + i);
-                if (STATS) st.tris++;
-                double t;
-                float u, v;
-                if (!triWatertight(w, tr, dtmin, dtmax, t, u, v)) continue;
-                uint32_t prim = __float_as_uint(tr.v0.w);
-                if (!found || hitBetter(t, curInst, prim, best)) {
-                    found = true;
-                    best.t = t; best.u = u; best.v = v; best.inst = curInst; best.prim = prim;
-                    if (ANY) return true;
+            if (cur < 0 && leaf == 0) {
+                leaf = cur;
+                cur = pop();
+            }
+            if (!__any(leaf == 0)) break;
+        }
+        if (leaf == 0 && cur < 0) {
+            leaf = cur;
+            cur = pop();
+        }
+        if (leaf != 0) {
+            if (curInst == 0xFFFFFFFFu) {
+                // TLAS leaf: one instance.  Enter its BLAS in object space.
+                uint32_t ii = leafFirst(leaf);
+                const InstTrav it = gloadInst(sc.inst + ii);
+                f3 oo, od;
+                objectRay(it.w2o, org, dir, oo, od);
+                w = setupWRay(oo, od);
+                br = makeBoxRay(oo, od);
+                nodes = it.nodes;
+                tris = it.tris;
+                curInst = ii;
+                if (STATS) st.insts++;
+                if (cur != kDone && !push(cur)) { st.overflow = 1; return found; }
+                if (!push(kExitMarker)) { st.overflow = 1; return found; }
+                cur = 0;
+            } else {
+                uint32_t first = leafFirst(leaf), cnt = leafCount(leaf);
+                for (uint32_t i = first; i < first + cnt; i++) {
+                    Tri tr = gloadTri(tris + i);
+                    if (STATS) st.tris++;
+                    TriHit th;
+                    if (!triWatertight(w, tr, dtmin, best.t, th)) continue;
+                    uint32_t prim = __float_as_uint(tr.v0.w);
+                    if (!found || hitBetter(th.t, curInst, prim, best)) {
+                        found = true;
+                        best.t = th.t;
+                        best.u = (float)(th.V / th.det);
+                        best.v = (float)(th.W / th.det);
+                        best.inst = curInst;
+                        best.prim = prim;
+                        if (ANY) return true;
+                        tlim = (float)th.t * kTRoundUp;
+                    }
                 }
             }
+            leaf = 0;
         }
-        if (stk.sp == 0) break;
-        cur = stk.pop();
         if (cur == kExitMarker) {
             curInst = 0xFFFFFFFFu;
             nodes = sc.tlas;
-            br = makeBoxRay(org, dir);
-            pad = wpad;
-            if (stk.sp == 0) break;
-            cur = stk.pop();
+            br = makeBoxRayPadded(org, dir, wpad);
+            cur = pop();
         }
+        if (cur == kDone) break;
+        if (++steps > kMaxSteps) { st.overflow = 1; return found; }
     }
     return found;
 }

@@ -26,6 +26,7 @@ namespace aph {
 namespace {
 
 constexpr int kWB = 128;
+static_assert(kWB == kStackLanes, "the traversal stack stride is the block size");
 constexpr int kSlots = 6;           // shadow slots per path: 0..3 distant lights, 4 MIS material, 5 MIS light
 constexpr int kSlotMisMat = 4, kSlotMisLight = 5;
 
@@ -203,8 +204,6 @@ __global__ __launch_bounds__(kWB) void kExtend(const DevReq* reqs, const SegInfo
     __shared__ uint32_t sStack[kLdsStack * kWB];
     ShortStack stk;
     stk.lds = sStack + threadIdx.x;
-    stk.stride = kWB;
-    stk.sp = 0;
     TraceStats ts = {};
     uint32_t n = Q.counts[cur], done = 0;
     for (uint32_t i = blockIdx.x * kWB + threadIdx.x; i < n; i += gridDim.x * kWB) {
@@ -429,8 +428,6 @@ __global__ __launch_bounds__(kWB) void kShadow(const DevReq* reqs, const SegInfo
     __shared__ uint32_t sStack[kLdsStack * kWB];
     ShortStack stk;
     stk.lds = sStack + threadIdx.x;
-    stk.stride = kWB;
-    stk.sp = 0;
     TraceStats ts = {};
     uint32_t n = Q.counts[2], done = 0;
     for (uint32_t i = blockIdx.x * kWB + threadIdx.x; i < n; i += gridDim.x * kWB) {
