@@ -54,8 +54,8 @@ __global__ __launch_bounds__(kBlock) void kRenderMega(RenderScene rs, aph_frame 
                                                      float4* accum, DevCounters* ctr)
 {
     __shared__ uint32_t sStack[kLdsStack * kBlock];
-    ShortStack stk;
-    stk.lds = sStack + threadIdx.x;
+    uint32_t spill[kSpillStack];
+    const StackMem stk{sStack + threadIdx.x, spill};
     PathCounters pc = {};
     // 16x8 tile per block, row-major tiles
     uint32_t tilesX = (W + 15) / 16;
@@ -86,8 +86,8 @@ __global__ __launch_bounds__(kBlock) void kTrace(TraceScene sc, const aph_ray* r
                                                 uint8_t* occ, DevCounters* ctr)
 {
     __shared__ uint32_t sStack[kLdsStack * kBlock];
-    ShortStack stk;
-    stk.lds = sStack + threadIdx.x;
+    uint32_t spill[kSpillStack];
+    const StackMem stk{sStack + threadIdx.x, spill};
     PathCounters pc = {};
     uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i < n) {

@@ -27,7 +27,7 @@ struct PathCounters {
 };
 
 template <bool STATS>
-__device__ __forceinline__ float traceShadow(const RenderScene& rs, f3 org, f3 dir, ShortStack stk, PathCounters& pc)
+__device__ __forceinline__ float traceShadow(const RenderScene& rs, f3 org, f3 dir, const StackMem& stk, PathCounters& pc)
 {
     HitRec h;
     pc.shadow++;
@@ -89,7 +89,7 @@ __device__ __forceinline__ f3 shadeDirectional(const aph_distant_light& l, const
 // The whole path of one pixel-sample; returns RGBA before accumulation.
 template <bool STATS>
 __device__ float4 pathSample(const RenderScene& rs, const aph_frame& fp, uint32_t W, uint32_t H, uint32_t px,
-                             uint32_t py, uint32_t sampleIndex, ShortStack stk, PathCounters& pc)
+                             uint32_t py, uint32_t sampleIndex, const StackMem& stk, PathCounters& pc)
 {
     const DevEnv& env = rs.env;
     const int maxDepth = fp.trace_depth;

@@ -47,12 +47,13 @@ struct TraceStats {
 // Per-lane traversal stack: a ring of the newest kLdsStack entries in LDS,
 // slot-major with a fixed stride of kStackLanes (the block size of every kernel
 // that traces), so the 64 lanes of a wave touch 64 consecutive dwords; older
-// entries move to a private spill array.  The caller hands in this lane's LDS
-// column (its __shared__ array + threadIdx.x); the stack pointer and the spill
-// array are locals of the traversal, so that only spills touch scratch memory.
+// entries move to a private spill array of kSpillStack entries.  The kernel
+// owns both (its __shared__ array + threadIdx.x, and a local array); the stack
+// pointer lives in the traversal state, so only spills touch scratch memory.
 constexpr int kStackLanes = 128;
-struct ShortStack {
+struct StackMem {
     uint32_t* lds;
+    uint32_t* spill;
 };
 
 // Loads through the global address space: the BVH pointers come out of device
@@ -246,141 +247,178 @@ APH_HD bool hitBetter(double t, uint32_t inst, uint32_t prim, const HitRec& b)
 constexpr int32_t kDone = 0x7FFFFFFE;  // stack exhausted: the ray is finished
 APH_HD bool isInner(int32_t r) { return (uint32_t)r < (uint32_t)kDone; }
 
-// Traces one ray.  ANY: stop at the first accepted hit (shadow rays).
-//
-// While-while traversal (Aila & Laine, HPG 2009) for 64-wide waves: the inner
-// loop steps through internal nodes; the first leaf a lane meets is parked and
-// the lane continues with its next stack entry, and the loop ends once every
-// lane still in it holds a parked leaf.  Leaves - instance entries in the TLAS,
-// triangle ranges in a BLAS - are then processed with the wave together.
-// Entering an instance pushes the pending TLAS entry and an exit marker; popping
-// the marker restores the world-space ray.  Traversal order never changes the
-// result: boxes only cull conservatively and the closest hit is a lexicographic
-// minimum.
-template <bool ANY, bool STATS>
-__device__ bool traceRay(const TraceScene& sc, f3 org, f3 dir, float tmin, float tmax, ShortStack stk,
-                         HitRec& best, TraceStats& st)
+// Everything one lane needs to resume a ray's traversal, held in registers.
+struct TraceState {
+    f3 org, dir;          // world ray
+    float tmin, wpad;     // ray start; pad of instance boxes for this origin
+    float tlim;           // box cull distance: the closest hit's t rounded up to float
+    BoxRay br;            // box-test ray of the current level (world or object space)
+    WRay w;               // watertight ray of the current instance
+    const BvhNode* nodes; // node array of the current level
+    const Tri* tris;      // triangles of the current instance
+    uint32_t curInst;     // 0xFFFFFFFF in the TLAS
+    int32_t cur, leaf;    // next entry; parked leaf (0 = none)
+    int sp;               // stack depth
+    uint32_t steps;
+    HitRec best;
+    bool found;
+};
+
+__device__ __forceinline__ bool stackPush(TraceState& s, const StackMem& m, int32_t v)
 {
-    best.t = (double)tmax;
-    best.u = best.v = 0.0f;
-    best.inst = best.prim = 0xFFFFFFFFu;
-    bool found = false;
+    if (s.sp >= kStackCap) return false;
+    uint32_t* slot = m.lds + (s.sp & (kLdsStack - 1)) * kStackLanes;
+    if (s.sp >= kLdsStack) m.spill[s.sp - kLdsStack] = *slot;
+    *slot = (uint32_t)v;
+    s.sp++;
+    return true;
+}
+// next stack entry, or kDone when the stack is empty
+__device__ __forceinline__ int32_t stackPop(TraceState& s, const StackMem& m)
+{
+    if (s.sp == 0) return kDone;
+    s.sp--;
+    uint32_t* slot = m.lds + (s.sp & (kLdsStack - 1)) * kStackLanes;
+    int32_t v = (int32_t)*slot;
+    if (s.sp >= kLdsStack) *slot = m.spill[s.sp - kLdsStack];
+    return v;
+}
+
+// Starts a ray.  Returns false when it is finished at once: an empty scene, or
+// a ray with a non-finite component, which misses (every triangle test rejects
+// it, but the NaN-ignoring slab test would accept every box).
+__device__ __forceinline__ bool traceBegin(TraceState& s, const TraceScene& sc, f3 org, f3 dir, float tmin,
+                                           float tmax)
+{
+    s.best.t = (double)tmax;
+    s.best.u = s.best.v = 0.0f;
+    s.best.inst = s.best.prim = 0xFFFFFFFFu;
+    s.found = false;
     if (sc.ninst == 0) return false;
-    // A ray with a non-finite component misses: every triangle test rejects it, but the
-    // NaN-ignoring slab test would accept every box.
     if (!(isfinite(org.x) && isfinite(org.y) && isfinite(org.z) && isfinite(dir.x) && isfinite(dir.y) &&
           isfinite(dir.z)))
         return false;
-    const float wpad = sc.padScale * (fabsf(org.x) + fabsf(org.y) + fabsf(org.z) + sc.extent);
-    BoxRay br = makeBoxRayPadded(org, dir, wpad);
-    const BvhNode* nodes = sc.tlas;
-    const Tri* tris = nullptr;
-    uint32_t curInst = 0xFFFFFFFFu;  // != 0xFFFFFFFF while inside a BLAS
-    WRay w;
-    w.ox = w.oy = w.oz = 0.0f;
-    w.Sx = w.Sy = w.Sz = 0.0;
-    w.perm = 0;
-    const double dtmin = (double)tmin;
-    float tlim = tmax;  // box cull distance: the closest hit's t rounded up to float
-    uint32_t* const lds = stk.lds;
-    int sp = 0;
-    uint32_t spill[kSpillStack];
-    auto push = [&](int32_t v) -> bool {
-        if (sp >= kStackCap) return false;
-        uint32_t* slot = lds + (sp & (kLdsStack - 1)) * kStackLanes;
-        if (sp >= kLdsStack) spill[sp - kLdsStack] = *slot;
-        *slot = (uint32_t)v;
-        sp++;
-        return true;
-    };
-    // next stack entry, or kDone when the stack is empty
-    auto pop = [&]() -> int32_t {
-        if (sp == 0) return kDone;
-        sp--;
-        uint32_t* slot = lds + (sp & (kLdsStack - 1)) * kStackLanes;
-        int32_t v = (int32_t)*slot;
-        if (sp >= kLdsStack) *slot = spill[sp - kLdsStack];
-        return v;
-    };
-    int32_t cur = 0, leaf = 0;
-    uint32_t steps = 0;
-    while (true) {
-        while (isInner(cur)) {
-            if (++steps > kMaxSteps) { st.overflow = 1; return found; }
-            const float4* np = reinterpret_cast<const float4*>(nodes + cur);
-            float4 a = gload4(np), b = gload4(np + 1), c = gload4(np + 2), d = gload4(np + 3);
-            if (STATS) st.nodes++;
-            float t0, t1;
-            bool h0 = slab(br, a, b, tmin, tlim, t0);
-            bool h1 = slab(br, c, d, tmin, tlim, t1);
-            int32_t c0 = __float_as_int(a.w), c1 = __float_as_int(b.w);
-            if (h0 && h1) {
-                bool swap = t1 < t0;
-                if (!push(swap ? c0 : c1)) { st.overflow = 1; return found; }
-                cur = swap ? c1 : c0;
-            } else if (h0 || h1) {
-                cur = h0 ? c0 : c1;
-            } else {
-                cur = pop();
-            }
-            if (cur < 0 && leaf == 0) {
-                leaf = cur;
-                cur = pop();
-            }
-            if (!__any(leaf == 0)) break;
+    s.org = org;
+    s.dir = dir;
+    s.tmin = tmin;
+    s.tlim = tmax;
+    s.wpad = sc.padScale * (fabsf(org.x) + fabsf(org.y) + fabsf(org.z) + sc.extent);
+    s.br = makeBoxRayPadded(org, dir, s.wpad);
+    s.w.ox = s.w.oy = s.w.oz = 0.0f;
+    s.w.Sx = s.w.Sy = s.w.Sz = 0.0;
+    s.w.perm = 0;
+    s.nodes = sc.tlas;
+    s.tris = nullptr;
+    s.curInst = 0xFFFFFFFFu;
+    s.cur = 0;
+    s.leaf = 0;
+    s.sp = 0;
+    s.steps = 0;
+    return true;
+}
+
+// One round of while-while traversal (Aila & Laine, HPG 2009) for 64-wide waves.
+// The inner loop steps through internal nodes; the first leaf a lane meets is
+// parked and the lane continues with its next stack entry, and the loop ends
+// once every lane still in it holds a parked leaf.  Then each lane processes one
+// leaf - an instance entry in the TLAS, a triangle range in a BLAS - so the wave
+// tests leaves together.  Entering an instance pushes the pending TLAS entry and
+// an exit marker; popping the marker restores the world-space ray.  Traversal
+// order never changes the result: boxes only cull conservatively and the
+// closest hit is a lexicographic minimum.  ANY: stop at the first accepted hit.
+// Returns true when the ray is finished (s.found / s.best hold the result).
+template <bool ANY, bool STATS>
+__device__ __forceinline__ bool traceRound(TraceState& s, const TraceScene& sc, const StackMem& m, TraceStats& st)
+{
+    while (isInner(s.cur)) {
+        if (++s.steps > kMaxSteps) { st.overflow = 1; return true; }
+        const float4* np = reinterpret_cast<const float4*>(s.nodes + s.cur);
+        float4 a = gload4(np), b = gload4(np + 1), c = gload4(np + 2), d = gload4(np + 3);
+        if (STATS) st.nodes++;
+        float t0, t1;
+        bool h0 = slab(s.br, a, b, s.tmin, s.tlim, t0);
+        bool h1 = slab(s.br, c, d, s.tmin, s.tlim, t1);
+        int32_t c0 = __float_as_int(a.w), c1 = __float_as_int(b.w);
+        if (h0 && h1) {
+            bool swap = t1 < t0;
+            if (!stackPush(s, m, swap ? c0 : c1)) { st.overflow = 1; return true; }
+            s.cur = swap ? c1 : c0;
+        } else if (h0 || h1) {
+            s.cur = h0 ? c0 : c1;
+        } else {
+            s.cur = stackPop(s, m);
         }
-        if (leaf == 0 && cur < 0) {
-            leaf = cur;
-            cur = pop();
+        if (s.cur < 0 && s.leaf == 0) {
+            s.leaf = s.cur;
+            s.cur = stackPop(s, m);
         }
-        if (leaf != 0) {
-            if (curInst == 0xFFFFFFFFu) {
-                // TLAS leaf: one instance.  Enter its BLAS in object space.
-                uint32_t ii = leafFirst(leaf);
-                const InstTrav it = gloadInst(sc.inst + ii);
-                f3 oo, od;
-                objectRay(it.w2o, org, dir, oo, od);
-                w = setupWRay(oo, od);
-                br = makeBoxRay(oo, od);
-                nodes = it.nodes;
-                tris = it.tris;
-                curInst = ii;
-                if (STATS) st.insts++;
-                if (cur != kDone && !push(cur)) { st.overflow = 1; return found; }
-                if (!push(kExitMarker)) { st.overflow = 1; return found; }
-                cur = 0;
-            } else {
-                uint32_t first = leafFirst(leaf), cnt = leafCount(leaf);
-                for (uint32_t i = first; i < first + cnt; i++) {
-                    Tri tr = gloadTri(tris + i);
-                    if (STATS) st.tris++;
-                    TriHit th;
-                    if (!triWatertight(w, tr, dtmin, best.t, th)) continue;
-                    uint32_t prim = __float_as_uint(tr.v0.w);
-                    if (!found || hitBetter(th.t, curInst, prim, best)) {
-                        found = true;
-                        best.t = th.t;
-                        best.u = (float)(th.V / th.det);
-                        best.v = (float)(th.W / th.det);
-                        best.inst = curInst;
-                        best.prim = prim;
-                        if (ANY) return true;
-                        tlim = (float)th.t * kTRoundUp;
-                    }
+        if (!__any(s.leaf == 0)) break;
+    }
+    if (s.leaf == 0 && s.cur < 0) {
+        s.leaf = s.cur;
+        s.cur = stackPop(s, m);
+    }
+    if (s.leaf != 0) {
+        if (s.curInst == 0xFFFFFFFFu) {
+            // TLAS leaf: one instance.  Enter its BLAS in object space.
+            uint32_t ii = leafFirst(s.leaf);
+            const InstTrav it = gloadInst(sc.inst + ii);
+            f3 oo, od;
+            objectRay(it.w2o, s.org, s.dir, oo, od);
+            s.w = setupWRay(oo, od);
+            s.br = makeBoxRay(oo, od);
+            s.nodes = it.nodes;
+            s.tris = it.tris;
+            s.curInst = ii;
+            if (STATS) st.insts++;
+            if (s.cur != kDone && !stackPush(s, m, s.cur)) { st.overflow = 1; return true; }
+            if (!stackPush(s, m, kExitMarker)) { st.overflow = 1; return true; }
+            s.cur = 0;
+        } else {
+            uint32_t first = leafFirst(s.leaf), cnt = leafCount(s.leaf);
+            const double dtmin = (double)s.tmin;
+            for (uint32_t i = first; i < first + cnt; i++) {
+                Tri tr = gloadTri(s.tris + i);
+                if (STATS) st.tris++;
+                TriHit th;
+                if (!triWatertight(s.w, tr, dtmin, s.best.t, th)) continue;
+                uint32_t prim = __float_as_uint(tr.v0.w);
+                if (!s.found || hitBetter(th.t, s.curInst, prim, s.best)) {
+                    s.found = true;
+                    s.best.t = th.t;
+                    s.best.u = (float)(th.V / th.det);
+                    s.best.v = (float)(th.W / th.det);
+                    s.best.inst = s.curInst;
+                    s.best.prim = prim;
+                    if (ANY) return true;
+                    s.tlim = (float)th.t * kTRoundUp;
                 }
             }
-            leaf = 0;
         }
-        if (cur == kExitMarker) {
-            curInst = 0xFFFFFFFFu;
-            nodes = sc.tlas;
-            br = makeBoxRayPadded(org, dir, wpad);
-            cur = pop();
-        }
-        if (cur == kDone) break;
-        if (++steps > kMaxSteps) { st.overflow = 1; return found; }
+        s.leaf = 0;
     }
-    return found;
+    if (s.cur == kExitMarker) {
+        s.curInst = 0xFFFFFFFFu;
+        s.nodes = sc.tlas;
+        s.br = makeBoxRayPadded(s.org, s.dir, s.wpad);
+        s.cur = stackPop(s, m);
+    }
+    if (s.cur == kDone) return true;
+    if (++s.steps > kMaxSteps) { st.overflow = 1; return true; }
+    return false;
+}
+
+// Traces one ray to completion.
+template <bool ANY, bool STATS>
+__device__ __forceinline__ bool traceRay(const TraceScene& sc, f3 org, f3 dir, float tmin, float tmax,
+                                         const StackMem& m, HitRec& best, TraceStats& st)
+{
+    TraceState s;
+    bool live = traceBegin(s, sc, org, dir, tmin, tmax);
+    while (live && !traceRound<ANY, STATS>(s, sc, m, st)) {
+    }
+    best = s.best;
+    return s.found;
 }
 
 }  // namespace aph

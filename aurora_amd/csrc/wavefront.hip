@@ -86,7 +86,7 @@ struct Queues {
     float* hitV;
     float4* shA;           // org.xyz, tmin
     float4* shB;           // dir.xyz, (path * kSlots + slot) bits
-    uint32_t* counts;      // [0],[1] ray queues, [2] shadow queue
+    uint32_t* counts;      // [0],[1] ray queues, [2] shadow queue, [3]/[4] extend/shadow fetch cursors
 };
 
 struct SegInfo {
@@ -149,6 +149,8 @@ __global__ void kReset(uint32_t* counts, uint32_t q0)
         counts[0] = q0;
         counts[1] = 0;
         counts[2] = 0;
+        counts[3] = 0;
+        counts[4] = 0;
     }
 }
 
@@ -157,6 +159,8 @@ __global__ void kResetNext(uint32_t* counts, int next)
     if (threadIdx.x == 0) {
         counts[next] = 0;
         counts[2] = 0;
+        counts[3] = 0;
+        counts[4] = 0;
     }
 }
 
@@ -196,31 +200,91 @@ __global__ __launch_bounds__(kWB) void kGen(const DevReq* reqs, const SegInfo* s
     addCounter(&ctr->paths, started);
 }
 
+// ---- persistent traversal of a ray queue ----
+constexpr uint32_t kNoRay = 0xFFFFFFFFu;
+constexpr uint32_t kRefillMin = 16;  // idle lanes of a wave that trigger a refill
+
+// Each lane keeps one ray in flight and advances it one traversal round per
+// iteration.  When at least kRefillMin lanes of the wave are idle (or all are),
+// the wave claims that many queue entries with one atomic on the fetch cursor,
+// so lanes do not wait for the slowest ray of their wave.  The loop is
+// wave-uniform: it ends once the cursor has passed n and no lane has a ray.
+// load(i, org, dir, tmin) returns entry i's scene; store(i, state) writes its result.
+template <bool ANY, bool STATS, class Load, class Store>
+__device__ __forceinline__ uint32_t traceQueue(uint32_t n, uint32_t* cursor, const StackMem& mem, TraceStats& ts,
+                                               Load load, Store store)
+{
+    const int lane = __lane_id();
+    const uint64_t below = (1ull << lane) - 1ull;
+    const uint32_t width = (uint32_t)__popcll(__ballot(1));
+    uint32_t ray = kNoRay, done = 0;
+    const TraceScene* sc = nullptr;
+    TraceState s;
+    bool drained = false;
+    while (true) {
+        const uint64_t idle = __ballot(ray == kNoRay);
+        const uint32_t nIdle = (uint32_t)__popcll(idle);
+        if (!drained && (nIdle >= kRefillMin || nIdle == width)) {
+            const int leader = __ffsll((unsigned long long)idle) - 1;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(cursor, nIdle);
+            base = __shfl(base, leader, 64);
+            if (base + nIdle >= n) drained = true;
+            if (ray == kNoRay) {
+                const uint32_t i = base + (uint32_t)__popcll(idle & below);
+                if (i < n) {
+                    f3 org, dir;
+                    float tmin;
+                    sc = load(i, org, dir, tmin);
+                    ray = i;
+                    if (!traceBegin(s, *sc, org, dir, tmin, INFINITY)) {
+                        store(i, s);
+                        ray = kNoRay;
+                        done++;
+                    }
+                }
+            }
+        }
+        if (ray != kNoRay && traceRound<ANY, STATS>(s, *sc, mem, ts)) {
+            store(ray, s);
+            ray = kNoRay;
+            done++;
+        }
+        if (drained && !__any(ray != kNoRay)) break;
+    }
+    return done;
+}
+
 // ---- extend: closest hit of the live queue ----
 template <bool STATS>
 __global__ __launch_bounds__(kWB) void kExtend(const DevReq* reqs, const SegInfo* segs, Paths P, Queues Q, int cur,
                                               DevCounters* ctr)
 {
     __shared__ uint32_t sStack[kLdsStack * kWB];
-    ShortStack stk;
-    stk.lds = sStack + threadIdx.x;
+    uint32_t spill[kSpillStack];
+    const StackMem mem{sStack + threadIdx.x, spill};
     TraceStats ts = {};
-    uint32_t n = Q.counts[cur], done = 0;
-    for (uint32_t i = blockIdx.x * kWB + threadIdx.x; i < n; i += gridDim.x * kWB) {
-        float4 a = Q.rayA[cur][i], b = Q.rayB[cur][i];
-        uint32_t p = __float_as_uint(b.w);
-        const TraceScene& sc = reqs[segs[P.seg[p]].req].rs.sc;
-        HitRec h;
-        bool found = traceRay<false, STATS>(sc, xyz(a), xyz(b), a.w, INFINITY, stk, h, ts);
-        if (found) {
-            Q.hitA[i] = make_float4((float)h.t, __uint_as_float(h.inst), __uint_as_float(h.prim), (float)h.u);
-            Q.hitV[i] = (float)h.v;
+    const uint32_t n = Q.counts[cur];
+    const float4* rayA = Q.rayA[cur];
+    const float4* rayB = Q.rayB[cur];
+    auto load = [&](uint32_t i, f3& org, f3& dir, float& tmin) -> const TraceScene* {
+        float4 a = rayA[i], b = rayB[i];
+        org = xyz(a);
+        dir = xyz(b);
+        tmin = a.w;
+        return &reqs[segs[P.seg[__float_as_uint(b.w)]].req].rs.sc;
+    };
+    auto store = [&](uint32_t i, const TraceState& s) {
+        if (s.found) {
+            Q.hitA[i] = make_float4((float)s.best.t, __uint_as_float(s.best.inst), __uint_as_float(s.best.prim),
+                                    s.best.u);
+            Q.hitV[i] = s.best.v;
         } else {
             Q.hitA[i] = make_float4(INFINITY, __uint_as_float(0xFFFFFFFFu), __uint_as_float(0xFFFFFFFFu), 0.0f);
             Q.hitV[i] = 0.0f;
         }
-        done++;
-    }
+    };
+    uint32_t done = traceQueue<false, STATS>(n, &Q.counts[3], mem, ts, load, store);
     addCounter(&ctr->ext, done);
     if (STATS) {
         addCounter(&ctr->nodes, ts.nodes);
@@ -426,20 +490,19 @@ __global__ __launch_bounds__(kWB) void kShadow(const DevReq* reqs, const SegInfo
                                               DevCounters* ctr)
 {
     __shared__ uint32_t sStack[kLdsStack * kWB];
-    ShortStack stk;
-    stk.lds = sStack + threadIdx.x;
+    uint32_t spill[kSpillStack];
+    const StackMem mem{sStack + threadIdx.x, spill};
     TraceStats ts = {};
-    uint32_t n = Q.counts[2], done = 0;
-    for (uint32_t i = blockIdx.x * kWB + threadIdx.x; i < n; i += gridDim.x * kWB) {
+    const uint32_t n = Q.counts[2];
+    auto load = [&](uint32_t i, f3& org, f3& dir, float& tmin) -> const TraceScene* {
         float4 a = Q.shA[i], b = Q.shB[i];
-        uint32_t slot = __float_as_uint(b.w);
-        uint32_t p = slot / kSlots;
-        const TraceScene& sc = reqs[segs[P.seg[p]].req].rs.sc;
-        HitRec h;
-        bool occ = traceRay<true, STATS>(sc, xyz(a), xyz(b), a.w, INFINITY, stk, h, ts);
-        P.vis[slot] = occ ? 0 : 1;
-        done++;
-    }
+        org = xyz(a);
+        dir = xyz(b);
+        tmin = a.w;
+        return &reqs[segs[P.seg[__float_as_uint(b.w) / kSlots]].req].rs.sc;
+    };
+    auto store = [&](uint32_t i, const TraceState& s) { P.vis[__float_as_uint(Q.shB[i].w)] = s.found ? 0 : 1; };
+    uint32_t done = traceQueue<true, STATS>(n, &Q.counts[4], mem, ts, load, store);
     addCounter(&ctr->shadow, done);
     if (STATS) {
         addCounter(&ctr->nodes, ts.nodes);
