@@ -107,6 +107,7 @@ SIGNATURES = {
     "aph_get_stream": (_i, [_P, C.POINTER(_P)]),
     "aph_mesh_upload": (_i, [_P, _P, _u32, _P, _u32, _P, _P, C.POINTER(_u64)]),
     "aph_blas_build": (_i, [_P, _u64, _i]),
+    "aph_blas_export": (_i, [_P, _u64, _P, _u64, _P, _u64, C.POINTER(_u32), C.POINTER(_u32)]),
     "aph_tlas_build": (_i, [_P, _P, _u32, C.POINTER(_u64)]),
     "aph_inverse3x4": (_i, [_P, _P]),
     "aph_materials_upload": (_i, [_P, _P, _u32]),

@@ -21,6 +21,8 @@ using namespace aph;
 
 namespace {
 
+constexpr uint32_t kSahMaxLeaf = 8;  // triangles per leaf of an SAH BLAS
+
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
@@ -396,15 +398,44 @@ int aph_blas_build(aph_ctx_t c, uint64_t mesh, int builder)
     HIPCALL(c, triBoxes(m->pos.as<float>(), m->idx.as<uint32_t>(), n, c->stream, lo.as<float4>(), hi.as<float4>()),
             "triangle boxes");
     BuildInfo info{};
-    HIPCALL(c, lbvhBuild(lo.as<float4>(), hi.as<float4>(), n, false, nullptr, c->stream, m->nodes.as<BvhNode>(),
-                         sorted.as<uint32_t>(), &info),
-            "BLAS build");
+    if (builder == APH_BUILDER_SAH)
+        HIPCALL(c, sahBuild(lo.as<float4>(), hi.as<float4>(), n, kSahMaxLeaf, false, nullptr, c->stream,
+                            m->nodes.as<BvhNode>(), sorted.as<uint32_t>(), &info),
+                "BLAS build (SAH)");
+    else
+        HIPCALL(c, lbvhBuild(lo.as<float4>(), hi.as<float4>(), n, false, nullptr, c->stream, m->nodes.as<BvhNode>(),
+                             sorted.as<uint32_t>(), &info),
+                "BLAS build (LBVH)");
     HIPCALL(c, reorderTris(m->pos.as<float>(), m->idx.as<uint32_t>(), sorted.as<uint32_t>(), n, c->stream,
                            m->tris.as<Tri>()),
             "reorder triangles");
     HIPCALL(c, hipStreamSynchronize(c->stream), "BLAS build");
     m->info = info;
     m->built = true;
+    return APH_OK;
+}
+
+int aph_blas_export(aph_ctx_t c, uint64_t mesh, void* nodes, uint64_t nodesBytes, void* tris, uint64_t trisBytes,
+                    uint32_t* nodeCount, uint32_t* depth)
+{
+    int rc = enter(c);
+    if (rc) return rc;
+    Mesh* m = lookup(c->meshes, mesh);
+    if (!m) return fail(c, APH_E_INVALID, "unknown mesh handle %llu", (unsigned long long)mesh);
+    if (!m->built) return fail(c, APH_E_STATE, "BLAS of mesh %llu not built", (unsigned long long)mesh);
+    if (nodeCount) *nodeCount = m->info.nodes;
+    if (depth) *depth = m->info.depth;
+    size_t nb = (size_t)m->info.nodes * sizeof(BvhNode), tb = (size_t)m->nt * sizeof(Tri);
+    if (nodes) {
+        if (nodesBytes < nb) return fail(c, APH_E_INVALID, "node buffer holds %llu of %zu bytes",
+                                         (unsigned long long)nodesBytes, nb);
+        HIPCALL(c, hipMemcpy(nodes, m->nodes.p, nb, hipMemcpyDeviceToHost), "export nodes");
+    }
+    if (tris) {
+        if (trisBytes < tb) return fail(c, APH_E_INVALID, "triangle buffer holds %llu of %zu bytes",
+                                        (unsigned long long)trisBytes, tb);
+        HIPCALL(c, hipMemcpy(tris, m->tris.p, tb, hipMemcpyDeviceToHost), "export triangles");
+    }
     return APH_OK;
 }
 

@@ -260,6 +260,7 @@ hipError_t lbvhBuild(const float4* primLo, const float4* primHi, uint32_t n, boo
                      const uint32_t* leafRemap, hipStream_t st, BvhNode* nodes, uint32_t* sortedOut, BuildInfo* info)
 {
     info->depth = 0;
+    info->nodes = 0;
     if (n == 0) return hipSuccess;
     if (n == 1) {
         uint32_t prim = 0;
@@ -279,6 +280,7 @@ hipError_t lbvhBuild(const float4* primLo, const float4* primHi, uint32_t n, boo
         info->rootLo[0] = lo.x; info->rootLo[1] = lo.y; info->rootLo[2] = lo.z;
         info->rootHi[0] = hi.x; info->rootHi[1] = hi.y; info->rootHi[2] = hi.z;
         info->depth = 1;
+        info->nodes = 1;
         return hipSuccess;
     }
     // scratch
@@ -331,6 +333,7 @@ hipError_t lbvhBuild(const float4* primLo, const float4* primHi, uint32_t n, boo
         info->rootLo[0] = rlo.x; info->rootLo[1] = rlo.y; info->rootLo[2] = rlo.z;
         info->rootHi[0] = rhi.x; info->rootHi[1] = rhi.y; info->rootHi[2] = rhi.z;
         info->depth = dep;
+        info->nodes = n - 1;
     } while (0);
     hipFree(bounds6); hipFree(keys); hipFree(keys2); hipFree(vals); hipFree(parent); hipFree(visits);
     hipFree(depth); hipFree(child); hipFree(boxLo); hipFree(boxHi); hipFree(sortTmp);

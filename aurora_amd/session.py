@@ -28,6 +28,7 @@ class Renderer:
         if rc != capi.APH_OK:
             raise AphError(rc, "aph_create failed (no usable HIP device?)")
         self._keep = []
+        self._mesh_tris = {}
 
     def close(self):
         if self.ctx:
@@ -54,10 +55,22 @@ class Renderer:
         nt = (len(idx.reshape(-1)) // 3) if idx is not None else len(pos.reshape(-1, 3)) // 3
         self._check(self.lib.aph_mesh_upload(self.ctx, capi.ptr(pos), len(pos.reshape(-1, 3)), capi.ptr(idx), nt,
                                              capi.ptr(nrm), capi.ptr(uv), C.byref(h)))
+        self._mesh_tris[h.value] = nt
         return h.value
 
     def build_blas(self, mesh, builder=capi.APH_BUILDER_LBVH):
         self._check(self.lib.aph_blas_build(self.ctx, mesh, builder))
+
+    def blas_export(self, mesh):
+        """(nodes float32[N, 16], tris float32[T, 12], depth) of a built BLAS."""
+        cnt, depth = C.c_uint32(), C.c_uint32()
+        self._check(self.lib.aph_blas_export(self.ctx, mesh, None, 0, None, 0, C.byref(cnt), C.byref(depth)))
+        nodes = np.zeros((max(cnt.value, 1), 16), dtype=np.float32)
+        nt = self._mesh_tris[mesh]
+        tris = np.zeros((max(nt, 1), 12), dtype=np.float32)
+        self._check(self.lib.aph_blas_export(self.ctx, mesh, nodes.ctypes.data, nodes.nbytes, tris.ctypes.data,
+                                             tris.nbytes, C.byref(cnt), C.byref(depth)))
+        return nodes[:cnt.value], tris[:nt], depth.value
 
     def tlas(self, instances):
         inst = np.ascontiguousarray(instances, dtype=capi.INSTANCE_DTYPE)

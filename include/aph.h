@@ -192,8 +192,18 @@ int aph_get_stream(aph_ctx_t ctx, void** stream);
 int aph_mesh_upload(aph_ctx_t ctx, const float* pos, uint32_t nv, const uint32_t* idx, uint32_t nt,
                     const float* nrm, const float* uv, uint64_t* mesh);
 /* BLAS build, replaces the driver build of HGIGeometry.cpp:127-160 /
- * PTGeometry.cpp:155-223. */
+ * PTGeometry.cpp:155-223.  APH_BUILDER_LBVH: Karras LBVH over 63-bit Morton
+ * codes, one triangle per leaf (fastest build); APH_BUILDER_SAH: top-down
+ * binned SAH (16 bins x 3 axes), up to 8 triangles per leaf (fastest trace). */
 int aph_blas_build(aph_ctx_t ctx, uint64_t mesh, int builder);
+/* Copies a built BLAS to host memory: *node_count 64-byte nodes (two child
+ * boxes; child references in a.w / b.w, leaf = bit 31 | (count-1) << 27 |
+ * first) and the triangles in leaf order (48 bytes, original primitive index in
+ * v0.w).  Null buffers query node_count and depth only.  No counterpart in the
+ * reference, whose BVH lives inside the driver: BVH inspection for tests and
+ * tools. */
+int aph_blas_export(aph_ctx_t ctx, uint64_t mesh, void* nodes, uint64_t nodes_bytes, void* tris,
+                    uint64_t tris_bytes, uint32_t* node_count, uint32_t* depth);
 /* TLAS build over instances, replaces HGIScene::rebuildAccelerationStructure
  * (HGIScene.cpp:377-449) / PTScene::buildTLAS (PTScene.cpp:1460-1552).
  * Instance IDs are the indices in inst[] (active order, HGIScene.cpp:392). */

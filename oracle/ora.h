@@ -95,6 +95,9 @@ ora_scene* ora_scene_create(const ora_mesh* meshes, uint32_t nmesh,
 void ora_scene_destroy(ora_scene* s);
 void ora_trace_closest(const ora_scene* s, const ora_ray* rays, uint32_t n, ora_hit* hits, int nthreads);
 void ora_trace_closest_brute(const ora_scene* s, const ora_ray* rays, uint32_t n, ora_hit* hits);
+/* SAH cost of a full-sweep SAH tree over n boxes (lo/hi float[3n]); checker for the BVH builders. */
+double ora_sah_sweep_cost(const float* lo, const float* hi, uint32_t n, uint32_t maxLeaf, float costNode,
+                          float costTri);
 void ora_trace_any(const ora_scene* s, const ora_ray* rays, uint32_t n, uint8_t* occluded, int nthreads);
 void ora_trace_any_brute(const ora_scene* s, const ora_ray* rays, uint32_t n, uint8_t* occluded);
 /* single triangle, FP64 watertight test; returns 1 on hit */

@@ -475,3 +475,99 @@ void ora_trace_any_brute(const ora_scene* s, const ora_ray* rays, uint32_t n, ui
         }
     }
 }
+
+/* ---------------- SAH reference cost (checker for the GPU builders) ----------------
+ * No reference function: the reference's BVH is built inside the driver.  A
+ * top-down full-sweep SAH build (MacDonald & Booth 1990; Wald 2007 section 3:
+ * every centroid-sorted split position on all three axes) returning the tree's
+ * SAH cost  sum_internal A(n)/A(root) * costNode + sum_leaves A(l)/A(root) *
+ * costTri * count.  Leaf rule as in the GPU builder: a non-root node of at most
+ * maxLeaf primitives becomes a leaf when costTri * count <= costNode + costTri *
+ * (A_L N_L + A_R N_R) / A. */
+typedef struct { float lo[3], hi[3], c[3]; } sah_prim;
+static const sah_prim* g_sah_prims;
+static int g_sah_axis_sort;
+
+static int sah_cmp(const void* a, const void* b)
+{
+    uint32_t i = *(const uint32_t*)a, j = *(const uint32_t*)b;
+    float ci = g_sah_prims[i].c[g_sah_axis_sort], cj = g_sah_prims[j].c[g_sah_axis_sort];
+    if (ci < cj) return -1;
+    if (ci > cj) return 1;
+    return i < j ? -1 : (i > j ? 1 : 0);
+}
+
+static double sah_area(const float lo[3], const float hi[3])
+{
+    double dx = (double)hi[0] - lo[0], dy = (double)hi[1] - lo[1], dz = (double)hi[2] - lo[2];
+    if (!(dx >= 0 && dy >= 0 && dz >= 0)) return 0.0;
+    return 2.0 * (dx * dy + dy * dz + dz * dx);
+}
+
+static void sah_grow(float lo[3], float hi[3], const sah_prim* p)
+{
+    for (int k = 0; k < 3; k++) {
+        if (p->lo[k] < lo[k]) lo[k] = p->lo[k];
+        if (p->hi[k] > hi[k]) hi[k] = p->hi[k];
+    }
+}
+
+static double sah_rec(uint32_t* idx, uint32_t n, uint32_t maxLeaf, double cn, double ct, double rootA, int root,
+                      double* rightA)
+{
+    float lo[3] = { INFINITY, INFINITY, INFINITY }, hi[3] = { -INFINITY, -INFINITY, -INFINITY };
+    for (uint32_t i = 0; i < n; i++) sah_grow(lo, hi, &g_sah_prims[idx[i]]);
+    double A = sah_area(lo, hi);
+    double leafCost = ct * (double)n;
+    if (n == 1) return A / rootA * leafCost;
+    double best = INFINITY;
+    int bestAxis = 0;
+    uint32_t bestK = n / 2;
+    for (int a = 0; a < 3; a++) {
+        g_sah_axis_sort = a;
+        qsort(idx, n, sizeof(uint32_t), sah_cmp);
+        float rl[3] = { INFINITY, INFINITY, INFINITY }, rh[3] = { -INFINITY, -INFINITY, -INFINITY };
+        for (uint32_t i = n; i-- > 1;) {
+            sah_grow(rl, rh, &g_sah_prims[idx[i]]);
+            rightA[i] = sah_area(rl, rh);
+        }
+        float ll[3] = { INFINITY, INFINITY, INFINITY }, lh[3] = { -INFINITY, -INFINITY, -INFINITY };
+        for (uint32_t k = 1; k < n; k++) {
+            sah_grow(ll, lh, &g_sah_prims[idx[k - 1]]);
+            double c = sah_area(ll, lh) * (double)k + rightA[k] * (double)(n - k);
+            if (c < best) { best = c; bestAxis = a; bestK = k; }
+        }
+    }
+    double splitCost = A > 0.0 ? cn + ct * best / A : cn + leafCost;
+    if (!root && n <= maxLeaf && leafCost <= splitCost) return A / rootA * leafCost;
+    g_sah_axis_sort = bestAxis;
+    qsort(idx, n, sizeof(uint32_t), sah_cmp);
+    return A / rootA * cn + sah_rec(idx, bestK, maxLeaf, cn, ct, rootA, 0, rightA) +
+           sah_rec(idx + bestK, n - bestK, maxLeaf, cn, ct, rootA, 0, rightA);
+}
+
+double ora_sah_sweep_cost(const float* lo, const float* hi, uint32_t n, uint32_t maxLeaf, float costNode,
+                          float costTri)
+{
+    if (n == 0) return 0.0;
+    sah_prim* p = (sah_prim*)malloc(sizeof(sah_prim) * n);
+    uint32_t* idx = (uint32_t*)malloc(sizeof(uint32_t) * n);
+    double* rightA = (double*)malloc(sizeof(double) * (n + 1));
+    float rlo[3] = { INFINITY, INFINITY, INFINITY }, rhi[3] = { -INFINITY, -INFINITY, -INFINITY };
+    for (uint32_t i = 0; i < n; i++) {
+        for (int k = 0; k < 3; k++) {
+            p[i].lo[k] = lo[3 * i + k];
+            p[i].hi[k] = hi[3 * i + k];
+            p[i].c[k] = 0.5f * (p[i].lo[k] + p[i].hi[k]);
+        }
+        idx[i] = i;
+        sah_grow(rlo, rhi, &p[i]);
+    }
+    g_sah_prims = p;
+    double rootA = sah_area(rlo, rhi);
+    double cost = rootA > 0.0 ? sah_rec(idx, n, maxLeaf, costNode, costTri, rootA, 1, rightA) : 0.0;
+    free(p);
+    free(idx);
+    free(rightA);
+    return cost;
+}

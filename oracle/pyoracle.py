@@ -61,6 +61,7 @@ def load():
         "ora_trace_closest_brute": (None, [P, P, u32, P]),
         "ora_trace_any": (None, [P, P, u32, P, i]),
         "ora_trace_any_brute": (None, [P, P, u32, P]),
+        "ora_sah_sweep_cost": (C.c_double, [P, P, u32, u32, C.c_float, C.c_float]),
         "ora_intersect_triangle": (i, [P, P, C.c_float, C.c_float, P, P, P, P, P, P]),
         "ora_material_defaults": (None, [P]),
         "ora_alias_build": (None, [P, u32, u32, P, P]),

@@ -83,11 +83,15 @@ def _scenes(teapot_path):
     ]
 
 
+BUILDERS = [capi.APH_BUILDER_LBVH, capi.APH_BUILDER_SAH]
+
+
+@pytest.mark.parametrize("builder", BUILDERS)
 @pytest.mark.parametrize("which", [0, 1, 2])
-def test_closest_and_any_match_oracle(renderer, teapot_path, which):
+def test_closest_and_any_match_oracle(renderer, teapot_path, which, builder):
     name, sc = _scenes(teapot_path)[which]
     rng = np.random.default_rng(100 + which)
-    tlas = renderer.load_scene(sc)
+    tlas = renderer.load_scene(sc, builder)
     ora = pyoracle.OracleScene(sc)
     ext = 15.0 if name != "teapot" else 3.0
     sets = [random_rays(rng, 20000, -ext, ext), targeted_rays(rng, sc, 3000), targeted_rays(rng, sc, 2000, 0.01),
@@ -111,7 +115,8 @@ def test_oracle_bvh_equals_brute_force_on_gpu_rays(renderer, teapot_path):
     assert_hits_equal(renderer.trace_closest(tlas, rays), ora.trace_closest(rays, brute=True))
 
 
-def test_shared_edge_and_vertex_rays_are_watertight(renderer):
+@pytest.mark.parametrize("builder", BUILDERS)
+def test_shared_edge_and_vertex_rays_are_watertight(renderer, builder):
     """A fan of triangles around a vertex: rays through shared edges and the shared
     vertex must hit exactly one triangle each (never fall through)."""
     n = 16
@@ -123,7 +128,7 @@ def test_shared_edge_and_vertex_rays_are_watertight(renderer):
     sc.instances = scenes._instances([(0, 0, scenes.affine())])
     sc.materials = [capi.default_material()]
     sc.env = {}
-    tlas = renderer.load_scene(sc)
+    tlas = renderer.load_scene(sc, builder)
     ora = pyoracle.OracleScene(sc)
     pts = [np.zeros(3)] + [0.5 * np.array([np.cos(a), np.sin(a), 0]) for a in ang]
     rays = np.zeros(len(pts) * 8, dtype=capi.RAY_DTYPE)
@@ -142,7 +147,8 @@ def test_shared_edge_and_vertex_rays_are_watertight(renderer):
     assert_hits_equal(g, ora.trace_closest(rays))
 
 
-def test_degenerate_inputs(renderer):
+@pytest.mark.parametrize("builder", BUILDERS)
+def test_degenerate_inputs(renderer, builder):
     """Zero-direction components, rays in the plane of an axis-aligned quad, tmin/tmax limits,
     an empty mesh and an instance-less TLAS."""
     gp, gn, gi = scenes.ground_quad(1.0)
@@ -152,7 +158,7 @@ def test_degenerate_inputs(renderer):
     sc.instances = scenes._instances([(0, 0, scenes.affine()), (1, 0, scenes.affine())])
     sc.materials = [capi.default_material()]
     sc.env = {}
-    tlas = renderer.load_scene(sc)
+    tlas = renderer.load_scene(sc, builder)
     ora = pyoracle.OracleScene(sc)
     rays = np.zeros(8, dtype=capi.RAY_DTYPE)
     rays["tmax"] = np.inf
@@ -180,7 +186,8 @@ def test_degenerate_inputs(renderer):
     assert (renderer.trace_any(t0, rays) == 0).all()
 
 
-def test_transformed_instances_scaled_rotated(renderer):
+@pytest.mark.parametrize("builder", BUILDERS)
+def test_transformed_instances_scaled_rotated(renderer, builder):
     """Non-uniform scale + rotation + translation; instance IDs and t must match the oracle."""
     rng = np.random.default_rng(11)
     p, f = scenes.icosphere(2)
@@ -193,7 +200,7 @@ def test_transformed_instances_scaled_rotated(renderer):
     sc.instances = scenes._instances(inst)
     sc.materials = [capi.default_material()]
     sc.env = {}
-    tlas = renderer.load_scene(sc)
+    tlas = renderer.load_scene(sc, builder)
     ora = pyoracle.OracleScene(sc)
     for rays in (random_rays(rng, 20000, -12, 12), targeted_rays(rng, sc, 3000)):
         assert_hits_equal(renderer.trace_closest(tlas, rays), ora.trace_closest(rays))

@@ -9,6 +9,7 @@ primitive) tie rule, and BVH traversal == brute force.
 import ctypes as C
 
 import numpy as np
+import pytest
 
 import pyoracle
 from aurora_amd import capi, scenes
@@ -137,3 +138,22 @@ def test_bvh_traversal_equals_brute_force(teapot_path):
         assert a.tobytes() == b.tobytes()
         assert (ora.trace_any(rays) == ora.trace_any(rays, brute=True)).all()
         assert ((a["inst"] != 0xFFFFFFFF) == (ora.trace_any(rays) == 1)).all()
+
+
+def test_sah_sweep_cost_closed_forms():
+    """The SAH checker: two separated boxes split at the root, cost = Cn + Ct (A1 + A2) / A_root;
+    eight unit boxes in a row split into halves down to the leaf limit."""
+    lib = pyoracle.load()
+    lo = np.array([[0, 0, 0], [3, 0, 0]], dtype=np.float32)
+    hi = lo + 1
+    a_root = 2 * (4 * 1 + 1 * 1 + 1 * 4)
+    exp = 1.0 + 1.2 * (6 + 6) / a_root
+    assert lib.ora_sah_sweep_cost(pyoracle._p(lo), pyoracle._p(hi), 2, 8, 1.0, 1.2) == pytest.approx(exp, rel=1e-6)  # costs pass as float32
+    lo = np.array([[2 * i, 0, 0] for i in range(8)], dtype=np.float32)
+    hi = lo + 1
+    # maxLeaf 1: a balanced tree; internal node areas for spans of 8, 4, 4, 2 x 4 boxes
+    def a(span):
+        w = 2 * span - 1
+        return 2 * (w + 1 + w)
+    exp = (1.0 * (a(8) + 2 * a(4) + 4 * a(2)) + 1.2 * 8 * a(1)) / a(8)
+    assert lib.ora_sah_sweep_cost(pyoracle._p(lo), pyoracle._p(hi), 8, 1, 1.0, 1.2) == pytest.approx(exp, rel=1e-6)  # costs pass as float32
