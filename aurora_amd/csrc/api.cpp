@@ -21,7 +21,7 @@ using namespace aph;
 
 namespace {
 
-constexpr uint32_t kSahMaxLeaf = 8;  // triangles per leaf of an SAH BLAS
+constexpr uint32_t kSahMaxLeaf = 4;  // triangles per leaf of an SAH BLAS
 
 struct DevBuf {
     void* p = nullptr;
@@ -524,8 +524,9 @@ int aph_tlas_build(aph_ctx_t c, const aph_instance* inst, uint32_t n, uint64_t* 
         HIPCALL(c, hipMemcpyAsync(hi.p, bhi.data(), (size_t)nv * 16, hipMemcpyHostToDevice, c->stream), "copy");
         HIPCALL(c, hipMemcpyAsync(rm.p, remap.data(), (size_t)nv * 4, hipMemcpyHostToDevice, c->stream), "copy");
         BuildInfo info{};
-        HIPCALL(c, lbvhBuild(lo.as<float4>(), hi.as<float4>(), nv, true, rm.as<uint32_t>(), c->stream,
-                             t->nodes.as<BvhNode>(), sorted.as<uint32_t>(), &info),
+        // binned SAH over the instance world boxes, one instance per leaf
+        HIPCALL(c, sahBuild(lo.as<float4>(), hi.as<float4>(), nv, 1, true, rm.as<uint32_t>(), c->stream,
+                            t->nodes.as<BvhNode>(), sorted.as<uint32_t>(), &info),
                 "TLAS build");
         HIPCALL(c, hipStreamSynchronize(c->stream), "TLAS build");
         t->depth = info.depth;

@@ -130,7 +130,7 @@ __device__ float4 pathSample(const RenderScene& rs, const aph_frame& fp, uint32_
                 Shading sh;
                 computeShading(ins, hit.prim, (float)hit.u, (float)hit.v, sh);
                 uint32_t mi = ins.material < rs.nmat ? ins.material : 0u;
-                const DevMaterial m = rs.mats[mi];
+                const DevMaterial& m = rs.mats[mi];
                 f3 transparency = splat(1.0f) - m.opacity;
                 float P = luminance(transparency);
                 bool hitLayer = random2D(rng).x >= P;

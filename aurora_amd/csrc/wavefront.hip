@@ -304,7 +304,9 @@ __device__ __forceinline__ void emitShadow(Queues& Q, bool emit, f3 org, f3 dir,
 }
 
 // ---- shade: closest-hit / miss programs without the shadow traces ----
-__global__ __launch_bounds__(kWB) void kShade(const DevReq* reqs, const SegInfo* segs, Paths P, Queues Q, int cur)
+// Four waves per SIMD (128 VGPRs, a few spills) hide the gathers of shading
+// data, materials and environment texels better than three without spills.
+__global__ __launch_bounds__(kWB) __attribute__((amdgpu_waves_per_eu(4, 8))) void kShade(const DevReq* reqs, const SegInfo* segs, Paths P, Queues Q, int cur)
 {
     const int nxt = cur ^ 1;
     uint32_t n = Q.counts[cur];
@@ -319,8 +321,6 @@ __global__ __launch_bounds__(kWB) void kShade(const DevReq* reqs, const SegInfo*
         f3 nOrg = splat(0.0f), nDir = splat(0.0f);
         float nTmin = kRayTMin;
         uint32_t p = 0;
-        bool sh[kSlots] = {false, false, false, false, false, false};
-        f3 shO[kSlots], shD[kSlots];
         if (live) {
             float4 ra = Q.rayA[cur][i], rb = Q.rayB[cur][i];
             p = __float_as_uint(rb.w);
@@ -337,6 +337,13 @@ __global__ __launch_bounds__(kWB) void kShade(const DevReq* reqs, const SegInfo*
             uint32_t inst = __float_as_uint(ha.y);
             uint32_t depth = f.depth();
             uint8_t mask = 0;
+            // A shadow request is queued where it is made (the appends work on
+            // whatever lanes are active), so no slot state stays live.
+            auto shadow = [&](int slot, f3 org, f3 sdir, f3 color) {
+                P.shColor[(size_t)p * kSlots + slot] = f4(color, 0.0f);
+                mask |= (uint8_t)(1u << slot);
+                emitShadow(Q, true, org, sdir, p * kSlots + (uint32_t)slot);
+            };
             if (inst == 0xFFFFFFFFu) {
                 bool bg = f.primary() || f.rayType() == (uint32_t)kTransmissionLobe ||
                           f.rayType() == (uint32_t)kGroundPlaneReflectionRay;
@@ -350,7 +357,7 @@ __global__ __launch_bounds__(kWB) void kShade(const DevReq* reqs, const SegInfo*
                 Shading shd;
                 computeShading(ins, __float_as_uint(ha.z), ha.w, Q.hitV[i], shd);
                 uint32_t mi = ins.material < rq.rs.nmat ? ins.material : 0u;
-                const DevMaterial m = rq.rs.mats[mi];
+                const DevMaterial& m = rq.rs.mats[mi];
                 f3 transparency = splat(1.0f) - m.opacity;
                 float Pt = luminance(transparency);
                 bool hitLayer = random2D(rng).x >= Pt;
@@ -370,10 +377,7 @@ __global__ __launch_bounds__(kWB) void kShade(const DevReq* reqs, const SegInfo*
                             f3 sd;
                             f3 lr = shadeDirectional(fp.lights[li], m, shd, V, rng, sd);
                             if (isBlack(lr)) continue;
-                            P.shColor[(size_t)p * kSlots + li] = f4(lr, 0.0f);
-                            sh[li] = true;
-                            shO[li] = so;
-                            shD[li] = sd;
+                            shadow(li, so, sd, lr);
                         }
                     } else if (fp.light_count > 0) {
                         const aph_distant_light& l0 = fp.lights[0];
@@ -385,10 +389,7 @@ __global__ __launch_bounds__(kWB) void kShade(const DevReq* reqs, const SegInfo*
                         if (l0.color_intensity[3] > 0.0f) {
                             f3 sd;
                             f3 dc = shadeDirectional(l0, m, shd, V, rng, sd) / fmaxf(selPdf, 1e-4f);
-                            P.shColor[(size_t)p * kSlots] = f4(dc, 0.0f);
-                            sh[0] = true;
-                            shO[0] = so;
-                            shD[0] = sd;
+                            shadow(0, so, sd, dc);
                         }
                     }
                     // environment MIS (ShadeFunctions.slang:135-259)
@@ -402,25 +403,16 @@ __global__ __launch_bounds__(kWB) void kShade(const DevReq* reqs, const SegInfo*
                             f3 Le = evalEnvironment(env, L, false, px, py, rq.W, rq.H);
                             if (!isBlack(Le) && !isBlack(fs)) {
                                 float lightPdf = env.hasLightTex ? luminance(Le) / env.lightIntegral : 0.25f * kPiInv;
-                                P.shColor[(size_t)p * kSlots + kSlotMisMat] =
-                                    f4(Le * misWeight(matPdf, lightPdf) * fs / matPdf, 0.0f);
-                                sh[kSlotMisMat] = true;
-                                shO[kSlotMisMat] = so;
-                                shD[kSlotMisMat] = L;
+                                shadow(kSlotMisMat, so, L, Le * misWeight(matPdf, lightPdf) * fs / matPdf);
                             }
                             float lightPdf;
                             Le = sampleEnvironment(env, r, L, lightPdf);
                             int evLobe;
                             fs = evalMaterialPdf(m, shd, V, L, r, matPdf, evLobe);
-                            if (!isBlack(Le) && !isBlack(fs)) {
-                                P.shColor[(size_t)p * kSlots + kSlotMisLight] =
-                                    f4(Le * misWeight(lightPdf, matPdf) * fs / lightPdf, 0.0f);
-                                sh[kSlotMisLight] = true;
-                                shO[kSlotMisLight] = so;
-                                shD[kSlotMisLight] = L;
-                            }
+                            if (!isBlack(Le) && !isBlack(fs))
+                                shadow(kSlotMisLight, so, L, Le * misWeight(lightPdf, matPdf) * fs / lightPdf);
                         }
-                        if (sh[kSlotMisMat] || sh[kSlotMisLight]) f.misLobe((uint32_t)lobe);
+                        if (mask & ((1u << kSlotMisMat) | (1u << kSlotMisLight))) f.misLobe((uint32_t)lobe);
                     }
                     contrib = contrib * (m.opacity / (1.0f - Pt));
                     f.alpha((f.alpha() == 0.0f && f.rayType() == (uint32_t)kTransmissionLobe) ? 0.0f : 1.0f);
@@ -456,7 +448,6 @@ __global__ __launch_bounds__(kWB) void kShade(const DevReq* reqs, const SegInfo*
                     f.event(kEvHit);
                     f.ended(ended);
                     contPath = !ended;
-                    for (int k = 0; k < kSlots; k++) mask |= sh[k] ? (uint8_t)(1u << k) : (uint8_t)0;
                 }
             }
             f.terminal(false);
@@ -479,8 +470,6 @@ __global__ __launch_bounds__(kWB) void kShade(const DevReq* reqs, const SegInfo*
             Q.rayA[nxt][at] = f4(nOrg, nTmin);
             Q.rayB[nxt][at] = f4(nDir, __uint_as_float(p));
         }
-#pragma unroll
-        for (int k = 0; k < kSlots; k++) emitShadow(Q, sh[k], shO[k], shD[k], p * kSlots + (uint32_t)k);
     }
 }
 

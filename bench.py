@@ -96,7 +96,7 @@ def measure_traffic(args, kname):
         d = tempfile.mkdtemp(prefix="aph_pmc_", dir="/tmp")
         cmd = ["timeout", "-s", "KILL", "240", prof, "--pmc", counter, "-d", d, "-o", "pmc", "--",
                sys.executable, "-u", os.path.abspath(__file__), "--pmc-child", "--spp", str(args.spp),
-               "--subdiv", str(args.subdiv), "--mode", args.mode]
+               "--subdiv", str(args.subdiv), "--mode", args.mode, "--builder", args.builder]
         log(f"pmc pass {counter}: {' '.join(cmd)}")
         env = dict(os.environ, TMPDIR="/tmp")
         p = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
@@ -178,6 +178,7 @@ def main():
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--subdiv", type=int, default=4, help="4: 1.02M instanced tris; 5: 4.1M")
     ap.add_argument("--mode", choices=["megakernel", "wavefront"], default="wavefront")
+    ap.add_argument("--builder", choices=["sah", "lbvh"], default="sah", help="BLAS builder")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 counter passes")
     ap.add_argument("--cpu-baseline-child", action="store_true",
@@ -193,6 +194,7 @@ def main():
     sc = scenes.config2(subdiv=args.subdiv, spp=args.spp)
     mode = capi.APH_MODE_MEGAKERNEL if args.mode == "megakernel" else capi.APH_MODE_WAVEFRONT
     kname = "render_megakernel" if args.mode == "megakernel" else "wf_extend"
+    builder = capi.APH_BUILDER_SAH if args.builder == "sah" else capi.APH_BUILDER_LBVH
     if args.cpu_baseline_child:
         print(json.dumps(cpu_baseline(sc)), flush=True)
         return
@@ -200,7 +202,7 @@ def main():
         copy_calibration()
         r = Renderer(0)
         tgt = r.target(sc.width, sc.height)
-        r.render([r.request(tgt, r.load_scene(sc), sc.frame, 0, sc.spp)], mode)
+        r.render([r.request(tgt, r.load_scene(sc, builder), sc.frame, 0, sc.spp)], mode)
         r.wait()
         r.close()
         print("pmc child done", flush=True)
@@ -209,7 +211,7 @@ def main():
     log(f"scene: {sc.instanced_triangles} instanced / {sc.unique_triangles} unique triangles, "
         f"{len(sc.instances)} instances, {sc.width}x{sc.height} x {sc.spp} spp, mode {args.mode}")
     r = Renderer(0)
-    tlas = r.load_scene(sc)
+    tlas = r.load_scene(sc, builder)
     tgt = r.target(sc.width, sc.height)
     req = [r.request(tgt, tlas, sc.frame, 0, sc.spp)]
 
@@ -275,6 +277,7 @@ def main():
                         f"icospheres (subdiv {args.subdiv}), {sc.instanced_triangles} instanced triangles, "
                         "2048x1024 HDR environment with 1e4 sun",
             "mode": args.mode,
+            "blas_builder": args.builder,
             "samples_per_s": round(sc.width * sc.height * sc.spp * args.steps / wall, 1),
             "extension_rays_per_frame": tst["extension_rays"] // args.steps,
             "shadow_rays_per_frame": tst["shadow_rays"] // args.steps,

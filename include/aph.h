@@ -194,7 +194,8 @@ int aph_mesh_upload(aph_ctx_t ctx, const float* pos, uint32_t nv, const uint32_t
 /* BLAS build, replaces the driver build of HGIGeometry.cpp:127-160 /
  * PTGeometry.cpp:155-223.  APH_BUILDER_LBVH: Karras LBVH over 63-bit Morton
  * codes, one triangle per leaf (fastest build); APH_BUILDER_SAH: top-down
- * binned SAH (16 bins x 3 axes), up to 8 triangles per leaf (fastest trace). */
+ * binned SAH (16 bins x 3 axes), up to 4 triangles per leaf (fastest trace).
+ * The TLAS is always a binned-SAH tree over the instance world boxes. */
 int aph_blas_build(aph_ctx_t ctx, uint64_t mesh, int builder);
 /* Copies a built BLAS to host memory: *node_count 64-byte nodes (two child
  * boxes; child references in a.w / b.w, leaf = bit 31 | (count-1) << 27 |

@@ -19,7 +19,7 @@ from aurora_amd import capi, scenes
 
 pytestmark = pytest.mark.gpu
 
-COST_NODE, COST_TRI, SAH_MAX_LEAF = 1.0, 1.2, 8
+COST_NODE, COST_TRI, SAH_MAX_LEAF = 1.0, 1.2, 4
 LEAF_BIT = 0x80000000
 
 
