@@ -93,6 +93,23 @@ class Stats(C.Structure):
                 ("shadow_inst_visits", C.c_uint64)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_uint64), ("max_active", C.c_uint32), ("step_samples", C.c_uint32),
+                ("mode", C.c_int), ("keep_results", C.c_int)]
+
+
+class Job(C.Structure):
+    _fields_ = [("tlas", C.c_uint64), ("frame", Frame), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("spp", C.c_uint32), ("seed_offset", C.c_uint32)]
+
+
+class JobStatus(C.Structure):
+    _fields_ = [("state", C.c_int), ("samples_done", C.c_uint32), ("t_arrival", C.c_double),
+                ("t_admit", C.c_double), ("t_done", C.c_double)]
+
+
+APH_JOB_PENDING, APH_JOB_ACTIVE, APH_JOB_DONE, APH_JOB_FAILED = 0, 1, 2, 3
+
 _P = C.c_void_p
 _u32 = C.c_uint32
 _u64 = C.c_uint64
@@ -131,6 +148,15 @@ SIGNATURES = {
     "aph_timing_enable": (_i, [_P, _i]),
     "aph_timing_get": (_i, [_P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(_u64)]),
     "aph_timing_reset": (_i, [_P]),
+    "aph_server_create": (_i, [_P, C.POINTER(ServerConfig), C.POINTER(_P)]),
+    "aph_server_destroy": (None, [_P]),
+    "aph_server_clock": (C.c_double, [_P]),
+    "aph_server_submit": (_i, [_P, C.POINTER(Job), C.c_double, C.POINTER(_u64)]),
+    "aph_server_step": (_i, [_P, C.POINTER(_u32), C.POINTER(_u32)]),
+    "aph_server_status": (_i, [_P, _u64, C.POINTER(JobStatus)]),
+    "aph_server_result": (_i, [_P, _u64, _P, C.c_size_t]),
+    "aph_server_release": (_i, [_P, _u64]),
+    "aph_server_replay": (_i, [_P, _P, _P, _u32, _P]),
 }
 
 _lib = None

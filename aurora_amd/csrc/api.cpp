@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <set>
 #include <memory>
 #include <string>
 #include <vector>
@@ -16,157 +17,13 @@
 #include "bvh_build.h"
 #include "kernels.h"
 #include "wavefront.h"
+#include "context.h"
 
 using namespace aph;
 
 namespace {
 
 constexpr uint32_t kSahMaxLeaf = 4;  // triangles per leaf of an SAH BLAS
-
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    ~DevBuf() { reset(); }
-    void reset()
-    {
-        if (p) hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    hipError_t alloc(size_t n)
-    {
-        reset();
-        if (n == 0) n = 16;
-        hipError_t e = hipMalloc(&p, n);
-        if (e == hipSuccess) bytes = n;
-        else p = nullptr;
-        return e;
-    }
-    // Grows (never shrinks) a scratch buffer.
-    hipError_t reserve(size_t n)
-    {
-        if (bytes >= n && p) return hipSuccess;
-        return alloc(n);
-    }
-    template <class T> T* as() const { return (T*)p; }
-};
-
-struct Mesh {
-    uint32_t nv = 0, nt = 0;
-    bool hasNrm = false;
-    DevBuf pos, nrm, uv, idx, nodes, tris;
-    bool built = false;
-    BuildInfo info{};
-};
-
-struct Tlas {
-    uint32_t n = 0, nvalid = 0;
-    DevBuf nodes, trav, shade;
-    TraceScene sc{};
-    uint32_t depth = 0;
-};
-
-struct Target {
-    uint32_t w = 0, h = 0;
-    DevBuf accum;
-};
-
-struct Timing {
-    double ms = 0.0;
-    uint64_t launches = 0;
-};
-
-}  // namespace
-
-struct aph_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::string err;
-    uint64_t nextHandle = 1;
-    std::map<uint64_t, std::unique_ptr<Mesh>> meshes;
-    std::map<uint64_t, std::unique_ptr<Tlas>> tlases;
-    std::map<uint64_t, std::unique_ptr<Target>> targets;
-    DevBuf mats;
-    uint32_t nmat = 0;
-    DevBuf envLight, envAlias, envBg;
-    DevEnv env{};
-    DevBuf counters;
-    int statsLevel = 0;
-    DevBuf scratchA, scratchB;
-    WavefrontState wf;
-    bool timing = false;
-    std::map<std::string, Timing> timings;
-    std::vector<PendingEvent> pending;
-};
-
-namespace {
-
-int fail(aph_ctx* c, int code, const char* fmt, ...)
-{
-    char buf[512];
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(buf, sizeof(buf), fmt, ap);
-    va_end(ap);
-    if (c) c->err = buf;
-    return code;
-}
-
-int hipFail(aph_ctx* c, hipError_t e, const char* what)
-{
-    return fail(c, e == hipErrorOutOfMemory ? APH_E_MEMORY : APH_E_DEVICE, "%s: %s", what, hipGetErrorString(e));
-}
-
-#define HIPCALL(ctx, x, what)                              \
-    do {                                                   \
-        hipError_t e_ = (x);                               \
-        if (e_ != hipSuccess) return hipFail(ctx, e_, what); \
-    } while (0)
-
-int enter(aph_ctx* c)
-{
-    if (!c) return APH_E_INVALID;
-    hipError_t e = hipSetDevice(c->device);
-    if (e != hipSuccess) return hipFail(c, e, "hipSetDevice");
-    c->err.clear();
-    return APH_OK;
-}
-
-void resolveTimings(aph_ctx* c)
-{
-    for (auto& p : c->pending) {
-        hipEventSynchronize(p.b);
-        float ms = 0.0f;
-        hipEventElapsedTime(&ms, p.a, p.b);
-        Timing& t = c->timings[p.name];
-        t.ms += ms;
-        t.launches++;
-        hipEventDestroy(p.a);
-        hipEventDestroy(p.b);
-    }
-    c->pending.clear();
-}
-
-struct TimedScope {
-    aph_ctx* c;
-    const char* name;
-    hipEvent_t a = nullptr, b = nullptr;
-    TimedScope(aph_ctx* ctx, const char* n) : c(ctx), name(n)
-    {
-        if (c->timing) {
-            hipEventCreate(&a);
-            hipEventCreate(&b);
-            hipEventRecord(a, c->stream);
-        }
-    }
-    ~TimedScope()
-    {
-        if (c->timing) {
-            hipEventRecord(b, c->stream);
-            c->pending.push_back(PendingEvent{name, a, b});
-        }
-    }
-};
 
 // Double-precision 4x4 inverse (row-major), rounded to float.
 bool inverse4(const float* mf, float* out)
@@ -214,12 +71,6 @@ bool inverse4(const float* mf, float* out)
 void identity4(float* m)
 {
     for (int i = 0; i < 16; i++) m[i] = (i % 5 == 0) ? 1.0f : 0.0f;
-}
-
-template <class T> T* lookup(std::map<uint64_t, std::unique_ptr<T>>& m, uint64_t h)
-{
-    auto it = m.find(h);
-    return it == m.end() ? nullptr : it->second.get();
 }
 
 void setDefaultEnv(aph_ctx* c)
@@ -454,9 +305,11 @@ int aph_tlas_build(aph_ctx_t c, const aph_instance* inst, uint32_t n, uint64_t* 
     std::vector<uint32_t> remap;
     double extent = 0.0, kappa = 1.0;
     uint32_t maxBlasDepth = 0;
+    std::set<const Mesh*> usedMeshes;
     for (uint32_t i = 0; i < n; i++) {
         const aph_instance& in = inst[i];
         Mesh* m = lookup(c->meshes, (uint64_t)in.mesh);
+        if (m) usedMeshes.insert(m);
         if (!m) return fail(c, APH_E_INVALID, "instance %u: unknown mesh handle %u", i, in.mesh);
         if (!m->built) return fail(c, APH_E_STATE, "instance %u: BLAS of mesh %u not built", i, in.mesh);
         float w2o[12];
@@ -541,6 +394,9 @@ int aph_tlas_build(aph_ctx_t c, const aph_instance* inst, uint32_t n, uint64_t* 
     t->sc.extent = (float)(3.0 * extent);
     t->sc.padScale = (float)(1e-5 * kappa);
     t->sc.ninst = nv;
+    t->bytes = t->nodes.bytes + t->trav.bytes + t->shade.bytes;
+    for (const Mesh* m : usedMeshes)
+        t->bytes += m->pos.bytes + m->nrm.bytes + m->uv.bytes + m->idx.bytes + m->nodes.bytes + m->tris.bytes;
     uint64_t h = c->nextHandle++;
     c->tlases[h] = std::move(t);
     *tlas = h;
@@ -771,16 +627,8 @@ int aph_render(aph_ctx_t c, const aph_request* reqs, uint32_t nreq, int mode)
             return fail(c, APH_E_INVALID, "request %u: sample range overflows", i);
         if (r.frame.light_count < 0 || r.frame.light_count > 4)
             return fail(c, APH_E_INVALID, "request %u: light_count must be in [0,4]", i);
-        aph_frame fp = r.frame;
-        fp.trace_depth = fp.trace_depth < 1 ? 1 : (fp.trace_depth > 10 ? 10 : fp.trace_depth);  // RendererBase.cpp:265
-        if (fp.rr_start_depth < 0) fp.rr_start_depth = 0;                                         // :274-275
-        RenderScene rs;
-        rs.sc = tl->sc;
-        rs.ishade = tl->shade.as<InstShade>();
-        rs.mats = c->mats.as<DevMaterial>();
-        rs.nmat = c->nmat;
-        rs.pad_ = 0;
-        rs.env = c->env;
+        aph_frame fp = clampFrame(r.frame);
+        RenderScene rs = renderScene(c, tl);
         if (mode == APH_MODE_MEGAKERNEL) {
             TimedScope ts(c, "render_megakernel");
             HIPCALL(c, launchRenderMega(rs, fp, tg->w, tg->h, r.sample_start, r.sample_count, r.seed_offset,

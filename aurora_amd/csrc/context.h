@@ -1,0 +1,205 @@
+// Internal state of a backend context, shared by the C-ABI translation units
+// (api.cpp: scene objects and rendering; server.cpp: the serving loop).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <map>
+#include <set>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/aph.h"
+#include "bvh_build.h"
+#include "kernels.h"
+#include "timing.h"
+#include "wavefront.h"
+
+namespace aph {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() { reset(); }
+    void reset()
+    {
+        if (p) hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    hipError_t alloc(size_t n)
+    {
+        reset();
+        if (n == 0) n = 16;
+        hipError_t e = hipMalloc(&p, n);
+        if (e == hipSuccess) bytes = n;
+        else p = nullptr;
+        return e;
+    }
+    // Grows (never shrinks) a scratch buffer.
+    hipError_t reserve(size_t n)
+    {
+        if (bytes >= n && p) return hipSuccess;
+        return alloc(n);
+    }
+    template <class T> T* as() const { return (T*)p; }
+};
+
+struct Mesh {
+    uint32_t nv = 0, nt = 0;
+    bool hasNrm = false;
+    DevBuf pos, nrm, uv, idx, nodes, tris;
+    bool built = false;
+    BuildInfo info{};
+};
+
+struct Tlas {
+    uint32_t n = 0, nvalid = 0;
+    DevBuf nodes, trav, shade;
+    TraceScene sc{};
+    uint32_t depth = 0;
+    size_t bytes = 0;  // device memory of the TLAS and the meshes/BLASes it references
+};
+
+struct Target {
+    uint32_t w = 0, h = 0;
+    DevBuf accum;
+};
+
+struct Timing {
+    double ms = 0.0;
+    uint64_t launches = 0;
+};
+
+}  // namespace aph
+
+// The opaque aph_ctx_t of the C-ABI.
+struct aph_ctx {
+    using DevBuf = aph::DevBuf;
+    using Mesh = aph::Mesh;
+    using Tlas = aph::Tlas;
+    using Target = aph::Target;
+    using Timing = aph::Timing;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    uint64_t nextHandle = 1;
+    std::map<uint64_t, std::unique_ptr<Mesh>> meshes;
+    std::map<uint64_t, std::unique_ptr<Tlas>> tlases;
+    std::map<uint64_t, std::unique_ptr<Target>> targets;
+    DevBuf mats;
+    uint32_t nmat = 0;
+    DevBuf envLight, envAlias, envBg;
+    aph::DevEnv env{};
+    DevBuf counters;
+    int statsLevel = 0;
+    DevBuf scratchA, scratchB;
+    aph::WavefrontState wf;
+    bool timing = false;
+    std::map<std::string, Timing> timings;
+    std::vector<aph::PendingEvent> pending;
+};
+
+namespace aph {
+
+inline int fail(aph_ctx* c, int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+inline int hipFail(aph_ctx* c, hipError_t e, const char* what)
+{
+    return fail(c, e == hipErrorOutOfMemory ? APH_E_MEMORY : APH_E_DEVICE, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define HIPCALL(ctx, x, what)                              \
+    do {                                                   \
+        hipError_t e_ = (x);                               \
+        if (e_ != hipSuccess) return hipFail(ctx, e_, what); \
+    } while (0)
+
+template <class T> T* lookup(std::map<uint64_t, std::unique_ptr<T>>& m, uint64_t h)
+{
+    auto it = m.find(h);
+    return it == m.end() ? nullptr : it->second.get();
+}
+
+// Frame parameters as RendererBase::updateFrameDataGPUStruct bounds them:
+// traceDepth in [1, 10] (RendererBase.cpp:265), RR start depth >= 0 (:274-275).
+inline aph_frame clampFrame(const aph_frame& in)
+{
+    aph_frame fp = in;
+    fp.trace_depth = fp.trace_depth < 1 ? 1 : (fp.trace_depth > 10 ? 10 : fp.trace_depth);
+    if (fp.rr_start_depth < 0) fp.rr_start_depth = 0;
+    return fp;
+}
+
+// Everything a render of one TLAS reads: geometry, instance records,
+// the context's material table and environment.
+inline RenderScene renderScene(const aph_ctx* c, const Tlas* tl)
+{
+    RenderScene rs;
+    rs.sc = tl->sc;
+    rs.ishade = tl->shade.as<InstShade>();
+    rs.mats = c->mats.as<DevMaterial>();
+    rs.nmat = c->nmat;
+    rs.pad_ = 0;
+    rs.env = c->env;
+    return rs;
+}
+
+inline int enter(aph_ctx* c)
+{
+    if (!c) return APH_E_INVALID;
+    hipError_t e = hipSetDevice(c->device);
+    if (e != hipSuccess) return hipFail(c, e, "hipSetDevice");
+    c->err.clear();
+    return APH_OK;
+}
+
+inline void resolveTimings(aph_ctx* c)
+{
+    for (auto& p : c->pending) {
+        hipEventSynchronize(p.b);
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, p.a, p.b);
+        Timing& t = c->timings[p.name];
+        t.ms += ms;
+        t.launches++;
+        hipEventDestroy(p.a);
+        hipEventDestroy(p.b);
+    }
+    c->pending.clear();
+}
+
+struct TimedScope {
+    aph_ctx* c;
+    const char* name;
+    hipEvent_t a = nullptr, b = nullptr;
+    TimedScope(aph_ctx* ctx, const char* n) : c(ctx), name(n)
+    {
+        if (c->timing) {
+            hipEventCreate(&a);
+            hipEventCreate(&b);
+            hipEventRecord(a, c->stream);
+        }
+    }
+    ~TimedScope()
+    {
+        if (c->timing) {
+            hipEventRecord(b, c->stream);
+            c->pending.push_back(PendingEvent{name, a, b});
+        }
+    }
+};
+
+}  // namespace aph

@@ -171,3 +171,74 @@ class Renderer:
 
     def reset_timing(self):
         self._check(self.lib.aph_timing_reset(self.ctx))
+
+
+class Server:
+    """Serving loop over a Renderer's resident scenes (aph_server_*): continuous
+    batching of render jobs with HBM-budget admission."""
+
+    def __init__(self, renderer, hbm_budget=0, max_active=0, step_samples=0, mode=capi.APH_MODE_WAVEFRONT,
+                 keep_results=True):
+        self.r = renderer
+        self.lib = renderer.lib
+        cfg = capi.ServerConfig(hbm_budget, max_active, step_samples, mode, 1 if keep_results else 0)
+        self.h = C.c_void_p()
+        renderer._check(self.lib.aph_server_create(renderer.ctx, C.byref(cfg), C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            self.lib.aph_server_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def job(tlas, frame, width, height, spp, seed_offset=0):
+        return capi.Job(tlas, frame, width, height, spp, seed_offset)
+
+    def clock(self):
+        return self.lib.aph_server_clock(self.h)
+
+    def submit(self, job, t_arrival=-1.0):
+        t = C.c_uint64()
+        self.r._check(self.lib.aph_server_submit(self.h, C.byref(job), t_arrival, C.byref(t)))
+        return t.value
+
+    def step(self):
+        a, p = C.c_uint32(), C.c_uint32()
+        self.r._check(self.lib.aph_server_step(self.h, C.byref(a), C.byref(p)))
+        return a.value, p.value
+
+    def status(self, ticket):
+        st = capi.JobStatus()
+        self.r._check(self.lib.aph_server_status(self.h, ticket, C.byref(st)))
+        return st
+
+    def result(self, ticket, width, height):
+        out = np.zeros((height, width, 4), dtype=np.float32)
+        self.r._check(self.lib.aph_server_result(self.h, ticket, out.ctypes.data, out.nbytes))
+        return out
+
+    def release(self, ticket):
+        self.r._check(self.lib.aph_server_release(self.h, ticket))
+
+    def run(self):
+        """Steps until no job is running or waiting."""
+        while True:
+            a, p = self.step()
+            if a == 0 and p == 0:
+                return
+
+    def replay(self, jobs, arrivals):
+        """Real-time replay of an arrival trace; returns the tickets."""
+        n = len(jobs)
+        arr = (capi.Job * n)(*jobs)
+        t = np.ascontiguousarray(arrivals, dtype=np.float64)
+        tickets = np.zeros(n, dtype=np.uint64)
+        self.r._check(self.lib.aph_server_replay(self.h, C.cast(arr, C.c_void_p), t.ctypes.data, n,
+                                                 tickets.ctypes.data))
+        return [int(x) for x in tickets]

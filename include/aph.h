@@ -265,6 +265,58 @@ int aph_timing_enable(aph_ctx_t ctx, int enable);
 int aph_timing_get(aph_ctx_t ctx, const char* kernel, double* total_ms, uint64_t* launches);
 int aph_timing_reset(aph_ctx_t ctx);
 
+/* ---- serving loop: continuous batching of render jobs against device-resident
+ * TLASes.  No reference counterpart: Aurora renders one request at a time
+ * (HGIRenderer::render, HGIRenderer.cpp:477-589, progressive
+ * render(sampleStart, sampleCount) driven by SampleCounter, Timer.h:97-286).
+ * Each aph_server_step admits waiting jobs in arrival order while a slot is
+ * free and the HBM budget (resident scenes + admitted jobs' RGBA32F
+ * accumulation buffers) allows, then renders the next run of samples of every
+ * admitted job in one shared wavefront.  A job's image is the same alone and in
+ * any mix.  Errors are reported through aph_last_error of the context. ---- */
+typedef struct aph_server* aph_server_t;
+
+typedef struct {
+    uint64_t hbm_budget;    /* bytes: resident scenes + job buffers; 0 = unlimited */
+    uint32_t max_active;    /* jobs sharing the wavefront at once; 0 = unlimited */
+    uint32_t step_samples;  /* pixel-samples per step over all active jobs; 0 = 8M */
+    int mode;               /* APH_MODE_WAVEFRONT: batched; APH_MODE_MEGAKERNEL: one job at a time */
+    int keep_results;       /* 1: a finished job keeps its image until aph_server_release */
+} aph_server_config;
+
+typedef struct {
+    uint64_t tlas;          /* scene (built TLAS of the context) */
+    aph_frame frame;        /* camera and options, as in aph_request */
+    uint32_t width, height; /* image size */
+    uint32_t spp;           /* samples 0 .. spp-1 */
+    uint32_t seed_offset;
+} aph_job;
+
+enum { APH_JOB_PENDING = 0, APH_JOB_ACTIVE = 1, APH_JOB_DONE = 2, APH_JOB_FAILED = 3 };
+
+typedef struct {
+    int state;
+    uint32_t samples_done;
+    double t_arrival, t_admit, t_done; /* seconds on the server clock */
+} aph_job_status;
+
+int aph_server_create(aph_ctx_t ctx, const aph_server_config* cfg, aph_server_t* out);
+void aph_server_destroy(aph_server_t s);
+/* Seconds since aph_server_create (steady clock). */
+double aph_server_clock(aph_server_t s);
+/* Queues a job; t_arrival < 0 stamps the current clock. */
+int aph_server_submit(aph_server_t s, const aph_job* job, double t_arrival, uint64_t* ticket);
+/* Admission + one batched step (synchronous); reports jobs running and waiting. */
+int aph_server_step(aph_server_t s, uint32_t* active, uint32_t* pending);
+int aph_server_status(aph_server_t s, uint64_t ticket, aph_job_status* st);
+/* RGBA32F image of a finished job kept with keep_results. */
+int aph_server_result(aph_server_t s, uint64_t ticket, float* dst, size_t bytes);
+/* Forgets a finished or waiting job and frees its buffer. */
+int aph_server_release(aph_server_t s, uint64_t ticket);
+/* Replays an arrival trace in real time: job i is submitted arrival[i] seconds
+ * (non-decreasing) after the call; returns when every job is done. */
+int aph_server_replay(aph_server_t s, const aph_job* jobs, const double* arrival, uint32_t n, uint64_t* tickets);
+
 #ifdef __cplusplus
 }
 #endif
