@@ -66,8 +66,6 @@ constexpr uint32_t kDefaultStepSamples = 8u << 20;
 
 size_t accumBytes(const aph_job& j) { return (size_t)j.width * j.height * sizeof(float4); }
 
-int serverFail(aph_server* s, int code, const char* what) { return fail(s->ctx, code, "%s", what); }
-
 // Admits waiting jobs in arrival order.  Returns APH_OK or an error for a job
 // that can never fit.
 int admit(aph_server* s)
@@ -272,6 +270,17 @@ int aph_server_result(aph_server_t s, uint64_t ticket, float* dst, size_t bytes)
     size_t need = accumBytes(j->desc);
     if (bytes < need) return fail(c, APH_E_INVALID, "destination too small (%zu < %zu)", bytes, need);
     HIPCALL(c, hipMemcpy(dst, j->accum.p, need, hipMemcpyDeviceToHost), "job result");
+    return APH_OK;
+}
+
+int aph_server_usage(aph_server_t s, uint64_t* residentBytes, uint64_t* jobBytes, uint32_t* active,
+                     uint32_t* pending)
+{
+    if (!s) return APH_E_INVALID;
+    if (residentBytes) *residentBytes = s->residentBytes;
+    if (jobBytes) *jobBytes = s->jobBytes;
+    if (active) *active = (uint32_t)s->active.size();
+    if (pending) *pending = (uint32_t)s->pending.size();
     return APH_OK;
 }
 

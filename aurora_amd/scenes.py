@@ -277,7 +277,8 @@ def config1(seed=1, width=512, height=512, spp=16, trace_depth=3):
     return sc
 
 
-def config2(seed=2, subdiv=4, width=1920, height=1080, spp=64, trace_depth=5, env_w=2048, env_h=1024):
+def config2(seed=2, subdiv=4, width=1920, height=1080, spp=64, trace_depth=5, env_w=2048, env_h=1024,
+            instances=200):
     """200 instances of 5 displaced icospheres, transmission/coat materials, 2048x1024 sun sky."""
     rng = np.random.Generator(np.random.PCG64(seed))
     sc = Scene()
@@ -289,7 +290,7 @@ def config2(seed=2, subdiv=4, width=1920, height=1080, spp=64, trace_depth=5, en
         sc.meshes.append((p, n, f))
     inst = [(0, 0, affine())]
     sc.materials = [_material((0.5, 0.5, 0.5), 0.0, 0.6)]
-    for i in range(200):
+    for i in range(instances):
         mesh = 1 + int(rng.integers(0, 5))
         s = rng.uniform(0.4, 1.2)
         c = (rng.uniform(-18, 18), rng.uniform(0.5, 3.0), rng.uniform(-18, 18))
@@ -303,6 +304,22 @@ def config2(seed=2, subdiv=4, width=1920, height=1080, spp=64, trace_depth=5, en
     sc.width, sc.height, sc.spp = width, height, spp
     sc.frame = look_at_frame((0.0, 14.0, 34.0), (0.0, 1.0, 0.0), (0, 1, 0), 50.0, width / height, trace_depth)
     return sc
+
+
+def serving_scenes(seed=7, env_w=1024, env_h=512):
+    """The four resident scenes of the serving workload, about 10k, 100k, 1M and 5M
+    instanced triangles: config1 (20,482), 80 x subdiv-3 (102,402), 200 x subdiv-4
+    (1,024,002) and 250 x subdiv-5 (5,120,002) displaced icospheres."""
+    return [config1(seed=seed), config2(seed=seed + 1, subdiv=3, instances=80, env_w=env_w, env_h=env_h),
+            config2(seed=seed + 2, subdiv=4, env_w=env_w, env_h=env_h),
+            config2(seed=seed + 3, subdiv=5, instances=250, env_w=env_w, env_h=env_h)]
+
+
+def orbit_frame(rng, width, height, radius=14.0, target=(0.0, 0.5, 0.0), trace_depth=5):
+    """A camera on a random orbit around the scene, looking at its centre."""
+    th = rng.uniform(0.0, 2.0 * math.pi)
+    eye = (radius * math.cos(th), float(rng.uniform(3.0, 9.0)), radius * math.sin(th))
+    return look_at_frame(eye, target, (0, 1, 0), 45.0, width / height, trace_depth)
 
 
 def teapot_scene(path, width=128, height=128, spp=4, trace_depth=5):

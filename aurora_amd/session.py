@@ -86,6 +86,26 @@ class Renderer:
         e, keep = scenes.environment_struct(env)
         self._check(self.lib.aph_environment_upload(self.ctx, C.byref(e)))
 
+    def load_scenes(self, scs, builder=capi.APH_BUILDER_LBVH):
+        """Uploads several scenes into this context, resident together: one material
+        table (each scene's instance material indices offset into it) and the first
+        scene's environment.  Returns one TLAS handle per scene."""
+        mats, tl = [], []
+        for sc in scs:
+            handles = []
+            for pos, nrm, idx in sc.meshes:
+                m = self.mesh(pos, idx, nrm)
+                self.build_blas(m, builder)
+                handles.append(m)
+            inst = sc.instances.copy()
+            inst["mesh"] = np.array([handles[i] for i in sc.instances["mesh"]], dtype=np.uint32)
+            inst["material"] = inst["material"] + len(mats)
+            tl.append(self.tlas(inst))
+            mats.extend(sc.materials)
+        self.materials(mats)
+        self.environment(scs[0].env or {})
+        return tl
+
     def load_scene(self, sc, builder=capi.APH_BUILDER_LBVH):
         """Uploads a scenes.Scene; returns the TLAS handle."""
         handles = []
@@ -222,6 +242,12 @@ class Server:
         out = np.zeros((height, width, 4), dtype=np.float32)
         self.r._check(self.lib.aph_server_result(self.h, ticket, out.ctypes.data, out.nbytes))
         return out
+
+    def usage(self):
+        """(resident scene bytes, admitted job bytes, running, waiting)."""
+        r, j, a, p = C.c_uint64(), C.c_uint64(), C.c_uint32(), C.c_uint32()
+        self.r._check(self.lib.aph_server_usage(self.h, C.byref(r), C.byref(j), C.byref(a), C.byref(p)))
+        return r.value, j.value, a.value, p.value
 
     def release(self, ticket):
         self.r._check(self.lib.aph_server_release(self.h, ticket))

@@ -170,6 +170,139 @@ def cpu_baseline_child(args):
     return json.loads(p.stdout.strip().splitlines()[-1])
 
 
+RESOLUTIONS = [(256, 256), (512, 512), (1024, 768), (1920, 1080)]
+
+
+def rays_of(st):
+    return st["extension_rays"] + st["shadow_rays"]
+
+
+def literal_config2(r, builder, spp):
+    """One frame of config2 as specified (icospheres subdiv 5: 4,096,002 instanced triangles)."""
+    sc = scenes.config2(subdiv=5, spp=spp)
+    tlas = r.load_scene(sc, builder)
+    tgt = r.target(sc.width, sc.height)
+    req = [r.request(tgt, tlas, sc.frame, 0, sc.spp)]
+    r.render(req, capi.APH_MODE_WAVEFRONT)
+    r.wait()
+    r.reset_stats()
+    t = time.perf_counter()
+    r.render(req, capi.APH_MODE_WAVEFRONT)
+    r.wait()
+    dt = time.perf_counter() - t
+    return {"instanced_triangles": sc.instanced_triangles, "unique_triangles": sc.unique_triangles,
+            "mrays_s": round(rays_of(r.stats()) / dt / 1e6, 2), "ms_per_frame": round(dt * 1e3, 1)}
+
+
+def c3_cameras(r, builder, seed=3, plain_sample=16):
+    """64 simultaneous cameras on the config1 scene, resolutions from RESOLUTIONS, spp U{1..64},
+    5 bounces: all in one batched wavefront call vs plain per-camera megakernel launches
+    (timed on a sample of the cameras; both are rates)."""
+    sc = scenes.config1()
+    tlas = r.load_scene(sc, builder)
+    rng = np.random.default_rng(seed)
+    cams = []
+    for _ in range(64):
+        w, h = RESOLUTIONS[int(rng.integers(0, 4))]
+        cams.append((w, h, int(rng.integers(1, 65)), scenes.orbit_frame(rng, w, h)))
+    tg = [r.target(w, h) for w, h, _, _ in cams]
+    reqs = [r.request(t, tlas, fr, 0, spp) for t, (w, h, spp, fr) in zip(tg, cams)]
+    r.render(reqs[:4], capi.APH_MODE_WAVEFRONT)  # warm
+    r.wait()
+    r.reset_stats()
+    t = time.perf_counter()
+    r.render(reqs, capi.APH_MODE_WAVEFRONT)
+    r.wait()
+    dt_b = time.perf_counter() - t
+    rays_b = rays_of(r.stats())
+    samples = sum(w * h * spp for w, h, spp, _ in cams)
+    pick = rng.choice(64, size=plain_sample, replace=False)
+    r.reset_stats()
+    t = time.perf_counter()
+    for k in pick:
+        r.render([reqs[k]], capi.APH_MODE_MEGAKERNEL)
+        r.wait()
+    dt_p = time.perf_counter() - t
+    rays_p = rays_of(r.stats())
+    psamples = sum(cams[k][0] * cams[k][1] * cams[k][2] for k in pick)
+    b, p = rays_b / dt_b / 1e6, rays_p / dt_p / 1e6
+    return {"cameras": 64, "samples": samples, "batched_mrays_s": round(b, 2), "plain_mrays_s": round(p, 2),
+            "batched_vs_plain": round(b / p, 2), "batched_s": round(dt_b, 3),
+            "batched_samples_per_s": round(samples / dt_b, 1), "plain_samples_per_s": round(psamples / dt_p, 1),
+            "plain_sample": f"{plain_sample} of the 64 cameras, one megakernel launch each"}
+
+
+def serving(r, builder, jobs=24, rates=(4.0, 16.0), max_spp=32, seed=5, budget_gib=24.0):
+    """Poisson arrivals of render jobs over the four resident serving scenes (about 10k, 100k,
+    1M and 5M triangles); resolution from RESOLUTIONS, spp U{1..max_spp}; HBM-budget admission.
+    Batched (continuous batching in shared wavefronts) vs plain (one job at a time, megakernel)."""
+    from aurora_amd.session import Server
+    scs = scenes.serving_scenes()
+    tl = r.load_scenes(scs, builder)
+    rng = np.random.default_rng(seed)
+    out = {"scenes_triangles": [s.instanced_triangles for s in scs], "jobs": jobs, "max_spp": max_spp,
+           "hbm_budget_gib": budget_gib, "rates": {}}
+    js = []
+    for _ in range(jobs):
+        k = int(rng.integers(0, 4))
+        w, h = RESOLUTIONS[int(rng.integers(0, 4))]
+        js.append(Server.job(tl[k], scenes.orbit_frame(rng, w, h, radius=30.0 if k else 14.0), w, h,
+                             int(rng.integers(1, max_spp + 1))))
+    for rate in rates:
+        arr = np.cumsum(rng.exponential(1.0 / rate, jobs))
+        res = {}
+        for name, mode in (("batched", capi.APH_MODE_WAVEFRONT), ("plain", capi.APH_MODE_MEGAKERNEL)):
+            srv = Server(r, hbm_budget=int(budget_gib * (1 << 30)), mode=mode, keep_results=False)
+            r.reset_stats()
+            t = time.perf_counter()
+            tickets = srv.replay(js, arr)
+            dt = time.perf_counter() - t
+            lat = np.array([srv.status(tk).t_done - srv.status(tk).t_arrival for tk in tickets])
+            resident = srv.usage()[0]
+            srv.close()
+            res[name] = {"p50_s": round(float(np.percentile(lat, 50)), 3),
+                         "p99_s": round(float(np.percentile(lat, 99)), 3),
+                         "jobs_per_s": round(jobs / dt, 2), "mrays_s": round(rays_of(r.stats()) / dt / 1e6, 2),
+                         "wall_s": round(dt, 2)}
+            log(f"serving rate {rate}/s {name}: {res[name]}")
+        out["rates"][str(rate)] = res
+    out["resident_scene_bytes"] = int(resident)
+    return out
+
+
+def equal_time_rmse(r, builder, width=480, height=270, ref_spp=4096, plain_spp=64):
+    """Equal-time RMSE vs a 4096-spp reference (config2 at reduced resolution): the plain
+    megakernel's 64-spp image against the wavefront image with as many samples as it renders
+    in the same time (same sampler, so images differ only by sample count)."""
+    sc = scenes.config2(width=width, height=height)
+    tlas = r.load_scene(sc, builder)
+    ref = r.target(width, height)
+    r.render([r.request(ref, tlas, sc.frame, 0, ref_spp)], capi.APH_MODE_WAVEFRONT)
+    r.wait()
+    ref_img = r.readback(ref, width, height)[..., :3].astype(np.float64)
+    tp = r.target(width, height)
+    t = time.perf_counter()
+    r.render([r.request(tp, tlas, sc.frame, 0, plain_spp)], capi.APH_MODE_MEGAKERNEL)
+    r.wait()
+    t_plain = time.perf_counter() - t
+    tw = r.target(width, height)
+    t = time.perf_counter()
+    r.render([r.request(tw, tlas, sc.frame, 0, plain_spp)], capi.APH_MODE_WAVEFRONT)
+    r.wait()
+    t_wf = time.perf_counter() - t
+    spp_eq = max(plain_spp, int(plain_spp * t_plain / t_wf))
+    if spp_eq > plain_spp:
+        r.render([r.request(tw, tlas, sc.frame, plain_spp, spp_eq - plain_spp)], capi.APH_MODE_WAVEFRONT)
+        r.wait()
+
+    def rmse(tgt):
+        return float(np.sqrt(np.mean((r.readback(tgt, width, height)[..., :3] - ref_img) ** 2)))
+    a, b = rmse(tp), rmse(tw)
+    return {"resolution": f"{width}x{height}", "reference_spp": ref_spp, "time_s": round(t_plain, 3),
+            "plain_spp": plain_spp, "plain_rmse": round(a, 5), "wavefront_spp": spp_eq, "wavefront_rmse": round(b, 5),
+            "rmse_ratio": round(a / b, 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -179,6 +312,9 @@ def main():
     ap.add_argument("--subdiv", type=int, default=4, help="4: 1.02M instanced tris; 5: 4.1M")
     ap.add_argument("--mode", choices=["megakernel", "wavefront"], default="wavefront")
     ap.add_argument("--builder", choices=["sah", "lbvh"], default="sah", help="BLAS builder")
+    ap.add_argument("--sections", default="literal,c3,serving,rmse",
+                    help="extra measurements after the headline line's frame: literal (config2 at subdiv 5), "
+                         "c3 (64 cameras), serving (Poisson replay), rmse (equal-time RMSE); '' for none")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 counter passes")
     ap.add_argument("--cpu-baseline-child", action="store_true",
@@ -306,6 +442,18 @@ def main():
         out["config"]["plain_megakernel_mrays_s"] = round(
             (pst["extension_rays"] + pst["shadow_rays"]) / t_plain / 1e6, 2)
         log(f"plain megakernel frame: {t_plain * 1e3:.1f} ms")
+    sections = [s for s in args.sections.split(",") if s]
+    if "literal" in sections:
+        out["config"]["config2_subdiv5"] = literal_config2(r, builder, sc.spp)
+        log(f"config2 subdiv 5: {out['config']['config2_subdiv5']}")
+    if "c3" in sections:
+        out["c3_cameras"] = c3_cameras(r, builder)
+        log(f"c3: {out['c3_cameras']}")
+    if "serving" in sections:
+        out["serving"] = serving(r, builder)
+    if "rmse" in sections:
+        out["equal_time_rmse"] = equal_time_rmse(r, builder)
+        log(f"rmse: {out['equal_time_rmse']}")
     r.close()
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle on host cores, child process)...")

@@ -311,6 +311,10 @@ int aph_server_step(aph_server_t s, uint32_t* active, uint32_t* pending);
 int aph_server_status(aph_server_t s, uint64_t ticket, aph_job_status* st);
 /* RGBA32F image of a finished job kept with keep_results. */
 int aph_server_result(aph_server_t s, uint64_t ticket, float* dst, size_t bytes);
+/* Device memory charged to the budget (resident scenes, admitted job buffers)
+ * and the jobs running / waiting. */
+int aph_server_usage(aph_server_t s, uint64_t* resident_bytes, uint64_t* job_bytes, uint32_t* active,
+                     uint32_t* pending);
 /* Forgets a finished or waiting job and frees its buffer. */
 int aph_server_release(aph_server_t s, uint64_t ticket);
 /* Replays an arrival trace in real time: job i is submitted arrival[i] seconds
