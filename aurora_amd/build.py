@@ -14,7 +14,8 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libaph.so")
 ARCH = os.environ.get("APH_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["api.cpp", "server.cpp", "alias.cpp", "bvh.hip", "sah.hip", "kernels.hip", "wavefront.hip"]
+SOURCES = ["api.cpp", "server.cpp", "guides.cpp", "alias.cpp", "bvh.hip", "sah.hip", "kernels.hip", "wavefront.hip",
+           "temporal.hip"]
 
 DEVICE_FLAGS = [
     f"--offload-arch={ARCH}",

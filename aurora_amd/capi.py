@@ -110,6 +110,18 @@ class JobStatus(C.Structure):
 
 APH_JOB_PENDING, APH_JOB_ACTIVE, APH_JOB_DONE, APH_JOB_FAILED = 0, 1, 2, 3
 
+
+class AovRequest(C.Structure):
+    _fields_ = [("aov", C.c_uint64), ("tlas", C.c_uint64), ("frame", Frame), ("view_proj", C.c_float * 16),
+                ("view_proj_prev", C.c_float * 16), ("inv_view_prev", C.c_float * 16), ("sky_view_z", C.c_float),
+                ("sample_index", C.c_uint32), ("seed_offset", C.c_uint32), ("_pad", C.c_uint32)]
+
+
+class TaaRequest(C.Structure):
+    _fields_ = [("current", C.c_uint64), ("aov", C.c_uint64), ("history_in", C.c_uint64),
+                ("history_out", C.c_uint64), ("is_reset", C.c_int32), ("blend_alpha_min", C.c_float),
+                ("blend_alpha_max", C.c_float), ("speed_reference", C.c_float)]
+
 _P = C.c_void_p
 _u32 = C.c_uint32
 _u64 = C.c_uint64
@@ -148,6 +160,12 @@ SIGNATURES = {
     "aph_timing_enable": (_i, [_P, _i]),
     "aph_timing_get": (_i, [_P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(_u64)]),
     "aph_timing_reset": (_i, [_P]),
+    "aph_aov_create": (_i, [_P, _u32, _u32, C.POINTER(_u64)]),
+    "aph_aov_destroy": (_i, [_P, _u64]),
+    "aph_aov_readback": (_i, [_P, _u64, _P, _P, _P, _P]),
+    "aph_render_aovs": (_i, [_P, C.POINTER(AovRequest)]),
+    "aph_temporal_resolve": (_i, [_P, C.POINTER(TaaRequest)]),
+    "aph_halton_jitter": (None, [_u32, _u32, _P]),
     "aph_server_create": (_i, [_P, C.POINTER(ServerConfig), C.POINTER(_P)]),
     "aph_server_destroy": (None, [_P]),
     "aph_server_clock": (C.c_double, [_P]),

@@ -69,6 +69,11 @@ struct Target {
     DevBuf accum;
 };
 
+struct AovSet {
+    uint32_t w = 0, h = 0;
+    DevBuf depth, normalRoughness, baseColorMetalness, motion;
+};
+
 struct Timing {
     double ms = 0.0;
     uint64_t launches = 0;
@@ -83,6 +88,7 @@ struct aph_ctx {
     using Tlas = aph::Tlas;
     using Target = aph::Target;
     using Timing = aph::Timing;
+    using AovSet = aph::AovSet;
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
@@ -90,6 +96,7 @@ struct aph_ctx {
     std::map<uint64_t, std::unique_ptr<Mesh>> meshes;
     std::map<uint64_t, std::unique_ptr<Tlas>> tlases;
     std::map<uint64_t, std::unique_ptr<Target>> targets;
+    std::map<uint64_t, std::unique_ptr<AovSet>> aovs;
     DevBuf mats;
     uint32_t nmat = 0;
     DevBuf envLight, envAlias, envBg;

@@ -155,6 +155,23 @@ def look_at_frame(eye, target, up, fov_y_deg, aspect, trace_depth=5, max_luminan
     return fr
 
 
+def view_proj(frame, near=0.1, far=1000.0):
+    """Row-major view-projection (clip = M (p, 1)) of an aph_frame's perspective camera:
+    the glm::perspective * glm::lookAt product FrameData.cameraViewProj holds
+    (RendererBase.cpp:237)."""
+    inv = np.array(frame.inv_view, dtype=np.float64).reshape(4, 4)  # rows right, up, back, eye
+    c2w = np.eye(4)
+    c2w[:3, 0], c2w[:3, 1], c2w[:3, 2], c2w[:3, 3] = inv[0, :3], inv[1, :3], inv[2, :3], inv[3, :3]
+    view = np.linalg.inv(c2w)
+    p = np.zeros((4, 4))
+    p[0, 0] = 2.0 / frame.view_size[0]
+    p[1, 1] = 2.0 / frame.view_size[1]
+    p[2, 2] = -(far + near) / (far - near)
+    p[2, 3] = -2.0 * far * near / (far - near)
+    p[3, 2] = -1.0
+    return (p @ view).astype(np.float32)
+
+
 # ---------------------------------------------------------------- environments
 
 

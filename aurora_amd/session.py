@@ -61,6 +61,43 @@ class Renderer:
     def build_blas(self, mesh, builder=capi.APH_BUILDER_LBVH):
         self._check(self.lib.aph_blas_build(self.ctx, mesh, builder))
 
+    # ---- guide images and temporal resolve
+    def aov(self, w, h):
+        a = C.c_uint64()
+        self._check(self.lib.aph_aov_create(self.ctx, w, h, C.byref(a)))
+        return a.value
+
+    def render_aovs(self, aov, tlas, frame, view_proj, view_proj_prev, inv_view_prev, sky_view_z=-2000.0,
+                    sample_index=0, seed_offset=0):
+        r = capi.AovRequest()
+        r.aov, r.tlas, r.frame = aov, tlas, frame
+        for name, m in (("view_proj", view_proj), ("view_proj_prev", view_proj_prev),
+                        ("inv_view_prev", inv_view_prev)):
+            getattr(r, name)[:] = [float(v) for v in np.asarray(m, dtype=np.float32).reshape(16)]
+        r.sky_view_z, r.sample_index, r.seed_offset = sky_view_z, sample_index, seed_offset
+        self._check(self.lib.aph_render_aovs(self.ctx, C.byref(r)))
+
+    def aov_readback(self, aov, w, h):
+        d = np.zeros((h, w), dtype=np.float32)
+        n, b, m = (np.zeros((h, w, 4), dtype=np.float32) for _ in range(3))
+        self._check(self.lib.aph_aov_readback(self.ctx, aov, d.ctypes.data, n.ctypes.data, b.ctypes.data,
+                                              m.ctypes.data))
+        return d, n, b, m
+
+    def temporal_resolve(self, current, aov, history_in, history_out, is_reset=False, alpha_min=1.0 / 16.0,
+                         alpha_max=0.6, speed_reference=6.0):
+        r = capi.TaaRequest(current, aov, history_in, history_out, int(is_reset), alpha_min, alpha_max,
+                            speed_reference)
+        self._check(self.lib.aph_temporal_resolve(self.ctx, C.byref(r)))
+
+    def target_upload(self, target, rgba):
+        """Writes an RGBA32F image into a target (test and tool helper, through the device pointer)."""
+        p = C.c_void_p()
+        self._check(self.lib.aph_target_device_ptr(self.ctx, target, C.byref(p)))
+        hip = C.CDLL("libamdhip64.so")
+        a = np.ascontiguousarray(rgba, dtype=np.float32)
+        assert hip.hipMemcpy(p, a.ctypes.data_as(C.c_void_p), C.c_size_t(a.nbytes), 1) == 0  # host to device
+
     def blas_export(self, mesh):
         """(nodes float32[N, 16], tris float32[T, 12], depth) of a built BLAS."""
         cnt, depth = C.c_uint32(), C.c_uint32()

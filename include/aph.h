@@ -265,6 +265,52 @@ int aph_timing_enable(aph_ctx_t ctx, int enable);
 int aph_timing_get(aph_ctx_t ctx, const char* kernel, double* total_ms, uint64_t* launches);
 int aph_timing_reset(aph_ctx_t ctx);
 
+/* ---- guide images (AOVs) and temporal resolve: the DirectX backend's denoising
+ * guides gDepthView, gNormalRoughness, gBaseColorMetalness, gMotionVectors
+ * (GlobalPipelineState.slang:110-172, written by the primary hit/miss,
+ * HitShaderEntryPoints.slang:150-215, MainEntryPoints.slang:385-405) and
+ * TemporalResolve.hlsl:156-249.  An AOV set holds four planes per pixel:
+ * view depth (float), normal+roughness, base colour+metalness and motion
+ * (float4 each). ---- */
+int aph_aov_create(aph_ctx_t ctx, uint32_t width, uint32_t height, uint64_t* aov);
+int aph_aov_destroy(aph_ctx_t ctx, uint64_t aov);
+/* Copies planes to host memory; any pointer may be NULL. */
+int aph_aov_readback(aph_ctx_t ctx, uint64_t aov, float* depth, float* normal_roughness,
+                     float* base_color_metalness, float* motion);
+
+typedef struct {
+    uint64_t aov;                 /* output set; its size is the image size */
+    uint64_t tlas;
+    aph_frame frame;              /* this frame's camera (frame.jitter with temporal_jitter) */
+    float view_proj[16];          /* FrameData.cameraViewProj, row-major: clip = M * (p, 1) */
+    float view_proj_prev[16];     /* cameraViewProjPrev */
+    float inv_view_prev[16];      /* cameraInvViewPrev, the layout of frame.inv_view */
+    float sky_view_z;             /* FrameData.skyViewZ (RendererBase.cpp:258) */
+    uint32_t sample_index;        /* the camera ray of this sample index (plus seed_offset) */
+    uint32_t seed_offset;
+    uint32_t _pad;
+} aph_aov_request;
+/* Traces the camera ray of one sample per pixel and writes the guides of its
+ * first segment (hit: view depth of origin + t*dir, normalized shading normal,
+ * max(specular roughness, 0.05), base colour, metalness, 2.5D motion; miss:
+ * sky_view_z, (0,0,0,0.1) twice, direction-only motion). */
+int aph_render_aovs(aph_ctx_t ctx, const aph_aov_request* req);
+
+typedef struct {
+    uint64_t current;             /* target: this frame's accumulation (gAccumulation) */
+    uint64_t aov;                 /* this frame's motion vectors and view depth */
+    uint64_t history_in;          /* target: last resolved frame */
+    uint64_t history_out;         /* target written (gTAAHistoryA/B ping-pong) */
+    int32_t is_reset;             /* discard history this frame */
+    float blend_alpha_min;        /* PTRenderer.cpp:2043: 1/16 */
+    float blend_alpha_max;        /* :2044: 0.6 */
+    float speed_reference;        /* :2046: 6 pixels per frame */
+} aph_taa_request;
+int aph_temporal_resolve(aph_ctx_t ctx, const aph_taa_request* req);
+/* Sub-pixel camera jitter of a frame, Halton(2,3) centred on the pixel:
+ * getTemporalJitter, PTRenderer.cpp:159-174 (phase_count 0: no wrap). */
+void aph_halton_jitter(uint32_t frame_index, uint32_t phase_count, float out[2]);
+
 /* ---- serving loop: continuous batching of render jobs against device-resident
  * TLASes.  No reference counterpart: Aurora renders one request at a time
  * (HGIRenderer::render, HGIRenderer.cpp:477-589, progressive

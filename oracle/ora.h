@@ -204,6 +204,21 @@ void ora_pixel_sample(const ora_scene* s, const ora_material_constants* mats, ui
                       ora_counters* counters);
 
 /* Post-processing, PostProcessing.glsl:37-64. */
+/* Temporal resolve (TemporalResolve.hlsl:156-249) over RGBA images. */
+void ora_temporal_resolve(const float* accum, const float* depth, const float* motion, const float* hist_in,
+                          float* hist_out, uint32_t W, uint32_t H, int is_reset, float alpha_min, float alpha_max,
+                          float speed_reference);
+float ora_view_z(const float inv_view[16], const float p[3]);
+void ora_screen_motion(const float view_proj[16], const float view_proj_prev[16], const float p[4], uint32_t W,
+                       uint32_t H, float out[2]);
+void ora_halton_jitter(uint32_t frame_index, uint32_t phase_count, float out[2]);
+/* Guides of the first segment of one camera ray (the AOV pass): out = depth,
+ * normal.xyz, roughness, base colour.xyz, metalness, motion.xyzw (13 floats). */
+void ora_aov_pixel(const ora_scene* s, const ora_material_constants* mats, uint32_t nmat,
+                   const ora_frame_params* fp, uint32_t W, uint32_t H, uint32_t x, uint32_t y,
+                   uint32_t sample_index, const float view_proj[16], const float view_proj_prev[16],
+                   const float inv_view_prev[16], float sky_view_z, float out[13]);
+
 void ora_post(const float* accum, uint32_t n, const float brightness[3], int tone_map, int gamma,
               float* out);
 

@@ -491,3 +491,47 @@ void ora_render(const ora_scene* s, const ora_material_constants* mats, uint32_t
     free(jobs);
     if (counters) *counters = tot;
 }
+
+/* The AOV pass: guides of the first segment of one camera ray, as the primary
+ * closest-hit / miss shaders write them (HitShaderEntryPoints.slang:184-214,
+ * MainEntryPoints.slang:391-405; motion vectors GlobalPipelineState.slang:126-172). */
+void ora_aov_pixel(const ora_scene* s, const ora_material_constants* mats, uint32_t nmat,
+                   const ora_frame_params* fp, uint32_t W, uint32_t H, uint32_t x, uint32_t y,
+                   uint32_t sample_index, const float view_proj[16], const float view_proj_prev[16],
+                   const float inv_view_prev[16], float sky_view_z, float out[13])
+{
+    uint32_t rng[2];
+    ora_pcg2d_init(sample_index, W, x, y, rng);
+    v3 org, dir;
+    camera_ray(fp, x, y, W, H, rng, &org, &dir);
+    ora_ray r;
+    v3store(r.org, org);
+    v3store(r.dir, dir);
+    r.tmin = 0.0f;
+    r.tmax = INFINITY;
+    ora_hit h;
+    ora_trace_closest_one(s, &r, &h);
+    if (h.inst == 0xFFFFFFFFu) {
+        float d4[4] = { dir.x, dir.y, dir.z, 0.0f }, mv[2];
+        ora_screen_motion(view_proj, view_proj_prev, d4, W, H, mv);
+        float o[13] = { sky_view_z, 0, 0, 0, 0.1f, 0, 0, 0, 0.1f, mv[0], mv[1], 0, 0 };
+        memcpy(out, o, sizeof(o));
+        return;
+    }
+    float pos[3] = { org.x + dir.x * h.t, org.y + dir.y * h.t, org.z + dir.z * h.t };
+    float vz = ora_view_z(fp->inv_view, pos);
+    ora_shading sh;
+    shading_data(s, h.inst, h.prim, h.u, h.v, &sh);
+    uint32_t mi = s->inst[h.inst].material;
+    if (mi >= nmat) mi = 0;
+    ora_material m;
+    ora_material_from_constants(&mats[mi], &m);
+    v3 n = normalize(sh.normal);
+    float p4[4] = { pos[0], pos[1], pos[2], 1.0f }, mv[2];
+    ora_screen_motion(view_proj, view_proj_prev, p4, W, H, mv);
+    float prevZ = ora_view_z(inv_view_prev, pos);
+    float rough = m.specularRoughness > 0.05f ? m.specularRoughness : 0.05f;
+    float o[13] = { vz, n.x, n.y, n.z, rough, m.baseColor.x, m.baseColor.y, m.baseColor.z, m.metalness,
+                    mv[0], mv[1], vz - prevZ, 0.0f };
+    memcpy(out, o, sizeof(o));
+}

@@ -76,6 +76,11 @@ def load():
         "ora_render": (None, [P, P, u32, P, P, u32, u32, u32, u32, P, u32, i, P, P]),
         "ora_pixel_sample": (None, [P, P, u32, P, P, u32, u32, u32, u32, u32, P, P]),
         "ora_post": (None, [P, u32, P, i, i, P]),
+        "ora_temporal_resolve": (None, [P, P, P, P, P, u32, u32, i, C.c_float, C.c_float, C.c_float]),
+        "ora_view_z": (C.c_float, [P, P]),
+        "ora_screen_motion": (None, [P, P, P, u32, u32, P]),
+        "ora_halton_jitter": (None, [u32, u32, P]),
+        "ora_aov_pixel": (None, [P, P, u32, P, u32, u32, u32, u32, u32, P, P, P, C.c_float, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -146,6 +151,21 @@ class OracleScene:
             self._env = make_env(self.scene.env or {})
         return self._env
 
+    def aovs(self, frame, width, height, sample_index, view_proj, view_proj_prev, inv_view_prev, sky_view_z):
+        """Guide images of one camera ray per pixel: depth [H,W], normal_roughness,
+        base_color_metalness, motion [H,W,4]."""
+        lib = load()
+        mats = (type(self.scene.materials[0]) * len(self.scene.materials))(*self.scene.materials)
+        vp, vpp, ivp = (np.ascontiguousarray(m, dtype=np.float32).reshape(16)
+                        for m in (view_proj, view_proj_prev, inv_view_prev))
+        out = np.zeros((height, width, 13), dtype=np.float32)
+        for y in range(height):
+            for x in range(width):
+                lib.ora_aov_pixel(self.ptr, C.cast(mats, C.c_void_p), len(self.scene.materials), C.byref(frame),
+                                  width, height, x, y, sample_index, _p(vp), _p(vpp), _p(ivp), sky_view_z,
+                                  out[y, x].ctypes.data_as(C.c_void_p))
+        return out[..., 0], out[..., 1:5], out[..., 5:9], out[..., 9:13]
+
     def render(self, frame, width, height, sample_start, sample_count, pixels=None, threads=8, accum=None):
         lib = load()
         mats = (type(self.scene.materials[0]) * len(self.scene.materials))(*self.scene.materials)
@@ -158,6 +178,17 @@ class OracleScene:
                        C.byref(frame), width, height, sample_start, sample_count, _p(pix),
                        0 if pix is None else len(pix), threads, _p(accum), C.byref(cnt))
         return accum, {"extension_rays": cnt.extension_rays, "shadow_rays": cnt.shadow_rays}
+
+
+def temporal_resolve(accum, depth, motion, hist_in, is_reset=0, alpha_min=1.0 / 16.0, alpha_max=0.6,
+                     speed_reference=6.0):
+    """TemporalResolve.hlsl over float32 images: accum/motion/hist [H,W,4], depth [H,W]."""
+    h, w = depth.shape
+    args = [np.ascontiguousarray(a, dtype=np.float32) for a in (accum, depth, motion, hist_in)]
+    out = np.zeros((h, w, 4), dtype=np.float32)
+    load().ora_temporal_resolve(*[_p(a) for a in args], _p(out), w, h, int(is_reset), alpha_min, alpha_max,
+                                speed_reference)
+    return out
 
 
 def alias_build(rgb):
