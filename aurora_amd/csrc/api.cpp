@@ -331,6 +331,8 @@ int aph_tlas_build(aph_ctx_t c, const aph_instance* inst, uint32_t n, uint64_t* 
         sh.idx = m->idx.as<uint32_t>();
         sh.material = in.material;
         sh.flags = in.flags;
+        sh.layerFirst = 0;
+        sh.layerCount = 0;
         if (m->nt == 0) continue;
         // condition number bound (infinity norm) of the linear part
         double na = 0.0, nb = 0.0;
@@ -359,6 +361,8 @@ int aph_tlas_build(aph_ctx_t c, const aph_instance* inst, uint32_t n, uint64_t* 
     uint32_t nv = (uint32_t)blo.size();
     t->nvalid = nv;
     HIPCALL(c, t->trav.alloc(sizeof(InstTrav) * (n ? n : 1)), "alloc instances");
+    t->shadeHost = shade;
+    HIPCALL(c, t->layerMats.alloc(16), "alloc layers");
     HIPCALL(c, t->shade.alloc(sizeof(InstShade) * (n ? n : 1)), "alloc instances");
     HIPCALL(c, hipMemcpyAsync(t->trav.p, trav.data(), sizeof(InstTrav) * (n ? n : 1), hipMemcpyHostToDevice, c->stream),
             "copy instances");
@@ -400,6 +404,36 @@ int aph_tlas_build(aph_ctx_t c, const aph_instance* inst, uint32_t n, uint64_t* 
     uint64_t h = c->nextHandle++;
     c->tlases[h] = std::move(t);
     *tlas = h;
+    return APH_OK;
+}
+
+int aph_tlas_set_layers(aph_ctx_t c, uint64_t tlas, const uint32_t* offsets, const uint32_t* materials)
+{
+    int rc = enter(c);
+    if (rc) return rc;
+    Tlas* t = lookup(c->tlases, tlas);
+    if (!t) return fail(c, APH_E_INVALID, "unknown tlas handle %llu", (unsigned long long)tlas);
+    if (!offsets) return fail(c, APH_E_INVALID, "null layer offsets");
+    uint32_t n = t->n;
+    if (offsets[0] != 0) return fail(c, APH_E_INVALID, "layer offsets must start at 0");
+    for (uint32_t i = 0; i < n; i++)
+        if (offsets[i + 1] < offsets[i]) return fail(c, APH_E_INVALID, "layer offsets decrease at instance %u", i);
+    uint32_t total = offsets[n];
+    if (total && !materials) return fail(c, APH_E_INVALID, "null layer materials");
+    if (total > (1u << 24)) return fail(c, APH_E_INVALID, "too many layers");
+    // Indices are resolved at render time against the material table then current;
+    // one beyond it reads material 0, like an instance's own material index.
+    HIPCALL(c, hipStreamSynchronize(c->stream), "layers");
+    HIPCALL(c, t->layerMats.alloc((size_t)(total ? total : 1) * 4), "alloc layers");
+    if (total) HIPCALL(c, hipMemcpy(t->layerMats.p, materials, (size_t)total * 4, hipMemcpyHostToDevice), "layers");
+    // Instance k of the TLAS is the k-th instance given to aph_tlas_build.
+    for (size_t k = 0; k < t->shadeHost.size() && k < n; k++) {
+        t->shadeHost[k].layerFirst = offsets[k];
+        t->shadeHost[k].layerCount = offsets[k + 1] - offsets[k];
+    }
+    HIPCALL(c, hipMemcpy(t->shade.p, t->shadeHost.data(), sizeof(InstShade) * t->shadeHost.size(),
+                         hipMemcpyHostToDevice),
+            "layers");
     return APH_OK;
 }
 

@@ -60,6 +60,8 @@ struct Tlas {
     uint32_t n = 0, nvalid = 0;
     DevBuf nodes, trav, shade;
     TraceScene sc{};
+    DevBuf layerMats;  // material indices of the instances' layers (aph_tlas_set_layers)
+    std::vector<InstShade> shadeHost;
     uint32_t depth = 0;
     size_t bytes = 0;  // device memory of the TLAS and the meshes/BLASes it references
 };
@@ -159,6 +161,7 @@ inline RenderScene renderScene(const aph_ctx* c, const Tlas* tl)
     rs.ishade = tl->shade.as<InstShade>();
     rs.mats = c->mats.as<DevMaterial>();
     rs.nmat = c->nmat;
+    rs.layerMats = tl->layerMats.as<uint32_t>();
     rs.pad_ = 0;
     rs.env = c->env;
     return rs;

@@ -15,10 +15,30 @@ struct RenderScene {
     TraceScene sc;
     const InstShade* ishade;
     const DevMaterial* mats;
+    const uint32_t* layerMats;  // material indices of all instances' layers
     uint32_t nmat;
     uint32_t pad_;
     DevEnv env;
 };
+
+// The material a hit shades with: material layers (HitShaderEntryPoints.slang:84-113,
+// DirectX semantics) are tried from the top layer down to the base material; each
+// draws its opacity coin, and a layer the coin passes through scales the throughput
+// by transparency / P.  Without layers this is the single coin of the HGI build.
+// Returns the hit layer's material, or null when every layer was passed through.
+__device__ __forceinline__ const DevMaterial* selectLayer(const RenderScene& rs, const InstShade& ins, Rng& rng,
+                                                          f3& contrib, float& P)
+{
+    for (int li = (int)ins.layerCount; li >= 0; li--) {
+        uint32_t mi = li == 0 ? ins.material : rs.layerMats[ins.layerFirst + (uint32_t)li - 1u];
+        const DevMaterial* m = &rs.mats[mi < rs.nmat ? mi : 0u];
+        f3 transparency = splat(1.0f) - m->opacity;
+        P = luminance(transparency);
+        if (random2D(rng).x >= P) return m;
+        contrib = contrib * (transparency / P);
+    }
+    return nullptr;
+}
 
 struct PathCounters {
     uint32_t ext, shadow;
@@ -129,12 +149,10 @@ __device__ float4 pathSample(const RenderScene& rs, const aph_frame& fp, uint32_
                 const InstShade& ins = rs.ishade[hit.inst];
                 Shading sh;
                 computeShading(ins, hit.prim, (float)hit.u, (float)hit.v, sh);
-                uint32_t mi = ins.material < rs.nmat ? ins.material : 0u;
-                const DevMaterial& m = rs.mats[mi];
-                f3 transparency = splat(1.0f) - m.opacity;
-                float P = luminance(transparency);
-                bool hitLayer = random2D(rng).x >= P;
-                if (hitLayer) {
+                float P;
+                const DevMaterial* mp = selectLayer(rs, ins, rng, contrib, P);
+                if (mp) {
+                    const DevMaterial& m = *mp;
                     direct = direct + m.emissionColor * m.emission * lerp(splat(1.0f), m.coatColor, m.coat);
                     f3 dirShadow = splat(0.0f), dirColor = splat(0.0f);
                     bool hasDir = false;
@@ -221,8 +239,7 @@ __device__ float4 pathSample(const RenderScene& rs, const aph_frame& fp, uint32_
                     if (nextRayType == kNoSecondaryRay) pathEnded = true;
                     isPrimary = false;
                 } else {
-                    contrib = contrib * (transparency / P);
-                    // continue straight through, as if nothing had been hit
+                    // every layer passed through: continue straight on, as if nothing had been hit
                     dir = -V;
                     org = sh.geomPosition;
                     skip = true;

@@ -50,6 +50,8 @@ struct alignas(16) InstShade {
     const uint32_t* idx;
     uint32_t material;
     uint32_t flags;
+    uint32_t layerFirst;  // material layers of the instance: layerMats[layerFirst + k], k < layerCount,
+    uint32_t layerCount;  // bottom (k = 0, just above the base) to top
 };
 
 struct TraceScene {

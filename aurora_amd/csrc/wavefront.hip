@@ -356,18 +356,15 @@ __global__ __launch_bounds__(kWB) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
                 const InstShade& ins = rq.rs.ishade[inst];
                 Shading shd;
                 computeShading(ins, __float_as_uint(ha.z), ha.w, Q.hitV[i], shd);
-                uint32_t mi = ins.material < rq.rs.nmat ? ins.material : 0u;
-                const DevMaterial& m = rq.rs.mats[mi];
-                f3 transparency = splat(1.0f) - m.opacity;
-                float Pt = luminance(transparency);
-                bool hitLayer = random2D(rng).x >= Pt;
-                if (!hitLayer) {
-                    contrib = contrib * (transparency / Pt);
+                float Pt;
+                const DevMaterial* mp = selectLayer(rq.rs, ins, rng, contrib, Pt);
+                if (!mp) {
                     nOrg = shd.geomPosition;
                     nDir = dir;
                     f.event(kEvSkip);
                     contPath = true;
                 } else {
+                    const DevMaterial& m = *mp;
                     f3 direct = m.emissionColor * m.emission * lerp(splat(1.0f), m.coatColor, m.coat);
                     f3 so = dot(V, shd.normal) < 0.0f ? shd.geomPosition : shd.position;
                     f.primary(false);

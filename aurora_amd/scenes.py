@@ -227,6 +227,7 @@ class Scene:
         self.instances = None   # INSTANCE_DTYPE array; mesh field = 0-based mesh index
         self.materials = []     # list of capi.Material
         self.env = None         # dict
+        self.layers = None      # per instance: material indices of its layers, bottom to top
         self.frame = None       # capi.Frame
         self.width = 0
         self.height = 0
@@ -316,6 +317,12 @@ def config2(seed=2, subdiv=4, width=1920, height=1080, spp=64, trace_depth=5, en
                                       float(rng.uniform(0.05, 0.8)), transmission=float(rng.uniform(0, 1)),
                                       coat=float(rng.uniform(0, 1)), coat_roughness=float(rng.uniform(0.02, 0.3))))
     sc.instances = _instances(inst)
+    # one decal layer (DirectX layer semantics): a partly opaque coat over every
+    # fourth sphere instance; constant opacity stands in for a decal's texture
+    sc.materials.append(_material((0.9, 0.35, 0.1), 0.0, 0.3))
+    sc.materials[-1].opacity[:] = [0.6, 0.6, 0.6]
+    decal = len(sc.materials) - 1
+    sc.layers = [[decal] if (k > 0 and k % 4 == 0) else [] for k in range(len(inst))]
     sky = sun_sky(env_w, env_h, rng)
     sc.env = dict(light_rgb=sky, background_rgb=sky)
     sc.width, sc.height, sc.spp = width, height, spp

@@ -115,6 +115,17 @@ class Renderer:
         self._check(self.lib.aph_tlas_build(self.ctx, capi.ptr(inst), len(inst), C.byref(h)))
         return h.value
 
+    def set_layers(self, tlas, layers, n):
+        """Per-instance material layer lists, bottom to top (aph_tlas_set_layers)."""
+        off = np.zeros(n + 1, dtype=np.uint32)
+        flat = []
+        for i in range(n):
+            ls = layers[i] if i < len(layers) and layers[i] else []
+            flat.extend(ls)
+            off[i + 1] = len(flat)
+        mats = np.ascontiguousarray(flat if flat else [0], dtype=np.uint32)
+        self._check(self.lib.aph_tlas_set_layers(self.ctx, tlas, off.ctypes.data, mats.ctypes.data))
+
     def materials(self, mats):
         arr = (capi.Material * len(mats))(*mats)
         self._check(self.lib.aph_materials_upload(self.ctx, C.cast(arr, C.c_void_p), len(mats)))
@@ -138,6 +149,8 @@ class Renderer:
             inst["mesh"] = np.array([handles[i] for i in sc.instances["mesh"]], dtype=np.uint32)
             inst["material"] = inst["material"] + len(mats)
             tl.append(self.tlas(inst))
+            if getattr(sc, "layers", None) is not None:
+                self.set_layers(tl[-1], [[m + len(mats) for m in (ls or [])] for ls in sc.layers], len(inst))
             mats.extend(sc.materials)
         self.materials(mats)
         self.environment(scs[0].env or {})
@@ -153,6 +166,8 @@ class Renderer:
         inst = sc.instances.copy()
         inst["mesh"] = np.array([handles[i] for i in sc.instances["mesh"]], dtype=np.uint32)
         t = self.tlas(inst)
+        if getattr(sc, "layers", None) is not None:
+            self.set_layers(t, sc.layers, len(inst))
         self.materials(sc.materials)
         self.environment(sc.env or {})
         return t

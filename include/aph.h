@@ -197,6 +197,14 @@ int aph_mesh_upload(aph_ctx_t ctx, const float* pos, uint32_t nv, const uint32_t
  * binned SAH (16 bins x 3 axes), up to 4 triangles per leaf (fastest trace).
  * The TLAS is always a binned-SAH tree over the instance world boxes. */
 int aph_blas_build(aph_ctx_t ctx, uint64_t mesh, int builder);
+/* Material layers of each instance of a TLAS: the DirectX backend's layer loop
+ * (HitShaderEntryPoints.slang:84-113) over IScene::addInstance's
+ * LayerDefinitions.  Instance i's layers are materials[offsets[i] ..
+ * offsets[i+1]) (indices into the material table), ordered from just above the
+ * base material to the top; a hit tries the top layer first, each layer's
+ * opacity coin passing the ray on (throughput *= transparency / P) or stopping
+ * at it.  Layer UV geometry only matters for textures, not supported here. */
+int aph_tlas_set_layers(aph_ctx_t ctx, uint64_t tlas, const uint32_t* offsets, const uint32_t* materials);
 /* Copies a built BLAS to host memory: *node_count 64-byte nodes (two child
  * boxes; child references in a.w / b.w, leaf = bit 31 | (count-1) << 27 |
  * first) and the triangles in leaf order (48 bytes, original primitive index in

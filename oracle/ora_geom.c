@@ -232,6 +232,17 @@ ora_scene* ora_scene_create(const ora_mesh* meshes, uint32_t nmesh, const ora_in
     return s;
 }
 
+void ora_scene_set_layers(ora_scene* s, const uint32_t* offsets, const uint32_t* materials)
+{
+    free(s->layer_off);
+    free(s->layer_mat);
+    s->layer_off = (uint32_t*)malloc(sizeof(uint32_t) * (s->ninst + 1));
+    memcpy(s->layer_off, offsets, sizeof(uint32_t) * (s->ninst + 1));
+    uint32_t total = offsets[s->ninst];
+    s->layer_mat = (uint32_t*)malloc(sizeof(uint32_t) * (total ? total : 1));
+    if (total) memcpy(s->layer_mat, materials, sizeof(uint32_t) * total);
+}
+
 void ora_scene_destroy(ora_scene* s)
 {
     if (!s) return;
@@ -241,6 +252,8 @@ void ora_scene_destroy(ora_scene* s)
     }
     free(s->tlas.nodes);
     free(s->tlas.prims);
+    free(s->layer_off);
+    free(s->layer_mat);
     free(s->blas);
     free(s->meshes);
     free(s->inst);

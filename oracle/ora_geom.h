@@ -32,6 +32,8 @@ struct ora_scene {
     ora_bvh* blas;
     ora_bvh tlas;
     double extent;   /* 3 x max |coordinate| of the instance world boxes */
+    uint32_t* layer_off; /* material layers: instance i has layer_mat[layer_off[i] .. layer_off[i+1]) */
+    uint32_t* layer_mat;
 };
 
 void ora_wray_setup(ora_wray* r, const double o[3], const double d[3], double tmin, double tmax);

@@ -198,16 +198,28 @@ static void closest_hit(const pt_ctx* c, payload* pl, v3 rayDir, const ora_hit* 
     v3 V = neg(rayDir);
     ora_shading sh;
     shading_data(c->s, hit->inst, hit->prim, hit->u, hit->v, &sh);
-    uint32_t mi = c->s->inst[hit->inst].material;
-    if (mi >= c->nmat) mi = 0;
+    /* Layer loop, HitShaderEntryPoints.slang:84-113 (DirectX semantics; HGI has no
+     * layers and runs it once): layers from the top down to the base material, each
+     * drawing its opacity coin (:235-240); a layer passed through scales the
+     * throughput by transparency / P (:439). */
+    const ora_scene* s = c->s;
+    uint32_t lfirst = s->layer_off ? s->layer_off[hit->inst] : 0;
+    uint32_t lcount = s->layer_off ? s->layer_off[hit->inst + 1] - lfirst : 0;
     ora_material m;
-    ora_material_from_constants(&c->mats[mi], &m);
-
-    v3 transparency = sub(v3s(1.0f), m.opacity);
-    float P = luminance(transparency);
+    float P = 0.0f;
+    int hitLayer = 0;
     float r[2];
-    ora_random2d(pl->rng, r);
-    int hitLayer = r[0] >= P;
+    for (int li = (int)lcount; li >= 0; li--) {
+        uint32_t mi = li == 0 ? s->inst[hit->inst].material : s->layer_mat[lfirst + (uint32_t)li - 1];
+        if (mi >= c->nmat) mi = 0;
+        ora_material_from_constants(&c->mats[mi], &m);
+        v3 transparency = sub(v3s(1.0f), m.opacity);
+        P = luminance(transparency);
+        ora_random2d(pl->rng, r);
+        hitLayer = r[0] >= P;
+        if (hitLayer) break;
+        pl->rayContribution = mul(pl->rayContribution, divs(transparency, P));
+    }
     v3 dirShadowDir = v3s(0.0f), dirLightColor = v3s(0.0f);
     int hasDirLight = 0;
     if (hitLayer) {
@@ -288,8 +300,6 @@ static void closest_hit(const pt_ctx* c, payload* pl, v3 rayDir, const ora_hit* 
         }
         if (pl->nextRayType == ORA_NO_SECONDARY_RAY) pl->pathEnded = 1;
         pl->isPrimaryRay = 0;
-    } else {
-        pl->rayContribution = mul(pl->rayContribution, divs(transparency, P));
     }
 
     if (!hitLayer) {

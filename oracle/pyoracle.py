@@ -57,6 +57,7 @@ def load():
         "ora_inverse3x4": (i, [P, P]),
         "ora_scene_create": (P, [P, u32, P, u32]),
         "ora_scene_destroy": (None, [P]),
+        "ora_scene_set_layers": (None, [P, P, P]),
         "ora_trace_closest": (None, [P, P, u32, P, i]),
         "ora_trace_closest_brute": (None, [P, P, u32, P]),
         "ora_trace_any": (None, [P, P, u32, P, i]),
@@ -120,6 +121,10 @@ class OracleScene:
         self._keep += [meshes, inst]
         self.ptr = lib.ora_scene_create(C.cast(meshes, C.c_void_p), len(sc.meshes), _p(inst), len(inst))
         self.scene = sc
+        layers = getattr(sc, "layers", None)
+        if layers is not None:
+            off, mats = layer_csr(layers, len(inst))
+            lib.ora_scene_set_layers(self.ptr, _p(off), _p(mats))
         self._env = None
 
     def __del__(self):
@@ -178,6 +183,17 @@ class OracleScene:
                        C.byref(frame), width, height, sample_start, sample_count, _p(pix),
                        0 if pix is None else len(pix), threads, _p(accum), C.byref(cnt))
         return accum, {"extension_rays": cnt.extension_rays, "shadow_rays": cnt.shadow_rays}
+
+
+def layer_csr(layers, n):
+    """(offsets[n+1], materials) of per-instance layer lists (bottom to top)."""
+    off = np.zeros(n + 1, dtype=np.uint32)
+    flat = []
+    for i in range(n):
+        ls = layers[i] if i < len(layers) and layers[i] else []
+        flat.extend(ls)
+        off[i + 1] = len(flat)
+    return off, np.ascontiguousarray(flat if flat else [0], dtype=np.uint32)
 
 
 def temporal_resolve(accum, depth, motion, hist_in, is_reset=0, alpha_min=1.0 / 16.0, alpha_max=0.6,
