@@ -93,6 +93,14 @@ class Stats(C.Structure):
                 ("shadow_inst_visits", C.c_uint64)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("chunk_paths", C.c_uint32), ("trace_blocks_per_cu", C.c_uint32),
+                ("shade_blocks_per_cu", C.c_uint32), ("ms", C.c_float)]
+
+
+APH_LAUNCH_GRAPH, APH_LAUNCH_EAGER = 0, 1
+
+
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget", C.c_uint64), ("max_active", C.c_uint32), ("step_samples", C.c_uint32),
                 ("mode", C.c_int), ("keep_results", C.c_int)]
@@ -161,6 +169,10 @@ SIGNATURES = {
     "aph_timing_enable": (_i, [_P, _i]),
     "aph_timing_get": (_i, [_P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(_u64)]),
     "aph_timing_reset": (_i, [_P]),
+    "aph_set_launch_mode": (_i, [_P, _i]),
+    "aph_get_tuning": (_i, [_P, C.POINTER(Tuning)]),
+    "aph_set_tuning": (_i, [_P, C.POINTER(Tuning)]),
+    "aph_wavefront_autotune": (_i, [_P, _P, _u32, C.POINTER(Tuning)]),
     "aph_aov_create": (_i, [_P, _u32, _u32, C.POINTER(_u64)]),
     "aph_aov_destroy": (_i, [_P, _u64]),
     "aph_aov_readback": (_i, [_P, _u64, _P, _P, _P, _P]),

@@ -690,6 +690,91 @@ int aph_render(aph_ctx_t c, const aph_request* reqs, uint32_t nreq, int mode)
     return APH_OK;
 }
 
+int aph_set_launch_mode(aph_ctx_t c, int mode)
+{
+    int rc = enter(c);
+    if (rc) return rc;
+    if (mode != APH_LAUNCH_GRAPH && mode != APH_LAUNCH_EAGER) return fail(c, APH_E_INVALID, "unknown launch mode");
+    c->wf.useGraph = mode == APH_LAUNCH_GRAPH;
+    return APH_OK;
+}
+
+int aph_get_tuning(aph_ctx_t c, aph_tuning* t)
+{
+    if (!c || !t) return APH_E_INVALID;
+    t->chunk_paths = c->wf.tuning.chunkPaths;
+    t->trace_blocks_per_cu = c->wf.tuning.tracePerCU;
+    t->shade_blocks_per_cu = c->wf.tuning.shadePerCU;
+    t->ms = 0.0f;
+    return APH_OK;
+}
+
+int aph_set_tuning(aph_ctx_t c, const aph_tuning* t)
+{
+    int rc = enter(c);
+    if (rc) return rc;
+    if (!t || t->chunk_paths < (1u << 16) || t->chunk_paths > (1u << 26) || t->trace_blocks_per_cu == 0 ||
+        t->trace_blocks_per_cu > 64 || t->shade_blocks_per_cu == 0 || t->shade_blocks_per_cu > 64)
+        return fail(c, APH_E_INVALID, "tuning out of range");
+    HIPCALL(c, hipStreamSynchronize(c->stream), "tuning");
+    c->wf.tuning.chunkPaths = t->chunk_paths;
+    c->wf.tuning.tracePerCU = t->trace_blocks_per_cu;
+    c->wf.tuning.shadePerCU = t->shade_blocks_per_cu;
+    return APH_OK;
+}
+
+int aph_wavefront_autotune(aph_ctx_t c, const aph_request* reqs, uint32_t nreq, aph_tuning* best)
+{
+    int rc = enter(c);
+    if (rc) return rc;
+    if (!reqs || nreq == 0) return fail(c, APH_E_INVALID, "autotune needs at least one request");
+    const uint32_t chunks[] = {2u << 20, 4u << 20, 8u << 20};
+    const uint32_t traces[] = {8, 16, 32};
+    const uint32_t shades[] = {4, 8, 16};
+    aph_tuning keep{};
+    aph_get_tuning(c, &keep);
+    aph_tuning win = keep;
+    win.ms = INFINITY;
+    hipEvent_t a, b;
+    HIPCALL(c, hipEventCreate(&a), "autotune");
+    HIPCALL(c, hipEventCreate(&b), "autotune");
+    int status = APH_OK;
+    for (uint32_t ch : chunks) {
+        for (uint32_t tr : traces) {
+            for (uint32_t sh : shades) {
+                aph_tuning t{ch, tr, sh, 0.0f};
+                if ((status = aph_set_tuning(c, &t)) != APH_OK) break;
+                // first run captures the graph for this launch shape; the second is timed
+                if ((status = aph_render(c, reqs, nreq, APH_MODE_WAVEFRONT)) != APH_OK) break;
+                if (hipEventRecord(a, c->stream) != hipSuccess) { status = APH_E_DEVICE; break; }
+                if ((status = aph_render(c, reqs, nreq, APH_MODE_WAVEFRONT)) != APH_OK) break;
+                if (hipEventRecord(b, c->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess) {
+                    status = APH_E_DEVICE;
+                    break;
+                }
+                float ms = 0.0f;
+                hipEventElapsedTime(&ms, a, b);
+                if (ms < win.ms) {
+                    win = t;
+                    win.ms = ms;
+                }
+            }
+            if (status) break;
+        }
+        if (status) break;
+    }
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    if (status) {
+        aph_set_tuning(c, &keep);
+        return status;
+    }
+    rc = aph_set_tuning(c, &win);
+    if (rc) return rc;
+    if (best) *best = win;
+    return APH_OK;
+}
+
 int aph_wait(aph_ctx_t c)
 {
     int rc = enter(c);

@@ -19,11 +19,34 @@ struct WavefrontRequest {
     float4* accum;
 };
 
-// Device buffers kept across calls (grown on demand).
+// Launch parameters of the pipeline (aph_wavefront_autotune picks them).
+struct WavefrontTuning {
+    uint32_t chunkPaths = 4u << 20;  // paths in flight per chunk
+    uint32_t tracePerCU = 16;        // blocks per CU of the persistent traversal kernels
+    uint32_t shadePerCU = 8;         // blocks per CU of the shade kernel
+};
+
+// Device buffers kept across calls (grown on demand) and the captured chunk graph.
 struct WavefrontState {
     void* buf = nullptr;
     size_t bytes = 0;
     uint32_t capacity = 0;  // paths in flight
+    WavefrontTuning tuning;
+    bool useGraph = true;
+    struct GraphKey {
+        void* buf;
+        void* ctr;
+        uint32_t depth;
+        bool stats;
+        unsigned gen, trace, shade, shadow, accum;
+        bool operator==(const GraphKey& o) const
+        {
+            return buf == o.buf && ctr == o.ctr && depth == o.depth && stats == o.stats && gen == o.gen &&
+                   trace == o.trace && shade == o.shade && shadow == o.shadow && accum == o.accum;
+        }
+    };
+    GraphKey graphKey{};
+    hipGraphExec_t graphExec = nullptr;
 };
 
 hipError_t wavefrontRender(WavefrontState& st, const WavefrontRequest* reqs, uint32_t nreq, DevCounters* ctr,

@@ -93,6 +93,12 @@ struct SegInfo {
     uint32_t req, sample, pathOffset, count;
 };
 
+// Sizes of the chunk in flight, written with its tables: the kernels read them
+// from device memory so that one captured launch graph serves every chunk.
+struct ChunkHdr {
+    uint32_t total, nseg, ncreq, totalPix;
+};
+
 struct ChunkReq {
     uint32_t req, firstSeg, nSeg, pixelBase;
 };
@@ -143,10 +149,10 @@ __device__ __forceinline__ void addCounter(unsigned long long* c, uint32_t v)
 __device__ __forceinline__ f3 xyz(float4 v) { return f3{v.x, v.y, v.z}; }
 __device__ __forceinline__ float4 f4(f3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
 
-__global__ void kReset(uint32_t* counts, uint32_t q0)
+__global__ void kReset(uint32_t* counts, const ChunkHdr* hdr)
 {
     if (threadIdx.x == 0) {
-        counts[0] = q0;
+        counts[0] = hdr->total;
         counts[1] = 0;
         counts[2] = 0;
         counts[3] = 0;
@@ -165,9 +171,10 @@ __global__ void kResetNext(uint32_t* counts, int next)
 }
 
 // ---- gen: RayGenShader prologue (MainEntryPoints.slang:83-103) ----
-__global__ __launch_bounds__(kWB) void kGen(const DevReq* reqs, const SegInfo* segs, uint32_t nseg, uint32_t total,
-                                           Paths P, Queues Q, DevCounters* ctr)
+__global__ __launch_bounds__(kWB) void kGen(const DevReq* reqs, const SegInfo* segs, const ChunkHdr* hdr, Paths P,
+                                           Queues Q, DevCounters* ctr)
 {
+    const uint32_t nseg = hdr->nseg, total = hdr->total;
     uint32_t started = 0;
     for (uint32_t p = blockIdx.x * kWB + threadIdx.x; p < total; p += gridDim.x * kWB) {
         uint32_t s = findSeg(segs, nseg, p);
@@ -576,9 +583,9 @@ __global__ __launch_bounds__(kWB) void kResolve(const DevReq* reqs, const SegInf
 }
 
 // ---- accum: running mean in sample order (Accumulation.glsl:22-30) ----
-__global__ void kAccum(const DevReq* reqs, const SegInfo* segs, const ChunkReq* creqs, uint32_t ncreq,
-                       uint32_t totalPix, Paths P)
+__global__ void kAccum(const DevReq* reqs, const SegInfo* segs, const ChunkReq* creqs, const ChunkHdr* hdr, Paths P)
 {
+    const uint32_t ncreq = hdr->ncreq, totalPix = hdr->totalPix;
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < totalPix; t += gridDim.x * blockDim.x) {
         uint32_t lo = 0, hi = ncreq;
         while (hi - lo > 1) {
@@ -610,23 +617,76 @@ size_t alignUp(size_t x) { return (x + 255) & ~(size_t)255; }
 
 }  // namespace
 
+namespace {
+
+// Launch sizes of one chunk: persistent kernels with grid-stride loops, sized by
+// the chunk capacity (not the chunk's own path count), so the same launches -
+// and the same captured graph - serve every chunk.
+struct Launch {
+    unsigned gen, trace, shade, light, shadow, accum;
+};
+
+// Everything one chunk runs after its tables are in place: generation, then per
+// bounce the reset, extension, shading, shadow and resolve stages, then
+// accumulation.  Timed per stage with HIP events when timing is given.
+void enqueueChunk(hipStream_t stream, const Launch& L, uint32_t maxDepth, bool stats, const DevReq* dreq,
+                  const SegInfo* gseg, const ChunkReq* gcr, const ChunkHdr* hdr, const Paths& P, const Queues& Q,
+                  DevCounters* ctr, std::vector<PendingEvent>* timing)
+{
+    {
+        EventScope ev(timing, "wf_gen", stream);
+        kReset<<<1, 64, 0, stream>>>(Q.counts, hdr);
+        kGen<<<L.gen, kWB, 0, stream>>>(dreq, gseg, hdr, P, Q, ctr);
+    }
+    int cur = 0;
+    for (uint32_t d = 0; d < maxDepth; d++) {
+        kResetNext<<<1, 64, 0, stream>>>(Q.counts, cur ^ 1);
+        {
+            EventScope ev(timing, "wf_extend", stream);
+            if (stats) kExtend<true><<<L.trace, kWB, 0, stream>>>(dreq, gseg, P, Q, cur, ctr);
+            else kExtend<false><<<L.trace, kWB, 0, stream>>>(dreq, gseg, P, Q, cur, ctr);
+        }
+        {
+            EventScope ev(timing, "wf_shade", stream);
+            kShade<<<L.shade, kWB, 0, stream>>>(dreq, gseg, P, Q, cur);
+        }
+        {
+            EventScope ev(timing, "wf_shadow", stream);
+            if (stats) kShadow<true><<<L.shadow, kWB, 0, stream>>>(dreq, gseg, P, Q, ctr);
+            else kShadow<false><<<L.shadow, kWB, 0, stream>>>(dreq, gseg, P, Q, ctr);
+        }
+        {
+            EventScope ev(timing, "wf_resolve", stream);
+            kResolve<<<L.light, kWB, 0, stream>>>(dreq, gseg, P, Q, cur);
+        }
+        cur ^= 1;
+    }
+    {
+        EventScope ev(timing, "wf_accum", stream);
+        kAccum<<<L.accum, 256, 0, stream>>>(dreq, gseg, gcr, hdr, P);
+    }
+}
+
+}  // namespace
+
 hipError_t wavefrontRender(WavefrontState& st, const WavefrontRequest* reqs, uint32_t nreq, DevCounters* ctr,
                            bool stats, hipStream_t stream, std::vector<PendingEvent>* timing)
 {
     if (nreq == 0) return hipSuccess;
+    const WavefrontTuning& tn = st.tuning;
     uint32_t maxPix = 0;
     for (uint32_t i = 0; i < nreq; i++) maxPix = std::max(maxPix, reqs[i].width * reqs[i].height);
-    // Chunk capacity: at least one full frame of the largest request, ~4M paths otherwise.
-    uint32_t cap = std::max<uint32_t>(maxPix, 4u << 20);
+    // Chunk capacity: at least one full frame of the largest request, the tuned path count otherwise.
+    uint32_t cap = std::max<uint32_t>(maxPix, tn.chunkPaths);
     size_t perPath = 4 + 4 + 8 + 16 + 4 + 16 * 4 + 16 * 3 + 16 * kSlots + 1 + kSlots + 16 +  // path state
                      16 * 4 + 16 + 4 + 32 * kSlots;                                              // queues
-    const size_t kTableBytes = 1u << 20;  // segment + chunk-request tables of one chunk
-    const uint32_t kMaxSegs = (uint32_t)(kTableBytes / 2 / sizeof(uint32_t) / 4);
-    size_t need = alignUp((size_t)cap * perPath) + 64 * 1024 + alignUp(sizeof(DevReq) * nreq) + kTableBytes;
-    if (st.bytes < need) {
-        if (st.buf) hipFree(st.buf);
-        st.buf = nullptr;
-        st.bytes = 0;
+    const size_t kTableBytes = 1u << 20;  // chunk header + segment + chunk-request tables of one chunk
+    const size_t kHdrBytes = 256;
+    const uint32_t kMaxSegs = (uint32_t)((kTableBytes / 2 - kHdrBytes) / sizeof(SegInfo));
+    const uint32_t kMaxCreqs = (uint32_t)(kTableBytes / 2 / sizeof(ChunkReq));
+    size_t need = alignUp((size_t)cap * perPath) + 64 * 1024 + kTableBytes + alignUp(sizeof(DevReq) * nreq);
+    if (st.bytes < need || st.capacity != cap) {
+        wavefrontRelease(st);
         hipError_t e = hipMalloc(&st.buf, need);
         if (e != hipSuccess) return e;
         st.bytes = need;
@@ -666,9 +726,12 @@ hipError_t wavefrontRender(WavefrontState& st, const WavefrontRequest* reqs, uin
     Q.shA = (float4*)take((size_t)cap * kSlots * 16);
     Q.shB = (float4*)take((size_t)cap * kSlots * 16);
     Q.counts = (uint32_t*)take(64);
-    DevReq* dreq = (DevReq*)take(sizeof(DevReq) * nreq);
     char* tbl = (char*)take(kTableBytes);
+    DevReq* dreq = (DevReq*)take(sizeof(DevReq) * nreq);  // last: its size varies, nothing after it moves
     if (off > need) return hipErrorOutOfMemory;
+    ChunkHdr* ghdr = (ChunkHdr*)tbl;
+    SegInfo* gseg = (SegInfo*)(tbl + kHdrBytes);
+    ChunkReq* gcr = (ChunkReq*)(tbl + kTableBytes / 2);
 
     // request table
     std::vector<DevReq> hreq(nreq);
@@ -686,6 +749,43 @@ hipError_t wavefrontRender(WavefrontState& st, const WavefrontRequest* reqs, uin
     hipError_t e = hipMemcpyAsync(dreq, hreq.data(), sizeof(DevReq) * nreq, hipMemcpyHostToDevice, stream);
     if (e != hipSuccess) return e;
 
+    int dev = 0, cus = 256;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    auto grid = [&](uint64_t work, unsigned perCU) {
+        return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((work + kWB - 1) / kWB, (uint64_t)cus * perCU));
+    };
+    Launch L;
+    L.gen = grid(cap, 16);
+    L.trace = grid(cap, tn.tracePerCU);
+    L.shade = grid(cap, tn.shadePerCU);
+    L.light = grid(cap, 8);
+    L.shadow = grid((uint64_t)cap * 2, tn.tracePerCU);
+    L.accum = std::max(1u, std::min<unsigned>((cap + 255) / 256, (unsigned)cus * 8));
+
+    // One graph per (buffers, depth, stats, launch sizes): captured on first use,
+    // replayed for every chunk.  Timed runs launch eagerly so each stage gets its
+    // own HIP events.
+    const bool graph = st.useGraph && timing == nullptr;
+    if (graph) {
+        WavefrontState::GraphKey key{st.buf, ctr, maxDepth, stats, L.gen, L.trace, L.shade, L.shadow, L.accum};
+        if (!st.graphExec || !(key == st.graphKey)) {
+            if (st.graphExec) hipGraphExecDestroy(st.graphExec);
+            st.graphExec = nullptr;
+            hipGraph_t g = nullptr;
+            if ((e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal)) != hipSuccess) return e;
+            enqueueChunk(stream, L, maxDepth, stats, dreq, gseg, gcr, ghdr, P, Q, ctr, nullptr);
+            if ((e = hipStreamEndCapture(stream, &g)) != hipSuccess) return e;
+            e = hipGraphInstantiate(&st.graphExec, g, nullptr, nullptr, 0);
+            hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                st.graphExec = nullptr;
+                return e;
+            }
+            st.graphKey = key;
+        }
+    }
+
     // Segments: (request, sample) pairs in request-major, sample-ascending order, cut into chunks.
     struct PendingSeg {
         uint32_t req, sample, count;
@@ -694,10 +794,6 @@ hipError_t wavefrontRender(WavefrontState& st, const WavefrontRequest* reqs, uin
     for (uint32_t i = 0; i < nreq; i++)
         for (uint32_t s = 0; s < reqs[i].sampleCount; s++)
             all.push_back(PendingSeg{i, reqs[i].sampleStart + s, reqs[i].width * reqs[i].height});
-    int dev = 0, cus = 256;
-    hipGetDevice(&dev);
-    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    const unsigned traceGrid = (unsigned)cus * 16, shadeGrid = (unsigned)cus * 8, lightGrid = (unsigned)cus * 8;
     size_t k = 0;
     while (k < all.size()) {
         std::vector<SegInfo> segs;
@@ -706,6 +802,7 @@ hipError_t wavefrontRender(WavefrontState& st, const WavefrontRequest* reqs, uin
         while (k < all.size() && total + all[k].count <= cap && segs.size() < kMaxSegs) {
             const PendingSeg& s = all[k];
             if (creqs.empty() || creqs.back().req != s.req) {
+                if (creqs.size() == kMaxCreqs) break;
                 creqs.push_back(ChunkReq{s.req, (uint32_t)segs.size(), 0, totalPix});
                 totalPix += s.count;
             }
@@ -715,49 +812,18 @@ hipError_t wavefrontRender(WavefrontState& st, const WavefrontRequest* reqs, uin
             k++;
         }
         if (segs.empty()) return hipErrorInvalidValue;
-        size_t sb = sizeof(SegInfo) * segs.size(), cb = sizeof(ChunkReq) * creqs.size();
-        SegInfo* gseg = (SegInfo*)tbl;
-        ChunkReq* gcr = (ChunkReq*)(tbl + kTableBytes / 2);
-        if ((e = hipMemcpyAsync(gseg, segs.data(), sb, hipMemcpyHostToDevice, stream)) != hipSuccess) return e;
-        if ((e = hipMemcpyAsync(gcr, creqs.data(), cb, hipMemcpyHostToDevice, stream)) != hipSuccess) return e;
-        uint32_t nseg = (uint32_t)segs.size();
-        {
-            EventScope ev(timing, "wf_gen", stream);
-            kReset<<<1, 64, 0, stream>>>(Q.counts, total);
-            unsigned g = std::min<unsigned>((total + kWB - 1) / kWB, (unsigned)cus * 16);
-            kGen<<<g, kWB, 0, stream>>>(dreq, gseg, nseg, total, P, Q, ctr);
-        }
-        int cur = 0;
-        unsigned tg = std::min<unsigned>((total + kWB - 1) / kWB, traceGrid);
-        unsigned sg = std::min<unsigned>((total + kWB - 1) / kWB, shadeGrid);
-        unsigned lg = std::min<unsigned>((total + kWB - 1) / kWB, lightGrid);
-        unsigned shg = std::min<unsigned>((total * 2 + kWB - 1) / kWB, traceGrid);
-        for (uint32_t d = 0; d < maxDepth; d++) {
-            kResetNext<<<1, 64, 0, stream>>>(Q.counts, cur ^ 1);
-            {
-                EventScope ev(timing, "wf_extend", stream);
-                if (stats) kExtend<true><<<tg, kWB, 0, stream>>>(dreq, gseg, P, Q, cur, ctr);
-                else kExtend<false><<<tg, kWB, 0, stream>>>(dreq, gseg, P, Q, cur, ctr);
-            }
-            {
-                EventScope ev(timing, "wf_shade", stream);
-                kShade<<<sg, kWB, 0, stream>>>(dreq, gseg, P, Q, cur);
-            }
-            {
-                EventScope ev(timing, "wf_shadow", stream);
-                if (stats) kShadow<true><<<shg, kWB, 0, stream>>>(dreq, gseg, P, Q, ctr);
-                else kShadow<false><<<shg, kWB, 0, stream>>>(dreq, gseg, P, Q, ctr);
-            }
-            {
-                EventScope ev(timing, "wf_resolve", stream);
-                kResolve<<<lg, kWB, 0, stream>>>(dreq, gseg, P, Q, cur);
-            }
-            cur ^= 1;
-        }
-        {
-            EventScope ev(timing, "wf_accum", stream);
-            unsigned g = std::min<unsigned>((totalPix + 255) / 256, (unsigned)cus * 8);
-            kAccum<<<g, 256, 0, stream>>>(dreq, gseg, gcr, (uint32_t)creqs.size(), totalPix, P);
+        ChunkHdr hdr{total, (uint32_t)segs.size(), (uint32_t)creqs.size(), totalPix};
+        if ((e = hipMemcpyAsync(ghdr, &hdr, sizeof(hdr), hipMemcpyHostToDevice, stream)) != hipSuccess) return e;
+        if ((e = hipMemcpyAsync(gseg, segs.data(), sizeof(SegInfo) * segs.size(), hipMemcpyHostToDevice, stream)) !=
+            hipSuccess)
+            return e;
+        if ((e = hipMemcpyAsync(gcr, creqs.data(), sizeof(ChunkReq) * creqs.size(), hipMemcpyHostToDevice, stream)) !=
+            hipSuccess)
+            return e;
+        if (graph) {
+            if ((e = hipGraphLaunch(st.graphExec, stream)) != hipSuccess) return e;
+        } else {
+            enqueueChunk(stream, L, maxDepth, stats, dreq, gseg, gcr, ghdr, P, Q, ctr, timing);
         }
         if ((e = hipGetLastError()) != hipSuccess) return e;
         // the tables are overwritten by the next chunk
@@ -770,6 +836,8 @@ hipError_t wavefrontRender(WavefrontState& st, const WavefrontRequest* reqs, uin
 
 void wavefrontRelease(WavefrontState& st)
 {
+    if (st.graphExec) hipGraphExecDestroy(st.graphExec);
+    st.graphExec = nullptr;
     if (st.buf) hipFree(st.buf);
     st.buf = nullptr;
     st.bytes = 0;

@@ -61,6 +61,25 @@ class Renderer:
     def build_blas(self, mesh, builder=capi.APH_BUILDER_LBVH):
         self._check(self.lib.aph_blas_build(self.ctx, mesh, builder))
 
+    # ---- launch control
+    def set_launch_mode(self, mode):
+        self._check(self.lib.aph_set_launch_mode(self.ctx, mode))
+
+    def tuning(self):
+        t = capi.Tuning()
+        self._check(self.lib.aph_get_tuning(self.ctx, C.byref(t)))
+        return t
+
+    def set_tuning(self, chunk_paths, trace_blocks_per_cu, shade_blocks_per_cu):
+        t = capi.Tuning(chunk_paths, trace_blocks_per_cu, shade_blocks_per_cu, 0.0)
+        self._check(self.lib.aph_set_tuning(self.ctx, C.byref(t)))
+
+    def autotune(self, requests):
+        arr = (capi.Request * len(requests))(*requests)
+        t = capi.Tuning()
+        self._check(self.lib.aph_wavefront_autotune(self.ctx, C.cast(arr, C.c_void_p), len(requests), C.byref(t)))
+        return t
+
     # ---- guide images and temporal resolve
     def aov(self, w, h):
         a = C.c_uint64()

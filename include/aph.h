@@ -273,6 +273,27 @@ int aph_timing_enable(aph_ctx_t ctx, int enable);
 int aph_timing_get(aph_ctx_t ctx, const char* kernel, double* total_ms, uint64_t* launches);
 int aph_timing_reset(aph_ctx_t ctx);
 
+/* ---- launch control of the wavefront pipeline.  No reference counterpart: the
+ * reference's driver schedules its own ray tracing. ---- */
+enum { APH_LAUNCH_GRAPH = 0, APH_LAUNCH_EAGER = 1 };
+/* APH_LAUNCH_GRAPH (default): each chunk's stage sequence (generation, per
+ * bounce reset/extend/shade/shadow/resolve, accumulation) is one captured HIP
+ * graph, replayed for every chunk; APH_LAUNCH_EAGER launches the kernels one by
+ * one.  With timing enabled the pipeline always launches eagerly. */
+int aph_set_launch_mode(aph_ctx_t ctx, int mode);
+typedef struct {
+    uint32_t chunk_paths;          /* paths in flight per chunk */
+    uint32_t trace_blocks_per_cu;  /* persistent traversal kernels: 128-thread blocks per CU */
+    uint32_t shade_blocks_per_cu;  /* shade kernel */
+    float ms;                      /* measured time of the winning configuration (autotune) */
+} aph_tuning;
+int aph_get_tuning(aph_ctx_t ctx, aph_tuning* t);
+int aph_set_tuning(aph_ctx_t ctx, const aph_tuning* t);
+/* Renders the given requests (into their targets) under each candidate launch
+ * configuration - chunk of 2M/4M/8M paths x 8/16/32 traversal blocks per CU x
+ * 4/8/16 shade blocks per CU - and keeps the fastest for this context. */
+int aph_wavefront_autotune(aph_ctx_t ctx, const aph_request* reqs, uint32_t nreq, aph_tuning* best);
+
 /* ---- guide images (AOVs) and temporal resolve: the DirectX backend's denoising
  * guides gDepthView, gNormalRoughness, gBaseColorMetalness, gMotionVectors
  * (GlobalPipelineState.slang:110-172, written by the primary hit/miss,

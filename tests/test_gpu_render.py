@@ -194,3 +194,56 @@ def test_material_layers_match_oracle(renderer, teapot_path):
     for mode in (capi.APH_MODE_MEGAKERNEL, capi.APH_MODE_WAVEFRONT):
         gpu, cpu = render_both(renderer, sc, 4, mode=mode)
         compare(gpu, cpu)
+
+
+def test_graph_replays_equal_eager_launches(renderer):
+    """Each chunk's stage sequence is one captured HIP graph.  With 64K-path chunks a
+    96x64x16 render replays it twice per call; every replay - with eager work and
+    synchronizations between them - must give the eager image bit for bit, and so
+    must a different launch configuration (tuning never changes results)."""
+    sc = scenes.config2(subdiv=2, width=96, height=64, trace_depth=5, env_w=256, env_h=128)
+    tlas = renderer.load_scene(sc)
+    renderer.set_tuning(1 << 16, 16, 8)
+
+    def render(spp=16, start=0, tgt=None):
+        tgt = tgt or renderer.target(96, 64)
+        renderer.render([renderer.request(tgt, tlas, sc.frame, start, spp)], capi.APH_MODE_WAVEFRONT)
+        renderer.wait()
+        return tgt, renderer.readback(tgt, 96, 64)
+
+    renderer.set_launch_mode(capi.APH_LAUNCH_EAGER)
+    _, eager = render()
+    renderer.set_launch_mode(capi.APH_LAUNCH_GRAPH)
+    for k in range(3):
+        _, g = render()
+        np.testing.assert_array_equal(g, eager)
+        other = renderer.target(96, 64)  # eager work between replays
+        renderer.render([renderer.request(other, tlas, sc.frame, 0, 1)], capi.APH_MODE_MEGAKERNEL)
+        renderer.wait()
+    t, _ = render(8, 0)
+    _, prog = render(8, 8, t)
+    np.testing.assert_array_equal(prog, eager)
+    renderer.set_tuning(1 << 17, 8, 16)
+    _, tuned = render()
+    np.testing.assert_array_equal(tuned, eager)
+    renderer.set_tuning(4 << 20, 16, 8)
+
+
+def test_autotune_keeps_results(renderer, teapot_path):
+    sc = scenes.teapot_scene(teapot_path, 64, 48)
+    tlas = renderer.load_scene(sc)
+    ref = renderer.target(64, 48)
+    renderer.render([renderer.request(ref, tlas, sc.frame, 0, 4)], capi.APH_MODE_WAVEFRONT)
+    renderer.wait()
+    before = renderer.readback(ref, 64, 48)
+    tgt = renderer.target(64, 48)
+    best = renderer.autotune([renderer.request(tgt, tlas, sc.frame, 0, 4)])
+    assert best.ms > 0 and best.chunk_paths in (2 << 20, 4 << 20, 8 << 20)
+    t = renderer.tuning()
+    assert (t.chunk_paths, t.trace_blocks_per_cu, t.shade_blocks_per_cu) == \
+        (best.chunk_paths, best.trace_blocks_per_cu, best.shade_blocks_per_cu)
+    after = renderer.target(64, 48)
+    renderer.render([renderer.request(after, tlas, sc.frame, 0, 4)], capi.APH_MODE_WAVEFRONT)
+    renderer.wait()
+    np.testing.assert_array_equal(renderer.readback(after, 64, 48), before)
+    renderer.set_tuning(4 << 20, 16, 8)
