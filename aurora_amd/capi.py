@@ -130,6 +130,33 @@ class TaaRequest(C.Structure):
                 ("history_out", C.c_uint64), ("is_reset", C.c_int32), ("blend_alpha_min", C.c_float),
                 ("blend_alpha_max", C.c_float), ("speed_reference", C.c_float)]
 
+class DnBuffer(C.Structure):
+    _fields_ = [("level", C.c_uint32), ("channels", C.c_uint32)]
+
+
+class DnOp(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("layer", C.c_int32), ("src", C.c_int32), ("src_offset", C.c_int32),
+                ("channels", C.c_int32), ("dst", C.c_int32), ("dst_offset", C.c_int32)]
+
+
+class DnLayer(C.Structure):
+    _fields_ = [("name", C.c_char * 16), ("cin", C.c_uint32), ("cin_padded", C.c_uint32), ("cout", C.c_uint32),
+                ("cout_padded", C.c_uint32), ("in_buffer", C.c_uint32), ("out_buffer", C.c_uint32),
+                ("out_offset", C.c_uint32), ("relu", C.c_uint32), ("weight_offset", C.c_uint64),
+                ("bias_offset", C.c_uint64)]
+
+
+class DenoiseExample(C.Structure):
+    _fields_ = [("noisy", C.c_uint64), ("aov", C.c_uint64), ("reference", C.c_uint64)]
+
+
+class DenoiseItem(C.Structure):
+    _fields_ = [("noisy", C.c_uint64), ("aov", C.c_uint64), ("output", C.c_uint64)]
+
+
+APH_DN_OP_CONV, APH_DN_OP_POOL, APH_DN_OP_UPSAMPLE = 0, 1, 2
+APH_DN_PARAMS, APH_DN_PARAM_GRADS, APH_DN_ACTIVATIONS, APH_DN_ACTIVATION_GRADS, APH_DN_PREDICTION = 0, 1, 2, 3, 4
+
 _P = C.c_void_p
 _u32 = C.c_uint32
 _u64 = C.c_uint64
@@ -189,6 +216,15 @@ SIGNATURES = {
     "aph_server_release": (_i, [_P, _u64]),
     "aph_server_usage": (_i, [_P, C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u32), C.POINTER(_u32)]),
     "aph_server_replay": (_i, [_P, _P, _P, _u32, _P]),
+    "aph_denoiser_arch": (_i, [_P, C.POINTER(_u32), _P, C.POINTER(_u32), _P, C.POINTER(_u32)]),
+    "aph_denoiser_param_count": (_u64, []),
+    "aph_denoiser_create": (_i, [_P, _u64, C.POINTER(_u64)]),
+    "aph_denoiser_destroy": (_i, [_P, _u64]),
+    "aph_denoiser_set_params": (_i, [_P, _u64, _P, _u64]),
+    "aph_denoiser_tensor_size": (_i, [_P, _u64, _i, _u32, C.POINTER(_u64)]),
+    "aph_denoiser_tensor": (_i, [_P, _u64, _i, _u32, _P, _u64]),
+    "aph_denoiser_train_step": (_i, [_P, _u64, _P, _u32, C.c_float, C.POINTER(C.c_float)]),
+    "aph_denoise": (_i, [_P, _u64, _P, _u32]),
 }
 
 _lib = None

@@ -18,6 +18,8 @@
 #include "timing.h"
 #include "wavefront.h"
 
+struct aph_denoiser;
+
 namespace aph {
 
 struct DevBuf {
@@ -81,6 +83,10 @@ struct Timing {
     uint64_t launches = 0;
 };
 
+struct DenoiserDeleter {
+    void operator()(aph_denoiser* d) const;  // denoiser.cpp
+};
+
 }  // namespace aph
 
 // The opaque aph_ctx_t of the C-ABI.
@@ -99,6 +105,7 @@ struct aph_ctx {
     std::map<uint64_t, std::unique_ptr<Tlas>> tlases;
     std::map<uint64_t, std::unique_ptr<Target>> targets;
     std::map<uint64_t, std::unique_ptr<AovSet>> aovs;
+    std::map<uint64_t, std::unique_ptr<aph_denoiser, aph::DenoiserDeleter>> denoisers;
     DevBuf mats;
     uint32_t nmat = 0;
     DevBuf envLight, envAlias, envBg;

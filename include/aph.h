@@ -396,6 +396,58 @@ int aph_server_release(aph_server_t s, uint64_t ticket);
  * (non-decreasing) after the call; returns when every job is done. */
 int aph_server_replay(aph_server_t s, const aph_job* jobs, const double* arrival, uint32_t n, uint64_t* tickets);
 
+
+/* ---- Learned denoiser ----
+ * A U-Net over the noisy radiance of a target and the guides of an AOV set
+ * (aph_render_aovs): log radiance, base colour + metalness, normal +
+ * roughness, view depth.  Convolutions run on bf16 MFMA with fp32
+ * accumulation; training (Adam, L1 loss on log radiance) runs on the GPU.
+ * No reference counterpart: the reference's DirectX backend hands the same
+ * guides to NVIDIA NRD (DirectX/Denoiser.h:24-35, Denoiser::denoise at :70)
+ * from PTRenderer.cpp:1052 (submitDenoising); aph_denoise takes that call's
+ * place.  Images are padded to multiples of 16 internally. */
+typedef struct { uint32_t level, channels; } aph_dn_buffer; /* NHWC bf16 at 1/2^level resolution */
+enum { APH_DN_OP_CONV = 0, APH_DN_OP_POOL = 1, APH_DN_OP_UPSAMPLE = 2 };
+/* conv: layer; pool (2x2 mean) and upsample (nearest, 2x): channels
+ * [src_offset, +channels) of buffer src to [dst_offset, +channels) of dst */
+typedef struct { int32_t kind, layer, src, src_offset, channels, dst, dst_offset; } aph_dn_op;
+typedef struct {
+    char name[16];
+    uint32_t cin, cin_padded, cout, cout_padded;
+    uint32_t in_buffer, out_buffer, out_offset, relu;
+    /* offsets into the parameter vector: weights [9][cout_padded][cin_padded]
+     * (tap 3*ky+kx, zero beyond cin/cout), bias [cout_padded] */
+    uint64_t weight_offset, bias_offset;
+} aph_dn_layer;
+/* The network's layers, activation buffers and forward schedule (pass a null
+ * array with a count pointer to query a count). */
+int aph_denoiser_arch(aph_dn_layer* layers, uint32_t* n_layers, aph_dn_buffer* buffers, uint32_t* n_buffers,
+                      aph_dn_op* ops, uint32_t* n_ops);
+uint64_t aph_denoiser_param_count(void);
+/* He-initialised weights from seed. */
+int aph_denoiser_create(aph_ctx_t ctx, uint64_t seed, uint64_t* denoiser);
+int aph_denoiser_destroy(aph_ctx_t ctx, uint64_t denoiser);
+/* Loads a parameter vector (count = aph_denoiser_param_count()); resets Adam. */
+int aph_denoiser_set_params(aph_ctx_t ctx, uint64_t denoiser, const float* params, uint64_t count);
+enum {
+    APH_DN_PARAMS = 0,           /* fp32 parameter vector */
+    APH_DN_PARAM_GRADS = 1,      /* its gradient from the last training step */
+    APH_DN_ACTIVATIONS = 2,      /* buffer `index` of the last forward pass, NHWC */
+    APH_DN_ACTIVATION_GRADS = 3, /* its gradient from the last training step */
+    APH_DN_PREDICTION = 4        /* fp32 [n][Hp][Wp][4] predicted log1p radiance */
+};
+/* Inspection of the last pass (synchronizes); count must equal the size. */
+int aph_denoiser_tensor_size(aph_ctx_t ctx, uint64_t denoiser, int kind, uint32_t index, uint64_t* count);
+int aph_denoiser_tensor(aph_ctx_t ctx, uint64_t denoiser, int kind, uint32_t index, float* out, uint64_t count);
+typedef struct { uint64_t noisy, aov, reference; } aph_denoise_example; /* targets and an AOV set of one size */
+/* One Adam step over a batch (all examples one size); returns the mean L1
+ * loss of log1p radiance before the step (synchronizes). */
+int aph_denoiser_train_step(aph_ctx_t ctx, uint64_t denoiser, const aph_denoise_example* ex, uint32_t n, float lr,
+                            float* loss);
+typedef struct { uint64_t noisy, aov, output; } aph_denoise_item; /* output may be the noisy target itself */
+/* Denoises n images of one size in one batch (asynchronous, on the context's stream). */
+int aph_denoise(aph_ctx_t ctx, uint64_t denoiser, const aph_denoise_item* items, uint32_t n);
+
 #ifdef __cplusplus
 }
 #endif

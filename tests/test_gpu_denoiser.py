@@ -1,0 +1,279 @@
+"""GPU denoiser (aph_denoise, aph_denoiser_train_step): every layer of the U-Net,
+forward and backward, against a plain PyTorch fp32 reference of the same op fed
+with the GPU's own inputs to that op.
+
+Bars: a convolution output is the fp32 result rounded to bf16, so it must lie
+within 2^-8 relative (one bf16 rounding) of torch's fp32 convolution of the same
+bf16 inputs and weights, plus 1e-3 of the tensor's largest magnitude for the
+summation order.  Weight and bias gradients are fp32 sums of exact bf16
+products: within 1e-3 of the largest reference magnitude.  Activation
+gradients pass through at most two bf16 roundings (a sum of a convolution's and
+a pooling's share): within 2^-7 of the shares' magnitudes.  Pooling is an fp32
+mean rounded once to bf16; upsampling, batching and the ReLU masks are exact.
+"""
+import numpy as np
+import pytest
+import torch
+import torch.nn.functional as F
+
+from aurora_amd import capi, scenes
+from aurora_amd.denoiser import Denoiser, padded
+from aurora_amd.session import AphError
+
+pytestmark = pytest.mark.gpu
+
+BF = 2.0 ** -8
+
+
+def bf16(a):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).bfloat16().float().numpy()
+
+
+def nchw(a):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).permute(0, 3, 1, 2)
+
+
+def nhwc(t):
+    return t.permute(0, 2, 3, 1).contiguous().numpy()
+
+
+def torch_w(w):
+    """[9][cout][cin] (tap 3*ky+kx) -> [cout][cin][3][3]."""
+    t = torch.from_numpy(np.ascontiguousarray(w, dtype=np.float32))
+    return t.reshape(3, 3, w.shape[1], w.shape[2]).permute(2, 3, 0, 1).contiguous()
+
+
+def frames(w, h):
+    cam = scenes.look_at_frame((0.0, 6.0, 14.0), (0.0, 0.5, 0.0), (0, 1, 0), 45.0, w / h, 3)
+    cam2 = scenes.look_at_frame((4.0, 5.0, 12.0), (0.0, 0.5, 0.0), (0, 1, 0), 45.0, w / h, 3)
+    return cam, cam2
+
+
+def render_inputs(r, tlas, cam, w, h, spp, seed=0):
+    noisy = r.target(w, h)
+    r.render([r.request(noisy, tlas, cam, 0, spp, seed)], capi.APH_MODE_WAVEFRONT)
+    aov = r.aov(w, h)
+    vp = scenes.view_proj(cam)
+    r.render_aovs(aov, tlas, cam, vp, vp, np.array(cam.inv_view, dtype=np.float32))
+    return noisy, aov
+
+
+@pytest.fixture(scope="module")
+def setup(renderer):
+    w, h = 70, 45  # padded to 80 x 48
+    sc = scenes.config1(width=w, height=h, trace_depth=3)
+    tlas = renderer.load_scene(sc)
+    cam, cam2 = frames(w, h)
+    a = render_inputs(renderer, tlas, cam, w, h, 2, 0)
+    b = render_inputs(renderer, tlas, cam2, w, h, 2, 7)
+    refs = []
+    for c in (cam, cam2):
+        t = renderer.target(w, h)
+        renderer.render([renderer.request(t, tlas, c, 0, 64, 3)], capi.APH_MODE_WAVEFRONT)
+        refs.append(t)
+    renderer.wait()
+    return dict(w=w, h=h, tlas=tlas, cams=(cam, cam2), inputs=(a, b), refs=refs)
+
+
+def expected_features(r, noisy, aov, w, h):
+    img = r.readback(noisy, w, h)
+    d, n, b, _ = r.aov_readback(aov, w, h)
+    f = np.zeros((padded(h), padded(w), 32), dtype=np.float32)
+    f[:h, :w, 0:3] = np.log1p(np.maximum(img[..., :3], 0))
+    f[:h, :w, 3:7] = b
+    f[:h, :w, 7:11] = n
+    f[:h, :w, 11] = np.log1p(np.abs(d)) * 0.2
+    return bf16(f)
+
+
+def check_conv_forward(dn, p, acts, pred, i):
+    L = dn.layers[i]
+    w, b = dn.layer_params(p, i)
+    x = acts[L.in_buffer]
+    ref = F.conv2d(nchw(x[..., :L.cin_padded]), torch_w(bf16(w)), torch.from_numpy(b.copy()), padding=1)
+    if L.relu:
+        ref = torch.relu(ref)
+    ref = nhwc(ref)[..., :L.cout]
+    if L.out_buffer == len(dn.buffers) - 1:
+        got = pred[..., :L.cout]
+        tol = 1e-5 * np.abs(ref) + 1e-3 * np.abs(ref).max()
+    else:
+        got = acts[L.out_buffer][..., L.out_offset:L.out_offset + L.cout]
+        tol = BF * np.abs(ref) + 1e-3 * np.abs(ref).max()
+    err = np.abs(got - ref)
+    assert (err <= tol).all(), (L.name.decode(), float(err.max()), float(np.abs(ref).max()))
+
+
+def test_forward_every_layer_matches_torch(renderer, setup):
+    r, w, h = renderer, setup["w"], setup["h"]
+    dn = Denoiser(r, seed=3)
+    (na, aa), (nb, ab) = setup["inputs"]
+    out_a, out_b = r.target(w, h), r.target(w, h)
+    dn.denoise([(na, aa, out_a), (nb, ab, out_b)])
+    hp, wp = padded(h), padded(w)
+    acts = {i: dn.activation(i, 2, hp, wp) for i in range(len(dn.buffers) - 1)}
+    pred = dn.tensor(capi.APH_DN_PREDICTION).reshape(2, hp, wp, 4)
+    p = dn.params()
+    # packed input features
+    for k, (nz, av) in enumerate(((na, aa), (nb, ab))):
+        np.testing.assert_allclose(acts[0][k], expected_features(r, nz, av, w, h), rtol=BF, atol=1e-6)
+    n_conv = 0
+    for op in dn.ops:
+        if op.kind == capi.APH_DN_OP_CONV:
+            check_conv_forward(dn, p, acts, pred, op.layer)
+            n_conv += 1
+        elif op.kind == capi.APH_DN_OP_POOL:
+            src = acts[op.src][..., op.src_offset:op.src_offset + op.channels]
+            ref = nhwc(F.avg_pool2d(nchw(src), 2))
+            got = acts[op.dst][..., op.dst_offset:op.dst_offset + op.channels]
+            np.testing.assert_allclose(got, ref, rtol=BF, atol=1e-6)
+        else:
+            src = acts[op.src][..., op.src_offset:op.src_offset + op.channels]
+            ref = src.repeat(2, axis=1).repeat(2, axis=2)
+            got = acts[op.dst][..., op.dst_offset:op.dst_offset + op.channels]
+            np.testing.assert_array_equal(got, ref)
+    assert n_conv == len(dn.layers) == 19
+    # output image: expm1 of the clamped prediction, alpha from the noisy input
+    for k, (nz, out) in enumerate(((na, out_a), (nb, out_b))):
+        got = r.readback(out, w, h)
+        ref = np.expm1(np.maximum(pred[k, :h, :w, :3], 0))
+        np.testing.assert_allclose(got[..., :3], ref, rtol=1e-5, atol=1e-6)
+        np.testing.assert_array_equal(got[..., 3], r.readback(nz, w, h)[..., 3])
+    dn.close()
+
+
+def test_batch_equals_single_images(renderer, setup):
+    r, w, h = renderer, setup["w"], setup["h"]
+    dn = Denoiser(r, seed=5)
+    (na, aa), (nb, ab) = setup["inputs"]
+    o1, o2, s1 = r.target(w, h), r.target(w, h), r.target(w, h)
+    dn.denoise([(na, aa, o1), (nb, ab, o2)])
+    dn.denoise([(nb, ab, s1)])
+    np.testing.assert_array_equal(r.readback(o2, w, h), r.readback(s1, w, h))
+    # in place
+    c = r.target(w, h)
+    r.target_upload(c, r.readback(nb, w, h))
+    dn.denoise([(c, ab, c)])
+    np.testing.assert_array_equal(r.readback(c, w, h), r.readback(s1, w, h))
+    dn.close()
+
+
+def test_backward_every_layer_matches_torch(renderer, setup):
+    r, w, h = renderer, setup["w"], setup["h"]
+    dn = Denoiser(r, seed=11)
+    (na, aa), (nb, ab) = setup["inputs"]
+    ra, rb = setup["refs"]
+    p = dn.params()
+    loss = dn.train_step([(na, aa, ra), (nb, ab, rb)], lr=0.0)
+    np.testing.assert_array_equal(dn.params(), p)  # lr 0 leaves the weights
+    hp, wp = padded(h), padded(w)
+    nbuf = len(dn.buffers)
+    acts = {i: dn.activation(i, 2, hp, wp) for i in range(nbuf - 1)}
+    grads = {i: dn.activation(i, 2, hp, wp, grad=True) for i in range(nbuf)}
+    pred = dn.tensor(capi.APH_DN_PREDICTION).reshape(2, hp, wp, 4)
+    pg = dn.tensor(capi.APH_DN_PARAM_GRADS)
+
+    # loss and its gradient
+    refl = np.zeros((2, hp, wp, 3), dtype=np.float32)
+    mask = np.zeros((2, hp, wp, 1), dtype=bool)
+    for k, t in enumerate((ra, rb)):
+        refl[k, :h, :w] = np.log1p(np.maximum(r.readback(t, w, h)[..., :3], 0))
+        mask[k, :h, :w] = True
+    d = (pred[..., :3] - refl) * mask
+    n = 3.0 * w * h * 2
+    assert abs(loss - np.abs(d).sum() / n) <= 1e-4 * loss
+    g_out = grads[nbuf - 1]
+    clear = np.abs(d) > 1e-5  # the sign of a near-zero difference depends on log1p's last bit
+    np.testing.assert_allclose(g_out[..., :3][clear], bf16(np.sign(d) / n)[clear], rtol=1e-6)
+    assert (g_out[..., 3:] == 0).all()
+
+    # expected activation gradients: shares of every consumer, masked where a ReLU produced the slice
+    shares = {i: [] for i in range(nbuf)}
+    relu_slices = {}
+    for op in dn.ops:
+        if op.kind == capi.APH_DN_OP_CONV:
+            L = dn.layers[op.layer]
+            wgt, _ = dn.layer_params(p, op.layer)
+            dy = grads[L.out_buffer][..., L.out_offset:L.out_offset + L.cout_padded]
+            if L.relu:
+                y = acts[L.out_buffer][..., L.out_offset:L.out_offset + L.cout_padded]
+                assert (dy[y <= 0] == 0).all(), L.name  # mask applied
+                relu_slices.setdefault(L.out_buffer, []).append((L.out_offset, L.cout_padded))
+            x = acts[L.in_buffer]
+            # weight and bias gradients
+            gw_ref = torch.nn.grad.conv2d_weight(nchw(x), (L.cout_padded, L.cin_padded, 3, 3), nchw(dy), padding=1)
+            gw_ref = gw_ref.permute(2, 3, 0, 1).reshape(9, L.cout_padded, L.cin_padded).numpy()
+            gw, gb = dn.layer_params(pg, op.layer)
+            scale = np.abs(gw_ref).max() + 1e-30
+            assert np.abs(gw - gw_ref).max() <= 1e-3 * scale, (L.name, float(np.abs(gw - gw_ref).max()), scale)
+            gb_ref = dy.sum(axis=(0, 1, 2))
+            assert np.abs(gb - gb_ref).max() <= 1e-3 * (np.abs(gb_ref).max() + 1e-30), L.name
+            if op.layer > 0:
+                gx = torch.nn.grad.conv2d_input(x.shape[:1] + (x.shape[3],) + x.shape[1:3], torch_w(bf16(wgt)),
+                                                nchw(dy), padding=1)
+                shares[L.in_buffer].append((0, nhwc(gx)))
+        elif op.kind == capi.APH_DN_OP_POOL:
+            gd = grads[op.dst][..., op.dst_offset:op.dst_offset + op.channels]
+            up = gd.repeat(2, axis=1).repeat(2, axis=2) * 0.25
+            shares[op.src].append((op.src_offset, up))
+        else:
+            gd = nchw(grads[op.dst][..., op.dst_offset:op.dst_offset + op.channels])
+            shares[op.src].append((op.src_offset, nhwc(F.avg_pool2d(gd, 2) * 4.0)))
+    checked = 0
+    for b in range(1, nbuf - 1):
+        if not shares[b]:
+            continue
+        shape = grads[b].shape
+        exp = np.zeros(shape, dtype=np.float32)
+        mag = np.zeros(shape, dtype=np.float32)
+        for off, s in shares[b]:
+            exp[..., off:off + s.shape[-1]] += s
+            mag[..., off:off + s.shape[-1]] += np.abs(s)
+        for off, c in relu_slices.get(b, []):
+            dead = acts[b][..., off:off + c] <= 0
+            exp[..., off:off + c][dead] = 0
+            mag[..., off:off + c][dead] = 0
+        err = np.abs(grads[b] - exp)
+        tol = 2 * BF * mag + 1e-3 * (mag.max() + 1e-30)
+        assert (err <= tol).all(), (b, float(err.max()), float(mag.max()))
+        checked += 1
+    assert checked == nbuf - 2  # every buffer but the input and the prediction gradient
+
+
+def test_training_reduces_loss(renderer, setup):
+    r = renderer
+    dn = Denoiser(r, seed=1)
+    (na, aa), (nb, ab) = setup["inputs"]
+    ra, rb = setup["refs"]
+    ex = [(na, aa, ra), (nb, ab, rb)]
+    losses = [dn.train_step(ex, lr=2e-3) for _ in range(40)]
+    assert all(np.isfinite(losses))
+    assert np.mean(losses[-5:]) < 0.6 * losses[0], losses
+    dn.close()
+
+
+def test_denoiser_errors(renderer, setup):
+    r, w, h = renderer, setup["w"], setup["h"]
+    dn = Denoiser(r, seed=2)
+    (na, aa), (nb, ab) = setup["inputs"]
+    with pytest.raises(AphError) as e:
+        dn.tensor(capi.APH_DN_PREDICTION)  # before any pass
+    assert e.value.code == capi.APH_E_STATE
+    small = r.target(w - 1, h)
+    with pytest.raises(AphError) as e:
+        dn.denoise([(na, aa, small)])
+    assert e.value.code == capi.APH_E_INVALID
+    with pytest.raises(AphError):
+        dn.denoise([(na, aa, nb), (nb, ab, na)])  # output aliases another input
+    with pytest.raises(AphError):
+        dn.set_params(np.zeros(10, dtype=np.float32))
+    dn.denoise([(na, aa, r.target(w, h))])
+    with pytest.raises(AphError) as e:
+        dn.tensor(capi.APH_DN_ACTIVATION_GRADS, 3)  # gradients only after a training step
+    assert e.value.code == capi.APH_E_STATE
+    # parameters round-trip
+    p = dn.params()
+    dn2 = Denoiser(r, seed=99, params=p)
+    np.testing.assert_array_equal(dn2.params(), p)
+    dn.close()
+    dn2.close()

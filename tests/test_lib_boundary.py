@@ -17,7 +17,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_functions():
     text = open(os.path.join(ROOT, "include", "aph.h")).read()
-    return sorted(set(re.findall(r"^\s*(?:int|void|double|const char\*)\s+(aph_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|void|double|uint64_t|const char\*)\s+(aph_\w+)\s*\(", text, re.M)))
 
 
 def test_library_exports_every_declared_entry_point():
