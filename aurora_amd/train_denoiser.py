@@ -1,0 +1,148 @@
+"""Trains the denoiser on the path tracer's own low-spp / 4096-spp pairs.
+
+Data: `--scenes` seeded scenes, alternately of the config-1 family (ground +
+16 icospheres, sky gradient) and the config-2 family (displaced icospheres of
+subdivision 3 or 4 with transmission, coat and a decal layer under a sun sky),
+each seen by one random camera (eye, target and field of view drawn from the
+scene's seed; config-2 seeds start at 1000, so the benchmark's seed-2 scene and
+its camera are never trained on).  Per scene
+the wavefront path tracer renders a 4096-spp reference and noisy inputs (two
+4-spp and one 1-spp realisation with seeds disjoint from the reference's), and
+the guide images of the same camera.  Everything stays resident on the GPU.
+
+Training: Adam on the L1 loss of log1p radiance (aph_denoiser_train_step),
+random batches of one image size, linear warm-up then cosine decay.  The last
+`--holdout` scenes are never trained on; their denoised error is reported.
+
+    python -m aurora_amd.train_denoiser --out aurora_amd/weights/denoiser_unet.npy
+"""
+import argparse
+import json
+import math
+import os
+import time
+
+import numpy as np
+
+from . import capi, scenes
+from .denoiser import DEFAULT_WEIGHTS, Denoiser
+from .session import Renderer
+
+
+def scene_for(i, size, seed0):
+    """Scene i of the training set and its camera."""
+    seed = seed0 + i
+    rng = np.random.Generator(np.random.PCG64(seed * 7919 + 13))
+    if i % 2 == 0:
+        sc = scenes.config1(seed=seed, width=size, height=size, trace_depth=5)
+        radius, target, yr = rng.uniform(8.0, 16.0), (0.0, 0.6, 0.0), (2.0, 9.0)
+        fov = float(rng.uniform(25.0, 60.0))
+    else:
+        sc = scenes.config2(seed=seed, subdiv=3 + (i // 2) % 2, width=size, height=size, env_w=1024, env_h=512,
+                            instances=int(rng.integers(60, 200)), trace_depth=5)
+        radius, target, yr = rng.uniform(16.0, 40.0), (0.0, 1.0, 0.0), (4.0, 18.0)
+        fov = float(rng.uniform(20.0, 55.0))
+    th = rng.uniform(0.0, 2.0 * math.pi)
+    eye = (radius * math.cos(th), float(rng.uniform(*yr)), radius * math.sin(th))
+    tgt = tuple(float(t + rng.uniform(-2.0, 2.0)) for t in target)
+    cam = scenes.look_at_frame(eye, tgt, (0, 1, 0), fov, 1.0, 5)
+    return sc, cam
+
+
+def render_pairs(r, n_scenes, size, ref_spp, seed0, log):
+    """[(noisy, aov, reference)] per scene (lists of examples per scene)."""
+    per_scene = []
+    t0 = time.perf_counter()
+    for i in range(n_scenes):
+        sc, cam = scene_for(i, size, seed0)
+        tlas = r.load_scene(sc)
+        ref = r.target(size, size)
+        r.render([r.request(ref, tlas, cam, 0, ref_spp, 1_000_003)], capi.APH_MODE_WAVEFRONT)
+        aov = r.aov(size, size)
+        vp = scenes.view_proj(cam)
+        r.render_aovs(aov, tlas, cam, vp, vp, np.array(cam.inv_view, dtype=np.float32))
+        ex = []
+        for k, spp in enumerate((4, 4, 1)):
+            t = r.target(size, size)
+            r.render([r.request(t, tlas, cam, 0, spp, 17 + 101 * k)], capi.APH_MODE_WAVEFRONT)
+            ex.append((t, aov, ref))
+        per_scene.append(ex)
+        if (i + 1) % 16 == 0 or i + 1 == n_scenes:
+            r.wait()
+            log(f"rendered {i + 1}/{n_scenes} scenes ({time.perf_counter() - t0:.1f} s)")
+    r.wait()
+    return per_scene
+
+
+def evaluate(r, dn, examples, size):
+    """Mean over examples of (noisy, denoised) L1 of log1p radiance and RMSE of radiance."""
+    out = r.target(size, size)
+    acc = np.zeros(4)
+    for noisy, aov, ref in examples:
+        dn.denoise([(noisy, aov, out)])
+        d = r.readback(out, size, size)[..., :3].astype(np.float64)
+        n = r.readback(noisy, size, size)[..., :3].astype(np.float64)
+        g = np.maximum(r.readback(ref, size, size)[..., :3].astype(np.float64), 0)
+        lg = np.log1p(g)
+        acc += [np.abs(np.log1p(np.maximum(n, 0)) - lg).mean(), np.abs(np.log1p(d) - lg).mean(),
+                np.sqrt(np.mean((n - g) ** 2)), np.sqrt(np.mean((d - g) ** 2))]
+    acc /= max(1, len(examples))
+    return dict(noisy_l1_log=acc[0], denoised_l1_log=acc[1], noisy_rmse=acc[2], denoised_rmse=acc[3])
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--scenes", type=int, default=384)
+    ap.add_argument("--holdout", type=int, default=16)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--ref-spp", type=int, default=4096)
+    ap.add_argument("--steps", type=int, default=16000)
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--seed", type=int, default=1000)
+    ap.add_argument("--init-seed", type=int, default=1)
+    ap.add_argument("--out", default=DEFAULT_WEIGHTS)
+    args = ap.parse_args()
+
+    def log(msg):
+        print(f"[train_denoiser] {msg}", flush=True)
+
+    r = Renderer(0)
+    per_scene = render_pairs(r, args.scenes, args.size, args.ref_spp, args.seed, log)
+    train = [e for ex in per_scene[:args.scenes - args.holdout] for e in ex]
+    held = [e for ex in per_scene[args.scenes - args.holdout:] for e in ex]
+    log(f"{len(train)} training pairs, {len(held)} held-out pairs of {args.size}x{args.size}")
+    dn = Denoiser(r, seed=args.init_seed)
+    rng = np.random.Generator(np.random.PCG64(args.seed))
+    warm = max(1, args.steps // 40)
+    losses = []
+    t0 = time.perf_counter()
+    for step in range(args.steps):
+        if step < warm:
+            lr = args.lr * (step + 1) / warm
+        else:
+            q = (step - warm) / max(1, args.steps - warm)
+            lr = 2e-5 + 0.5 * (args.lr - 2e-5) * (1.0 + math.cos(math.pi * q))
+        batch = [train[j] for j in rng.choice(len(train), args.batch, replace=False)]
+        losses.append(dn.train_step(batch, lr))
+        if (step + 1) % 500 == 0:
+            log(f"step {step + 1}/{args.steps} lr {lr:.2e} loss {np.mean(losses[-500:]):.5f} "
+                f"({time.perf_counter() - t0:.1f} s)")
+    t_train = time.perf_counter() - t0
+    ev = evaluate(r, dn, held, args.size)
+    log("held-out: " + json.dumps({k: round(v, 5) for k, v in ev.items()}))
+    dn.save(args.out)
+    summary = dict(scenes=args.scenes, holdout=args.holdout, size=args.size, ref_spp=args.ref_spp,
+                   noisy_spp=[4, 4, 1], steps=args.steps, batch=args.batch, lr=args.lr, seed=args.seed,
+                   init_seed=args.init_seed, train_seconds=round(t_train, 1),
+                   loss_curve=[round(float(np.mean(losses[i:i + 500])), 5) for i in range(0, len(losses), 500)],
+                   held_out={k: round(v, 5) for k, v in ev.items()})
+    with open(os.path.splitext(args.out)[0] + ".json", "w") as f:
+        json.dump(summary, f, indent=1)
+    log(f"saved {args.out}")
+    dn.close()
+    r.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -33,6 +33,7 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 from aurora_amd import capi, scenes  # noqa: E402
+from aurora_amd import denoiser as dnmod  # noqa: E402
 from aurora_amd.session import Renderer  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
@@ -303,6 +304,94 @@ def equal_time_rmse(r, builder, width=480, height=270, ref_spp=4096, plain_spp=6
             "rmse_ratio": round(a / b, 3)}
 
 
+def denoiser_flops(width, height):
+    """Algorithmic work of one denoise: 2 * 9 * cin * cout multiply-adds per output
+    pixel of every convolution, over the real (unpadded) channels."""
+    layers, bufs, _ = dnmod.arch()
+    hp, wp = dnmod.padded(height), dnmod.padded(width)
+    return sum(2 * 9 * L.cin * L.cout * (hp >> bufs[L.in_buffer].level) * (wp >> bufs[L.in_buffer].level)
+               for L in layers)
+
+
+def denoise_vs_plain(r, builder, width=1920, height=1080, ref_spp=4096, plain_spp=256):
+    """Denoised low-spp frames against the plain path tracer at 256 spp (config2, 1080p,
+    its own camera, which no training image uses): RMSE against a 4096-spp reference
+    rendered with disjoint seeds, and wall time with the device synchronized.  The
+    denoised path renders 4 (or 1) spp with the wavefront pipeline, the guide images,
+    and the U-Net with the committed weights; the plain path is one megakernel launch."""
+    sc = scenes.config2(width=width, height=height)
+    tlas = r.load_scene(sc, builder)
+    dn = dnmod.Denoiser(r)
+    dn.load()
+    w, h = width, height
+    ref = r.target(w, h)
+    step = 512
+    for s0 in range(0, ref_spp, step):
+        r.render([r.request(ref, tlas, sc.frame, s0, min(step, ref_spp - s0), 1_000_003)], capi.APH_MODE_WAVEFRONT)
+        r.wait()
+        log(f"denoise: reference {s0 + step}/{ref_spp} spp")
+    g = r.readback(ref, w, h)[..., :3].astype(np.float64)
+
+    def errors(tgt):
+        img = r.readback(tgt, w, h)[..., :3].astype(np.float64)
+        tm = lambda x: np.maximum(x, 0) / (1.0 + np.maximum(x, 0))  # noqa: E731
+        return (float(np.sqrt(np.mean((img - g) ** 2))), float(np.sqrt(np.mean((tm(img) - tm(g)) ** 2))))
+
+    vp = scenes.view_proj(sc.frame)
+    ivp = np.array(sc.frame.inv_view, dtype=np.float32)
+    aov, noisy, out = r.aov(w, h), r.target(w, h), r.target(w, h)
+
+    def denoised(spp):
+        r.render([r.request(noisy, tlas, sc.frame, 0, spp, 0)], capi.APH_MODE_WAVEFRONT)
+        r.render_aovs(aov, tlas, sc.frame, vp, vp, ivp)
+        dn.denoise([(noisy, aov, out)])
+        r.wait()
+
+    res = {"resolution": f"{w}x{h}", "reference_spp": ref_spp, "scene": "config2 (1M instanced triangles)",
+           "rmse_domain": "linear radiance; tonemapped = x/(1+x)"}
+    denoised(4)  # warm-up: activation buffers, graphs
+    tp = r.target(w, h)
+    t = time.perf_counter()
+    r.render([r.request(tp, tlas, sc.frame, 0, plain_spp, 0)], capi.APH_MODE_MEGAKERNEL)
+    r.wait()
+    res["plain_spp"] = plain_spp
+    res["plain_s"] = round(time.perf_counter() - t, 4)
+    res["plain_rmse"], res["plain_rmse_tonemapped"] = (round(v, 5) for v in errors(tp))
+    for spp in (4, 1):
+        t = time.perf_counter()
+        denoised(spp)
+        dt = time.perf_counter() - t
+        e = errors(out)
+        e_noisy = errors(noisy)
+        res[f"denoised{spp}_s"] = round(dt, 4)
+        res[f"denoised{spp}_rmse"], res[f"denoised{spp}_rmse_tonemapped"] = (round(v, 5) for v in e)
+        res[f"noisy{spp}_rmse"], res[f"noisy{spp}_rmse_tonemapped"] = (round(v, 5) for v in e_noisy)
+        log(f"denoise: {spp} spp done")
+    res["time_ratio_4spp_vs_plain"] = round(res["denoised4_s"] / res["plain_s"], 4)
+    res["rmse_ratio_4spp_vs_plain"] = round(res["denoised4_rmse"] / res["plain_rmse"], 3)
+    res["rmse_ratio_4spp_vs_plain_tonemapped"] = round(
+        res["denoised4_rmse_tonemapped"] / res["plain_rmse_tonemapped"], 3)
+    # plain samples that would match the denoised error (error ~ 1/sqrt(spp))
+    res["plain_spp_equal_rmse"] = int(plain_spp * (res["plain_rmse"] / res["denoised4_rmse"]) ** 2)
+    res["plain_spp_equal_rmse_tonemapped"] = int(
+        plain_spp * (res["plain_rmse_tonemapped"] / res["denoised4_rmse_tonemapped"]) ** 2)
+    # the U-Net alone: HIP events around aph_denoise, MFMA work over real channels
+    r.reset_timing()
+    r.enable_timing(True)
+    for _ in range(5):
+        dn.denoise([(noisy, aov, out)])
+    r.wait()
+    r.enable_timing(False)
+    ms, n = r.timing("denoise")
+    flops = denoiser_flops(w, h)
+    res["unet_ms"] = round(ms / max(n, 1), 3)
+    res["unet_tflops"] = round(flops / (ms / max(n, 1) * 1e-3) / 1e12, 1)
+    res["unet_mfma_frac_of_bf16_peak"] = round(res["unet_tflops"] / 2500.0, 4)
+    res["weights"] = os.path.relpath(dnmod.DEFAULT_WEIGHTS, ROOT)
+    dn.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -312,9 +401,10 @@ def main():
     ap.add_argument("--subdiv", type=int, default=4, help="4: 1.02M instanced tris; 5: 4.1M")
     ap.add_argument("--mode", choices=["megakernel", "wavefront"], default="wavefront")
     ap.add_argument("--builder", choices=["sah", "lbvh"], default="sah", help="BLAS builder")
-    ap.add_argument("--sections", default="literal,c3,serving,rmse",
+    ap.add_argument("--sections", default="literal,c3,serving,rmse,denoise",
                     help="extra measurements after the headline line's frame: literal (config2 at subdiv 5), "
-                         "c3 (64 cameras), serving (Poisson replay), rmse (equal-time RMSE); '' for none")
+                         "c3 (64 cameras), serving (Poisson replay), rmse (equal-time RMSE), denoise (denoised "
+                         "4 spp vs plain 256 spp at 1080p); '' for none")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 counter passes")
     ap.add_argument("--cpu-baseline-child", action="store_true",
@@ -458,6 +548,9 @@ def main():
     if "rmse" in sections:
         out["equal_time_rmse"] = equal_time_rmse(r, builder)
         log(f"rmse: {out['equal_time_rmse']}")
+    if "denoise" in sections:
+        out["denoise"] = denoise_vs_plain(r, builder)
+        log(f"denoise: {out['denoise']}")
     r.close()
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle on host cores, child process)...")
