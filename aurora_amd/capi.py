@@ -147,7 +147,8 @@ class DnLayer(C.Structure):
 
 
 class DenoiseExample(C.Structure):
-    _fields_ = [("noisy", C.c_uint64), ("aov", C.c_uint64), ("reference", C.c_uint64)]
+    _fields_ = [("noisy", C.c_uint64), ("aov", C.c_uint64), ("reference", C.c_uint64), ("transform", C.c_uint32),
+                ("_pad", C.c_uint32)]
 
 
 class DenoiseItem(C.Structure):
@@ -155,6 +156,7 @@ class DenoiseItem(C.Structure):
 
 
 APH_DN_OP_CONV, APH_DN_OP_POOL, APH_DN_OP_UPSAMPLE = 0, 1, 2
+APH_DN_LOSS_L1_LOG, APH_DN_LOSS_L2_TONEMAPPED = 0, 1
 APH_DN_PARAMS, APH_DN_PARAM_GRADS, APH_DN_ACTIVATIONS, APH_DN_ACTIVATION_GRADS, APH_DN_PREDICTION = 0, 1, 2, 3, 4
 
 _P = C.c_void_p
@@ -221,6 +223,7 @@ SIGNATURES = {
     "aph_denoiser_create": (_i, [_P, _u64, C.POINTER(_u64)]),
     "aph_denoiser_destroy": (_i, [_P, _u64]),
     "aph_denoiser_set_params": (_i, [_P, _u64, _P, _u64]),
+    "aph_denoiser_set_loss": (_i, [_P, _u64, _i]),
     "aph_denoiser_tensor_size": (_i, [_P, _u64, _i, _u32, C.POINTER(_u64)]),
     "aph_denoiser_tensor": (_i, [_P, _u64, _i, _u32, _P, _u64]),
     "aph_denoiser_train_step": (_i, [_P, _u64, _P, _u32, C.c_float, C.POINTER(C.c_float)]),

@@ -10,6 +10,8 @@ namespace aph {
 constexpr int kDnInChannels = 12;  // log radiance 3, albedo 3, metalness, normal 3, roughness, depth
 constexpr int kDnAlign = 16;       // image sides are padded to this (four 2x2 poolings)
 constexpr int kDnPredStride = 4;   // fp32 prediction: log radiance 3 + pad
+constexpr int kDnLossL1Log = 0;        // L1 of log1p radiance
+constexpr int kDnLossL2Tonemapped = 1; // L2 of x / (1 + x)
 
 // 3x3 convolution, stride 1, zero padding 1, NHWC.  Input channels [0, cinP)
 // of a buffer with stride inStride (cinP a multiple of 32); weights
@@ -47,12 +49,14 @@ hipError_t launchUpsample(const __bf16* in, int inStride, __bf16* out, int outSt
                           int Wo, int C, hipStream_t st);
 hipError_t launchUpsampleBack(const __bf16* dout, int doStride, int doOff, __bf16* din, int diStride, int N, int Hi,
                               int Wi, int C, hipStream_t st);
+// transform: destination (x, y) reads source (sx, sy), (u, v) = bit 2 ? (y, x) : (x, y),
+// sx = bit 0 ? W-1-u : u, sy = bit 1 ? H-1-v : v (transposition needs a square image)
 hipError_t launchPack(const float4* noisy, const float* depth, const float4* normalRough, const float4* albedo, int W,
-                      int H, int Wp, int Hp, int img, __bf16* out, int stride, hipStream_t st);
+                      int H, int Wp, int Hp, int img, uint32_t transform, __bf16* out, int stride, hipStream_t st);
 hipError_t launchUnpack(const float* pred, int W, int H, int Wp, int Hp, int img, const float4* noisy, float4* out,
                         hipStream_t st);
 hipError_t launchLoss(const float* pred, const float4* ref, int W, int H, int Wp, int Hp, int img, int nImg,
-                      __bf16* dpred, int dStride, float* loss, hipStream_t st);
+                      uint32_t transform, int kind, __bf16* dpred, int dStride, float* loss, hipStream_t st);
 hipError_t launchAdam(float* w, float* m, float* v, const float* g, int64_t n, float lr, int step, hipStream_t st);
 hipError_t launchConvWeights(const float* w, int coutP, int cinP, __bf16* wf, __bf16* wt, hipStream_t st);
 hipError_t launchBf16ToF32(const __bf16* in, float* out, int64_t n, hipStream_t st);

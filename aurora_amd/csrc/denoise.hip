@@ -225,9 +225,22 @@ __global__ void kUpsample2Back(const __bf16* dout, int doStride, int doOff, __bf
     }
 }
 
+// Source pixel of destination (x, y) under a training transform: bit 2
+// transposes (square images), bit 0 mirrors x, bit 1 mirrors y.
+__device__ __forceinline__ void sourcePixel(int x, int y, int W, int H, uint32_t xf, int& sx, int& sy)
+{
+    if (xf & 4u) {
+        int t = x;
+        x = y;
+        y = t;
+    }
+    sx = (xf & 1u) ? W - 1 - x : x;
+    sy = (xf & 2u) ? H - 1 - y : y;
+}
+
 // ---- input packing: log1p radiance, albedo, metalness, normal, roughness, depth ----
 __global__ void kPack(const float4* noisy, const float* depth, const float4* normalRough, const float4* albedo,
-                      int W, int H, int Wp, int Hp, int img, __bf16* out, int stride)
+                      int W, int H, int Wp, int Hp, int img, uint32_t xf, __bf16* out, int stride)
 {
     const int64_t n = (int64_t)Wp * Hp;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
@@ -235,7 +248,9 @@ __global__ void kPack(const float4* noisy, const float* depth, const float4* nor
         __bf16* o = out + ((int64_t)img * n + i) * stride;
         float f[kDnInChannels] = {};
         if (x < W && y < H) {
-            int64_t s = (int64_t)y * W + x;
+            int sx, sy;
+            sourcePixel(x, y, W, H, xf, sx, sy);
+            int64_t s = (int64_t)sy * W + sx;
             float4 c = noisy[s], nr = normalRough[s], al = albedo[s];
             f[0] = log1pf(fmaxf(c.x, 0.0f));
             f[1] = log1pf(fmaxf(c.y, 0.0f));
@@ -269,10 +284,12 @@ __global__ void kUnpack(const float* pred, int W, int H, int Wp, int Hp, int img
     }
 }
 
-// ---- L1 loss in log space against the reference, and its gradient ----
-// loss = mean over the unpadded pixels, 3 channels and nImg images.
-__global__ void kLossL1(const float* pred, const float4* ref, int W, int H, int Wp, int Hp, int img, int nImg,
-                        __bf16* dpred, int dStride, float* loss)
+// ---- loss against the reference and its gradient, mean over the unpadded
+// pixels, 3 channels and nImg images.  With p the predicted log1p radiance and
+// q = log1p(reference): L1 of p - q, or L2 of the tonemapped values
+// x / (1 + x) = 1 - exp(-p) ----
+__global__ void kLoss(const float* pred, const float4* ref, int W, int H, int Wp, int Hp, int img, int nImg,
+                      uint32_t xf, int kind, __bf16* dpred, int dStride, float* loss)
 {
     const int64_t n = (int64_t)Wp * Hp;
     const float scale = 1.0f / (3.0f * (float)W * (float)H * (float)nImg);
@@ -280,14 +297,28 @@ __global__ void kLossL1(const float* pred, const float4* ref, int W, int H, int 
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         int x = (int)(i % Wp), y = (int)(i / Wp);
         const int64_t q = (int64_t)img * n + i;
+        float4 r = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        const bool inside = x < W && y < H;
+        if (inside) {
+            int sx, sy;
+            sourcePixel(x, y, W, H, xf, sx, sy);
+            r = ref[(int64_t)sy * W + sx];
+        }
         for (int c = 0; c < dStride; c++) {
             float g = 0.0f;
-            if (c < 3 && x < W && y < H) {
-                float4 r = ref[(int64_t)y * W + x];
+            if (c < 3 && inside) {
                 float rv = log1pf(fmaxf(c == 0 ? r.x : (c == 1 ? r.y : r.z), 0.0f));
-                float d = pred[q * kDnPredStride + c] - rv;
-                part += fabsf(d) * scale;
-                g = (d > 0.0f ? 1.0f : (d < 0.0f ? -1.0f : 0.0f)) * scale;
+                float p = pred[q * kDnPredStride + c];
+                if (kind == kDnLossL2Tonemapped) {
+                    float ep = expf(-p);
+                    float d = expf(-rv) - ep;  // (1 - e^-p) - (1 - e^-q)
+                    part += d * d * scale;
+                    g = 2.0f * d * ep * scale;
+                } else {
+                    float d = p - rv;
+                    part += fabsf(d) * scale;
+                    g = (d > 0.0f ? 1.0f : (d < 0.0f ? -1.0f : 0.0f)) * scale;
+                }
             }
             dpred[q * dStride + c] = toBf(g);
         }
@@ -404,10 +435,10 @@ hipError_t launchUpsampleBack(const __bf16* dout, int doStride, int doOff, __bf1
 }
 
 hipError_t launchPack(const float4* noisy, const float* depth, const float4* normalRough, const float4* albedo, int W,
-                      int H, int Wp, int Hp, int img, __bf16* out, int stride, hipStream_t st)
+                      int H, int Wp, int Hp, int img, uint32_t transform, __bf16* out, int stride, hipStream_t st)
 {
-    kPack<<<gridFor((int64_t)Wp * Hp), 256, 0, st>>>(noisy, depth, normalRough, albedo, W, H, Wp, Hp, img, out,
-                                                     stride);
+    kPack<<<gridFor((int64_t)Wp * Hp), 256, 0, st>>>(noisy, depth, normalRough, albedo, W, H, Wp, Hp, img, transform,
+                                                     out, stride);
     return hipGetLastError();
 }
 
@@ -419,9 +450,10 @@ hipError_t launchUnpack(const float* pred, int W, int H, int Wp, int Hp, int img
 }
 
 hipError_t launchLoss(const float* pred, const float4* ref, int W, int H, int Wp, int Hp, int img, int nImg,
-                      __bf16* dpred, int dStride, float* loss, hipStream_t st)
+                      uint32_t transform, int kind, __bf16* dpred, int dStride, float* loss, hipStream_t st)
 {
-    kLossL1<<<gridFor((int64_t)Wp * Hp), 256, 0, st>>>(pred, ref, W, H, Wp, Hp, img, nImg, dpred, dStride, loss);
+    kLoss<<<gridFor((int64_t)Wp * Hp), 256, 0, st>>>(pred, ref, W, H, Wp, Hp, img, nImg, transform, kind, dpred,
+                                                     dStride, loss);
     return hipGetLastError();
 }
 

@@ -22,6 +22,9 @@
 #include "context.h"
 #include "denoise.h"
 
+static_assert(APH_DN_LOSS_L1_LOG == aph::kDnLossL1Log && APH_DN_LOSS_L2_TONEMAPPED == aph::kDnLossL2Tonemapped,
+              "loss kinds of the C-ABI and the kernels agree");
+
 using namespace aph;
 
 namespace aph {
@@ -152,6 +155,7 @@ struct aph_denoiser {
     int N = 0, Hp = 0, Wp = 0;  // shape of the last forward pass
     bool haveGrads = false;     // actGrads hold the last training step's gradients
     int step = 0;
+    int lossKind = APH_DN_LOSS_L1_LOG;
 
     int levelH(int l) const { return Hp >> l; }
     int levelW(int l) const { return Wp >> l; }
@@ -353,12 +357,13 @@ int resolveInputs(aph_ctx* c, const uint64_t* noisy, const uint64_t* aovs, uint3
 }
 
 hipError_t packInputs(aph_denoiser* d, const std::vector<Target*>& tg, const std::vector<AovSet*>& av, uint32_t W,
-                      uint32_t H, hipStream_t st)
+                      uint32_t H, const std::vector<uint32_t>& xf, hipStream_t st)
 {
     for (size_t i = 0; i < tg.size(); i++) {
         hipError_t e = launchPack(tg[i]->accum.as<float4>(), av[i]->depth.as<float>(),
                                   av[i]->normalRoughness.as<float4>(), av[i]->baseColorMetalness.as<float4>(), (int)W,
-                                  (int)H, d->Wp, d->Hp, (int)i, d->act(B_IN), kBufs[B_IN].channels, st);
+                                  (int)H, d->Wp, d->Hp, (int)i, xf.empty() ? 0u : xf[i], d->act(B_IN),
+                                  kBufs[B_IN].channels, st);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
@@ -482,6 +487,18 @@ int aph_denoiser_set_params(aph_ctx_t c, uint64_t dn, const float* params, uint6
     return APH_OK;
 }
 
+int aph_denoiser_set_loss(aph_ctx_t c, uint64_t dn, int kind)
+{
+    int rc = enter(c);
+    if (rc) return rc;
+    aph_denoiser* d = findDenoiser(c, dn);
+    if (!d) return fail(c, APH_E_INVALID, "unknown denoiser %llu", (unsigned long long)dn);
+    if (kind != APH_DN_LOSS_L1_LOG && kind != APH_DN_LOSS_L2_TONEMAPPED)
+        return fail(c, APH_E_INVALID, "unknown loss %d", kind);
+    d->lossKind = kind;
+    return APH_OK;
+}
+
 int aph_denoiser_tensor_size(aph_ctx_t c, uint64_t dn, int kind, uint32_t index, uint64_t* count)
 {
     int rc = enter(c);
@@ -542,15 +559,21 @@ int aph_denoiser_train_step(aph_ctx_t c, uint64_t dn, const aph_denoise_example*
     if (!ex || n == 0 || n > 256) return fail(c, APH_E_INVALID, "need 1..256 training examples");
     if (!(lr >= 0.0f) || !std::isfinite(lr)) return fail(c, APH_E_INVALID, "bad learning rate");
     std::vector<uint64_t> noisy(n), aov(n);
+    std::vector<uint32_t> xf(n);
     for (uint32_t i = 0; i < n; i++) {
         noisy[i] = ex[i].noisy;
         aov[i] = ex[i].aov;
+        xf[i] = ex[i].transform;
     }
     std::vector<Target*> tg;
     std::vector<AovSet*> av;
     uint32_t W = 0, H = 0;
     rc = resolveInputs(c, noisy.data(), aov.data(), n, tg, av, W, H);
     if (rc) return rc;
+    for (uint32_t i = 0; i < n; i++) {
+        if (xf[i] > 7u) return fail(c, APH_E_INVALID, "example %u: transform %u out of range", i, xf[i]);
+        if ((xf[i] & 4u) && W != H) return fail(c, APH_E_INVALID, "example %u: transposing needs a square image", i);
+    }
     std::vector<Target*> refs(n);
     for (uint32_t i = 0; i < n; i++) {
         refs[i] = lookup(c->targets, ex[i].reference);
@@ -562,13 +585,14 @@ int aph_denoiser_train_step(aph_ctx_t c, uint64_t dn, const aph_denoise_example*
     hipStream_t st = c->stream;
     TimedScope ts(c, "denoiser_train_step");
     HIPCALL(c, reserveShape(d, (int)n, padded(H), padded(W), true), "alloc denoiser activations");
-    HIPCALL(c, packInputs(d, tg, av, W, H, st), "denoiser pack");
+    HIPCALL(c, packInputs(d, tg, av, W, H, xf, st), "denoiser pack");
     HIPCALL(c, forward(d, st), "denoiser forward");
     HIPCALL(c, hipMemsetAsync(d->loss.p, 0, 4, st), "clear loss");
     HIPCALL(c, hipMemsetAsync(d->grads.p, 0, o.nParams * 4, st), "clear gradients");
     for (uint32_t i = 0; i < n; i++)
         HIPCALL(c, launchLoss(d->pred.as<float>(), refs[i]->accum.as<float4>(), (int)W, (int)H, d->Wp, d->Hp, (int)i,
-                              (int)n, d->grad(B_OUT), kBufs[B_OUT].channels, d->loss.as<float>(), st),
+                              (int)n, xf[i], d->lossKind, d->grad(B_OUT), kBufs[B_OUT].channels,
+                              d->loss.as<float>(), st),
                 "denoiser loss");
     HIPCALL(c, backward(d, st), "denoiser backward");
     d->haveGrads = true;
@@ -615,7 +639,7 @@ int aph_denoise(aph_ctx_t c, uint64_t dn, const aph_denoise_item* items, uint32_
     hipStream_t st = c->stream;
     TimedScope ts(c, "denoise");
     HIPCALL(c, reserveShape(d, (int)n, padded(H), padded(W), false), "alloc denoiser activations");
-    HIPCALL(c, packInputs(d, tg, av, W, H, st), "denoiser pack");
+    HIPCALL(c, packInputs(d, tg, av, W, H, {}, st), "denoiser pack");
     HIPCALL(c, forward(d, st), "denoiser forward");
     for (uint32_t i = 0; i < n; i++)
         HIPCALL(c, launchUnpack(d->pred.as<float>(), (int)W, (int)H, d->Wp, d->Hp, (int)i, tg[i]->accum.as<float4>(),

@@ -3,8 +3,8 @@
 A U-Net over the noisy radiance of a target and the guides of an AOV set,
 trained and run by the bf16 MFMA kernels of libaph.so.  The reference hands
 the same guides to NVIDIA NRD (DirectX/Denoiser.h:24-35); this network is
-its own (no reference counterpart), and `train_pairs` trains it on the path
-tracer's own low-spp / high-spp renders.
+its own (no reference counterpart); train_denoiser.py trains it on the path
+tracer's own low-spp / 4096-spp renders.
 """
 import ctypes as C
 import os
@@ -49,7 +49,6 @@ class Denoiser:
         self.r._check(self.lib.aph_denoiser_create(self.r.ctx, seed, C.byref(h)))
         self.h = h.value
         self.layers, self.buffers, self.ops = arch()
-        self.shape = None  # (n, Hp, Wp) of the last pass
         if params is not None:
             self.set_params(params)
 
@@ -74,8 +73,11 @@ class Denoiser:
         self.set_params(np.load(path, allow_pickle=False))
 
     # ---- training and inference
+    def set_loss(self, kind):
+        self.r._check(self.lib.aph_denoiser_set_loss(self.r.ctx, self.h, kind))
+
     def train_step(self, examples, lr):
-        """examples: [(noisy_target, aov, reference_target)] of one size; returns the loss."""
+        """examples: [(noisy_target, aov, reference_target[, transform])] of one size; returns the loss."""
         arr = (capi.DenoiseExample * len(examples))(*[capi.DenoiseExample(*e) for e in examples])
         loss = C.c_float()
         self.r._check(self.lib.aph_denoiser_train_step(self.r.ctx, self.h, C.cast(arr, C.c_void_p), len(examples),

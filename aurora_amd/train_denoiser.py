@@ -10,8 +10,10 @@ the wavefront path tracer renders a 4096-spp reference and noisy inputs (two
 4-spp and one 1-spp realisation with seeds disjoint from the reference's), and
 the guide images of the same camera.  Everything stays resident on the GPU.
 
-Training: Adam on the L1 loss of log1p radiance (aph_denoiser_train_step),
-random batches of one image size, linear warm-up then cosine decay.  The last
+Training: Adam (aph_denoiser_train_step) on the L1 loss of log1p radiance or
+the L2 loss of tonemapped radiance x / (1 + x), random batches of one image
+size, each pair randomly mirrored / transposed, linear warm-up then cosine
+decay.  The last
 `--holdout` scenes are never trained on; their denoised error is reported.
 
     python -m aurora_amd.train_denoiser --out aurora_amd/weights/denoiser_unet.npy
@@ -29,16 +31,19 @@ from .denoiser import DEFAULT_WEIGHTS, Denoiser
 from .session import Renderer
 
 
+LOSSES = {"l1log": capi.APH_DN_LOSS_L1_LOG, "l2tm": capi.APH_DN_LOSS_L2_TONEMAPPED}
+
+
 def scene_for(i, size, seed0):
-    """Scene i of the training set and its camera."""
+    """Scene i of the training set and its camera (every fourth of the config-1 family)."""
     seed = seed0 + i
     rng = np.random.Generator(np.random.PCG64(seed * 7919 + 13))
-    if i % 2 == 0:
+    if i % 4 == 0:
         sc = scenes.config1(seed=seed, width=size, height=size, trace_depth=5)
         radius, target, yr = rng.uniform(8.0, 16.0), (0.0, 0.6, 0.0), (2.0, 9.0)
         fov = float(rng.uniform(25.0, 60.0))
     else:
-        sc = scenes.config2(seed=seed, subdiv=3 + (i // 2) % 2, width=size, height=size, env_w=1024, env_h=512,
+        sc = scenes.config2(seed=seed, subdiv=3 + i % 2, width=size, height=size, env_w=1024, env_h=512,
                             instances=int(rng.integers(60, 200)), trace_depth=5)
         radius, target, yr = rng.uniform(16.0, 40.0), (0.0, 1.0, 0.0), (4.0, 18.0)
         fov = float(rng.uniform(20.0, 55.0))
@@ -75,19 +80,57 @@ def render_pairs(r, n_scenes, size, ref_spp, seed0, log):
 
 
 def evaluate(r, dn, examples, size):
-    """Mean over examples of (noisy, denoised) L1 of log1p radiance and RMSE of radiance."""
+    """Means over examples of the noisy and denoised errors: L1 of log1p radiance,
+    RMSE of radiance and RMSE of x / (1 + x)."""
     out = r.target(size, size)
-    acc = np.zeros(4)
+    names = ["l1_log", "rmse", "rmse_tonemapped"]
+    acc = np.zeros((2, 3))
+    tm = lambda x: np.maximum(x, 0) / (1.0 + np.maximum(x, 0))  # noqa: E731
     for noisy, aov, ref in examples:
         dn.denoise([(noisy, aov, out)])
-        d = r.readback(out, size, size)[..., :3].astype(np.float64)
-        n = r.readback(noisy, size, size)[..., :3].astype(np.float64)
         g = np.maximum(r.readback(ref, size, size)[..., :3].astype(np.float64), 0)
-        lg = np.log1p(g)
-        acc += [np.abs(np.log1p(np.maximum(n, 0)) - lg).mean(), np.abs(np.log1p(d) - lg).mean(),
-                np.sqrt(np.mean((n - g) ** 2)), np.sqrt(np.mean((d - g) ** 2))]
+        for k, t in enumerate((noisy, out)):
+            x = np.maximum(r.readback(t, size, size)[..., :3].astype(np.float64), 0)
+            acc[k] += [np.abs(np.log1p(x) - np.log1p(g)).mean(), np.sqrt(np.mean((x - g) ** 2)),
+                       np.sqrt(np.mean((tm(x) - tm(g)) ** 2))]
     acc /= max(1, len(examples))
-    return dict(noisy_l1_log=acc[0], denoised_l1_log=acc[1], noisy_rmse=acc[2], denoised_rmse=acc[3])
+    return {f"{w}_{n}": float(acc[k, j]) for k, w in enumerate(("noisy", "denoised")) for j, n in enumerate(names)}
+
+
+def train(r, args, train_set, held, loss_name, out_path, log):
+    dn = Denoiser(r, seed=args.init_seed)
+    dn.set_loss(LOSSES[loss_name])
+    rng = np.random.Generator(np.random.PCG64(args.seed))
+    warm = max(1, args.steps // 40)
+    losses = []
+    t0 = time.perf_counter()
+    for step in range(args.steps):
+        if step < warm:
+            lr = args.lr * (step + 1) / warm
+        else:
+            q = (step - warm) / max(1, args.steps - warm)
+            lr = 2e-5 + 0.5 * (args.lr - 2e-5) * (1.0 + math.cos(math.pi * q))
+        pick = rng.choice(len(train_set), args.batch, replace=False)
+        xf = rng.integers(0, 8 if args.augment else 1, args.batch)
+        batch = [train_set[j] + (int(t),) for j, t in zip(pick, xf)]
+        losses.append(dn.train_step(batch, lr))
+        if (step + 1) % 500 == 0:
+            log(f"[{loss_name}] step {step + 1}/{args.steps} lr {lr:.2e} loss {np.mean(losses[-500:]):.6f} "
+                f"({time.perf_counter() - t0:.1f} s)")
+    t_train = time.perf_counter() - t0
+    ev = evaluate(r, dn, held, args.size)
+    log(f"[{loss_name}] held-out: " + json.dumps({k: round(v, 5) for k, v in ev.items()}))
+    dn.save(out_path)
+    summary = dict(scenes=args.scenes, holdout=args.holdout, size=args.size, ref_spp=args.ref_spp,
+                   noisy_spp=[4, 4, 1], loss=loss_name, augment=bool(args.augment), steps=args.steps,
+                   batch=args.batch, lr=args.lr, seed=args.seed, init_seed=args.init_seed,
+                   train_seconds=round(t_train, 1),
+                   loss_curve=[float(np.mean(losses[i:i + 500])) for i in range(0, len(losses), 500)],
+                   held_out={k: round(v, 6) for k, v in ev.items()})
+    with open(os.path.splitext(out_path)[0] + ".json", "w") as f:
+        json.dump(summary, f, indent=1)
+    log(f"saved {out_path}")
+    dn.close()
 
 
 def main():
@@ -101,46 +144,29 @@ def main():
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=1000)
     ap.add_argument("--init-seed", type=int, default=1)
+    ap.add_argument("--loss", default="l1log",
+                    help="l1log, l2tm, or a comma-separated list: each trains its own network on the same data "
+                         "and saves <out>_<loss>.npy")
+    ap.add_argument("--no-augment", dest="augment", action="store_false",
+                    help="no random mirroring / transposition of the training pairs")
     ap.add_argument("--out", default=DEFAULT_WEIGHTS)
     args = ap.parse_args()
 
     def log(msg):
         print(f"[train_denoiser] {msg}", flush=True)
 
+    losses = [x for x in args.loss.split(",") if x]
+    for x in losses:
+        if x not in LOSSES:
+            raise SystemExit(f"unknown loss {x}")
     r = Renderer(0)
     per_scene = render_pairs(r, args.scenes, args.size, args.ref_spp, args.seed, log)
-    train = [e for ex in per_scene[:args.scenes - args.holdout] for e in ex]
+    train_set = [e for ex in per_scene[:args.scenes - args.holdout] for e in ex]
     held = [e for ex in per_scene[args.scenes - args.holdout:] for e in ex]
-    log(f"{len(train)} training pairs, {len(held)} held-out pairs of {args.size}x{args.size}")
-    dn = Denoiser(r, seed=args.init_seed)
-    rng = np.random.Generator(np.random.PCG64(args.seed))
-    warm = max(1, args.steps // 40)
-    losses = []
-    t0 = time.perf_counter()
-    for step in range(args.steps):
-        if step < warm:
-            lr = args.lr * (step + 1) / warm
-        else:
-            q = (step - warm) / max(1, args.steps - warm)
-            lr = 2e-5 + 0.5 * (args.lr - 2e-5) * (1.0 + math.cos(math.pi * q))
-        batch = [train[j] for j in rng.choice(len(train), args.batch, replace=False)]
-        losses.append(dn.train_step(batch, lr))
-        if (step + 1) % 500 == 0:
-            log(f"step {step + 1}/{args.steps} lr {lr:.2e} loss {np.mean(losses[-500:]):.5f} "
-                f"({time.perf_counter() - t0:.1f} s)")
-    t_train = time.perf_counter() - t0
-    ev = evaluate(r, dn, held, args.size)
-    log("held-out: " + json.dumps({k: round(v, 5) for k, v in ev.items()}))
-    dn.save(args.out)
-    summary = dict(scenes=args.scenes, holdout=args.holdout, size=args.size, ref_spp=args.ref_spp,
-                   noisy_spp=[4, 4, 1], steps=args.steps, batch=args.batch, lr=args.lr, seed=args.seed,
-                   init_seed=args.init_seed, train_seconds=round(t_train, 1),
-                   loss_curve=[round(float(np.mean(losses[i:i + 500])), 5) for i in range(0, len(losses), 500)],
-                   held_out={k: round(v, 5) for k, v in ev.items()})
-    with open(os.path.splitext(args.out)[0] + ".json", "w") as f:
-        json.dump(summary, f, indent=1)
-    log(f"saved {args.out}")
-    dn.close()
+    log(f"{len(train_set)} training pairs, {len(held)} held-out pairs of {args.size}x{args.size}")
+    for x in losses:
+        out = args.out if len(losses) == 1 else f"{os.path.splitext(args.out)[0]}_{x}.npy"
+        train(r, args, train_set, held, x, out, log)
     r.close()
 
 

@@ -313,16 +313,17 @@ def denoiser_flops(width, height):
                for L in layers)
 
 
-def denoise_vs_plain(r, builder, width=1920, height=1080, ref_spp=4096, plain_spp=256):
+def denoise_vs_plain(r, builder, width=1920, height=1080, ref_spp=4096, plain_spp=256, weights=None):
     """Denoised low-spp frames against the plain path tracer at 256 spp (config2, 1080p,
     its own camera, which no training image uses): RMSE against a 4096-spp reference
     rendered with disjoint seeds, and wall time with the device synchronized.  The
     denoised path renders 4 (or 1) spp with the wavefront pipeline, the guide images,
-    and the U-Net with the committed weights; the plain path is one megakernel launch."""
+    and the U-Net with the committed weights; the plain path is one megakernel launch.
+    Further weight files, if given, are scored on the same images under "variants"."""
+    weights = list(weights or [dnmod.DEFAULT_WEIGHTS])
     sc = scenes.config2(width=width, height=height)
     tlas = r.load_scene(sc, builder)
     dn = dnmod.Denoiser(r)
-    dn.load()
     w, h = width, height
     ref = r.target(w, h)
     step = 512
@@ -335,7 +336,8 @@ def denoise_vs_plain(r, builder, width=1920, height=1080, ref_spp=4096, plain_sp
     def errors(tgt):
         img = r.readback(tgt, w, h)[..., :3].astype(np.float64)
         tm = lambda x: np.maximum(x, 0) / (1.0 + np.maximum(x, 0))  # noqa: E731
-        return (float(np.sqrt(np.mean((img - g) ** 2))), float(np.sqrt(np.mean((tm(img) - tm(g)) ** 2))))
+        return (round(float(np.sqrt(np.mean((img - g) ** 2))), 5),
+                round(float(np.sqrt(np.mean((tm(img) - tm(g)) ** 2))), 5))
 
     vp = scenes.view_proj(sc.frame)
     ivp = np.array(sc.frame.inv_view, dtype=np.float32)
@@ -349,6 +351,7 @@ def denoise_vs_plain(r, builder, width=1920, height=1080, ref_spp=4096, plain_sp
 
     res = {"resolution": f"{w}x{h}", "reference_spp": ref_spp, "scene": "config2 (1M instanced triangles)",
            "rmse_domain": "linear radiance; tonemapped = x/(1+x)"}
+    dn.load(weights[0])
     denoised(4)  # warm-up: activation buffers, graphs
     tp = r.target(w, h)
     t = time.perf_counter()
@@ -356,25 +359,35 @@ def denoise_vs_plain(r, builder, width=1920, height=1080, ref_spp=4096, plain_sp
     r.wait()
     res["plain_spp"] = plain_spp
     res["plain_s"] = round(time.perf_counter() - t, 4)
-    res["plain_rmse"], res["plain_rmse_tonemapped"] = (round(v, 5) for v in errors(tp))
-    for spp in (4, 1):
-        t = time.perf_counter()
-        denoised(spp)
-        dt = time.perf_counter() - t
-        e = errors(out)
-        e_noisy = errors(noisy)
-        res[f"denoised{spp}_s"] = round(dt, 4)
-        res[f"denoised{spp}_rmse"], res[f"denoised{spp}_rmse_tonemapped"] = (round(v, 5) for v in e)
-        res[f"noisy{spp}_rmse"], res[f"noisy{spp}_rmse_tonemapped"] = (round(v, 5) for v in e_noisy)
-        log(f"denoise: {spp} spp done")
-    res["time_ratio_4spp_vs_plain"] = round(res["denoised4_s"] / res["plain_s"], 4)
-    res["rmse_ratio_4spp_vs_plain"] = round(res["denoised4_rmse"] / res["plain_rmse"], 3)
-    res["rmse_ratio_4spp_vs_plain_tonemapped"] = round(
-        res["denoised4_rmse_tonemapped"] / res["plain_rmse_tonemapped"], 3)
-    # plain samples that would match the denoised error (error ~ 1/sqrt(spp))
-    res["plain_spp_equal_rmse"] = int(plain_spp * (res["plain_rmse"] / res["denoised4_rmse"]) ** 2)
-    res["plain_spp_equal_rmse_tonemapped"] = int(
-        plain_spp * (res["plain_rmse_tonemapped"] / res["denoised4_rmse_tonemapped"]) ** 2)
+    res["plain_rmse"], res["plain_rmse_tonemapped"] = errors(tp)
+
+    def score(into):
+        for spp in (4, 1):
+            t = time.perf_counter()
+            denoised(spp)
+            into[f"denoised{spp}_s"] = round(time.perf_counter() - t, 4)
+            into[f"denoised{spp}_rmse"], into[f"denoised{spp}_rmse_tonemapped"] = errors(out)
+            into[f"noisy{spp}_rmse"], into[f"noisy{spp}_rmse_tonemapped"] = errors(noisy)
+        into["time_ratio_4spp_vs_plain"] = round(into["denoised4_s"] / res["plain_s"], 4)
+        into["rmse_ratio_4spp_vs_plain"] = round(into["denoised4_rmse"] / res["plain_rmse"], 3)
+        into["rmse_ratio_4spp_vs_plain_tonemapped"] = round(
+            into["denoised4_rmse_tonemapped"] / res["plain_rmse_tonemapped"], 3)
+        # plain samples that would match the denoised error (error ~ 1/sqrt(spp))
+        into["plain_spp_equal_rmse"] = int(plain_spp * (res["plain_rmse"] / into["denoised4_rmse"]) ** 2)
+        into["plain_spp_equal_rmse_tonemapped"] = int(
+            plain_spp * (res["plain_rmse_tonemapped"] / into["denoised4_rmse_tonemapped"]) ** 2)
+
+    score(res)
+    res["weights"] = os.path.relpath(weights[0], ROOT)
+    if len(weights) > 1:
+        res["variants"] = {}
+        for wpath in weights[1:]:
+            dn.load(wpath)
+            v = {}
+            score(v)
+            res["variants"][os.path.basename(wpath)] = v
+            log(f"denoise: {os.path.basename(wpath)} {v}")
+        dn.load(weights[0])
     # the U-Net alone: HIP events around aph_denoise, MFMA work over real channels
     r.reset_timing()
     r.enable_timing(True)
@@ -387,7 +400,6 @@ def denoise_vs_plain(r, builder, width=1920, height=1080, ref_spp=4096, plain_sp
     res["unet_ms"] = round(ms / max(n, 1), 3)
     res["unet_tflops"] = round(flops / (ms / max(n, 1) * 1e-3) / 1e12, 1)
     res["unet_mfma_frac_of_bf16_peak"] = round(res["unet_tflops"] / 2500.0, 4)
-    res["weights"] = os.path.relpath(dnmod.DEFAULT_WEIGHTS, ROOT)
     dn.close()
     return res
 
@@ -405,6 +417,9 @@ def main():
                     help="extra measurements after the headline line's frame: literal (config2 at subdiv 5), "
                          "c3 (64 cameras), serving (Poisson replay), rmse (equal-time RMSE), denoise (denoised "
                          "4 spp vs plain 256 spp at 1080p); '' for none")
+    ap.add_argument("--denoiser-weights", default="",
+                    help="comma-separated weight files for the denoise section (default: the committed weights); "
+                         "the first is the reported one, the rest are scored under denoise.variants")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 counter passes")
     ap.add_argument("--cpu-baseline-child", action="store_true",
@@ -549,7 +564,8 @@ def main():
         out["equal_time_rmse"] = equal_time_rmse(r, builder)
         log(f"rmse: {out['equal_time_rmse']}")
     if "denoise" in sections:
-        out["denoise"] = denoise_vs_plain(r, builder)
+        out["denoise"] = denoise_vs_plain(r, builder,
+                                          weights=[x for x in args.denoiser_weights.split(",") if x] or None)
         log(f"denoise: {out['denoise']}")
     r.close()
     if not args.no_cpu_baseline:

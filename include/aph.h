@@ -439,9 +439,18 @@ enum {
 /* Inspection of the last pass (synchronizes); count must equal the size. */
 int aph_denoiser_tensor_size(aph_ctx_t ctx, uint64_t denoiser, int kind, uint32_t index, uint64_t* count);
 int aph_denoiser_tensor(aph_ctx_t ctx, uint64_t denoiser, int kind, uint32_t index, float* out, uint64_t count);
-typedef struct { uint64_t noisy, aov, reference; } aph_denoise_example; /* targets and an AOV set of one size */
-/* One Adam step over a batch (all examples one size); returns the mean L1
- * loss of log1p radiance before the step (synchronizes). */
+/* targets and an AOV set of one size; transform (augmentation, applied to all
+ * three): destination pixel (x, y) reads source (sx, sy) with (u, v) = (y, x)
+ * if bit 2 (transpose, square images only) else (x, y), sx = W-1-u if bit 0
+ * else u, sy = H-1-v if bit 1 else v */
+typedef struct { uint64_t noisy, aov, reference; uint32_t transform, _pad; } aph_denoise_example;
+enum {
+    APH_DN_LOSS_L1_LOG = 0,        /* mean |p - log1p(ref)|, p the predicted log1p radiance (default) */
+    APH_DN_LOSS_L2_TONEMAPPED = 1  /* mean (x/(1+x) - ref/(1+ref))^2 = (exp(-log1p(ref)) - exp(-p))^2 */
+};
+int aph_denoiser_set_loss(aph_ctx_t ctx, uint64_t denoiser, int kind);
+/* One Adam step over a batch (all examples one size); returns the mean loss
+ * before the step (synchronizes). */
 int aph_denoiser_train_step(aph_ctx_t ctx, uint64_t denoiser, const aph_denoise_example* ex, uint32_t n, float lr,
                             float* loss);
 typedef struct { uint64_t noisy, aov, output; } aph_denoise_item; /* output may be the noisy target itself */

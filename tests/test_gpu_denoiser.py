@@ -240,6 +240,51 @@ def test_backward_every_layer_matches_torch(renderer, setup):
     assert checked == nbuf - 2  # every buffer but the input and the prediction gradient
 
 
+@pytest.mark.parametrize("transform", [0, 1, 2, 5, 7])
+def test_transforms_and_tonemapped_loss(renderer, transform):
+    # square image: every transform applies; the packed input and the loss see
+    # the same mirrored / transposed pixels
+    r, s = renderer, 48
+    sc = scenes.config1(width=s, height=s, trace_depth=3)
+    tlas = r.load_scene(sc)
+    cam, _ = frames(s, s)
+    noisy, aov = render_inputs(r, tlas, cam, s, s, 2, 1)
+    ref = r.target(s, s)
+    r.render([r.request(ref, tlas, cam, 0, 32, 5)], capi.APH_MODE_WAVEFRONT)
+    r.wait()
+
+    def apply(img):
+        # destination (x, y) reads source (sx, sy): (u, v) = (y, x) when
+        # transposing, else (x, y); sx = u or W-1-u (bit 0), sy = v or H-1-v (bit 1)
+        ys, xs = np.meshgrid(np.arange(s), np.arange(s), indexing="ij")
+        u, v = (ys, xs) if transform & 4 else (xs, ys)
+        sx = s - 1 - u if transform & 1 else u
+        sy = s - 1 - v if transform & 2 else v
+        return np.ascontiguousarray(img[sy, sx])
+
+    dn = Denoiser(r, seed=4)
+    dn.set_loss(capi.APH_DN_LOSS_L2_TONEMAPPED)
+    loss = dn.train_step([(noisy, aov, ref, transform)], lr=0.0)
+    x = dn.activation(0, 1, s, s)[0]
+    np.testing.assert_allclose(x, apply(expected_features(r, noisy, aov, s, s)), rtol=BF, atol=1e-6)
+    pred = dn.tensor(capi.APH_DN_PREDICTION).reshape(s, s, 4)[..., :3].astype(np.float64)
+    q = np.log1p(np.maximum(apply(r.readback(ref, s, s))[..., :3], 0)).astype(np.float64)
+    d = np.exp(-q) - np.exp(-pred)
+    n = 3.0 * s * s
+    assert abs(loss - (d * d).sum() / n) <= 1e-4 * loss + 1e-9
+    g = dn.activation(len(dn.buffers) - 1, 1, s, s, grad=True)[0]
+    gref = bf16(2.0 * d * np.exp(-pred) / n)
+    # fp32 exp differences cancel where the two tonemapped values nearly agree
+    np.testing.assert_allclose(g[..., :3], gref, rtol=2 * BF, atol=1e-3 * np.abs(gref).max())
+    dn.close()
+    with pytest.raises(AphError):
+        bad = Denoiser(r, seed=4)
+        try:
+            bad.train_step([(noisy, aov, ref, 8)], lr=0.0)
+        finally:
+            bad.close()
+
+
 def test_training_reduces_loss(renderer, setup):
     r = renderer
     dn = Denoiser(r, seed=1)
@@ -267,6 +312,10 @@ def test_denoiser_errors(renderer, setup):
         dn.denoise([(na, aa, nb), (nb, ab, na)])  # output aliases another input
     with pytest.raises(AphError):
         dn.set_params(np.zeros(10, dtype=np.float32))
+    with pytest.raises(AphError):
+        dn.set_loss(7)
+    with pytest.raises(AphError):  # transposing a non-square image
+        dn.train_step([(na, aa, setup["refs"][0], 4)], lr=0.0)
     dn.denoise([(na, aa, r.target(w, h))])
     with pytest.raises(AphError) as e:
         dn.tensor(capi.APH_DN_ACTIVATION_GRADS, 3)  # gradients only after a training step
