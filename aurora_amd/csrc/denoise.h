@@ -27,19 +27,20 @@ struct ConvArgs {
     int inStride, cinP, coutP, coutStore, outStride, outOffset;
 };
 
-// dW[9][coutP][cinP] += sum over pixels of dY (x) shifted X.
+// dW[9][coutP][cinP] += sum over pixels of dY (x) shifted X; db[coutP] +=
+// sum over pixels of dY (when db is not null).
 struct WGradArgs {
     const __bf16* dy;
     const __bf16* x;
     float* dw;
+    float* db;
     int N, H, W;
     int dyStride, xStride, coutP, cinP;
-    int chunk;  // pixels per wave (a multiple of 32)
+    int chunk;  // pixels per block (a multiple of 128)
 };
 
 hipError_t launchConv(const ConvArgs& a, bool relu, bool outF32, hipStream_t st);
 hipError_t launchWGrad(const WGradArgs& a, hipStream_t st);
-hipError_t launchBiasGrad(const __bf16* dy, int64_t npix, int stride, int coutP, float* db, hipStream_t st);
 hipError_t launchReluMask(__bf16* dy, int dyStride, const __bf16* y, int yStride, int64_t npix, int C, hipStream_t st);
 hipError_t launchAvgPool(const __bf16* in, int inStride, int inOff, __bf16* out, int outStride, int outOff, int N,
                          int H, int W, int C, hipStream_t st);

@@ -36,123 +36,196 @@ __device__ __forceinline__ bf16x8 load8(const __bf16* p)
 }
 
 // ---- convolution (forward, and data gradient with flipped weights) ----
-// One wave: 16 consecutive pixels x up to 8 output tiles of 16 channels.
-// Block: 4 waves (64 pixels); blockIdx.y selects a group of 128 output channels.
-template <bool RELU, bool OUT_F32>
+// One wave: P subtiles of 16 consecutive pixels x T subtiles of 16 output
+// channels (P * T accumulators); each A fragment (8 input channels of one
+// pixel) feeds T MFMAs and each B fragment (8 weights) feeds P.  Block: 4
+// waves; blockIdx.y selects 16 * T output channels.
+template <bool RELU, bool OUT_F32, int P, int T>
 __global__ __launch_bounds__(256) void kConv3x3(ConvArgs a)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t npix = (int64_t)a.N * a.H * a.W;
-    const int64_t p0 = ((int64_t)blockIdx.x * 4 + wave) * 16;
+    const int64_t p0 = ((int64_t)blockIdx.x * 4 + wave) * (16 * P);
     if (p0 >= npix) return;
-    const int coBase = blockIdx.y * 128;
-    const int nTiles = min(8, (a.coutP - coBase) / 16);
+    const int coBase = blockIdx.y * (16 * T);
+    const int nTiles = min(T, (a.coutP - coBase) / 16);
     const int row = lane & 15, kg = lane >> 4;
-    // the pixel this lane feeds into A
-    const int64_t pa = p0 + row;
-    const bool paValid = pa < npix;
-    int n = 0, y = 0, x = 0;
-    if (paValid) {
-        n = (int)(pa / ((int64_t)a.H * a.W));
-        int r = (int)(pa - (int64_t)n * a.H * a.W);
-        y = r / a.W;
-        x = r - y * a.W;
-    }
-    f32x4 acc[8];
+    const int HW = a.H * a.W;
+    // pixel fed into A by this lane in each pixel subtile
+    int n[P], y[P], x[P];
 #pragma unroll
-    for (int j = 0; j < 8; j++) acc[j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int i = 0; i < P; i++) {
+        const int64_t pa = p0 + i * 16 + row;
+        if (pa < npix) {
+            n[i] = (int)(pa / HW);
+            const int r = (int)(pa - (int64_t)n[i] * HW);
+            y[i] = r / a.W;
+            x[i] = r - y[i] * a.W;
+        } else {
+            n[i] = 0;
+            y[i] = -4;  // every tap out of the image
+            x[i] = 0;
+        }
+    }
+    f32x4 acc[P][T];
+#pragma unroll
+    for (int i = 0; i < P; i++)
+#pragma unroll
+        for (int j = 0; j < T; j++) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     for (int t = 0; t < 9; t++) {
-        const int sy = y + t / 3 - 1, sx = x + t % 3 - 1;
-        const bool ok = paValid && sy >= 0 && sy < a.H && sx >= 0 && sx < a.W;
-        const __bf16* src = a.in + (((int64_t)n * a.H + sy) * a.W + sx) * a.inStride + kg * 8;
+        const int ty = t / 3 - 1, tx = t % 3 - 1;
+        const __bf16* src[P];
+        bool ok[P];
+#pragma unroll
+        for (int i = 0; i < P; i++) {
+            const int sy = y[i] + ty, sx = x[i] + tx;
+            ok[i] = sy >= 0 && sy < a.H && sx >= 0 && sx < a.W;
+            src[i] = a.in + (((int64_t)n[i] * a.H + (ok[i] ? sy : 0)) * a.W + (ok[i] ? sx : 0)) * a.inStride + kg * 8;
+        }
         const __bf16* wt = a.w + ((int64_t)t * a.coutP + coBase + row) * a.cinP + kg * 8;
         for (int c0 = 0; c0 < a.cinP; c0 += 32) {
-            bf16x8 av = ok ? load8(src + c0) : bf16x8{};
+            bf16x8 av[P], bv[T];
 #pragma unroll
-            for (int j = 0; j < 8; j++) {
+            for (int i = 0; i < P; i++) av[i] = ok[i] ? load8(src[i] + c0) : bf16x8{};
+#pragma unroll
+            for (int j = 0; j < T; j++) bv[j] = j < nTiles ? load8(wt + (int64_t)j * 16 * a.cinP + c0) : bf16x8{};
+#pragma unroll
+            for (int j = 0; j < T; j++) {
                 if (j < nTiles) {
-                    bf16x8 bv = load8(wt + (int64_t)j * 16 * a.cinP + c0);
-                    acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[j], 0, 0, 0);
+#pragma unroll
+                    for (int i = 0; i < P; i++)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bv[j], acc[i][j], 0, 0, 0);
                 }
             }
         }
     }
 #pragma unroll
-    for (int j = 0; j < 8; j++) {
+    for (int j = 0; j < T; j++) {
         if (j >= nTiles) continue;
         const int co = coBase + j * 16 + row;
+        if (co >= a.coutStore) continue;
         const float b = a.bias ? a.bias[co] : 0.0f;
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int64_t q = p0 + kg * 4 + i;
-            if (q >= npix || co >= a.coutStore) continue;
-            float v = acc[j][i] + b;
-            if (RELU) v = fmaxf(v, 0.0f);
-            if (OUT_F32) a.outF[q * a.outStride + a.outOffset + co] = v;
-            else a.out[q * a.outStride + a.outOffset + co] = toBf(v);
+        for (int i = 0; i < P; i++) {
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int64_t q = p0 + i * 16 + kg * 4 + e;
+                if (q >= npix) continue;
+                float v = acc[i][j][e] + b;
+                if (RELU) v = fmaxf(v, 0.0f);
+                if (OUT_F32) a.outF[q * a.outStride + a.outOffset + co] = v;
+                else a.out[q * a.outStride + a.outOffset + co] = toBf(v);
+            }
         }
     }
 }
 
 // ---- weight gradient: dW[t][co][ci] += sum_p dY[p][co] X[p + off(t)][ci] ----
-// One wave per (16 co x 16 ci tile, tap, pixel chunk); K = pixels, 32 per MFMA,
-// fragments gathered (8 pixels per lane); partial sums added atomically.
-__global__ __launch_bounds__(64) void kConvWGrad(WGradArgs a)
+// Block: one tap, 16*TC output x 16*TI input channels, one pixel chunk; its 4
+// waves take interleaved steps of 32 pixels (K = pixels, 8 per lane,
+// gathered), each wave adding its TC x TI tile of partial sums atomically.
+// The centre tap's blocks of the first input-channel group also sum dY into
+// the bias gradient db[co].
+template <int TC, int TI>
+__global__ __launch_bounds__(256) void kConvWGrad(WGradArgs a)
 {
-    const int lane = threadIdx.x, row = lane & 15, kg = lane >> 4;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, row = lane & 15, kg = lane >> 4;
     const int t = blockIdx.y;
-    const int tilesCi = a.cinP / 16;
-    const int coT = blockIdx.z / tilesCi, ciT = blockIdx.z % tilesCi;
-    const int co = coT * 16 + row, ci = ciT * 16 + row;
-    const int64_t npix = (int64_t)a.N * a.H * a.W;
+    const int ciGroups = (a.cinP + 16 * TI - 1) / (16 * TI);
+    const int coBase = (blockIdx.z / ciGroups) * 16 * TC, ciBase = (blockIdx.z % ciGroups) * 16 * TI;
+    const int nTc = min(TC, (a.coutP - coBase) / 16), nTi = min(TI, (a.cinP - ciBase) / 16);
+    const int HW = a.H * a.W;
+    const int64_t npix = (int64_t)a.N * HW;
     const int64_t c0 = (int64_t)blockIdx.x * a.chunk, c1 = min(npix, c0 + a.chunk);
-    const int dy = t / 3 - 1, dx = t % 3 - 1;
-    f32x4 acc = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    for (int64_t p = c0; p < c1; p += 32) {
-        bf16x8 av, bv;
+    const int ty = t / 3 - 1, tx = t % 3 - 1;
+    const int64_t shift = (int64_t)ty * a.W + tx;
+    f32x4 acc[TC][TI];
+#pragma unroll
+    for (int i = 0; i < TC; i++)
+#pragma unroll
+        for (int j = 0; j < TI; j++) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    const __bf16* dyBase = a.dy + coBase + row;
+    const __bf16* xBase = a.x + ciBase + row;
+    const bool biasBlock = a.db && t == 4 && ciBase == 0;
+    float bsum[TC] = {};
+    for (int64_t p = c0 + wave * 32; p < c1; p += 128) {
+        const int64_t q0 = p + kg * 8;
+        int n = (int)(q0 / HW);
+        int r = (int)(q0 - (int64_t)n * HW);
+        int y = r / a.W, x = r - (r / a.W) * a.W;
+        bf16x8 av[TC], bv[TI];
 #pragma unroll
         for (int e = 0; e < 8; e++) {
-            const int64_t q = p + kg * 8 + e;
-            float gy = 0.0f, xv = 0.0f;
-            if (q < c1) {
-                gy = fromBf(a.dy[q * a.dyStride + co]);
-                int n = (int)(q / ((int64_t)a.H * a.W));
-                int r = (int)(q - (int64_t)n * a.H * a.W);
-                int y = r / a.W, x = r - (r / a.W) * a.W;
-                int sy = y + dy, sx = x + dx;
-                if (sy >= 0 && sy < a.H && sx >= 0 && sx < a.W)
-                    xv = fromBf(a.x[(((int64_t)n * a.H + sy) * a.W + sx) * a.xStride + ci]);
+            const int64_t q = q0 + e;
+            const bool valid = q < c1;
+            const int sy = y + ty, sx = x + tx;
+            const bool okx = valid && sy >= 0 && sy < a.H && sx >= 0 && sx < a.W;
+#pragma unroll
+            for (int i = 0; i < TC; i++)
+                av[i][e] = (valid && i < nTc) ? dyBase[q * a.dyStride + i * 16] : toBf(0.0f);
+#pragma unroll
+            for (int j = 0; j < TI; j++)
+                bv[j][e] = (okx && j < nTi) ? xBase[(q + shift) * a.xStride + j * 16] : toBf(0.0f);
+            if (++x == a.W) {
+                x = 0;
+                if (++y == a.H) y = 0;
             }
-            av[e] = toBf(gy);
-            bv[e] = toBf(xv);
         }
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc, 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < TC; i++)
+#pragma unroll
+            for (int j = 0; j < TI; j++)
+                if (i < nTc && j < nTi)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bv[j], acc[i][j], 0, 0, 0);
+        if (biasBlock) {
+#pragma unroll
+            for (int i = 0; i < TC; i++)
+#pragma unroll
+                for (int e = 0; e < 8; e++) bsum[i] += fromBf(av[i][e]);
+        }
+    }
+    if (biasBlock) {
+        // lanes row, row+16, row+32, row+48 hold the same channel
+#pragma unroll
+        for (int i = 0; i < TC; i++) {
+            float v = bsum[i];
+            v += __shfl_xor(v, 16);
+            v += __shfl_xor(v, 32);
+            if (kg == 0 && i < nTc) atomicAdd(&a.db[coBase + i * 16 + row], v);
+        }
     }
     // D rows are co (A rows), columns ci (B columns)
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-        const int dco = coT * 16 + kg * 4 + i, dci = ciT * 16 + row;
-        atomicAdd(&a.dw[((int64_t)t * a.coutP + dco) * a.cinP + dci], acc[i]);
-    }
+    for (int i = 0; i < TC; i++)
+#pragma unroll
+        for (int j = 0; j < TI; j++) {
+            if (i >= nTc || j >= nTi) continue;
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int dco = coBase + i * 16 + kg * 4 + e, dci = ciBase + j * 16 + row;
+                atomicAdd(&a.dw[((int64_t)t * a.coutP + dco) * a.cinP + dci], acc[i][j][e]);
+            }
+        }
 }
 
-__global__ void kBiasGrad(const __bf16* dy, int64_t npix, int stride, int coutP, float* db)
-{
-    const int co = blockIdx.y * blockDim.x + threadIdx.x;
-    if (co >= coutP) return;
-    float s = 0.0f;
-    for (int64_t p = blockIdx.x; p < npix; p += gridDim.x) s += fromBf(dy[p * stride + co]);
-    atomicAdd(&db[co], s);
-}
+__device__ __forceinline__ void store8(__bf16* p, bf16x8 v) { *reinterpret_cast<bf16x8*>(p) = v; }
+
+// The elementwise kernels below take 8 channels (16 bytes) per thread; channel
+// counts, strides and offsets are multiples of 8.
 
 // dY *= (Y > 0) for a ReLU output Y: channels [0, C) of two strided buffers
 __global__ void kReluMask(__bf16* dy, int dyStride, const __bf16* y, int yStride, int64_t npix, int C)
 {
-    const int64_t n = npix * C;
+    const int G = C / 8;
+    const int64_t n = npix * G;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        int c = (int)(i % C);
-        int64_t p = i / C;
-        if (!(fromBf(y[p * yStride + c]) > 0.0f)) dy[p * dyStride + c] = toBf(0.0f);
+        const int64_t p = i / G;
+        const int c = (int)(i - p * G) * 8;
+        bf16x8 g = load8(dy + p * dyStride + c), v = load8(y + p * yStride + c);
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            if (!(fromBf(v[k]) > 0.0f)) g[k] = toBf(0.0f);
+        store8(dy + p * dyStride + c, g);
     }
 }
 
@@ -166,62 +239,80 @@ __global__ void kBf16ToF32(const __bf16* in, float* out, int64_t n)
 __global__ void kAvgPool2(const __bf16* in, int inStride, int inOff, __bf16* out, int outStride, int outOff, int N,
                           int H, int W, int C)
 {
-    const int Ho = H / 2, Wo = W / 2;
-    const int64_t n = (int64_t)N * Ho * Wo * C;
+    const int Ho = H / 2, Wo = W / 2, G = C / 8;
+    const int64_t n = (int64_t)N * Ho * Wo * G;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        int c = (int)(i % C);
-        int64_t p = i / C;
-        int xo = (int)(p % Wo), yo = (int)((p / Wo) % Ho), b = (int)(p / ((int64_t)Wo * Ho));
-        float s = 0.0f;
+        const int64_t p = i / G;
+        const int c = (int)(i - p * G) * 8;
+        const int xo = (int)(p % Wo), yo = (int)((p / Wo) % Ho), b = (int)(p / ((int64_t)Wo * Ho));
+        float s[8] = {};
+#pragma unroll
         for (int k = 0; k < 4; k++) {
-            int yy = 2 * yo + k / 2, xx = 2 * xo + k % 2;
-            s += fromBf(in[(((int64_t)b * H + yy) * W + xx) * inStride + inOff + c]);
+            const bf16x8 v = load8(in + (((int64_t)b * H + 2 * yo + k / 2) * W + 2 * xo + k % 2) * inStride + inOff + c);
+#pragma unroll
+            for (int e = 0; e < 8; e++) s[e] += fromBf(v[e]);
         }
-        out[p * outStride + outOff + c] = toBf(0.25f * s);
+        bf16x8 o;
+#pragma unroll
+        for (int e = 0; e < 8; e++) o[e] = toBf(0.25f * s[e]);
+        store8(out + p * outStride + outOff + c, o);
     }
 }
 
 __global__ void kAvgPool2Back(const __bf16* dout, int doStride, __bf16* din, int diStride, int diOff, int N, int H,
                               int W, int C, int accumulate)
 {
-    const int64_t n = (int64_t)N * H * W * C;
+    const int G = C / 8;
+    const int64_t n = (int64_t)N * H * W * G;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        int c = (int)(i % C);
-        int64_t p = i / C;
-        int x = (int)(p % W), y = (int)((p / W) % H), b = (int)(p / ((int64_t)W * H));
-        float g = 0.25f * fromBf(dout[(((int64_t)b * (H / 2) + y / 2) * (W / 2) + x / 2) * doStride + c]);
-        __bf16* d = &din[p * diStride + diOff + c];
-        *d = toBf(accumulate ? fromBf(*d) + g : g);
+        const int64_t p = i / G;
+        const int c = (int)(i - p * G) * 8;
+        const int x = (int)(p % W), y = (int)((p / W) % H), b = (int)(p / ((int64_t)W * H));
+        const bf16x8 g = load8(dout + (((int64_t)b * (H / 2) + y / 2) * (W / 2) + x / 2) * doStride + c);
+        __bf16* d = din + p * diStride + diOff + c;
+        bf16x8 o = accumulate ? load8(d) : bf16x8{};
+#pragma unroll
+        for (int e = 0; e < 8; e++) o[e] = toBf((accumulate ? fromBf(o[e]) : 0.0f) + 0.25f * fromBf(g[e]));
+        store8(d, o);
     }
 }
 
 __global__ void kUpsample2(const __bf16* in, int inStride, __bf16* out, int outStride, int outOff, int N, int Ho,
                            int Wo, int C)
 {
-    const int64_t n = (int64_t)N * Ho * Wo * C;
+    const int G = C / 8;
+    const int64_t n = (int64_t)N * Ho * Wo * G;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        int c = (int)(i % C);
-        int64_t p = i / C;
-        int x = (int)(p % Wo), y = (int)((p / Wo) % Ho), b = (int)(p / ((int64_t)Wo * Ho));
-        out[p * outStride + outOff + c] = in[(((int64_t)b * (Ho / 2) + y / 2) * (Wo / 2) + x / 2) * inStride + c];
+        const int64_t p = i / G;
+        const int c = (int)(i - p * G) * 8;
+        const int x = (int)(p % Wo), y = (int)((p / Wo) % Ho), b = (int)(p / ((int64_t)Wo * Ho));
+        store8(out + p * outStride + outOff + c,
+               load8(in + (((int64_t)b * (Ho / 2) + y / 2) * (Wo / 2) + x / 2) * inStride + c));
     }
 }
 
 __global__ void kUpsample2Back(const __bf16* dout, int doStride, int doOff, __bf16* din, int diStride, int N, int Hi,
                                int Wi, int C)
 {
-    const int64_t n = (int64_t)N * Hi * Wi * C;
+    const int G = C / 8;
+    const int64_t n = (int64_t)N * Hi * Wi * G;
     const int Ho = 2 * Hi, Wo = 2 * Wi;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        int c = (int)(i % C);
-        int64_t p = i / C;
-        int x = (int)(p % Wi), y = (int)((p / Wi) % Hi), b = (int)(p / ((int64_t)Wi * Hi));
-        float s = 0.0f;
+        const int64_t p = i / G;
+        const int c = (int)(i - p * G) * 8;
+        const int x = (int)(p % Wi), y = (int)((p / Wi) % Hi), b = (int)(p / ((int64_t)Wi * Hi));
+        float s[8] = {};
+#pragma unroll
         for (int k = 0; k < 4; k++) {
-            int yy = 2 * y + k / 2, xx = 2 * x + k % 2;
-            s += fromBf(dout[(((int64_t)b * Ho + yy) * Wo + xx) * doStride + doOff + c]);
+            const bf16x8 v =
+                load8(dout + (((int64_t)b * Ho + 2 * y + k / 2) * Wo + 2 * x + k % 2) * doStride + doOff + c);
+#pragma unroll
+            for (int e = 0; e < 8; e++) s[e] += fromBf(v[e]);
         }
-        din[p * diStride + c] = toBf(s);
+        bf16x8 o;
+#pragma unroll
+        for (int e = 0; e < 8; e++) o[e] = toBf(s[e]);
+        store8(din + p * diStride + c, o);
     }
 }
 
@@ -361,39 +452,45 @@ unsigned gridFor(int64_t n) { return (unsigned)std::min<int64_t>((n + 255) / 256
 
 }  // namespace
 
+template <int P, int T>
+void convLaunch(const ConvArgs& a, bool relu, bool outF32, hipStream_t st)
+{
+    const int64_t npix = (int64_t)a.N * a.H * a.W;
+    dim3 grid((unsigned)((npix + 64 * P - 1) / (64 * P)), (unsigned)((a.coutP + 16 * T - 1) / (16 * T)));
+    if (outF32) {
+        if (relu) kConv3x3<true, true, P, T><<<grid, 256, 0, st>>>(a);
+        else kConv3x3<false, true, P, T><<<grid, 256, 0, st>>>(a);
+    } else {
+        if (relu) kConv3x3<true, false, P, T><<<grid, 256, 0, st>>>(a);
+        else kConv3x3<false, false, P, T><<<grid, 256, 0, st>>>(a);
+    }
+}
+
 hipError_t launchConv(const ConvArgs& a, bool relu, bool outF32, hipStream_t st)
 {
     const int64_t npix = (int64_t)a.N * a.H * a.W;
     if (npix == 0) return hipSuccess;
-    dim3 grid((unsigned)((npix + 63) / 64), (unsigned)((a.coutP + 127) / 128));
-    if (outF32) {
-        if (relu) kConv3x3<true, true><<<grid, 256, 0, st>>>(a);
-        else kConv3x3<false, true><<<grid, 256, 0, st>>>(a);
-    } else {
-        if (relu) kConv3x3<true, false><<<grid, 256, 0, st>>>(a);
-        else kConv3x3<false, false><<<grid, 256, 0, st>>>(a);
-    }
+    // 16 accumulators per wave: 8 pixel x 2 channel subtiles for 32 output
+    // channels, 4 x 4 for 64, 2 x 8 above
+    if (a.coutP <= 32) convLaunch<8, 2>(a, relu, outF32, st);
+    else if (a.coutP <= 64) convLaunch<4, 4>(a, relu, outF32, st);
+    else convLaunch<2, 8>(a, relu, outF32, st);
     return hipGetLastError();
 }
 
 hipError_t launchWGrad(const WGradArgs& a, hipStream_t st)
 {
+    constexpr int TC = 4, TI = 4;
     const int64_t npix = (int64_t)a.N * a.H * a.W;
-    dim3 grid((unsigned)((npix + a.chunk - 1) / a.chunk), 9, (unsigned)((a.coutP / 16) * (a.cinP / 16)));
-    kConvWGrad<<<grid, 64, 0, st>>>(a);
-    return hipGetLastError();
-}
-
-hipError_t launchBiasGrad(const __bf16* dy, int64_t npix, int stride, int coutP, float* db, hipStream_t st)
-{
-    dim3 grid((unsigned)std::min<int64_t>(npix, 1024), (unsigned)((coutP + 63) / 64));
-    kBiasGrad<<<grid, 64, 0, st>>>(dy, npix, stride, coutP, db);
+    const unsigned groups = (unsigned)(((a.coutP + 16 * TC - 1) / (16 * TC)) * ((a.cinP + 16 * TI - 1) / (16 * TI)));
+    dim3 grid((unsigned)((npix + a.chunk - 1) / a.chunk), 9, groups);
+    kConvWGrad<TC, TI><<<grid, 256, 0, st>>>(a);
     return hipGetLastError();
 }
 
 hipError_t launchReluMask(__bf16* dy, int dyStride, const __bf16* y, int yStride, int64_t npix, int C, hipStream_t st)
 {
-    kReluMask<<<gridFor(npix * C), 256, 0, st>>>(dy, dyStride, y, yStride, npix, C);
+    kReluMask<<<gridFor(npix * C / 8), 256, 0, st>>>(dy, dyStride, y, yStride, npix, C);
     return hipGetLastError();
 }
 
@@ -406,7 +503,7 @@ hipError_t launchBf16ToF32(const __bf16* in, float* out, int64_t n, hipStream_t 
 hipError_t launchAvgPool(const __bf16* in, int inStride, int inOff, __bf16* out, int outStride, int outOff, int N,
                          int H, int W, int C, hipStream_t st)
 {
-    kAvgPool2<<<gridFor((int64_t)N * H * W * C / 4), 256, 0, st>>>(in, inStride, inOff, out, outStride, outOff, N, H,
+    kAvgPool2<<<gridFor((int64_t)N * H * W * C / 32), 256, 0, st>>>(in, inStride, inOff, out, outStride, outOff, N, H,
                                                                    W, C);
     return hipGetLastError();
 }
@@ -414,7 +511,7 @@ hipError_t launchAvgPool(const __bf16* in, int inStride, int inOff, __bf16* out,
 hipError_t launchAvgPoolBack(const __bf16* dout, int doStride, __bf16* din, int diStride, int diOff, int N, int H,
                              int W, int C, bool accumulate, hipStream_t st)
 {
-    kAvgPool2Back<<<gridFor((int64_t)N * H * W * C), 256, 0, st>>>(dout, doStride, din, diStride, diOff, N, H, W, C,
+    kAvgPool2Back<<<gridFor((int64_t)N * H * W * C / 8), 256, 0, st>>>(dout, doStride, din, diStride, diOff, N, H, W, C,
                                                                    accumulate ? 1 : 0);
     return hipGetLastError();
 }
@@ -422,14 +519,14 @@ hipError_t launchAvgPoolBack(const __bf16* dout, int doStride, __bf16* din, int 
 hipError_t launchUpsample(const __bf16* in, int inStride, __bf16* out, int outStride, int outOff, int N, int Ho,
                           int Wo, int C, hipStream_t st)
 {
-    kUpsample2<<<gridFor((int64_t)N * Ho * Wo * C), 256, 0, st>>>(in, inStride, out, outStride, outOff, N, Ho, Wo, C);
+    kUpsample2<<<gridFor((int64_t)N * Ho * Wo * C / 8), 256, 0, st>>>(in, inStride, out, outStride, outOff, N, Ho, Wo, C);
     return hipGetLastError();
 }
 
 hipError_t launchUpsampleBack(const __bf16* dout, int doStride, int doOff, __bf16* din, int diStride, int N, int Hi,
                               int Wi, int C, hipStream_t st)
 {
-    kUpsample2Back<<<gridFor((int64_t)N * Hi * Wi * C), 256, 0, st>>>(dout, doStride, doOff, din, diStride, N, Hi, Wi,
+    kUpsample2Back<<<gridFor((int64_t)N * Hi * Wi * C / 8), 256, 0, st>>>(dout, doStride, doOff, din, diStride, N, Hi, Wi,
                                                                       C);
     return hipGetLastError();
 }

@@ -283,6 +283,7 @@ hipError_t backward(aph_denoiser* d, hipStream_t st)
             g.dy = dy;
             g.x = d->act(L.in);
             g.dw = d->grads.as<float>() + o.w[l];
+            g.db = d->grads.as<float>() + o.b[l];
             g.N = d->N;
             g.H = d->levelH(lv);
             g.W = d->levelW(lv);
@@ -290,14 +291,13 @@ hipError_t backward(aph_denoiser* d, hipStream_t st)
             g.xStride = kBufs[L.in].channels;
             g.coutP = L.coutP;
             g.cinP = L.cinP;
-            const int64_t waves = 9ll * (L.coutP / 16) * (L.cinP / 16);
-            const int64_t chunks = std::max<int64_t>(1, 16384 / waves);
+            // about 2048 blocks of (tap, 64 x 64 channel group, pixel chunk)
+            const int64_t groups = 9ll * ((L.coutP + 63) / 64) * ((L.cinP + 63) / 64);
+            const int64_t chunks = std::max<int64_t>(1, 2048 / groups);
             int64_t chunk = (npix + chunks - 1) / chunks;
-            chunk = std::max<int64_t>(256, (chunk + 31) / 32 * 32);
+            chunk = std::max<int64_t>(512, (chunk + 127) / 128 * 128);
             g.chunk = (int)std::min<int64_t>(chunk, 1 << 30);
             e = launchWGrad(g, st);
-            if (e != hipSuccess) return e;
-            e = launchBiasGrad(dy, npix, dyStride, L.coutP, d->grads.as<float>() + o.b[l], st);
             if (e != hipSuccess) return e;
             if (l == 0) continue;  // no gradient of the packed input
             ConvArgs a{};
