@@ -152,11 +152,13 @@ class DenoiseExample(C.Structure):
 
 
 class DenoiseItem(C.Structure):
-    _fields_ = [("noisy", C.c_uint64), ("aov", C.c_uint64), ("output", C.c_uint64)]
+    _fields_ = [("noisy", C.c_uint64), ("aov", C.c_uint64), ("output", C.c_uint64), ("flags", C.c_uint32),
+                ("_pad", C.c_uint32)]
 
 
 APH_DN_OP_CONV, APH_DN_OP_POOL, APH_DN_OP_UPSAMPLE = 0, 1, 2
 APH_DN_LOSS_L1_LOG, APH_DN_LOSS_L2_TONEMAPPED = 0, 1
+APH_DN_ENSEMBLE = 1
 APH_DN_PARAMS, APH_DN_PARAM_GRADS, APH_DN_ACTIVATIONS, APH_DN_ACTIVATION_GRADS, APH_DN_PREDICTION = 0, 1, 2, 3, 4
 
 _P = C.c_void_p

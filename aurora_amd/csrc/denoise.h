@@ -54,8 +54,9 @@ hipError_t launchUpsampleBack(const __bf16* dout, int doStride, int doOff, __bf1
 // sx = bit 0 ? W-1-u : u, sy = bit 1 ? H-1-v : v (transposition needs a square image)
 hipError_t launchPack(const float4* noisy, const float* depth, const float4* normalRough, const float4* albedo, int W,
                       int H, int Wp, int Hp, int img, uint32_t transform, __bf16* out, int stride, hipStream_t st);
-hipError_t launchUnpack(const float* pred, int W, int H, int Wp, int Hp, int img, const float4* noisy, float4* out,
-                        hipStream_t st);
+// k: index of this pass in an ensemble over transforms (running mean into out)
+hipError_t launchUnpack(const float* pred, int W, int H, int Wp, int Hp, int img, uint32_t transform, int k,
+                        const float4* noisy, float4* out, hipStream_t st);
 hipError_t launchLoss(const float* pred, const float4* ref, int W, int H, int Wp, int Hp, int img, int nImg,
                       uint32_t transform, int kind, __bf16* dpred, int dStride, float* loss, hipStream_t st);
 hipError_t launchAdam(float* w, float* m, float* v, const float* g, int64_t n, float lr, int step, hipStream_t st);

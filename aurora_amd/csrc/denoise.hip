@@ -363,15 +363,27 @@ __global__ void kPack(const float4* noisy, const float* depth, const float4* nor
     }
 }
 
-// ---- output: expm1 of the predicted log radiance, alpha from the noisy input ----
-__global__ void kUnpack(const float* pred, int W, int H, int Wp, int Hp, int img, const float4* noisy, float4* out)
+// ---- output: expm1 of the predicted log radiance, alpha from the noisy input.
+// The prediction of batch image img was made on the input under transform xf:
+// its pixel (x, y) belongs to source pixel sourcePixel(x, y).  Pass k of an
+// ensemble folds into the running mean of the k passes before it ----
+__global__ void kUnpack(const float* pred, int W, int H, int Wp, int Hp, int img, uint32_t xf, int k,
+                        const float4* noisy, float4* out)
 {
     const int64_t n = (int64_t)W * H;
+    const float wNew = 1.0f / (float)(k + 1);
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         int x = (int)(i % W), y = (int)(i / W);
         const float* p = pred + ((int64_t)img * Wp * Hp + (int64_t)y * Wp + x) * kDnPredStride;
-        out[i] = make_float4(expm1f(fmaxf(p[0], 0.0f)), expm1f(fmaxf(p[1], 0.0f)), expm1f(fmaxf(p[2], 0.0f)),
-                             noisy[i].w);
+        int sx, sy;
+        sourcePixel(x, y, W, H, xf, sx, sy);
+        const int64_t s = (int64_t)sy * W + sx;
+        float3 v = make_float3(expm1f(fmaxf(p[0], 0.0f)), expm1f(fmaxf(p[1], 0.0f)), expm1f(fmaxf(p[2], 0.0f)));
+        if (k > 0) {
+            const float4 o = out[s];
+            v = make_float3(o.x + (v.x - o.x) * wNew, o.y + (v.y - o.y) * wNew, o.z + (v.z - o.z) * wNew);
+        }
+        out[s] = make_float4(v.x, v.y, v.z, noisy[s].w);
     }
 }
 
@@ -539,10 +551,10 @@ hipError_t launchPack(const float4* noisy, const float* depth, const float4* nor
     return hipGetLastError();
 }
 
-hipError_t launchUnpack(const float* pred, int W, int H, int Wp, int Hp, int img, const float4* noisy, float4* out,
-                        hipStream_t st)
+hipError_t launchUnpack(const float* pred, int W, int H, int Wp, int Hp, int img, uint32_t transform, int k,
+                        const float4* noisy, float4* out, hipStream_t st)
 {
-    kUnpack<<<gridFor((int64_t)W * H), 256, 0, st>>>(pred, W, H, Wp, Hp, img, noisy, out);
+    kUnpack<<<gridFor((int64_t)W * H), 256, 0, st>>>(pred, W, H, Wp, Hp, img, transform, k, noisy, out);
     return hipGetLastError();
 }
 

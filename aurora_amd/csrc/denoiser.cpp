@@ -635,16 +635,35 @@ int aph_denoise(aph_ctx_t c, uint64_t dn, const aph_denoise_item* items, uint32_
             if (j != i && (outs[i] == tg[j] || (j < i && outs[i] == outs[j])))
                 return fail(c, APH_E_INVALID, "item %u: output aliases another item's image", i);
     }
-    if ((uint64_t)n * padded(W) * padded(H) > (1ull << 28)) return fail(c, APH_E_INVALID, "denoise batch too large");
+    // batch images: one per item, or one per transform of an ensemble item
+    std::vector<Target*> btg;
+    std::vector<AovSet*> bav;
+    std::vector<uint32_t> bxf;
+    std::vector<uint32_t> bitem;
+    for (uint32_t i = 0; i < n; i++) {
+        if (items[i].flags & ~(uint32_t)APH_DN_ENSEMBLE) return fail(c, APH_E_INVALID, "item %u: unknown flags", i);
+        const uint32_t K = (items[i].flags & APH_DN_ENSEMBLE) ? (W == H ? 8u : 4u) : 1u;
+        for (uint32_t k = 0; k < K; k++) {
+            btg.push_back(tg[i]);
+            bav.push_back(av[i]);
+            bxf.push_back(k);
+            bitem.push_back(i);
+        }
+    }
+    const uint32_t nb = (uint32_t)btg.size();
+    if ((uint64_t)nb * padded(W) * padded(H) > (1ull << 28)) return fail(c, APH_E_INVALID, "denoise batch too large");
     hipStream_t st = c->stream;
     TimedScope ts(c, "denoise");
-    HIPCALL(c, reserveShape(d, (int)n, padded(H), padded(W), false), "alloc denoiser activations");
-    HIPCALL(c, packInputs(d, tg, av, W, H, {}, st), "denoiser pack");
+    HIPCALL(c, reserveShape(d, (int)nb, padded(H), padded(W), false), "alloc denoiser activations");
+    HIPCALL(c, packInputs(d, btg, bav, W, H, bxf, st), "denoiser pack");
     HIPCALL(c, forward(d, st), "denoiser forward");
-    for (uint32_t i = 0; i < n; i++)
-        HIPCALL(c, launchUnpack(d->pred.as<float>(), (int)W, (int)H, d->Wp, d->Hp, (int)i, tg[i]->accum.as<float4>(),
-                                outs[i]->accum.as<float4>(), st),
+    for (uint32_t b = 0, k = 0; b < nb; b++) {
+        const uint32_t i = bitem[b];
+        k = (b > 0 && bitem[b - 1] == i) ? k + 1 : 0;
+        HIPCALL(c, launchUnpack(d->pred.as<float>(), (int)W, (int)H, d->Wp, d->Hp, (int)b, bxf[b], (int)k,
+                                tg[i]->accum.as<float4>(), outs[i]->accum.as<float4>(), st),
                 "denoiser output");
+    }
     return APH_OK;
 }
 

@@ -85,7 +85,8 @@ class Denoiser:
         return float(loss.value)
 
     def denoise(self, items):
-        """items: [(noisy_target, aov, output_target)] of one size (asynchronous)."""
+        """items: [(noisy_target, aov, output_target[, flags])] of one size (asynchronous);
+        flags capi.APH_DN_ENSEMBLE averages over mirrored / transposed passes."""
         arr = (capi.DenoiseItem * len(items))(*[capi.DenoiseItem(*i) for i in items])
         self.r._check(self.lib.aph_denoise(self.r.ctx, self.h, C.cast(arr, C.c_void_p), len(items)))
 

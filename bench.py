@@ -343,14 +343,15 @@ def denoise_vs_plain(r, builder, width=1920, height=1080, ref_spp=4096, plain_sp
     ivp = np.array(sc.frame.inv_view, dtype=np.float32)
     aov, noisy, out = r.aov(w, h), r.target(w, h), r.target(w, h)
 
-    def denoised(spp):
+    def denoised(spp, flags=0):
         r.render([r.request(noisy, tlas, sc.frame, 0, spp, 0)], capi.APH_MODE_WAVEFRONT)
         r.render_aovs(aov, tlas, sc.frame, vp, vp, ivp)
-        dn.denoise([(noisy, aov, out)])
+        dn.denoise([(noisy, aov, out, flags)])
         r.wait()
 
     res = {"resolution": f"{w}x{h}", "reference_spp": ref_spp, "scene": "config2 (1M instanced triangles)",
-           "rmse_domain": "linear radiance; tonemapped = x/(1+x)"}
+           "rmse_domain": "linear radiance; tonemapped = x/(1+x)",
+           "denoised_path": "wavefront render + guide images + U-Net (ensemble: mean over the 4 mirrored inputs)"}
     dn.load(weights[0])
     denoised(4)  # warm-up: activation buffers, graphs
     tp = r.target(w, h)
@@ -362,16 +363,20 @@ def denoise_vs_plain(r, builder, width=1920, height=1080, ref_spp=4096, plain_sp
     res["plain_rmse"], res["plain_rmse_tonemapped"] = errors(tp)
 
     def score(into):
-        for spp in (4, 1):
+        for spp, flags, key in ((4, 0, "denoised4"), (1, 0, "denoised1"),
+                                (4, capi.APH_DN_ENSEMBLE, "denoised4_ensemble")):
+            denoised(spp, flags)  # first use of a batch shape allocates
             t = time.perf_counter()
-            denoised(spp)
-            into[f"denoised{spp}_s"] = round(time.perf_counter() - t, 4)
-            into[f"denoised{spp}_rmse"], into[f"denoised{spp}_rmse_tonemapped"] = errors(out)
+            denoised(spp, flags)
+            into[f"{key}_s"] = round(time.perf_counter() - t, 4)
+            into[f"{key}_rmse"], into[f"{key}_rmse_tonemapped"] = errors(out)
             into[f"noisy{spp}_rmse"], into[f"noisy{spp}_rmse_tonemapped"] = errors(noisy)
-        into["time_ratio_4spp_vs_plain"] = round(into["denoised4_s"] / res["plain_s"], 4)
-        into["rmse_ratio_4spp_vs_plain"] = round(into["denoised4_rmse"] / res["plain_rmse"], 3)
-        into["rmse_ratio_4spp_vs_plain_tonemapped"] = round(
-            into["denoised4_rmse_tonemapped"] / res["plain_rmse_tonemapped"], 3)
+        for key in ("denoised4", "denoised4_ensemble"):
+            tag = key.replace("denoised4", "4spp")
+            into[f"time_ratio_{tag}_vs_plain"] = round(into[f"{key}_s"] / res["plain_s"], 4)
+            into[f"rmse_ratio_{tag}_vs_plain"] = round(into[f"{key}_rmse"] / res["plain_rmse"], 3)
+            into[f"rmse_ratio_{tag}_vs_plain_tonemapped"] = round(
+                into[f"{key}_rmse_tonemapped"] / res["plain_rmse_tonemapped"], 3)
         # plain samples that would match the denoised error (error ~ 1/sqrt(spp))
         into["plain_spp_equal_rmse"] = int(plain_spp * (res["plain_rmse"] / into["denoised4_rmse"]) ** 2)
         into["plain_spp_equal_rmse_tonemapped"] = int(

@@ -453,7 +453,11 @@ int aph_denoiser_set_loss(aph_ctx_t ctx, uint64_t denoiser, int kind);
  * before the step (synchronizes). */
 int aph_denoiser_train_step(aph_ctx_t ctx, uint64_t denoiser, const aph_denoise_example* ex, uint32_t n, float lr,
                             float* loss);
-typedef struct { uint64_t noisy, aov, output; } aph_denoise_item; /* output may be the noisy target itself */
+/* output may be the noisy target itself; flags: APH_DN_ENSEMBLE averages the
+ * outputs of the network over the mirrored (and, for square images,
+ * transposed) inputs, 4 or 8 passes in one batch */
+typedef struct { uint64_t noisy, aov, output; uint32_t flags, _pad; } aph_denoise_item;
+enum { APH_DN_ENSEMBLE = 1 };
 /* Denoises n images of one size in one batch (asynchronous, on the context's stream). */
 int aph_denoise(aph_ctx_t ctx, uint64_t denoiser, const aph_denoise_item* items, uint32_t n);
 

@@ -285,6 +285,45 @@ def test_transforms_and_tonemapped_loss(renderer, transform):
             bad.close()
 
 
+def source_index(s, transform):
+    """(sy, sx) index arrays: destination (y, x) of a transformed image reads source (sy, sx)."""
+    ys, xs = np.meshgrid(np.arange(s), np.arange(s), indexing="ij")
+    u, v = (ys, xs) if transform & 4 else (xs, ys)
+    return (s - 1 - v if transform & 2 else v), (s - 1 - u if transform & 1 else u)
+
+
+def test_ensemble_is_the_mean_over_transforms(renderer, setup):
+    r, s = renderer, 48
+    sc = scenes.config1(width=s, height=s, trace_depth=3)
+    tlas = r.load_scene(sc)
+    cam, _ = frames(s, s)
+    noisy, aov = render_inputs(r, tlas, cam, s, s, 2, 3)
+    dn = Denoiser(r, seed=6)
+    single, ens = r.target(s, s), r.target(s, s)
+    dn.denoise([(noisy, aov, single)])
+    p1 = dn.tensor(capi.APH_DN_PREDICTION).reshape(1, s, s, 4)
+    dn.denoise([(noisy, aov, ens, capi.APH_DN_ENSEMBLE)])
+    pk = dn.tensor(capi.APH_DN_PREDICTION).reshape(8, s, s, 4)  # square: 8 transforms
+    np.testing.assert_array_equal(pk[0, ..., :3], p1[0, ..., :3])  # channel 3 is padding
+    expected = np.zeros((s, s, 3))
+    for k in range(8):
+        sy, sx = source_index(s, k)
+        img = np.empty((s, s, 3))
+        img[sy, sx] = np.expm1(np.maximum(pk[k, ..., :3].astype(np.float64), 0))
+        expected += img / 8
+    got = r.readback(ens, s, s)
+    np.testing.assert_allclose(got[..., :3], expected, rtol=1e-5, atol=1e-6)
+    np.testing.assert_array_equal(got[..., 3], r.readback(noisy, s, s)[..., 3])
+    # a non-square image takes the 4 mirrors
+    (na, aa), _ = setup["inputs"]
+    w, h = setup["w"], setup["h"]
+    dn.denoise([(na, aa, r.target(w, h), capi.APH_DN_ENSEMBLE)])
+    assert dn.tensor(capi.APH_DN_PREDICTION).size == 4 * padded(h) * padded(w) * 4
+    with pytest.raises(AphError):
+        dn.denoise([(na, aa, r.target(w, h), 2)])  # unknown flag
+    dn.close()
+
+
 def test_training_reduces_loss(renderer, setup):
     r = renderer
     dn = Denoiser(r, seed=1)
