@@ -46,7 +46,7 @@ def scene_for(i, size, seed0):
         sc = scenes.config2(seed=seed, subdiv=3 + i % 2, width=size, height=size, env_w=1024, env_h=512,
                             instances=int(rng.integers(60, 200)), trace_depth=5)
         radius, target, yr = rng.uniform(16.0, 40.0), (0.0, 1.0, 0.0), (4.0, 18.0)
-        fov = float(rng.uniform(12.0, 45.0))
+        fov = float(rng.uniform(8.0, 32.0))
     th = rng.uniform(0.0, 2.0 * math.pi)
     eye = (radius * math.cos(th), float(rng.uniform(*yr)), radius * math.sin(th))
     tgt = tuple(float(t + rng.uniform(-2.0, 2.0)) for t in target)

@@ -304,6 +304,32 @@ def equal_time_rmse(r, builder, width=480, height=270, ref_spp=4096, plain_spp=6
             "rmse_ratio": round(a / b, 3)}
 
 
+def autotune_section(r, sc, tlas, spp=8):
+    """aph_wavefront_autotune on the headline frame at reduced spp: the chosen launch
+    configuration and its frame time against the default one."""
+    tgt = r.target(sc.width, sc.height)
+    req = [r.request(tgt, tlas, sc.frame, 0, spp)]
+    default = r.tuning()
+
+    def frame_ms():
+        r.render(req, capi.APH_MODE_WAVEFRONT)
+        r.wait()
+        t = time.perf_counter()
+        r.render(req, capi.APH_MODE_WAVEFRONT)
+        r.wait()
+        return (time.perf_counter() - t) * 1e3
+
+    base = frame_ms()
+    best = r.autotune(req)
+    tuned = frame_ms()
+    out = {"spp": spp, "default": [default.chunk_paths, default.trace_blocks_per_cu, default.shade_blocks_per_cu],
+           "default_ms": round(base, 2),
+           "chosen": [best.chunk_paths, best.trace_blocks_per_cu, best.shade_blocks_per_cu],
+           "chosen_ms": round(tuned, 2), "candidates": 27}
+    r.set_tuning(default.chunk_paths, default.trace_blocks_per_cu, default.shade_blocks_per_cu)
+    return out
+
+
 def denoiser_flops(width, height):
     """Algorithmic work of one denoise: 2 * 9 * cin * cout multiply-adds per output
     pixel of every convolution, over the real (unpadded) channels."""
@@ -418,10 +444,11 @@ def main():
     ap.add_argument("--subdiv", type=int, default=4, help="4: 1.02M instanced tris; 5: 4.1M")
     ap.add_argument("--mode", choices=["megakernel", "wavefront"], default="wavefront")
     ap.add_argument("--builder", choices=["sah", "lbvh"], default="sah", help="BLAS builder")
-    ap.add_argument("--sections", default="literal,c3,serving,rmse,denoise",
+    ap.add_argument("--sections", default="literal,c3,serving,rmse,denoise,autotune",
                     help="extra measurements after the headline line's frame: literal (config2 at subdiv 5), "
                          "c3 (64 cameras), serving (Poisson replay), rmse (equal-time RMSE), denoise (denoised "
-                         "4 spp vs plain 256 spp at 1080p); '' for none")
+                         "4 spp vs plain 256 spp at 1080p), autotune (launch configuration search on the "
+                         "headline frame); '' for none")
     ap.add_argument("--denoiser-weights", default="",
                     help="comma-separated weight files for the denoise section (default: the committed weights); "
                          "the first is the reported one, the rest are scored under denoise.variants")
@@ -557,6 +584,9 @@ def main():
             (pst["extension_rays"] + pst["shadow_rays"]) / t_plain / 1e6, 2)
         log(f"plain megakernel frame: {t_plain * 1e3:.1f} ms")
     sections = [s for s in args.sections.split(",") if s]
+    if "autotune" in sections and args.mode == "wavefront":
+        out["autotune"] = autotune_section(r, sc, tlas)
+        log(f"autotune: {out['autotune']}")
     if "literal" in sections:
         out["config"]["config2_subdiv5"] = literal_config2(r, builder, sc.spp)
         log(f"config2 subdiv 5: {out['config']['config2_subdiv5']}")
