@@ -120,6 +120,100 @@ __global__ __launch_bounds__(256) void kConv3x3(ConvArgs a)
     }
 }
 
+// ---- convolution through LDS (32 or 64 output channels) ----
+// Block: 4 waves over a tile of 4P rows x 16 columns of one image and 16*T
+// output channels.  For each chunk of 32 input channels the block stages the
+// tile's input with a one-pixel halo and the chunk's weights of all 9 taps in
+// LDS; every wave then takes its A fragments (P rows of 16 pixels) and B
+// fragments from there.  Pixels are 40 bf16 apart in LDS (80 B: the 16 lanes
+// of a fragment read 16 disjoint 4-bank groups).
+constexpr int kLdsPix = 40;
+
+template <bool RELU, bool OUT_F32, int P, int T>
+__global__ __launch_bounds__(256) void kConvLds(ConvArgs a)
+{
+    constexpr int TR = 4 * P, HR = TR + 2, HC = 18;
+    __shared__ __attribute__((aligned(16))) __bf16 tile[HR * HC * kLdsPix];
+    __shared__ __attribute__((aligned(16))) __bf16 wts[9 * 16 * T * kLdsPix];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, row = lane & 15, kg = lane >> 4;
+    const int tilesX = (a.W + 15) / 16, tilesY = (a.H + TR - 1) / TR;
+    const int tx0 = (int)(blockIdx.x % tilesX) * 16;
+    const int rest = (int)(blockIdx.x / tilesX);
+    const int ty0 = (rest % tilesY) * TR, n = rest / tilesY;
+    const int coBase = blockIdx.y * (16 * T);
+    const int nTiles = min(T, (a.coutP - coBase) / 16);
+    f32x4 acc[P][T];
+#pragma unroll
+    for (int i = 0; i < P; i++)
+#pragma unroll
+        for (int j = 0; j < T; j++) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int c0 = 0; c0 < a.cinP; c0 += 32) {
+        // stage the haloed input tile (4 segments of 8 channels per pixel)
+        for (int sgi = tid; sgi < HR * HC * 4; sgi += 256) {
+            const int px = sgi >> 2, part = sgi & 3;
+            const int hy = px / HC, hx = px - hy * HC;
+            const int gy = ty0 + hy - 1, gx = tx0 + hx - 1;
+            bf16x8 v{};
+            if (gy >= 0 && gy < a.H && gx >= 0 && gx < a.W)
+                v = load8(a.in + (((int64_t)n * a.H + gy) * a.W + gx) * a.inStride + c0 + part * 8);
+            *reinterpret_cast<bf16x8*>(&tile[px * kLdsPix + part * 8]) = v;
+        }
+        // stage the chunk's weights: 9 taps x 16*T output channels x 32 inputs
+        for (int sgi = tid; sgi < 9 * 16 * T * 4; sgi += 256) {
+            const int r = sgi >> 2, part = sgi & 3;
+            const int t = r / (16 * T), co = r - t * (16 * T);
+            bf16x8 v{};
+            if (co < nTiles * 16) v = load8(a.w + ((int64_t)t * a.coutP + coBase + co) * a.cinP + c0 + part * 8);
+            *reinterpret_cast<bf16x8*>(&wts[r * kLdsPix + part * 8]) = v;
+        }
+        __syncthreads();
+#pragma unroll 1
+        for (int t = 0; t < 9; t++) {
+            const int dy = t / 3, dx = t % 3;  // halo offsets of the tap
+            bf16x8 av[P], bv[T];
+#pragma unroll
+            for (int i = 0; i < P; i++)
+                av[i] = *reinterpret_cast<const bf16x8*>(
+                    &tile[((wave * P + i + dy) * HC + row + dx) * kLdsPix + kg * 8]);
+#pragma unroll
+            for (int j = 0; j < T; j++)
+                bv[j] = *reinterpret_cast<const bf16x8*>(&wts[(t * 16 * T + j * 16 + row) * kLdsPix + kg * 8]);
+#pragma unroll
+            for (int j = 0; j < T; j++) {
+                if (j < nTiles) {
+#pragma unroll
+                    for (int i = 0; i < P; i++)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bv[j], acc[i][j], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // D of subtile i: rows = the 16 pixels of tile row wave*P + i, columns = channels
+#pragma unroll
+    for (int j = 0; j < T; j++) {
+        if (j >= nTiles) continue;
+        const int co = coBase + j * 16 + row;
+        if (co >= a.coutStore) continue;
+        const float b = a.bias ? a.bias[co] : 0.0f;
+#pragma unroll
+        for (int i = 0; i < P; i++) {
+            const int y = ty0 + wave * P + i;
+            if (y >= a.H) continue;
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int x = tx0 + kg * 4 + e;
+                if (x >= a.W) continue;
+                const int64_t q = ((int64_t)n * a.H + y) * a.W + x;
+                float v = acc[i][j][e] + b;
+                if (RELU) v = fmaxf(v, 0.0f);
+                if (OUT_F32) a.outF[q * a.outStride + a.outOffset + co] = v;
+                else a.out[q * a.outStride + a.outOffset + co] = toBf(v);
+            }
+        }
+    }
+}
+
 // ---- weight gradient: dW[t][co][ci] += sum_p dY[p][co] X[p + off(t)][ci] ----
 // Block: one tap, 16*TC output x 16*TI input channels, one pixel chunk; its 4
 // waves take interleaved steps of 32 pixels (K = pixels, 8 per lane,
@@ -478,14 +572,29 @@ void convLaunch(const ConvArgs& a, bool relu, bool outF32, hipStream_t st)
     }
 }
 
+template <int P, int T>
+void convLdsLaunch(const ConvArgs& a, bool relu, bool outF32, hipStream_t st)
+{
+    const unsigned tiles = (unsigned)((a.W + 15) / 16) * (unsigned)((a.H + 4 * P - 1) / (4 * P)) * (unsigned)a.N;
+    dim3 grid(tiles, (unsigned)((a.coutP + 16 * T - 1) / (16 * T)));
+    if (outF32) {
+        if (relu) kConvLds<true, true, P, T><<<grid, 256, 0, st>>>(a);
+        else kConvLds<false, true, P, T><<<grid, 256, 0, st>>>(a);
+    } else {
+        if (relu) kConvLds<true, false, P, T><<<grid, 256, 0, st>>>(a);
+        else kConvLds<false, false, P, T><<<grid, 256, 0, st>>>(a);
+    }
+}
+
 hipError_t launchConv(const ConvArgs& a, bool relu, bool outF32, hipStream_t st)
 {
     const int64_t npix = (int64_t)a.N * a.H * a.W;
     if (npix == 0) return hipSuccess;
-    // 16 accumulators per wave: 8 pixel x 2 channel subtiles for 32 output
-    // channels, 4 x 4 for 64, 2 x 8 above
-    if (a.coutP <= 32) convLaunch<8, 2>(a, relu, outF32, st);
-    else if (a.coutP <= 64) convLaunch<4, 4>(a, relu, outF32, st);
+    // 32 or 64 output channels (the full- and half-resolution layers): LDS
+    // tiles of 32 x 16 or 16 x 16 pixels; wider layers: 2 pixel x 8 channel
+    // subtiles per wave from global memory
+    if (a.coutP <= 32) convLdsLaunch<8, 2>(a, relu, outF32, st);
+    else if (a.coutP <= 64) convLdsLaunch<4, 4>(a, relu, outF32, st);
     else convLaunch<2, 8>(a, relu, outF32, st);
     return hipGetLastError();
 }

@@ -46,7 +46,7 @@ def scene_for(i, size, seed0):
         sc = scenes.config2(seed=seed, subdiv=3 + i % 2, width=size, height=size, env_w=1024, env_h=512,
                             instances=int(rng.integers(60, 200)), trace_depth=5)
         radius, target, yr = rng.uniform(16.0, 40.0), (0.0, 1.0, 0.0), (4.0, 18.0)
-        fov = float(rng.uniform(8.0, 32.0))
+        fov = float(rng.uniform(6.0, 26.0))
     th = rng.uniform(0.0, 2.0 * math.pi)
     eye = (radius * math.cos(th), float(rng.uniform(*yr)), radius * math.sin(th))
     tgt = tuple(float(t + rng.uniform(-2.0, 2.0)) for t in target)
@@ -135,11 +135,11 @@ def train(r, args, train_set, held, loss_name, out_path, log):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--scenes", type=int, default=384)
+    ap.add_argument("--scenes", type=int, default=448)
     ap.add_argument("--holdout", type=int, default=16)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--ref-spp", type=int, default=4096)
-    ap.add_argument("--steps", type=int, default=16000)
+    ap.add_argument("--steps", type=int, default=40000)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=1000)
