@@ -175,31 +175,36 @@ aph_denoiser* findDenoiser(aph_ctx* c, uint64_t h)
 }
 
 // Lays out the activation (and, for training, gradient) buffers of N images of
-// Hp x Wp; grows the allocations when needed.
+// Hp x Wp; grows the allocations when needed.  The shape is recorded only once
+// every buffer is in place (a failed allocation leaves no shape behind).
 hipError_t reserveShape(aph_denoiser* d, int N, int Hp, int Wp, bool training)
 {
+    auto pix = [&](int level) { return (int64_t)N * (Hp >> level) * (Wp >> level); };
+    size_t actOff[kNumBufs], gradOff[kNumBufs];
+    size_t a = 0, g = 0;
+    for (int b = 0; b < kNumBufs; b++) {
+        const size_t bytes = ((size_t)pix(kBufs[b].level) * kBufs[b].channels * 2 + 255) & ~(size_t)255;
+        actOff[b] = a;
+        gradOff[b] = g;
+        if (b != B_OUT) a += bytes;
+        g += bytes;
+    }
+    d->N = 0;
+    d->haveGrads = false;
+    hipError_t e = d->acts.reserve(a);
+    if (e == hipSuccess) e = d->pred.reserve((size_t)pix(0) * kDnPredStride * 4);
+    if (e == hipSuccess && training) e = d->actGrads.reserve(g);
+    if (e != hipSuccess) return e;
+    std::memcpy(d->actOff, actOff, sizeof(actOff));
+    d->actBytes = a;
+    if (training) {
+        std::memcpy(d->gradOff, gradOff, sizeof(gradOff));
+        d->gradBytes = g;
+    }
     d->N = N;
     d->Hp = Hp;
     d->Wp = Wp;
-    size_t off = 0;
-    for (int b = 0; b < kNumBufs; b++) {
-        d->actOff[b] = off;
-        if (b != B_OUT) off += ((size_t)d->levelPix(kBufs[b].level) * kBufs[b].channels * 2 + 255) & ~(size_t)255;
-    }
-    d->actBytes = off;
-    hipError_t e = d->acts.reserve(off);
-    if (e != hipSuccess) return e;
-    e = d->pred.reserve((size_t)d->levelPix(0) * kDnPredStride * 4);
-    if (e != hipSuccess) return e;
-    d->haveGrads = false;
-    if (!training) return hipSuccess;
-    off = 0;
-    for (int b = 0; b < kNumBufs; b++) {
-        d->gradOff[b] = off;
-        off += ((size_t)d->levelPix(kBufs[b].level) * kBufs[b].channels * 2 + 255) & ~(size_t)255;
-    }
-    d->gradBytes = off;
-    return d->actGrads.reserve(off);
+    return hipSuccess;
 }
 
 hipError_t refreshWeights(aph_denoiser* d, hipStream_t st)
