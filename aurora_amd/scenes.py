@@ -12,6 +12,10 @@ Workloads:
     triangles; subdivision 5 gives 4.1 M), random rotations, materials with
     transmission and coat, 2048x1024 HDR environment with a 1e4 sun,
     1920x1080, 64 spp, 5 bounces.
+  * ``config5`` - the denoiser / temporal-resolve sequence: 10,000 instances
+    of five 2,048-triangle rocks on a 4.19 M-triangle terrain (24.7 M
+    instanced triangles), a sun sky, and a seeded 120-frame camera spline
+    (``camera_spline``).
 """
 import math
 
@@ -73,6 +77,65 @@ def displaced_icosphere(subdiv, rng, amplitude=0.15, octaves=6):
     p *= (1.0 + disp)[:, None]
     pos = p.astype(np.float32)
     return pos, vertex_normals(pos, f), f
+
+
+def rock(rng, rings=33, segments=32, amplitude=0.25, octaves=5):
+    """A UV sphere of 2 * segments * (rings - 1) triangles (2,048 by default) displaced
+    along the normal by seeded smooth noise."""
+    th = np.linspace(0.0, math.pi, rings + 1)[1:-1]
+    ph = np.arange(segments) * (2 * math.pi / segments)
+    ring = np.stack([np.sin(th)[:, None] * np.cos(ph)[None, :], np.cos(th)[:, None] * np.ones_like(ph)[None, :],
+                     np.sin(th)[:, None] * np.sin(ph)[None, :]], -1).reshape(-1, 3)
+    p = np.concatenate([[[0.0, 1.0, 0.0]], ring, [[0.0, -1.0, 0.0]]])
+    disp = np.zeros(len(p))
+    for o in range(octaves):
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        disp += (amplitude / (1.7 ** o)) * np.sin(1.3 * (1.9 ** o) * (p @ d) * math.pi + rng.uniform(0, 2 * math.pi))
+    p *= (1.0 + disp)[:, None]
+    nr = rings - 1  # latitude rows of ring vertices
+    f = []
+    j = np.arange(segments)
+    jn = (j + 1) % segments
+    f.append(np.stack([np.zeros(segments, dtype=np.int64), 1 + jn, 1 + j], 1))  # top cap
+    for r in range(nr - 1):
+        a, b = 1 + r * segments, 1 + (r + 1) * segments
+        f.append(np.stack([a + j, a + jn, b + jn], 1))
+        f.append(np.stack([a + j, b + jn, b + j], 1))
+    last, bottom = 1 + (nr - 1) * segments, len(p) - 1
+    f.append(np.stack([np.full(segments, bottom), last + j, last + jn], 1))  # bottom cap
+    idx = np.concatenate(f).astype(np.uint32)
+    pos = p.astype(np.float32)
+    return pos, vertex_normals(pos, idx), idx
+
+
+def terrain_height(x, z, seed, amplitude=6.0):
+    """Seeded smooth height field (sum of rotated sinusoids)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    h = np.zeros(np.broadcast(x, z).shape)
+    for o in range(6):
+        a = rng.uniform(0, 2 * math.pi)
+        k = 0.012 * (2.0 ** o)
+        h += (amplitude / (1.8 ** o)) * np.sin(k * (math.cos(a) * x + math.sin(a) * z) + rng.uniform(0, 2 * math.pi))
+    return h
+
+
+def terrain(n, half, seed):
+    """n x n quads (2 n^2 triangles) over [-half, half]^2 with analytic-gradient normals."""
+    g = np.linspace(-half, half, n + 1)
+    x, z = np.meshgrid(g, g, indexing="xy")
+    y = terrain_height(x, z, seed)
+    e = 1e-3 * half
+    dx = (terrain_height(x + e, z, seed) - terrain_height(x - e, z, seed)) / (2 * e)
+    dz = (terrain_height(x, z + e, seed) - terrain_height(x, z - e, seed)) / (2 * e)
+    nrm = np.stack([-dx, np.ones_like(dx), -dz], -1)
+    nrm /= np.linalg.norm(nrm, axis=-1, keepdims=True)
+    pos = np.stack([x, y, z], -1).reshape(-1, 3).astype(np.float32)
+    i = np.arange(n)
+    a = (i[:, None] * (n + 1) + i[None, :]).reshape(-1)  # row-major quad corners
+    b, c, d = a + 1, a + n + 1, a + n + 2
+    idx = np.concatenate([np.stack([a, c, b], 1), np.stack([b, c, d], 1)]).astype(np.uint32)
+    return pos, nrm.reshape(-1, 3).astype(np.float32), idx
 
 
 def ground_quad(half=10.0):
@@ -328,6 +391,62 @@ def config2(seed=2, subdiv=4, width=1920, height=1080, spp=64, trace_depth=5, en
     sc.width, sc.height, sc.spp = width, height, spp
     sc.frame = look_at_frame((0.0, 14.0, 34.0), (0.0, 1.0, 0.0), (0, 1, 0), 50.0, width / height, trace_depth)
     return sc
+
+
+def config5(seed=5, rocks=10000, terrain_quads=1448, half=200.0, width=1920, height=1080, spp=4, trace_depth=5,
+            env_w=2048, env_h=1024):
+    """10,000 instances of five 2,048-triangle rocks scattered on a 1448^2-quad
+    (4.19 M-triangle) terrain, sun sky; the camera is frame 0 of camera_spline."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sc = Scene()
+    sc.name = "config5"
+    tp, tn, ti = terrain(terrain_quads, half, seed)
+    sc.meshes = [(tp, tn, ti)] + [rock(rng) for _ in range(5)]
+    sc.materials = [_material((0.35, 0.3, 0.25), 0.0, 0.8)]
+    for _ in range(31):
+        sc.materials.append(_material(tuple(rng.uniform(0.1, 0.9, 3)), float(rng.uniform() < 0.2),
+                                      float(rng.uniform(0.1, 0.9)), coat=float(rng.uniform() < 0.3)))
+    inst = [(0, 0, affine())]
+    xz = rng.uniform(-0.95 * half, 0.95 * half, (rocks, 2))
+    ys = terrain_height(xz[:, 0], xz[:, 1], seed)
+    for k in range(rocks):
+        s = float(rng.uniform(0.3, 2.5))
+        inst.append((1 + int(rng.integers(0, 5)), 1 + int(rng.integers(0, 31)),
+                     affine(random_rotation(rng), s, (xz[k, 0], ys[k] + 0.3 * s, xz[k, 1]))))
+    sc.instances = _instances(inst)
+    sky = sun_sky(env_w, env_h, rng)
+    sc.env = dict(light_rgb=sky, background_rgb=sky)
+    sc.width, sc.height, sc.spp = width, height, spp
+    sc.seed = seed
+    sc.frame = camera_spline(seed, 1, width, height, trace_depth, half)[0]
+    return sc
+
+
+def camera_spline(seed, frames, width, height, trace_depth=5, half=200.0, fov_y_deg=50.0):
+    """A seeded closed Catmull-Rom loop of 6 eye points above the terrain of config5,
+    each frame looking at a point further along the loop."""
+    rng = np.random.Generator(np.random.PCG64(seed * 31 + 7))
+    ang = np.sort(rng.uniform(0, 2 * math.pi, 6))
+    rad = rng.uniform(0.2, 0.6, 6) * half
+    ctrl = np.stack([rad * np.cos(ang), np.zeros(6), rad * np.sin(ang)], -1)
+    ctrl[:, 1] = terrain_height(ctrl[:, 0], ctrl[:, 2], seed) + rng.uniform(8.0, 25.0, 6)
+
+    def at(u):
+        u = u % 6.0
+        i = int(u)
+        t = u - i
+        p0, p1, p2, p3 = (ctrl[(i + k) % 6] for k in (-1, 0, 1, 2))
+        return 0.5 * ((2 * p1) + (-p0 + p2) * t + (2 * p0 - 5 * p1 + 4 * p2 - p3) * t * t
+                      + (-p0 + 3 * p1 - 3 * p2 + p3) * t * t * t)
+
+    out = []
+    for f in range(frames):
+        u = 6.0 * f / max(frames, 1) * 0.25  # a quarter of the loop over the sequence
+        eye = at(u)
+        look = at(u + 0.35)
+        look[1] = terrain_height(look[0], look[2], seed) + 1.0
+        out.append(look_at_frame(tuple(eye), tuple(look), (0, 1, 0), fov_y_deg, width / height, trace_depth))
+    return out
 
 
 def serving_scenes(seed=7, env_w=1024, env_h=512):

@@ -135,11 +135,11 @@ def train(r, args, train_set, held, loss_name, out_path, log):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--scenes", type=int, default=448)
+    ap.add_argument("--scenes", type=int, default=512)
     ap.add_argument("--holdout", type=int, default=16)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--ref-spp", type=int, default=4096)
-    ap.add_argument("--steps", type=int, default=40000)
+    ap.add_argument("--steps", type=int, default=44000)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=1000)

@@ -97,7 +97,8 @@ def measure_traffic(args, kname):
         d = tempfile.mkdtemp(prefix="aph_pmc_", dir="/tmp")
         cmd = ["timeout", "-s", "KILL", "240", prof, "--pmc", counter, "-d", d, "-o", "pmc", "--",
                sys.executable, "-u", os.path.abspath(__file__), "--pmc-child", "--spp", str(args.spp),
-               "--subdiv", str(args.subdiv), "--mode", args.mode, "--builder", args.builder]
+               "--subdiv", str(args.subdiv), "--mode", args.mode, "--builder", args.builder,
+               "--tuning", args.tuning]
         log(f"pmc pass {counter}: {' '.join(cmd)}")
         env = dict(os.environ, TMPDIR="/tmp")
         p = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
@@ -118,6 +119,12 @@ def measure_traffic(args, kname):
     detail = {"fetch_kib_per_dispatch": round(f_kib, 1), "fetch_factor": round(f_fac, 4),
               "write_kib_per_dispatch": round(w_kib, 1), "write_factor": round(w_fac, 4), "dispatches": nd}
     return int((f_kib * f_fac + w_kib * w_fac) * 1024), detail
+
+
+def apply_tuning(r, tuning):
+    if tuning:
+        c, t, sh = (int(v) for v in tuning.split(","))
+        r.set_tuning(c, t, sh)
 
 
 def cpu_baseline(sc, budget_s=12.0):
@@ -154,6 +161,22 @@ def cpu_baseline(sc, budget_s=12.0):
                   f"{total_rays} rays in {total_time:.1f} s on {threads} host threads",
         "samples_per_s": round(total_pix / total_time, 1),
     }
+
+
+def autotune_child(args):
+    """aph_wavefront_autotune on the headline frame in a child process (before this
+    process touches the GPU): the launch configuration the measured line uses."""
+    cmd = [sys.executable, "-u", os.path.abspath(__file__), "--autotune-child", "--spp", str(args.spp),
+           "--subdiv", str(args.subdiv), "--builder", args.builder]
+    log("autotune (child process)...")
+    try:
+        p = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=300)
+    except subprocess.TimeoutExpired:
+        return None
+    if p.returncode != 0:
+        log(p.stderr[-3000:])
+        return None
+    return json.loads(p.stdout.strip().splitlines()[-1])
 
 
 def cpu_baseline_child(args):
@@ -306,7 +329,8 @@ def equal_time_rmse(r, builder, width=480, height=270, ref_spp=4096, plain_spp=6
 
 def autotune_section(r, sc, tlas, spp=8):
     """aph_wavefront_autotune on the headline frame at reduced spp: the chosen launch
-    configuration and its frame time against the default one."""
+    configuration and its frame time against the default one (the context keeps
+    the chosen one)."""
     tgt = r.target(sc.width, sc.height)
     req = [r.request(tgt, tlas, sc.frame, 0, spp)]
     default = r.tuning()
@@ -326,8 +350,48 @@ def autotune_section(r, sc, tlas, spp=8):
            "default_ms": round(base, 2),
            "chosen": [best.chunk_paths, best.trace_blocks_per_cu, best.shade_blocks_per_cu],
            "chosen_ms": round(tuned, 2), "candidates": 27}
-    r.set_tuning(default.chunk_paths, default.trace_blocks_per_cu, default.shade_blocks_per_cu)
     return out
+
+
+def config5_sequence(r, builder, frames=120, spp=4, width=1920, height=1080):
+    """Config 5 (24.7 M instanced triangles) along its 120-frame camera spline at 1080p:
+    per frame a 4-spp wavefront render, the guide images, the denoiser (committed
+    weights) and the temporal resolve, each frame synchronized; latency p50/p99."""
+    t = time.perf_counter()
+    sc = scenes.config5(width=width, height=height)
+    t_gen = time.perf_counter() - t
+    t = time.perf_counter()
+    tlas = r.load_scene(sc, builder)
+    r.wait()
+    t_load = time.perf_counter() - t
+    cams = scenes.camera_spline(sc.seed, frames, width, height)
+    dn = dnmod.Denoiser(r)
+    dn.load()
+    noisy, den, aov = r.target(width, height), r.target(width, height), r.aov(width, height)
+    hist = [r.target(width, height), r.target(width, height)]
+    lat = []
+    prev = cams[0]
+    r.reset_stats()
+    for i, fr in enumerate(cams):
+        t = time.perf_counter()
+        r.render([r.request(noisy, tlas, fr, 0, spp, i)], capi.APH_MODE_WAVEFRONT)
+        r.render_aovs(aov, tlas, fr, scenes.view_proj(fr), scenes.view_proj(prev),
+                      np.array(prev.inv_view, dtype=np.float32))
+        dn.denoise([(noisy, aov, den)])
+        r.temporal_resolve(den, aov, hist[i & 1], hist[(i + 1) & 1], is_reset=(i == 0))
+        r.wait()
+        lat.append(time.perf_counter() - t)
+        prev = fr
+    dn.close()
+    lat = np.array(lat[1:]) * 1e3  # frame 0 allocates
+    return {"scene": f"config5: {len(sc.instances) - 1} rocks of 2048 triangles on a {len(sc.meshes[0][2])}-triangle "
+                     f"terrain, {sc.instanced_triangles} instanced triangles",
+            "resolution": f"{width}x{height}", "spp": spp, "frames": frames,
+            "pipeline": "wavefront render + guide images + U-Net + temporal resolve",
+            "generate_s": round(t_gen, 2), "load_build_s": round(t_load, 3),
+            "frame_ms_p50": round(float(np.percentile(lat, 50)), 2),
+            "frame_ms_p99": round(float(np.percentile(lat, 99)), 2),
+            "fps": round(1e3 / float(np.mean(lat)), 1)}
 
 
 def denoiser_flops(width, height):
@@ -444,11 +508,17 @@ def main():
     ap.add_argument("--subdiv", type=int, default=4, help="4: 1.02M instanced tris; 5: 4.1M")
     ap.add_argument("--mode", choices=["megakernel", "wavefront"], default="wavefront")
     ap.add_argument("--builder", choices=["sah", "lbvh"], default="sah", help="BLAS builder")
-    ap.add_argument("--sections", default="literal,c3,serving,rmse,denoise,autotune",
+    ap.add_argument("--sections", default="literal,c3,serving,rmse,denoise,config5",
                     help="extra measurements after the headline line's frame: literal (config2 at subdiv 5), "
                          "c3 (64 cameras), serving (Poisson replay), rmse (equal-time RMSE), denoise (denoised "
-                         "4 spp vs plain 256 spp at 1080p), autotune (launch configuration search on the "
-                         "headline frame); '' for none")
+                         "4 spp vs plain 256 spp at 1080p), config5 (24.7M-triangle 120-frame sequence with "
+                         "denoiser and temporal resolve); '' for none")
+    ap.add_argument("--tuning", default="",
+                    help="wavefront launch configuration 'chunk_paths,trace_blocks_per_cu,shade_blocks_per_cu'; "
+                         "default: the autotuner's choice on the headline frame (8 spp, child process)")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the library's default launch configuration")
+    ap.add_argument("--autotune-child", action="store_true",
+                    help="internal: run the autotuner on the headline frame and print its choice")
     ap.add_argument("--denoiser-weights", default="",
                     help="comma-separated weight files for the denoise section (default: the committed weights); "
                          "the first is the reported one, the rest are scored under denoise.variants")
@@ -471,19 +541,32 @@ def main():
     if args.cpu_baseline_child:
         print(json.dumps(cpu_baseline(sc)), flush=True)
         return
+    if args.autotune_child:
+        r = Renderer(0)
+        print(json.dumps(autotune_section(r, sc, r.load_scene(sc, builder))), flush=True)
+        r.close()
+        return
     if args.pmc_child:
         copy_calibration()
         r = Renderer(0)
+        apply_tuning(r, args.tuning)
         tgt = r.target(sc.width, sc.height)
         r.render([r.request(tgt, r.load_scene(sc, builder), sc.frame, 0, sc.spp)], mode)
         r.wait()
         r.close()
         print("pmc child done", flush=True)
         return
+    tune_info = None
+    if not args.tuning and not args.no_autotune and args.mode == "wavefront":
+        tune_info = autotune_child(args)
+        if tune_info:
+            args.tuning = ",".join(str(v) for v in tune_info["chosen"])
+            log(f"autotune: {tune_info}")
     traffic, traffic_detail = (None, "skipped (--no-traffic)") if args.no_traffic else measure_traffic(args, kname)
     log(f"scene: {sc.instanced_triangles} instanced / {sc.unique_triangles} unique triangles, "
         f"{len(sc.instances)} instances, {sc.width}x{sc.height} x {sc.spp} spp, mode {args.mode}")
     r = Renderer(0)
+    apply_tuning(r, args.tuning)
     tlas = r.load_scene(sc, builder)
     tgt = r.target(sc.width, sc.height)
     req = [r.request(tgt, tlas, sc.frame, 0, sc.spp)]
@@ -555,6 +638,10 @@ def main():
                         "2048x1024 HDR environment with 1e4 sun",
             "mode": args.mode,
             "blas_builder": args.builder,
+            "launch_config": {"chunk_paths": r.tuning().chunk_paths,
+                              "trace_blocks_per_cu": r.tuning().trace_blocks_per_cu,
+                              "shade_blocks_per_cu": r.tuning().shade_blocks_per_cu,
+                              "source": "autotuner" if tune_info else ("--tuning" if args.tuning else "default")},
             "samples_per_s": round(sc.width * sc.height * sc.spp * args.steps / wall, 1),
             "extension_rays_per_frame": tst["extension_rays"] // args.steps,
             "shadow_rays_per_frame": tst["shadow_rays"] // args.steps,
@@ -584,9 +671,8 @@ def main():
             (pst["extension_rays"] + pst["shadow_rays"]) / t_plain / 1e6, 2)
         log(f"plain megakernel frame: {t_plain * 1e3:.1f} ms")
     sections = [s for s in args.sections.split(",") if s]
-    if "autotune" in sections and args.mode == "wavefront":
-        out["autotune"] = autotune_section(r, sc, tlas)
-        log(f"autotune: {out['autotune']}")
+    if tune_info:
+        out["autotune"] = tune_info
     if "literal" in sections:
         out["config"]["config2_subdiv5"] = literal_config2(r, builder, sc.spp)
         log(f"config2 subdiv 5: {out['config']['config2_subdiv5']}")
@@ -598,6 +684,9 @@ def main():
     if "rmse" in sections:
         out["equal_time_rmse"] = equal_time_rmse(r, builder)
         log(f"rmse: {out['equal_time_rmse']}")
+    if "config5" in sections:
+        out["config5_sequence"] = config5_sequence(r, builder)
+        log(f"config5: {out['config5_sequence']}")
     if "denoise" in sections:
         out["denoise"] = denoise_vs_plain(r, builder,
                                           weights=[x for x in args.denoiser_weights.split(",") if x] or None)
