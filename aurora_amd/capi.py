@@ -130,6 +130,39 @@ class TaaRequest(C.Structure):
                 ("history_out", C.c_uint64), ("is_reset", C.c_int32), ("blend_alpha_min", C.c_float),
                 ("blend_alpha_max", C.c_float), ("speed_reference", C.c_float)]
 
+class ImageDesc(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("format", C.c_int32), ("linearize", C.c_int32),
+                ("pixels", C.c_void_p)]
+
+
+class TextureBinding(C.Structure):
+    _fields_ = [("image", C.c_uint64), ("address_u", C.c_int32), ("address_v", C.c_int32)]
+
+
+APH_IMAGE_RGBA8, APH_IMAGE_RGBA32F, APH_IMAGE_RGB32F = 0, 1, 2
+APH_ADDRESS_WRAP, APH_ADDRESS_MIRROR, APH_ADDRESS_CLAMP, APH_ADDRESS_MIRROR_ONCE, APH_ADDRESS_BORDER = 0, 1, 2, 3, 4
+APH_TEX_BASE_COLOR, APH_TEX_SPECULAR_ROUGHNESS, APH_TEX_EMISSION_COLOR, APH_TEX_OPACITY, APH_TEX_NORMAL = 0, 1, 2, 3, 4
+APH_TEX_SLOTS = 5
+# word offsets (from byte 224, Material.texture_block) of each slot's flag, offset, scale,
+# pivot and rotation (DefaultMaterialUniformBuffer.slang:43-71)
+TEX_WORDS = [(0, 1, 4, 6, 8), (9, 10, 12, 14, 16), (17, 18, 20, 22, 24), (25, 26, 28, 30, 32), (33, 34, 36, 38, 40)]
+
+
+def set_texture_slot(m, slot, offset=(0.0, 0.0), scale=(1.0, 1.0), pivot=(0.0, 0.0), rotation=0.0, enabled=True):
+    """Sets a material's has_*_image flag and UV transform (the 400-byte layout)."""
+    fl, off, sc, pv, rot = TEX_WORDS[slot]
+    words = np.zeros(44, dtype=np.int32)
+    words[:] = list(m.texture_block)
+    f = words.view(np.float32)
+    words[fl] = 1 if enabled else 0
+    f[off:off + 2] = offset
+    f[sc:sc + 2] = scale
+    f[pv:pv + 2] = pivot
+    f[rot] = rotation
+    for i in range(44):
+        m.texture_block[i] = int(words[i])
+
+
 class DnBuffer(C.Structure):
     _fields_ = [("level", C.c_uint32), ("channels", C.c_uint32)]
 
@@ -220,6 +253,9 @@ SIGNATURES = {
     "aph_server_release": (_i, [_P, _u64]),
     "aph_server_usage": (_i, [_P, C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u32), C.POINTER(_u32)]),
     "aph_server_replay": (_i, [_P, _P, _P, _u32, _P]),
+    "aph_image_create": (_i, [_P, C.POINTER(ImageDesc), C.POINTER(_u64)]),
+    "aph_image_destroy": (_i, [_P, _u64]),
+    "aph_materials_bind_textures": (_i, [_P, _P, _u32]),
     "aph_denoiser_arch": (_i, [_P, C.POINTER(_u32), _P, C.POINTER(_u32), _P, C.POINTER(_u32)]),
     "aph_denoiser_param_count": (_u64, []),
     "aph_denoiser_create": (_i, [_P, _u64, C.POINTER(_u64)]),

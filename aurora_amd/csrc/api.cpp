@@ -23,6 +23,8 @@ using namespace aph;
 
 namespace {
 
+int uploadMaterialTextures(aph_ctx* c);
+
 constexpr uint32_t kSahMaxLeaf = 4;  // triangles per leaf of an SAH BLAS
 
 // Double-precision 4x4 inverse (row-major), rounded to float.
@@ -131,6 +133,8 @@ int aph_create(int device, aph_ctx_t* out)
     if (c->mats.alloc(sizeof(DevMaterial)) != hipSuccess) return APH_E_MEMORY;
     if (hipMemcpy(c->mats.p, &d, sizeof(d), hipMemcpyHostToDevice) != hipSuccess) return APH_E_DEVICE;
     c->nmat = 1;
+    c->matsHost.assign(1, dm);
+    if (uploadMaterialTextures(c.get()) != APH_OK) return APH_E_DEVICE;
     *out = c.release();
     return APH_OK;
 }
@@ -214,10 +218,15 @@ int aph_mesh_upload(aph_ctx_t c, const float* pos, uint32_t nv, const uint32_t* 
         HIPCALL(c, m->nrm.alloc((size_t)nv * 12), "alloc normals");
         HIPCALL(c, hipMemcpyAsync(m->nrm.p, nrm, (size_t)nv * 12, hipMemcpyHostToDevice, c->stream), "copy normals");
     }
-    if (uv) {
-        HIPCALL(c, m->uv.alloc((size_t)nv * 8), "alloc uv");
-        HIPCALL(c, hipMemcpyAsync(m->uv.p, uv, (size_t)nv * 8, hipMemcpyHostToDevice, c->stream), "copy uv");
+    // Texture coordinates; without them u = v = i / vertex count (GeometryBase.cpp:114-124).
+    std::vector<float> genUv;
+    if (!uv) {
+        genUv.resize((size_t)nv * 2);
+        for (uint32_t i = 0; i < nv; i++) genUv[2 * (size_t)i] = genUv[2 * (size_t)i + 1] = (float)i / (float)nv;
+        uv = genUv.data();
     }
+    HIPCALL(c, m->uv.alloc((size_t)nv * 8), "alloc uv");
+    HIPCALL(c, hipMemcpyAsync(m->uv.p, uv, (size_t)nv * 8, hipMemcpyHostToDevice, c->stream), "copy uv");
     HIPCALL(c, hipStreamSynchronize(c->stream), "mesh upload");
     uint64_t h = c->nextHandle++;
     c->meshes[h] = std::move(m);
@@ -328,6 +337,7 @@ int aph_tlas_build(aph_ctx_t c, const aph_instance* inst, uint32_t n, uint64_t* 
         std::memcpy(sh.o2w, in.o2w, sizeof(sh.o2w));
         sh.pos = m->pos.as<float>();
         sh.nrm = m->hasNrm ? m->nrm.as<float>() : nullptr;
+        sh.uv = m->uv.as<float>();
         sh.idx = m->idx.as<uint32_t>();
         sh.material = in.material;
         sh.flags = in.flags;
@@ -463,6 +473,65 @@ void aph_material_defaults(aph_material* m)
     m->coat_ior = 1.5f;
 }
 
+namespace {
+
+// Word offsets (from offset 224) of each texture slot's flag, offset, scale,
+// pivot and rotation in MaterialConstants (DefaultMaterialUniformBuffer.slang:43-71).
+const int kTexWord[kTexSlots][5] = {
+    {0, 1, 4, 6, 8}, {9, 10, 12, 14, 16}, {17, 18, 20, 22, 24}, {25, 26, 28, 30, 32}, {33, 34, 36, 38, 40}};
+
+float blockFloat(const aph_material& m, int w)
+{
+    float f;
+    std::memcpy(&f, &m.texture_block[w], 4);
+    return f;
+}
+
+// Device texture table and per-material slots from the current materials and bindings.
+int uploadMaterialTextures(aph_ctx* c)
+{
+    const uint32_t n = c->nmat;
+    std::vector<DevMatTex> mt(n ? n : 1);
+    std::vector<DevTexture> tex;
+    c->texturesMissing.clear();
+    for (uint32_t i = 0; i < n; i++) {
+        const aph_material& m = c->matsHost[i];
+        DevMatTex& d = mt[i];
+        std::memset(&d, 0, sizeof(d));
+        for (int k = 0; k < kTexSlots; k++) {
+            const int* w = kTexWord[k];
+            d.tex[k] = -1;
+            d.xf[k][0] = blockFloat(m, w[1]);
+            d.xf[k][1] = blockFloat(m, w[1] + 1);
+            d.xf[k][2] = blockFloat(m, w[2]);
+            d.xf[k][3] = blockFloat(m, w[2] + 1);
+            d.xf[k][4] = blockFloat(m, w[3]);
+            d.xf[k][5] = blockFloat(m, w[3] + 1);
+            d.xf[k][6] = blockFloat(m, w[4]);
+            if (!m.texture_block[w[0]]) continue;
+            const aph_texture_binding* b = c->texBindings.empty() ? nullptr : &c->texBindings[(size_t)i * kTexSlots + k];
+            Image* img = b && b->image ? lookup(c->images, b->image) : nullptr;
+            if (!img) {
+                if (k != kTexEmissionColor) c->texturesMissing.push_back(i * kTexSlots + k);
+                continue;
+            }
+            d.tex[k] = (int32_t)tex.size();
+            tex.push_back(DevTexture{img->texels.as<float4>(), img->w, img->h, b->address_u, b->address_v});
+        }
+    }
+    HIPCALL(c, hipStreamSynchronize(c->stream), "material textures");
+    HIPCALL(c, c->matTex.alloc(sizeof(DevMatTex) * mt.size()), "alloc material textures");
+    HIPCALL(c, hipMemcpy(c->matTex.p, mt.data(), sizeof(DevMatTex) * mt.size(), hipMemcpyHostToDevice),
+            "copy material textures");
+    HIPCALL(c, c->textures.alloc(sizeof(DevTexture) * (tex.empty() ? 1 : tex.size())), "alloc texture table");
+    if (!tex.empty())
+        HIPCALL(c, hipMemcpy(c->textures.p, tex.data(), sizeof(DevTexture) * tex.size(), hipMemcpyHostToDevice),
+                "copy texture table");
+    return APH_OK;
+}
+
+}  // namespace
+
 int aph_materials_upload(aph_ctx_t c, const aph_material* mats, uint32_t n)
 {
     int rc = enter(c);
@@ -470,17 +539,90 @@ int aph_materials_upload(aph_ctx_t c, const aph_material* mats, uint32_t n)
     if (!mats || n == 0) return fail(c, APH_E_INVALID, "need at least one material");
     std::vector<DevMaterial> d(n);
     for (uint32_t i = 0; i < n; i++) {
-        // has_*_image flags at byte offsets 224, 260, 292, 324, 356
-        const int texFlags[5] = {0, 9, 17, 25, 33};
-        for (int f : texFlags)
-            if (mats[i].texture_block[f] != 0)
-                return fail(c, APH_E_UNSUPPORTED, "material %u: image textures are not supported yet", i);
         d[i] = toDev(mats[i]);
+        for (int k = 0; k < kTexSlots; k++)
+            if (mats[i].texture_block[kTexWord[k][0]]) d[i].texMask |= 1 << k;
     }
     HIPCALL(c, hipStreamSynchronize(c->stream), "materials");
     HIPCALL(c, c->mats.alloc(sizeof(DevMaterial) * n), "alloc materials");
     HIPCALL(c, hipMemcpy(c->mats.p, d.data(), sizeof(DevMaterial) * n, hipMemcpyHostToDevice), "copy materials");
     c->nmat = n;
+    c->matsHost.assign(mats, mats + n);
+    c->texBindings.clear();
+    return uploadMaterialTextures(c);
+}
+
+int aph_materials_bind_textures(aph_ctx_t c, const aph_texture_binding* bindings, uint32_t n)
+{
+    int rc = enter(c);
+    if (rc) return rc;
+    if (n != c->nmat) return fail(c, APH_E_INVALID, "%u bindings' materials for a table of %u", n, c->nmat);
+    if (!bindings && n) return fail(c, APH_E_INVALID, "null bindings");
+    for (uint32_t i = 0; i < n * (uint32_t)kTexSlots; i++) {
+        const aph_texture_binding& b = bindings[i];
+        if (b.image && !lookup(c->images, b.image))
+            return fail(c, APH_E_INVALID, "material %u slot %u: unknown image %llu", i / kTexSlots, i % kTexSlots,
+                        (unsigned long long)b.image);
+        if (b.address_u < APH_ADDRESS_WRAP || b.address_u > APH_ADDRESS_BORDER || b.address_v < APH_ADDRESS_WRAP ||
+            b.address_v > APH_ADDRESS_BORDER)
+            return fail(c, APH_E_INVALID, "material %u slot %u: unknown address mode", i / kTexSlots,
+                        i % kTexSlots);
+    }
+    c->texBindings.assign(bindings, bindings + (size_t)n * kTexSlots);
+    return uploadMaterialTextures(c);
+}
+
+int aph_image_create(aph_ctx_t c, const aph_image_desc* desc, uint64_t* image)
+{
+    int rc = enter(c);
+    if (rc) return rc;
+    if (!desc || !image || !desc->pixels) return fail(c, APH_E_INVALID, "null image description or output");
+    *image = 0;
+    const uint32_t w = desc->width, h = desc->height;
+    if (w == 0 || h == 0 || w > 32768 || h > 32768) return fail(c, APH_E_INVALID, "bad image size %ux%u", w, h);
+    if (desc->format < APH_IMAGE_RGBA8 || desc->format > APH_IMAGE_RGB32F)
+        return fail(c, APH_E_INVALID, "unknown image format %d", desc->format);
+    const size_t n = (size_t)w * h;
+    std::vector<float4> px(n);
+    if (desc->format == APH_IMAGE_RGBA8) {
+        // HgiFormatUNorm8Vec4(srgb): unorm, and with linearize the sRGB transfer
+        // function undone per channel (alpha stays linear) before filtering.
+        const uint8_t* p = (const uint8_t*)desc->pixels;
+        float lut[256], lin[256];
+        for (int v = 0; v < 256; v++) {
+            lut[v] = (float)v / 255.0f;
+            lin[v] = srgbToLinear(lut[v]);
+        }
+        const float* rgb = desc->linearize ? lin : lut;
+        for (size_t i = 0; i < n; i++)
+            px[i] = make_float4(rgb[p[4 * i]], rgb[p[4 * i + 1]], rgb[p[4 * i + 2]], lut[p[4 * i + 3]]);
+    } else if (desc->format == APH_IMAGE_RGBA32F) {
+        std::memcpy(px.data(), desc->pixels, n * 16);
+    } else {
+        const float* p = (const float*)desc->pixels;
+        for (size_t i = 0; i < n; i++) px[i] = make_float4(p[3 * i], p[3 * i + 1], p[3 * i + 2], 1.0f);
+    }
+    std::unique_ptr<Image> img(new Image());
+    img->w = w;
+    img->h = h;
+    HIPCALL(c, img->texels.alloc(n * 16), "alloc image");
+    HIPCALL(c, hipMemcpy(img->texels.p, px.data(), n * 16, hipMemcpyHostToDevice), "copy image");
+    uint64_t hd = c->nextHandle++;
+    c->images[hd] = std::move(img);
+    *image = hd;
+    return APH_OK;
+}
+
+int aph_image_destroy(aph_ctx_t c, uint64_t image)
+{
+    int rc = enter(c);
+    if (rc) return rc;
+    if (!lookup(c->images, image)) return fail(c, APH_E_INVALID, "unknown image %llu", (unsigned long long)image);
+    for (const aph_texture_binding& b : c->texBindings)
+        if (b.image == image) return fail(c, APH_E_STATE, "image %llu is bound to the material table",
+                                          (unsigned long long)image);
+    HIPCALL(c, hipStreamSynchronize(c->stream), "image destroy");
+    c->images.erase(image);
     return APH_OK;
 }
 
@@ -648,6 +790,7 @@ int aph_render(aph_ctx_t c, const aph_request* reqs, uint32_t nreq, int mode)
     int rc = enter(c);
     if (rc) return rc;
     if (nreq == 0) return APH_OK;
+    if ((rc = checkTextures(c))) return rc;
     if (!reqs) return fail(c, APH_E_INVALID, "requests pointer is null");
     if (mode != APH_MODE_MEGAKERNEL && mode != APH_MODE_WAVEFRONT) return fail(c, APH_E_INVALID, "unknown mode %d", mode);
     std::vector<WavefrontRequest> wreq;
@@ -727,6 +870,7 @@ int aph_wavefront_autotune(aph_ctx_t c, const aph_request* reqs, uint32_t nreq, 
 {
     int rc = enter(c);
     if (rc) return rc;
+    if ((rc = checkTextures(c))) return rc;
     if (!reqs || nreq == 0) return fail(c, APH_E_INVALID, "autotune needs at least one request");
     const uint32_t chunks[] = {2u << 20, 4u << 20, 8u << 20};
     const uint32_t traces[] = {8, 16, 32};

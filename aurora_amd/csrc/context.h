@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <map>
@@ -73,6 +74,17 @@ struct Target {
     DevBuf accum;
 };
 
+struct Image {
+    uint32_t w = 0, h = 0;
+    DevBuf texels;  // linear RGBA float
+};
+
+// sRGB transfer function undone (IEC 61966-2-1), the decode of an sRGB texture format.
+inline float srgbToLinear(float c)
+{
+    return c <= 0.04045f ? c / 12.92f : std::pow((c + 0.055f) / 1.055f, 2.4f);
+}
+
 struct AovSet {
     uint32_t w = 0, h = 0;
     DevBuf depth, normalRoughness, baseColorMetalness, motion;
@@ -97,6 +109,7 @@ struct aph_ctx {
     using Target = aph::Target;
     using Timing = aph::Timing;
     using AovSet = aph::AovSet;
+    using Image = aph::Image;
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
@@ -108,6 +121,11 @@ struct aph_ctx {
     std::map<uint64_t, std::unique_ptr<aph_denoiser, aph::DenoiserDeleter>> denoisers;
     DevBuf mats;
     uint32_t nmat = 0;
+    std::vector<aph_material> matsHost;            // the uploaded table (texture flags and transforms)
+    std::vector<aph_texture_binding> texBindings;  // APH_TEX_SLOTS per material, or empty
+    std::vector<uint32_t> texturesMissing;         // material * APH_TEX_SLOTS + slot: flagged, unbound
+    DevBuf matTex, textures;                       // DevMatTex per material, DevTexture table
+    std::map<uint64_t, std::unique_ptr<Image>> images;
     DevBuf envLight, envAlias, envBg;
     aph::DevEnv env{};
     DevBuf counters;
@@ -167,11 +185,23 @@ inline RenderScene renderScene(const aph_ctx* c, const Tlas* tl)
     rs.sc = tl->sc;
     rs.ishade = tl->shade.as<InstShade>();
     rs.mats = c->mats.as<DevMaterial>();
+    rs.matTex = c->matTex.as<DevMatTex>();
+    rs.textures = c->textures.as<DevTexture>();
     rs.nmat = c->nmat;
     rs.layerMats = tl->layerMats.as<uint32_t>();
     rs.pad_ = 0;
     rs.env = c->env;
     return rs;
+}
+
+// A render reads every flagged texture slot; an unbound one (other than
+// emission, which reads 0) cannot be sampled.
+inline int checkTextures(aph_ctx* c)
+{
+    if (c->texturesMissing.empty()) return APH_OK;
+    const uint32_t k = c->texturesMissing[0];
+    return fail(c, APH_E_STATE, "material %u: texture slot %u is flagged in the material but has no bound image",
+                k / APH_TEX_SLOTS, k % APH_TEX_SLOTS);
 }
 
 inline int enter(aph_ctx* c)

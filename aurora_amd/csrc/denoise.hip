@@ -453,7 +453,17 @@ __global__ void kPack(const float4* noisy, const float* depth, const float4* nor
             for (int k = 0; k < kDnInChannels; k++)
                 if (!(f[k] == f[k]) || fabsf(f[k]) > 1e30f) f[k] = 0.0f;
         }
-        for (int c = 0; c < stride; c++) o[c] = toBf(c < kDnInChannels ? f[c] : 0.0f);
+        // 16-byte stores of 8 channels: the 12 features, then zeros to the stride
+        static_assert(kDnInChannels > 8 && kDnInChannels <= 16, "features fill two 8-channel groups");
+        bf16x8 v0, v1;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            v0[k] = toBf(f[k]);
+            v1[k] = toBf(8 + k < kDnInChannels ? f[(8 + k) % kDnInChannels] : 0.0f);
+        }
+        store8(o, v0);
+        store8(o + 8, v1);
+        for (int c0 = 16; c0 < stride; c0 += 8) store8(o + c0, bf16x8{});
     }
 }
 

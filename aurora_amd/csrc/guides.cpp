@@ -89,6 +89,7 @@ int aph_render_aovs(aph_ctx_t c, const aph_aov_request* r)
     int rc = enter(c);
     if (rc) return rc;
     if (!r) return fail(c, APH_E_INVALID, "null AOV request");
+    if ((rc = checkTextures(c))) return rc;
     AovSet* a = lookup(c->aovs, r->aov);
     Tlas* tl = lookup(c->tlases, r->tlas);
     if (!a) return fail(c, APH_E_INVALID, "unknown AOV set %llu", (unsigned long long)r->aov);

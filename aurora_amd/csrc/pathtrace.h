@@ -2,7 +2,7 @@
 // loop (Shaders/MainEntryPoints.slang:70-334) with the closest-hit
 // (Shaders/HitShaderEntryPoints.slang:35-486) and miss (:354-426) programs
 // inlined and traversal from trace.h.  HGI semantics: opaque shadows, no
-// layers, no ground plane, no AOVs.
+// ground plane; material layers (DirectX semantics) when an instance has them.
 #pragma once
 
 #include "../../include/aph.h"
@@ -15,29 +15,100 @@ struct RenderScene {
     TraceScene sc;
     const InstShade* ishade;
     const DevMaterial* mats;
+    const DevMatTex* matTex;      // per material: bound textures and UV transforms
+    const DevTexture* textures;
     const uint32_t* layerMats;  // material indices of all instances' layers
     uint32_t nmat;
     uint32_t pad_;
     DevEnv env;
 };
 
+// Texture coordinate of a hit (Geometry.slang:199-203): the barycentric mix of
+// the vertex UVs, computed on first use (only textured materials read it).
+struct HitUV {
+    const InstShade* ins;
+    uint32_t prim;
+    float bu, bv;
+    bool have;
+    f2 uv;
+    __device__ HitUV(const InstShade& i, uint32_t p, float u, float v) : ins(&i), prim(p), bu(u), bv(v), have(false) {}
+    __device__ f2 get()
+    {
+        if (!have) {
+            const float b[3] = {1.0f - bu - bv, bu, bv};
+            f2 t = f2{0.0f, 0.0f};
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const uint32_t vi = ins->idx[3 * prim + k];
+                t = f2{t.x + ins->uv[2 * (size_t)vi] * b[k], t.y + ins->uv[2 * (size_t)vi + 1] * b[k]};
+            }
+            uv = t;
+            have = true;
+        }
+        return uv;
+    }
+};
+
+constexpr uint32_t kNoMaterial = 0xFFFFFFFFu;
+
+// Texture slot of material mi at the hit (an unbound slot reads 0: only the
+// emission slot can be flagged and unbound at render time).
+__device__ __forceinline__ f3 sampleSlot(const RenderScene& rs, uint32_t mi, int slot, f2 uv)
+{
+    const DevMatTex& mt = rs.matTex[mi];
+    const int t = mt.tex[slot];
+    if (t < 0) return splat(0.0f);
+    const float4 c = sampleTexture(rs.textures[t], applyUVTransform(uv, mt.xf[slot]));
+    return f3{c.x, c.y, c.z};
+}
+
 // The material a hit shades with: material layers (HitShaderEntryPoints.slang:84-113,
 // DirectX semantics) are tried from the top layer down to the base material; each
-// draws its opacity coin, and a layer the coin passes through scales the throughput
-// by transparency / P.  Without layers this is the single coin of the HGI build.
-// Returns the hit layer's material, or null when every layer was passed through.
-__device__ __forceinline__ const DevMaterial* selectLayer(const RenderScene& rs, const InstShade& ins, Rng& rng,
-                                                          f3& contrib, float& P)
+// draws its opacity coin (its opacity texture, if flagged, in place of the
+// constant), and a layer the coin passes through scales the throughput by
+// transparency / P.  Without layers this is the single coin of the HGI build.
+// Returns the hit layer's material index and opacity, or kNoMaterial when every
+// layer was passed through.
+__device__ __forceinline__ uint32_t selectLayer(const RenderScene& rs, const InstShade& ins, HitUV& huv, Rng& rng,
+                                                f3& contrib, float& P, f3& opacity)
 {
     for (int li = (int)ins.layerCount; li >= 0; li--) {
         uint32_t mi = li == 0 ? ins.material : rs.layerMats[ins.layerFirst + (uint32_t)li - 1u];
-        const DevMaterial* m = &rs.mats[mi < rs.nmat ? mi : 0u];
-        f3 transparency = splat(1.0f) - m->opacity;
+        mi = mi < rs.nmat ? mi : 0u;
+        const DevMaterial& m = rs.mats[mi];
+        f3 op = m.opacity;
+        if (m.texMask & (1 << kTexOpacity)) op = sampleSlot(rs, mi, kTexOpacity, huv.get());
+        f3 transparency = splat(1.0f) - op;
         P = luminance(transparency);
-        if (random2D(rng).x >= P) return m;
+        if (random2D(rng).x >= P) {
+            opacity = op;
+            return mi;
+        }
         contrib = contrib * (transparency / P);
     }
-    return nullptr;
+    return kNoMaterial;
+}
+
+// The textured part of evaluateDefaultMaterial (DefaultMaterial.slang:171-233)
+// for the hit material: base colour, specular roughness and emission colour
+// into sh / emissionColor, and a normal map replacing the shading normal and
+// frame (HitShaderEntryPoints.slang:251-255, buildBasis).
+__device__ __forceinline__ void materialTextures(const RenderScene& rs, uint32_t mi, const DevMaterial& m, HitUV& huv,
+                                                 Shading& sh, f3& emissionColor)
+{
+    sh.baseColor = m.baseColor;
+    sh.specularRoughness = m.specularRoughness;
+    emissionColor = m.emissionColor;
+    const int mask = m.texMask;
+    if (!mask) return;
+    const f2 uv = huv.get();
+    if (mask & (1 << kTexBaseColor)) sh.baseColor = sampleSlot(rs, mi, kTexBaseColor, uv);
+    if (mask & (1 << kTexSpecularRoughness)) sh.specularRoughness = sampleSlot(rs, mi, kTexSpecularRoughness, uv).x;
+    if (mask & (1 << kTexEmissionColor)) emissionColor = sampleSlot(rs, mi, kTexEmissionColor, uv);
+    if (mask & (1 << kTexNormal)) {
+        sh.normal = normalFromMap(sampleSlot(rs, mi, kTexNormal, uv), sh.normal, sh.tangent);
+        buildBasis(sh.normal, sh.tangent, sh.bitangent);
+    }
 }
 
 struct PathCounters {
@@ -149,11 +220,15 @@ __device__ float4 pathSample(const RenderScene& rs, const aph_frame& fp, uint32_
                 const InstShade& ins = rs.ishade[hit.inst];
                 Shading sh;
                 computeShading(ins, hit.prim, (float)hit.u, (float)hit.v, sh);
+                HitUV huv(ins, hit.prim, (float)hit.u, (float)hit.v);
                 float P;
-                const DevMaterial* mp = selectLayer(rs, ins, rng, contrib, P);
-                if (mp) {
-                    const DevMaterial& m = *mp;
-                    direct = direct + m.emissionColor * m.emission * lerp(splat(1.0f), m.coatColor, m.coat);
+                f3 opacity;
+                const uint32_t mi = selectLayer(rs, ins, huv, rng, contrib, P, opacity);
+                if (mi != kNoMaterial) {
+                    const DevMaterial& m = rs.mats[mi];
+                    f3 emissionColor;
+                    materialTextures(rs, mi, m, huv, sh, emissionColor);
+                    direct = direct + emissionColor * m.emission * lerp(splat(1.0f), m.coatColor, m.coat);
                     f3 dirShadow = splat(0.0f), dirColor = splat(0.0f);
                     bool hasDir = false;
                     if (fp.light_count > 1) {
@@ -208,7 +283,7 @@ __device__ float4 pathSample(const RenderScene& rs, const aph_frame& fp, uint32_
                             }
                         }
                     }
-                    contrib = contrib * (m.opacity / (1.0f - P));
+                    contrib = contrib * (opacity / (1.0f - P));
                     outputAlpha = (outputAlpha == 0.0f && rayType == kTransmissionLobe) ? 0.0f : 1.0f;
                     // ---- setupIndirectLightRay, ShadeFunctions.slang:263-310 ----
                     {

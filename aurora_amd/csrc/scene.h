@@ -47,6 +47,7 @@ struct alignas(16) InstShade {
     float o2w[12];
     const float* pos;
     const float* nrm;
+    const float* uv;  // 2 floats per vertex (given, or u = v = i / vertex count)
     const uint32_t* idx;
     uint32_t material;
     uint32_t flags;
@@ -63,7 +64,7 @@ struct TraceScene {
     uint32_t pad_;
 };
 
-// Runtime material: the Material struct of Material.slang:20-57 (no textures).
+// Runtime material: the Material struct of Material.slang:20-57; textures in DevMatTex.
 struct DevMaterial {
     float base; f3 baseColor;
     float diffuseRoughness, metalness, specular, specularRoughness;
@@ -74,9 +75,30 @@ struct DevMaterial {
     f3 sheenColor; float coat;
     f3 coatColor; float coatRoughness;
     float coatAnisotropy, coatRotation, coatIOR, coatAffectColor;
-    float coatAffectRoughness, emission; int thinWalled; int pad0;
+    float coatAffectRoughness, emission; int thinWalled; int texMask;  // bit k: slot k's has_*_image flag
     f3 emissionColor; float pad1;
     f3 opacity; float pad2;
+};
+
+// Texture slots of a material (in the order of its has_*_image flags).
+constexpr int kTexBaseColor = 0, kTexSpecularRoughness = 1, kTexEmissionColor = 2, kTexOpacity = 3,
+              kTexNormal = 4, kTexSlots = 5;
+constexpr int kAddrWrap = 0, kAddrMirror = 1, kAddrClamp = 2, kAddrMirrorOnce = 3, kAddrBorder = 4;
+
+// An image with its sampler's address modes.
+struct alignas(16) DevTexture {
+    const float4* texels;  // linear RGBA, row-major
+    uint32_t w, h;
+    int32_t addrU, addrV;
+};
+
+// Texture slots of one material: the bound texture (or -1) and the UV
+// transform from the material's texture block (offset.xy, scale.xy, pivot.xy,
+// rotation in degrees, pad).
+struct DevMatTex {
+    int32_t tex[kTexSlots];
+    int32_t pad[3];
+    float xf[kTexSlots][8];
 };
 
 struct alignas(16) AliasEntry {

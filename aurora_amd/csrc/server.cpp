@@ -237,6 +237,7 @@ int aph_server_step(aph_server_t s, uint32_t* active, uint32_t* pending)
     if (!s) return APH_E_INVALID;
     int rc = enter(s->ctx);
     if (rc) return rc;
+    if ((rc = checkTextures(s->ctx))) return rc;
     rc = stepOnce(s);
     if (active) *active = (uint32_t)s->active.size();
     if (pending) *pending = (uint32_t)s->pending.size();

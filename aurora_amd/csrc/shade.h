@@ -165,9 +165,84 @@ __device__ __forceinline__ f3 sampleEnvironment(const DevEnv& e, f2 r, f3& L, fl
     return lerp(e.lightTop, e.lightBottom, uv.y);
 }
 
+// ---------------- material textures, DefaultMaterial.slang:84-127 ----------------
+// applyUVTransform (:97-104; rotateUV :86-93): about the pivot, scale, then
+// rotate by the angle in degrees, then subtract the offset.  xf = offset.xy,
+// scale.xy, pivot.xy, rotation.
+APH_HD f2 applyUVTransform(f2 uv, const float* xf)
+{
+    float sx = (uv.x - xf[4]) / xf[2], sy = (uv.y - xf[5]) / xf[3];
+    float r = xf[6] * (kPi / 180.0f);
+    float sa = sinf(r), ca = cosf(r);
+    float rx = ca * sx + sa * sy, ry = -sa * sx + ca * sy;
+    return f2{(rx - xf[0]) + xf[4], (ry - xf[1]) + xf[5]};
+}
+
+// Address mode of an integer texel coordinate (Vulkan rules); border flags a
+// coordinate outside the image under APH_ADDRESS_BORDER.
+APH_HD int addressTexel(int i, int n, int mode, bool& border)
+{
+    if (mode == kAddrMirror) {
+        int t = i % (2 * n);
+        if (t < 0) t += 2 * n;
+        return t < n ? t : 2 * n - 1 - t;
+    }
+    if (mode == kAddrClamp) return i < 0 ? 0 : (i >= n ? n - 1 : i);
+    if (mode == kAddrMirrorOnce) {
+        int j = i < 0 ? -1 - i : i;
+        return j >= n ? n - 1 : j;
+    }
+    if (mode == kAddrBorder) {
+        if (i < 0 || i >= n) border = true;
+        return i < 0 ? 0 : (i >= n ? n - 1 : i);
+    }
+    int t = i % n;  // wrap
+    return t < 0 ? t + n : t;
+}
+
+// Bilinear sample (one mip level) of a material texture.  Non-finite
+// coordinates read as 0 and texel coordinates are clamped to +-2^30 before the
+// conversion to int, on both sides of the parity check.
+__device__ __forceinline__ float4 sampleTexture(const DevTexture& t, f2 uv)
+{
+    if (!isfinite(uv.x)) uv.x = 0.0f;
+    if (!isfinite(uv.y)) uv.y = 0.0f;
+    float x = uv.x * (float)t.w - 0.5f, y = uv.y * (float)t.h - 0.5f;
+    float x0f = floorf(x), y0f = floorf(y);
+    float fx = x - x0f, fy = y - y0f;
+    x0f = fminf(fmaxf(x0f, -1073741824.0f), 1073741824.0f);
+    y0f = fminf(fmaxf(y0f, -1073741824.0f), 1073741824.0f);
+    int x0 = (int)x0f, y0 = (int)y0f;
+    float4 c[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        bool border = false;
+        int xi = addressTexel(x0 + (k & 1), (int)t.w, t.addrU, border);
+        int yi = addressTexel(y0 + (k >> 1), (int)t.h, t.addrV, border);
+        c[k] = border ? make_float4(1.0f, 1.0f, 1.0f, 1.0f) : t.texels[(size_t)yi * t.w + xi];
+    }
+    auto mix = [](float4 a, float4 b, float w) {
+        return make_float4(a.x + (b.x - a.x) * w, a.y + (b.y - a.y) * w, a.z + (b.z - a.z) * w, a.w + (b.w - a.w) * w);
+    };
+    return mix(mix(c[0], c[1], fx), mix(c[2], c[3], fx), fy);
+}
+
+// calculateNormalFromMap (:107-127), tangent space: the texel remapped to
+// [-1, 1] in the frame (T, normalize(N x T), N).
+APH_HD f3 normalFromMap(f3 texel, f3 N, f3 T)
+{
+    f3 v = texel * 2.0f - splat(1.0f);
+    f3 B = normalize(cross(N, T));
+    return normalize(T * v.x + B * v.y + N * v.z);
+}
+
 // ---------------- shading data, Geometry.slang:131-265 ----------------
 struct Shading {
     f3 normal, position, geomPosition, tangent, bitangent;
+    // the hit material's base colour and specular roughness after its textures
+    // (the BSDF reads these, not the material's constants)
+    f3 baseColor;
+    float specularRoughness;
 };
 
 __device__ __forceinline__ f3 ld3(const float* p) { return f3{p[0], p[1], p[2]}; }
@@ -498,13 +573,13 @@ __device__ __forceinline__ Prologue prologue(const DevMaterial& m, const Shading
 {
     Prologue p;
     float caf = m.coatAffectRoughness * m.coat * m.coatRoughness;
-    float car = lerp(m.specularRoughness, 1.0f, caf);
+    float car = lerp(sh.specularRoughness, 1.0f, caf);
     float coatGamma = clampf(m.coat, 0.0f, 1.0f) * m.coatAffectColor + 1.0f;
     p.subsurfaceColor = pow3(m.subsurfaceColor, coatGamma);
     p.mainRough = roughAniso(car, m.specularAnisotropy);
     p.mainTangent = rotateVector(sh.tangent, m.specularRotation * 360.0f, sh.normal);
     p.coatTangent = rotateVector(sh.tangent, m.coatRotation * 360.0f, sh.normal);
-    p.diffuseColor = pow3(m.baseColor, coatGamma);
+    p.diffuseColor = pow3(sh.baseColor, coatGamma);
     return p;
 }
 
@@ -526,7 +601,7 @@ __device__ f3 evalMaterial(const DevMaterial& m, const Shading& sh, f3 Vw, f3 Lw
                               sheenDiff);
     f3 metalMix = spec;
     if (m.metalness > 0.0f) {
-        f3 metal = conductorBrdf(Lw, Vw, 1.0f, m.baseColor * m.base, m.specularColor * m.specular, p.mainRough, N,
+        f3 metal = conductorBrdf(Lw, Vw, 1.0f, sh.baseColor * m.base, m.specularColor * m.specular, p.mainRough, N,
                                  p.mainTangent);
         metalMix = mxMix(spec, metal, m.metalness);
     }
@@ -574,11 +649,11 @@ __device__ f3 evalMaterialPdf(const DevMaterial& m, const Shading& sh, f3 Vw, f3
         lobe = kCoatLobe;
     } else {
         r.x = (r.x - coatLum) / (1.0f - coatLum);
-        f2 sa = roughAniso(m.specularRoughness, m.specularAnisotropy);
+        f2 sa = roughAniso(sh.specularRoughness, m.specularAnisotropy);
         if (r.x < m.metalness) {
             r.x = r.x / m.metalness;
             pdf = ggxPdf(V, sa, L);
-            res = conductorBrdf(Lw, Vw, 1.0f, m.baseColor * m.base, m.specularColor * m.specular, p.mainRough, N,
+            res = conductorBrdf(Lw, Vw, 1.0f, sh.baseColor * m.base, m.specularColor * m.specular, p.mainRough, N,
                                 p.mainTangent) /
                   fabsf(L.z);
             lobe = kMetalLobe;
@@ -636,11 +711,11 @@ __device__ f3 sampleMaterial(const DevMaterial& m, const Shading& sh, f3 Vw, f2 
         lobe = kCoatLobe;
     } else {
         r.x = (r.x - coatLum) / (1.0f - coatLum);
-        f2 sa = roughAniso(m.specularRoughness, m.specularAnisotropy);
+        f2 sa = roughAniso(sh.specularRoughness, m.specularAnisotropy);
         if (r.x < m.metalness) {
             r.x = r.x / m.metalness;
             L = ggxDirPdf(sh, V, sa, r, Lw, pdf);
-            res = conductorBrdf(Lw, Vw, 1.0f, m.baseColor * m.base, m.specularColor * m.specular, p.mainRough, N,
+            res = conductorBrdf(Lw, Vw, 1.0f, sh.baseColor * m.base, m.specularColor * m.specular, p.mainRough, N,
                                 p.mainTangent) /
                   fabsf(L.z);
             lobe = kMetalLobe;

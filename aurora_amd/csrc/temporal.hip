@@ -91,12 +91,15 @@ __global__ __launch_bounds__(kAovBlock) void kAovs(RenderScene rs, AovParams p, 
     computeShading(ins, h.prim, h.u, h.v, sh);
     uint32_t mi = ins.material < rs.nmat ? ins.material : 0u;
     const DevMaterial& m = rs.mats[mi];
+    HitUV huv(ins, h.prim, h.u, h.v);
+    f3 emissionColor;
+    materialTextures(rs, mi, m, huv, sh, emissionColor);  // base colour, roughness, normal map
     f3 nrm = normalize(sh.normal);
     f2 mv = screenMotion(p.viewProj, p.viewProjPrev, pos.x, pos.y, pos.z, 1.0f, p.W, p.H);
     float prevZ = viewZ(p.invViewPrev, pos);
     out.depth[i] = vz;
-    out.normalRoughness[i] = make_float4(nrm.x, nrm.y, nrm.z, fmaxf(m.specularRoughness, 0.05f));
-    out.baseColorMetalness[i] = make_float4(m.baseColor.x, m.baseColor.y, m.baseColor.z, m.metalness);
+    out.normalRoughness[i] = make_float4(nrm.x, nrm.y, nrm.z, fmaxf(sh.specularRoughness, 0.05f));
+    out.baseColorMetalness[i] = make_float4(sh.baseColor.x, sh.baseColor.y, sh.baseColor.z, m.metalness);
     out.motion[i] = make_float4(mv.x, mv.y, vz - prevZ, 0.0f);
 }
 

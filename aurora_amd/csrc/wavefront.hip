@@ -362,17 +362,23 @@ __global__ __launch_bounds__(kWB) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
                 f3 V = -dir;
                 const InstShade& ins = rq.rs.ishade[inst];
                 Shading shd;
-                computeShading(ins, __float_as_uint(ha.z), ha.w, Q.hitV[i], shd);
+                const uint32_t prim = __float_as_uint(ha.z);
+                const float hv = Q.hitV[i];
+                computeShading(ins, prim, ha.w, hv, shd);
+                HitUV huv(ins, prim, ha.w, hv);
                 float Pt;
-                const DevMaterial* mp = selectLayer(rq.rs, ins, rng, contrib, Pt);
-                if (!mp) {
+                f3 opacity;
+                const uint32_t mi = selectLayer(rq.rs, ins, huv, rng, contrib, Pt, opacity);
+                if (mi == kNoMaterial) {
                     nOrg = shd.geomPosition;
                     nDir = dir;
                     f.event(kEvSkip);
                     contPath = true;
                 } else {
-                    const DevMaterial& m = *mp;
-                    f3 direct = m.emissionColor * m.emission * lerp(splat(1.0f), m.coatColor, m.coat);
+                    const DevMaterial& m = rq.rs.mats[mi];
+                    f3 emissionColor;
+                    materialTextures(rq.rs, mi, m, huv, shd, emissionColor);
+                    f3 direct = emissionColor * m.emission * lerp(splat(1.0f), m.coatColor, m.coat);
                     f3 so = dot(V, shd.normal) < 0.0f ? shd.geomPosition : shd.position;
                     f.primary(false);
                     if (fp.light_count > 1) {
@@ -418,7 +424,7 @@ __global__ __launch_bounds__(kWB) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
                         }
                         if (mask & ((1u << kSlotMisMat) | (1u << kSlotMisLight))) f.misLobe((uint32_t)lobe);
                     }
-                    contrib = contrib * (m.opacity / (1.0f - Pt));
+                    contrib = contrib * (opacity / (1.0f - Pt));
                     f.alpha((f.alpha() == 0.0f && f.rayType() == (uint32_t)kTransmissionLobe) ? 0.0f : 1.0f);
                     f3 nextContrib = splat(1.0f);
                     uint32_t nextType;

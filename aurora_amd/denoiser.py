@@ -110,3 +110,45 @@ class Denoiser:
         nw = 9 * L.cout_padded * L.cin_padded
         w = p[L.weight_offset:L.weight_offset + nw].reshape(9, L.cout_padded, L.cin_padded)
         return w, p[L.bias_offset:L.bias_offset + L.cout_padded]
+
+
+def main():
+    """Times aph_denoise on a 4-spp 1080p frame of config2 (the committed weights):
+    python -m aurora_amd.denoiser [--runs N] [--ensemble]."""
+    import argparse
+    import time
+
+    from . import scenes
+    from .session import Renderer
+
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--runs", type=int, default=20)
+    ap.add_argument("--ensemble", action="store_true")
+    args = ap.parse_args()
+    r = Renderer(0)
+    sc = scenes.config2()
+    w, h = sc.width, sc.height
+    tlas = r.load_scene(sc, capi.APH_BUILDER_SAH)
+    r.set_launch_mode(capi.APH_LAUNCH_EAGER)
+    noisy, out, aov = r.target(w, h), r.target(w, h), r.aov(w, h)
+    r.render([r.request(noisy, tlas, sc.frame, 0, 4)], capi.APH_MODE_WAVEFRONT)
+    vp = scenes.view_proj(sc.frame)
+    r.render_aovs(aov, tlas, sc.frame, vp, vp, np.array(sc.frame.inv_view, dtype=np.float32))
+    dn = Denoiser(r)
+    dn.load()
+    item = [(noisy, aov, out, capi.APH_DN_ENSEMBLE if args.ensemble else 0)]
+    dn.denoise(item)
+    r.wait()
+    t = time.perf_counter()
+    for _ in range(args.runs):
+        dn.denoise(item)
+    r.wait()
+    ms = (time.perf_counter() - t) / args.runs * 1e3
+    print(f"denoise {w}x{h}{' ensemble' if args.ensemble else ''}: {ms:.3f} ms per call over {args.runs} calls",
+          flush=True)
+    dn.close()
+    r.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -291,6 +291,9 @@ class Scene:
         self.materials = []     # list of capi.Material
         self.env = None         # dict
         self.layers = None      # per instance: material indices of its layers, bottom to top
+        self.uvs = {}           # mesh index -> (nv, 2) texture coordinates (others: u = v = i / nv)
+        self.images = []        # dicts: pixels (h, w, 4) uint8 / (h, w, 4|3) float32, linearize
+        self.textures = {}      # (material, slot) -> (image index, address_u, address_v)
         self.frame = None       # capi.Frame
         self.width = 0
         self.height = 0
@@ -447,6 +450,88 @@ def camera_spline(seed, frames, width, height, trace_depth=5, half=200.0, fov_y_
         look[1] = terrain_height(look[0], look[2], seed) + 1.0
         out.append(look_at_frame(tuple(eye), tuple(look), (0, 1, 0), fov_y_deg, width / height, trace_depth))
     return out
+
+
+def sphere_uvs(pos):
+    """Spherical texture coordinates of points on a unit sphere."""
+    p = np.asarray(pos, dtype=np.float64)
+    u = 0.5 + np.arctan2(p[:, 2], p[:, 0]) / (2 * math.pi)
+    v = np.arccos(np.clip(p[:, 1] / np.linalg.norm(p, axis=1), -1, 1)) / math.pi
+    return np.stack([u, v], 1).astype(np.float32)
+
+
+def texture_images(rng):
+    """Five test images: an sRGB RGBA8 checker, an RGBA32F roughness ramp, an RGB32F
+    emission pattern, an RGBA8 opacity stripe image and an RGBA8 bump normal map."""
+    y, x = np.mgrid[0:64, 0:64]
+    a, b = rng.integers(0, 256, 4), rng.integers(0, 256, 4)
+    checker = np.where((((x // 8) + (y // 8)) % 2 == 0)[..., None], a, b).astype(np.uint8)
+    checker[..., 3] = 255
+    y32, x32 = np.mgrid[0:32, 0:32]
+    rough = np.stack([(x32 + 0.5) / 32.0, (y32 + 0.5) / 32.0, np.zeros((32, 32)), np.ones((32, 32))], -1)
+    y16, x16 = np.mgrid[0:16, 0:16]
+    emis = np.stack([4.0 * ((x16 + y16) % 4 == 0), 2.0 * (x16 % 3 == 0), rng.uniform(0, 1, (16, 16))], -1)
+    stripes = np.zeros((32, 32, 4), np.uint8)
+    stripes[..., :3] = np.where(((x32 // 4) % 2 == 0)[..., None], 255, 64)
+    stripes[..., 3] = 255
+    nx = 0.4 * np.sin(x * 0.4) * np.cos(y * 0.3)
+    ny = 0.4 * np.cos(x * 0.25) * np.sin(y * 0.5)
+    n = np.stack([nx, ny, np.ones_like(nx)], -1)
+    n /= np.linalg.norm(n, axis=-1, keepdims=True)
+    nmap = np.concatenate([np.round((n * 0.5 + 0.5) * 255), np.full((64, 64, 1), 255)], -1).astype(np.uint8)
+    return [dict(pixels=checker, linearize=True), dict(pixels=rough.astype(np.float32)),
+            dict(pixels=emis.astype(np.float32)), dict(pixels=stripes, linearize=False),
+            dict(pixels=nmap, linearize=False)]
+
+
+def textured_scene(seed=11, width=96, height=64, spp=4, trace_depth=3):
+    """config1 with material textures in every slot, UV transform, address mode and
+    image format: a textured ground quad (UVs 0..1, base colour checker repeated and
+    rotated, mirrored normal map) and 16 spheres, half with spherical UVs and half
+    with the generated u = v = i / nv, cycling through base colour (clamp),
+    roughness (mirror once / wrap), emission (border), opacity (wrap / clamp) and
+    normal map + base colour (wrap); sphere 13 flags an emission image without
+    binding one (reads 0)."""
+    sc = config1(seed=seed, width=width, height=height, spp=spp, trace_depth=trace_depth)
+    rng = np.random.Generator(np.random.PCG64(seed + 100))
+    sc.name = "textured"
+    gp, gn, gi = sc.meshes[0]
+    sp, sn, sf = sc.meshes[1]
+    sc.meshes = [(gp, gn, gi), (sp, sn, sf), (sp.copy(), sn.copy(), sf.copy())]
+    sc.uvs = {0: np.array([[0, 0], [1, 0], [1, 1], [0, 1]], np.float32), 1: sphere_uvs(sp)}
+    for k in range(9, 17):
+        sc.instances[k]["mesh"] = 2
+    sc.images = texture_images(rng)
+    A = capi
+    g = sc.materials[0]
+    A.set_texture_slot(g, A.APH_TEX_BASE_COLOR, offset=(0.1, 0.2), scale=(0.25, 0.25), pivot=(0.5, 0.5),
+                       rotation=30.0)
+    A.set_texture_slot(g, A.APH_TEX_NORMAL, scale=(0.5, 0.5))
+    sc.textures = {(0, A.APH_TEX_BASE_COLOR): (0, A.APH_ADDRESS_WRAP, A.APH_ADDRESS_WRAP),
+                   (0, A.APH_TEX_NORMAL): (4, A.APH_ADDRESS_MIRROR, A.APH_ADDRESS_MIRROR)}
+    for i in range(1, 17):
+        m = sc.materials[i]
+        kind = (i - 1) % 5
+        if kind == 0:
+            A.set_texture_slot(m, A.APH_TEX_BASE_COLOR, scale=(0.5, 0.5))
+            sc.textures[(i, A.APH_TEX_BASE_COLOR)] = (0, A.APH_ADDRESS_CLAMP, A.APH_ADDRESS_CLAMP)
+        elif kind == 1:
+            A.set_texture_slot(m, A.APH_TEX_SPECULAR_ROUGHNESS, rotation=45.0, pivot=(0.5, 0.5))
+            sc.textures[(i, A.APH_TEX_SPECULAR_ROUGHNESS)] = (1, A.APH_ADDRESS_MIRROR_ONCE, A.APH_ADDRESS_WRAP)
+        elif kind == 2:
+            m.emission = 0.5
+            A.set_texture_slot(m, A.APH_TEX_EMISSION_COLOR, scale=(0.7, 0.7), offset=(-0.1, 0.05))
+            if i != 13:
+                sc.textures[(i, A.APH_TEX_EMISSION_COLOR)] = (2, A.APH_ADDRESS_BORDER, A.APH_ADDRESS_BORDER)
+        elif kind == 3:
+            A.set_texture_slot(m, A.APH_TEX_OPACITY, scale=(2.0, 1.0))
+            sc.textures[(i, A.APH_TEX_OPACITY)] = (3, A.APH_ADDRESS_WRAP, A.APH_ADDRESS_CLAMP)
+        else:
+            A.set_texture_slot(m, A.APH_TEX_NORMAL)
+            A.set_texture_slot(m, A.APH_TEX_BASE_COLOR, rotation=-20.0)
+            sc.textures[(i, A.APH_TEX_NORMAL)] = (4, A.APH_ADDRESS_WRAP, A.APH_ADDRESS_WRAP)
+            sc.textures[(i, A.APH_TEX_BASE_COLOR)] = (0, A.APH_ADDRESS_WRAP, A.APH_ADDRESS_WRAP)
+    return sc
 
 
 def serving_scenes(seed=7, env_w=1024, env_h=512):

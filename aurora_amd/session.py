@@ -149,6 +149,38 @@ class Renderer:
         arr = (capi.Material * len(mats))(*mats)
         self._check(self.lib.aph_materials_upload(self.ctx, C.cast(arr, C.c_void_p), len(mats)))
 
+    def image(self, pixels, linearize=True):
+        """aph_image_create from an (h, w, 4) uint8 (RGBA8), (h, w, 4) float32 (RGBA32F)
+        or (h, w, 3) float32 (RGB32F) array."""
+        a = np.ascontiguousarray(pixels)
+        if a.dtype == np.uint8:
+            fmt = capi.APH_IMAGE_RGBA8
+        else:
+            a = np.ascontiguousarray(a, dtype=np.float32)
+            fmt = capi.APH_IMAGE_RGBA32F if a.shape[2] == 4 else capi.APH_IMAGE_RGB32F
+        d = capi.ImageDesc(a.shape[1], a.shape[0], fmt, int(linearize), a.ctypes.data)
+        h = C.c_uint64()
+        self._check(self.lib.aph_image_create(self.ctx, C.byref(d), C.byref(h)))
+        return h.value
+
+    def destroy_image(self, image):
+        self._check(self.lib.aph_image_destroy(self.ctx, image))
+
+    def bind_textures(self, nmat, slots):
+        """slots: {(material, slot): (image handle, address_u, address_v)} for a table of nmat."""
+        arr = (capi.TextureBinding * (nmat * capi.APH_TEX_SLOTS))()
+        for (mi, k), (img, au, av) in slots.items():
+            arr[mi * capi.APH_TEX_SLOTS + k] = capi.TextureBinding(img, au, av)
+        self._check(self.lib.aph_materials_bind_textures(self.ctx, C.cast(arr, C.c_void_p), nmat))
+
+    def _scene_textures(self, sc, mat_base=0, nmat=None):
+        """Images and texture bindings of a scene (Scene.images / Scene.textures)."""
+        textures = getattr(sc, "textures", None)
+        if not textures:
+            return {}
+        handles = [self.image(im["pixels"], im.get("linearize", True)) for im in sc.images]
+        return {(mi + mat_base, k): (handles[ii], au, av) for (mi, k), (ii, au, av) in textures.items()}
+
     def environment(self, env):
         e, keep = scenes.environment_struct(env)
         self._check(self.lib.aph_environment_upload(self.ctx, C.byref(e)))
@@ -157,11 +189,11 @@ class Renderer:
         """Uploads several scenes into this context, resident together: one material
         table (each scene's instance material indices offset into it) and the first
         scene's environment.  Returns one TLAS handle per scene."""
-        mats, tl = [], []
+        mats, tl, slots = [], [], {}
         for sc in scs:
             handles = []
-            for pos, nrm, idx in sc.meshes:
-                m = self.mesh(pos, idx, nrm)
+            for k, (pos, nrm, idx) in enumerate(sc.meshes):
+                m = self.mesh(pos, idx, nrm, sc.uvs.get(k))
                 self.build_blas(m, builder)
                 handles.append(m)
             inst = sc.instances.copy()
@@ -170,16 +202,19 @@ class Renderer:
             tl.append(self.tlas(inst))
             if getattr(sc, "layers", None) is not None:
                 self.set_layers(tl[-1], [[m + len(mats) for m in (ls or [])] for ls in sc.layers], len(inst))
+            slots.update(self._scene_textures(sc, len(mats)))
             mats.extend(sc.materials)
         self.materials(mats)
+        if slots:
+            self.bind_textures(len(mats), slots)
         self.environment(scs[0].env or {})
         return tl
 
     def load_scene(self, sc, builder=capi.APH_BUILDER_LBVH):
         """Uploads a scenes.Scene; returns the TLAS handle."""
         handles = []
-        for pos, nrm, idx in sc.meshes:
-            m = self.mesh(pos, idx, nrm)
+        for k, (pos, nrm, idx) in enumerate(sc.meshes):
+            m = self.mesh(pos, idx, nrm, sc.uvs.get(k))
             self.build_blas(m, builder)
             handles.append(m)
         inst = sc.instances.copy()
@@ -188,6 +223,9 @@ class Renderer:
         if getattr(sc, "layers", None) is not None:
             self.set_layers(t, sc.layers, len(inst))
         self.materials(sc.materials)
+        slots = self._scene_textures(sc)
+        if slots:
+            self.bind_textures(len(sc.materials), slots)
         self.environment(sc.env or {})
         return t
 

@@ -225,6 +225,46 @@ int aph_inverse3x4(const float m[12], float out[12]);
 int aph_materials_upload(aph_ctx_t ctx, const aph_material* mats, uint32_t n);
 void aph_material_defaults(aph_material* m);
 
+/* ---- material textures: evaluateDefaultMaterial (DefaultMaterial.slang:171-233)
+ * samples up to five images per material, each through its own UV transform
+ * (applyUVTransform, :97-104: scale, then rotation in degrees, then offset,
+ * about the pivot) read from the material's texture block (offsets 224-399),
+ * only where its has_*_image flag is set.  Images are sampled as HGI's samplers
+ * do (HGISampler.cpp:22-60): bilinear, one mip level (HGIImage.cpp:35), address
+ * mode per binding, opaque-white border; an RGBA8 image with linearize set is
+ * sRGB-decoded texel by texel before filtering (HgiFormatUNorm8Vec4srgb,
+ * HGIImage.cpp:98-100).  Texel (x, y) of a w x h image is pixels[y*w + x], at
+ * uv ((x + 0.5)/w, (y + 0.5)/h).  The normal image is a tangent-space normal map
+ * (calculateNormalFromMap, :107-127) whose result replaces the shading normal
+ * and frame of the hit (HitShaderEntryPoints.slang:251-255). ---- */
+enum { APH_IMAGE_RGBA8 = 0, APH_IMAGE_RGBA32F = 1, APH_IMAGE_RGB32F = 2 };
+typedef struct {
+    uint32_t width, height;
+    int32_t format;     /* APH_IMAGE_* */
+    int32_t linearize;  /* RGBA8 only: decode sRGB (IImage::InitData::linearize) */
+    const void* pixels; /* row-major, rows of width pixels */
+} aph_image_desc;
+int aph_image_create(aph_ctx_t ctx, const aph_image_desc* desc, uint64_t* image);
+/* Fails with APH_E_STATE while the image is bound to the material table. */
+int aph_image_destroy(aph_ctx_t ctx, uint64_t image);
+/* Sampler address modes (Names::AddressModes, HGISampler.cpp:45-60). */
+enum { APH_ADDRESS_WRAP = 0, APH_ADDRESS_MIRROR = 1, APH_ADDRESS_CLAMP = 2, APH_ADDRESS_MIRROR_ONCE = 3,
+       APH_ADDRESS_BORDER = 4 };
+/* Texture slots, in the order of the material's has_*_image flags. */
+enum { APH_TEX_BASE_COLOR = 0, APH_TEX_SPECULAR_ROUGHNESS = 1, APH_TEX_EMISSION_COLOR = 2, APH_TEX_OPACITY = 3,
+       APH_TEX_NORMAL = 4, APH_TEX_SLOTS = 5 };
+typedef struct {
+    uint64_t image;              /* 0: unbound */
+    int32_t address_u, address_v; /* APH_ADDRESS_* (default wrap) */
+} aph_texture_binding;
+/* Binds images to the material table: bindings[APH_TEX_SLOTS * i + slot] for
+ * material i, n = the table's material count (HGIScene.cpp:226-380 gathers the
+ * same per-instance texture and sampler indices).  A flagged emission slot
+ * without an image reads as 0 (InstanceData.glsl:147-153); any other flagged
+ * slot without an image makes renders fail with APH_E_STATE.  Uploading
+ * materials clears the bindings. */
+int aph_materials_bind_textures(aph_ctx_t ctx, const aph_texture_binding* bindings, uint32_t n);
+
 /* ---- environment: EnvironmentBase::updateGPUStruct (EnvironmentBase.cpp:52-71)
  *      + AliasMap::build (AliasMap.cpp:31-166) ---- */
 int aph_environment_upload(aph_ctx_t ctx, const aph_environment* env);

@@ -121,6 +121,23 @@ typedef struct {
 } ora_material_constants;
 void ora_material_defaults(ora_material_constants* m); /* MaterialBase.cpp:109-167 */
 
+/* ---- material textures (DefaultMaterial.slang:84-233; HGISampler.cpp:22-60) ---- */
+#define ORA_TEX_SLOTS 5 /* base colour, specular roughness, emission colour, opacity, normal */
+typedef struct {
+    const float* rgba;       /* linear RGBA float, w*h*4, row-major */
+    uint32_t w, h;
+    int32_t addr_u, addr_v;  /* 0 wrap, 1 mirror, 2 clamp, 3 mirror once, 4 border (opaque white) */
+} ora_texture;
+float ora_srgb_to_linear(float c);
+void ora_uv_transform(const float uv[2], const float xf[7], float out[2]);
+void ora_texture_sample(const ora_texture* t, const float uv[2], float out[4]);
+void ora_normal_from_map(const float texel[3], const float N[3], const float T[3], float out[3]);
+/* Interpolated texture coordinate of a hit (tests). */
+void ora_hit_uv(const ora_scene* s, uint32_t inst, uint32_t prim, float bu, float bv, float out[2]);
+/* slot_tex[5*m + slot]: texture index of material m's slot or -1; textures are copied. */
+void ora_scene_set_textures(ora_scene* s, const ora_texture* tex, uint32_t ntex, const int32_t* slot_tex,
+                            uint32_t nmat);
+
 /* ---- environment (EnvironmentBase.cpp:52-71, Environment.slang) ---- */
 typedef struct { uint32_t alias; float prob; float pdf; float pad; } ora_alias_entry;
 void ora_alias_build(const float* rgb, uint32_t w, uint32_t h, ora_alias_entry* out, float* integral);

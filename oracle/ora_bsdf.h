@@ -19,6 +19,7 @@ typedef struct {
 /* ShadingData fields used by the path, Shaders/Geometry.slang:44-104. */
 typedef struct {
     v3 normal, position, geomPosition, tangent, bitangent;
+    float texCoord[2];
 } ora_shading;
 
 void ora_material_from_constants(const ora_material_constants* c, ora_material* m);

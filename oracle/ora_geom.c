@@ -175,6 +175,16 @@ ora_scene* ora_scene_create(const ora_mesh* meshes, uint32_t nmesh, const ora_in
     s->blas = (ora_bvh*)calloc(nmesh ? nmesh : 1, sizeof(ora_bvh));
     memcpy(s->meshes, meshes, sizeof(ora_mesh) * nmesh);
     memcpy(s->inst, inst, sizeof(ora_instance) * ninst);
+    /* texture coordinates: without them u = v = i / vertex count (GeometryBase.cpp:114-124) */
+    s->uv_gen = (float**)calloc(nmesh ? nmesh : 1, sizeof(float*));
+    for (uint32_t m = 0; m < nmesh; m++) {
+        if (s->meshes[m].uv) continue;
+        uint32_t nv = s->meshes[m].nv;
+        float* g = (float*)malloc(sizeof(float) * 2 * (nv ? nv : 1));
+        for (uint32_t i = 0; i < nv; i++) g[2 * i] = g[2 * i + 1] = (float)i / (float)nv;
+        s->uv_gen[m] = g;
+        s->meshes[m].uv = g;
+    }
     for (uint32_t m = 0; m < nmesh; m++) {
         const ora_mesh* me = &meshes[m];
         prim_ref* refs = (prim_ref*)malloc(sizeof(prim_ref) * (me->nt ? me->nt : 1));
@@ -254,6 +264,11 @@ void ora_scene_destroy(ora_scene* s)
     free(s->tlas.prims);
     free(s->layer_off);
     free(s->layer_mat);
+    for (uint32_t m = 0; m < s->nmesh; m++) free(s->uv_gen[m]);
+    free(s->uv_gen);
+    for (uint32_t i = 0; i < s->ntex; i++) free((void*)s->tex[i].rgba);
+    free(s->tex);
+    free(s->slot_tex);
     free(s->blas);
     free(s->meshes);
     free(s->inst);

@@ -34,7 +34,17 @@ struct ora_scene {
     double extent;   /* 3 x max |coordinate| of the instance world boxes */
     uint32_t* layer_off; /* material layers: instance i has layer_mat[layer_off[i] .. layer_off[i+1]) */
     uint32_t* layer_mat;
+    float** uv_gen;      /* per mesh: generated texture coordinates (u = v = i / nv) or NULL */
+    ora_texture* tex;    /* material textures (ora_scene_set_textures) */
+    uint32_t ntex;
+    int32_t* slot_tex;   /* ORA_TEX_SLOTS per material: texture index or -1 */
+    uint32_t nmat_tex;
 };
+
+/* Material texture slot at a texture coordinate (0 and return 0 when unbound). */
+int ora_tex_flag(const ora_material_constants* mc, int slot);
+int ora_tex_slot(const ora_scene* s, const ora_material_constants* mc, uint32_t mi, int slot, const float uv[2],
+                 float out[3]);
 
 void ora_wray_setup(ora_wray* r, const double o[3], const double d[3], double tmin, double tmax);
 int ora_wray_triangle(const ora_wray* r, const float* p0, const float* p1, const float* p2,

@@ -30,9 +30,13 @@ static void shading_data(const ora_scene* s, uint32_t ii, uint32_t prim, float b
     const uint32_t* tri = me->idx + 3 * (size_t)prim;
     v3 p[3], n[3];
     v3 geom = v3s(0.0f), nrm = v3s(0.0f);
+    sh->texCoord[0] = sh->texCoord[1] = 0.0f;
     for (int i = 0; i < 3; i++) {
         p[i] = v3load(me->pos + 3 * (size_t)tri[i]);
         geom = add(geom, scale(p[i], b[i]));
+        /* texCoord, Geometry.slang:199-203 */
+        sh->texCoord[0] = sh->texCoord[0] + me->uv[2 * (size_t)tri[i]] * b[i];
+        sh->texCoord[1] = sh->texCoord[1] + me->uv[2 * (size_t)tri[i] + 1] * b[i];
         if (me->nrm) {
             n[i] = v3load(me->nrm + 3 * (size_t)tri[i]);
             nrm = add(nrm, scale(n[i], b[i]));
@@ -46,6 +50,42 @@ static void shading_data(const ora_scene* s, uint32_t ii, uint32_t prim, float b
     v3 up = fabsf(sh->normal.y) < 0.999f ? V3(0.0f, 1.0f, 0.0f) : V3(1.0f, 0.0f, 0.0f);
     sh->tangent = normalize(cross(sh->normal, up));
     sh->bitangent = cross(sh->normal, sh->tangent);
+}
+
+/* buildBasis, Globals.slang:205-210 */
+static void build_basis(v3 n, v3* t, v3* b);
+
+/* The textured part of evaluateDefaultMaterial (DefaultMaterial.slang:171-233) for
+ * material mi: base colour (and the metal colour copied from it), specular
+ * roughness and emission colour into m; with a normal map, the tangent-space
+ * normal (calculateNormalFromMap, :107-127) replaces the shading normal and frame
+ * (HitShaderEntryPoints.slang:251-255).  Opacity is applied by the layer loop. */
+static void material_textures(const ora_scene* s, const ora_material_constants* mc, uint32_t mi, ora_material* m,
+                              ora_shading* sh)
+{
+    float t[3];
+    if (ora_tex_flag(mc, 0)) {
+        ora_tex_slot(s, mc, mi, 0, sh->texCoord, t);
+        m->baseColor = V3(t[0], t[1], t[2]);
+        m->metalColor = m->baseColor;
+    }
+    if (ora_tex_flag(mc, 1)) {
+        ora_tex_slot(s, mc, mi, 1, sh->texCoord, t);
+        m->specularRoughness = t[0];
+    }
+    if (ora_tex_flag(mc, 2)) {
+        ora_tex_slot(s, mc, mi, 2, sh->texCoord, t); /* unbound: 0 (InstanceData.glsl:147-153) */
+        m->emissionColor = V3(t[0], t[1], t[2]);
+    }
+    if (ora_tex_flag(mc, 4)) {
+        ora_tex_slot(s, mc, mi, 4, sh->texCoord, t);
+        float N[3], T[3], n[3];
+        v3store(N, sh->normal);
+        v3store(T, sh->tangent);
+        ora_normal_from_map(t, N, T, n);
+        sh->normal = v3load(n);
+        build_basis(sh->normal, &sh->tangent, &sh->bitangent);
+    }
 }
 
 /* adjustRadiance, PathTracingCommon.slang:59-82 */
@@ -208,11 +248,18 @@ static void closest_hit(const pt_ctx* c, payload* pl, v3 rayDir, const ora_hit* 
     ora_material m;
     float P = 0.0f;
     int hitLayer = 0;
+    uint32_t hitMat = 0;
     float r[2];
     for (int li = (int)lcount; li >= 0; li--) {
         uint32_t mi = li == 0 ? s->inst[hit->inst].material : s->layer_mat[lfirst + (uint32_t)li - 1];
         if (mi >= c->nmat) mi = 0;
         ora_material_from_constants(&c->mats[mi], &m);
+        if (ora_tex_flag(&c->mats[mi], 3)) { /* opacity texture */
+            float t[3];
+            ora_tex_slot(s, &c->mats[mi], mi, 3, sh.texCoord, t);
+            m.opacity = V3(t[0], t[1], t[2]);
+        }
+        hitMat = mi;
         v3 transparency = sub(v3s(1.0f), m.opacity);
         P = luminance(transparency);
         ora_random2d(pl->rng, r);
@@ -223,6 +270,7 @@ static void closest_hit(const pt_ctx* c, payload* pl, v3 rayDir, const ora_hit* 
     v3 dirShadowDir = v3s(0.0f), dirLightColor = v3s(0.0f);
     int hasDirLight = 0;
     if (hitLayer) {
+        material_textures(s, &c->mats[hitMat], hitMat, &m, &sh);
         /* shadeEmission, ShadeFunctions.slang:31-48 */
         v3 ecb = lerp3(v3s(1.0f), m.coatColor, m.coat);
         pl->direct = add(pl->direct, mul(scale(m.emissionColor, m.emission), ecb));
@@ -502,6 +550,15 @@ void ora_render(const ora_scene* s, const ora_material_constants* mats, uint32_t
     if (counters) *counters = tot;
 }
 
+/* Texture coordinate of a hit (Geometry.slang:199-203), for tests. */
+void ora_hit_uv(const ora_scene* s, uint32_t inst, uint32_t prim, float bu, float bv, float out[2])
+{
+    ora_shading sh;
+    shading_data(s, inst, prim, bu, bv, &sh);
+    out[0] = sh.texCoord[0];
+    out[1] = sh.texCoord[1];
+}
+
 /* The AOV pass: guides of the first segment of one camera ray, as the primary
  * closest-hit / miss shaders write them (HitShaderEntryPoints.slang:184-214,
  * MainEntryPoints.slang:391-405; motion vectors GlobalPipelineState.slang:126-172). */
@@ -536,6 +593,7 @@ void ora_aov_pixel(const ora_scene* s, const ora_material_constants* mats, uint3
     if (mi >= nmat) mi = 0;
     ora_material m;
     ora_material_from_constants(&mats[mi], &m);
+    material_textures(s, &mats[mi], mi, &m, &sh); /* base colour, roughness, normal map */
     v3 n = normalize(sh.normal);
     float p4[4] = { pos[0], pos[1], pos[2], 1.0f }, mv[2];
     ora_screen_motion(view_proj, view_proj_prev, p4, W, H, mv);

@@ -21,6 +21,29 @@ class Mesh(C.Structure):
                 ("nv", C.c_uint32), ("nt", C.c_uint32)]
 
 
+class Texture(C.Structure):
+    _fields_ = [("rgba", C.c_void_p), ("w", C.c_uint32), ("h", C.c_uint32), ("addr_u", C.c_int32),
+                ("addr_v", C.c_int32)]
+
+
+def texture_texels(pixels, linearize=True):
+    """Linear RGBA float texels of an image as the oracle reads it: RGBA8 as unorm, with
+    linearize the sRGB transfer function undone per colour channel (ora_srgb_to_linear)."""
+    a = np.asarray(pixels)
+    if a.dtype == np.uint8:
+        lib = load()
+        unorm = np.arange(256, dtype=np.float32) / np.float32(255.0)
+        lin = np.array([lib.ora_srgb_to_linear(float(v)) for v in unorm], dtype=np.float32)
+        out = unorm[a].astype(np.float32)
+        if linearize:
+            out[..., :3] = lin[a[..., :3]]
+        return np.ascontiguousarray(out)
+    a = a.astype(np.float32)
+    if a.shape[2] == 3:
+        a = np.concatenate([a, np.ones(a.shape[:2] + (1,), np.float32)], axis=2)
+    return np.ascontiguousarray(a)
+
+
 class Env(C.Structure):
     _fields_ = [("light_top", C.c_float * 3), ("light_bottom", C.c_float * 3), ("bg_top", C.c_float * 3),
                 ("bg_bottom", C.c_float * 3), ("light_xform", C.c_float * 16), ("light_xform_inv", C.c_float * 16),
@@ -82,6 +105,12 @@ def load():
         "ora_screen_motion": (None, [P, P, P, u32, u32, P]),
         "ora_halton_jitter": (None, [u32, u32, P]),
         "ora_aov_pixel": (None, [P, P, u32, P, u32, u32, u32, u32, u32, P, P, P, C.c_float, P]),
+        "ora_srgb_to_linear": (C.c_float, [C.c_float]),
+        "ora_uv_transform": (None, [P, P, P]),
+        "ora_texture_sample": (None, [P, P, P]),
+        "ora_scene_set_textures": (None, [P, P, u32, P, u32]),
+        "ora_normal_from_map": (None, [P, P, P, P]),
+        "ora_hit_uv": (None, [P, u32, u32, C.c_float, C.c_float, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -106,14 +135,17 @@ class OracleScene:
         lib = load()
         self._keep = []
         meshes = (Mesh * len(sc.meshes))()
+        uvs = getattr(sc, "uvs", {}) or {}
         for k, (pos, nrm, idx) in enumerate(sc.meshes):
             pos = np.ascontiguousarray(pos, dtype=np.float32)
             idx = np.ascontiguousarray(idx, dtype=np.uint32)
             nrm = None if nrm is None else np.ascontiguousarray(nrm, dtype=np.float32)
-            self._keep += [pos, idx, nrm]
+            uv = uvs.get(k)
+            uv = None if uv is None else np.ascontiguousarray(uv, dtype=np.float32)
+            self._keep += [pos, idx, nrm, uv]
             meshes[k].pos = _p(pos)
             meshes[k].nrm = _p(nrm)
-            meshes[k].uv = None
+            meshes[k].uv = _p(uv)
             meshes[k].idx = _p(idx)
             meshes[k].nv = len(pos.reshape(-1, 3))
             meshes[k].nt = len(idx.reshape(-1)) // 3
@@ -125,6 +157,20 @@ class OracleScene:
         if layers is not None:
             off, mats = layer_csr(layers, len(inst))
             lib.ora_scene_set_layers(self.ptr, _p(off), _p(mats))
+        textures = getattr(sc, "textures", None)
+        if textures:
+            # one oracle texture per (material, slot) binding, as the library's table
+            nmat = len(sc.materials)
+            slot_tex = np.full(nmat * 5, -1, dtype=np.int32)
+            tex = (Texture * len(textures))()
+            texels = []
+            for n, ((mi, k), (ii, au, av)) in enumerate(sorted(textures.items())):
+                im = sc.images[ii]
+                t = texture_texels(im["pixels"], im.get("linearize", True))
+                texels.append(t)
+                tex[n] = Texture(_p(t), t.shape[1], t.shape[0], au, av)
+                slot_tex[mi * 5 + k] = n
+            lib.ora_scene_set_textures(self.ptr, C.cast(tex, C.c_void_p), len(textures), _p(slot_tex), nmat)
         self._env = None
 
     def __del__(self):

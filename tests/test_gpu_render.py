@@ -247,3 +247,93 @@ def test_autotune_keeps_results(renderer, teapot_path):
     renderer.wait()
     np.testing.assert_array_equal(renderer.readback(after, 64, 48), before)
     renderer.set_tuning(4 << 20, 16, 8)
+
+
+def test_textured_scene_matches_oracle(renderer):
+    """Every texture slot, UV transform, address mode and image format (scenes.textured_scene):
+    the GPU's sampling (device sinf/cosf in the UV rotation) against the oracle's, same bar
+    as the untextured scenes."""
+    sc = scenes.textured_scene(width=96, height=64)
+    for mode in (capi.APH_MODE_MEGAKERNEL, capi.APH_MODE_WAVEFRONT):
+        gpu, cpu = render_both(renderer, sc, 4, mode=mode)
+        compare(gpu, cpu)
+
+
+@pytest.mark.parametrize("slot, value", [
+    (capi.APH_TEX_BASE_COLOR, (0.7, 0.2, 0.4, 1.0)),
+    (capi.APH_TEX_SPECULAR_ROUGHNESS, (0.35, 0.0, 0.0, 1.0)),
+    (capi.APH_TEX_EMISSION_COLOR, (0.9, 0.6, 0.1, 1.0)),
+    (capi.APH_TEX_OPACITY, (0.6, 0.6, 0.6, 1.0)),
+])
+def test_constant_texture_equals_constant_material(renderer, slot, value):
+    """A constant image in a slot renders bit-identically to the material constant it replaces."""
+    from test_oracle_textures import constant_scene
+
+    tex, plain = constant_scene(slot, value)
+    for mode in (capi.APH_MODE_MEGAKERNEL, capi.APH_MODE_WAVEFRONT):
+        images = []
+        for sc in (tex, plain):
+            tlas = renderer.load_scene(sc)
+            tgt = renderer.target(sc.width, sc.height)
+            renderer.render([renderer.request(tgt, tlas, sc.frame, 0, 2)], mode)
+            renderer.wait()
+            images.append(renderer.readback(tgt, sc.width, sc.height))
+        np.testing.assert_array_equal(images[0], images[1])
+
+
+def test_textured_guides_match_oracle(renderer):
+    """Guide images of a textured scene: base colour and roughness from the textures, the
+    normal from the normal map.  The UV rotation uses the device's and the host's
+    sinf/cosf; where the bilinear filter crosses a sharp texture edge (the checker) a
+    last-bit difference of the coordinate becomes up to (texels per unit) x (edge
+    contrast) x (coordinate ulp).  Bar: >= 99.9 % of values within 1e-4 relative +
+    1e-5 absolute, all within 1e-3 absolute."""
+    w, h = 64, 40
+    sc = scenes.textured_scene(width=w, height=h)
+    tlas = renderer.load_scene(sc)
+    fr = sc.frame
+    vp = scenes.view_proj(fr)
+    ivp = np.array(fr.inv_view, dtype=np.float32)
+    aov = renderer.aov(w, h)
+    renderer.render_aovs(aov, tlas, fr, vp, vp, ivp, sample_index=2)
+    gd, gn, gb, _ = renderer.aov_readback(aov, w, h)
+    od, on, ob, _ = pyoracle.OracleScene(sc).aovs(fr, w, h, 2, vp, vp, ivp, -2000.0)
+    np.testing.assert_allclose(gd, od, rtol=1e-5)
+    for g, o in ((gb, ob), (gn, on)):
+        assert np.isclose(g, o, rtol=1e-4, atol=1e-5).mean() >= 0.999
+        assert np.abs(g - o).max() <= 1e-3
+    assert np.unique(np.round(gb[..., :3], 3).reshape(-1, 3), axis=0).shape[0] > 20  # textured, not constant
+
+
+def test_texture_errors(renderer):
+    from aurora_amd.session import AphError
+
+    r = renderer
+    sc = scenes.textured_scene(width=16, height=16)
+    tlas = r.load_scene(sc)
+    tgt = r.target(16, 16)
+    img = r.image(np.zeros((4, 4, 4), np.uint8))
+    n = len(sc.materials)
+    with pytest.raises(AphError) as e:  # a flagged slot loses its image
+        r.bind_textures(n, {(0, capi.APH_TEX_BASE_COLOR): (img, 0, 0)})
+        r.render([r.request(tgt, tlas, sc.frame, 0, 1)])
+    assert e.value.code == capi.APH_E_STATE
+    with pytest.raises(AphError) as e:
+        r.bind_textures(n, {(0, capi.APH_TEX_BASE_COLOR): (img, 7, 0)})
+    assert e.value.code == capi.APH_E_INVALID
+    with pytest.raises(AphError) as e:
+        r.bind_textures(n - 1, {})
+    assert e.value.code == capi.APH_E_INVALID
+    r.load_scene(sc)  # the full bindings again
+    bound = img
+    r.bind_textures(n, {**{k: v for k, v in
+                           r._scene_textures(sc).items()}, (1, capi.APH_TEX_NORMAL): (bound, 0, 0)})
+    with pytest.raises(AphError) as e:
+        r.destroy_image(bound)
+    assert e.value.code == capi.APH_E_STATE
+    r.materials(sc.materials)  # uploading clears the bindings
+    r.destroy_image(bound)
+    d = capi.ImageDesc(4, 4, 9, 0, np.zeros(64, np.uint8).ctypes.data)
+    import ctypes as C
+    h = C.c_uint64()
+    assert r.lib.aph_image_create(r.ctx, C.byref(d), C.byref(h)) == capi.APH_E_INVALID
