@@ -244,6 +244,7 @@ typedef struct {
     int32_t linearize;  /* RGBA8 only: decode sRGB (IImage::InitData::linearize) */
     const void* pixels; /* row-major, rows of width pixels */
 } aph_image_desc;
+/* IRenderer::createImagePointer (Aurora.h:1116; HGIRenderer.cpp:434, HGIImage.cpp:23-112). */
 int aph_image_create(aph_ctx_t ctx, const aph_image_desc* desc, uint64_t* image);
 /* Fails with APH_E_STATE while the image is bound to the material table. */
 int aph_image_destroy(aph_ctx_t ctx, uint64_t image);
