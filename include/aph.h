@@ -331,8 +331,10 @@ typedef struct {
 int aph_get_tuning(aph_ctx_t ctx, aph_tuning* t);
 int aph_set_tuning(aph_ctx_t ctx, const aph_tuning* t);
 /* Renders the given requests (into their targets) under each candidate launch
- * configuration - chunk of 2M/4M/8M paths x 8/16/32 traversal blocks per CU x
- * 4/8/16 shade blocks per CU - and keeps the fastest for this context. */
+ * configuration - chunk of 4M/8M/16M/32M paths x 4/8/16 traversal blocks per
+ * CU x 4/8/16 shade blocks per CU - and keeps the fastest for this context.
+ * A chunk holds about 0.55 KB of device memory per path (17 GB at 32M paths),
+ * allocated at the tuned size even for smaller frames. */
 int aph_wavefront_autotune(aph_ctx_t ctx, const aph_request* reqs, uint32_t nreq, aph_tuning* best);
 
 /* ---- guide images (AOVs) and temporal resolve: the DirectX backend's denoising

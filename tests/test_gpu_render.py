@@ -238,7 +238,9 @@ def test_autotune_keeps_results(renderer, teapot_path):
     before = renderer.readback(ref, 64, 48)
     tgt = renderer.target(64, 48)
     best = renderer.autotune([renderer.request(tgt, tlas, sc.frame, 0, 4)])
-    assert best.ms > 0 and best.chunk_paths in (2 << 20, 4 << 20, 8 << 20)
+    # the candidate set of aph_wavefront_autotune
+    assert best.ms > 0 and best.chunk_paths in (4 << 20, 8 << 20, 16 << 20, 32 << 20)
+    assert best.trace_blocks_per_cu in (4, 8, 16) and best.shade_blocks_per_cu in (4, 8, 16)
     t = renderer.tuning()
     assert (t.chunk_paths, t.trace_blocks_per_cu, t.shade_blocks_per_cu) == \
         (best.chunk_paths, best.trace_blocks_per_cu, best.shade_blocks_per_cu)
