@@ -565,21 +565,29 @@ APH_HD f3 toTangent(f3 i, const Shading& s) { return f3{dot(i, s.tangent), dot(i
 APH_HD f3 toWorld(f3 i, const Shading& s) { return s.tangent * i.x + s.bitangent * i.y + s.normal * i.z; }
 
 // Quantities shared by evaluate / sample (common prologue of :598-629, :862-905, :1093-1143).
+// A gamma of 1 and a rotation of 0 are identities (pow(x, 1) = x; a rotation by
+// sin 0 = 0, cos 0 = 1 returns v up to the sign of zero components), so the
+// common material skips the powf and sincos work the general case needs.
 struct Prologue {
     f2 mainRough;
-    f3 mainTangent, coatTangent, diffuseColor, subsurfaceColor;
+    f3 mainTangent, coatTangent, diffuseColor;
+    float coatGamma;
 };
+__device__ __forceinline__ f3 coatGammaPow(f3 c, float gamma) { return gamma == 1.0f ? c : pow3(c, gamma); }
+__device__ __forceinline__ f3 tangentRotation(const Shading& sh, float turns)
+{
+    return turns == 0.0f ? sh.tangent : rotateVector(sh.tangent, turns * 360.0f, sh.normal);
+}
 __device__ __forceinline__ Prologue prologue(const DevMaterial& m, const Shading& sh)
 {
     Prologue p;
     float caf = m.coatAffectRoughness * m.coat * m.coatRoughness;
     float car = lerp(sh.specularRoughness, 1.0f, caf);
-    float coatGamma = clampf(m.coat, 0.0f, 1.0f) * m.coatAffectColor + 1.0f;
-    p.subsurfaceColor = pow3(m.subsurfaceColor, coatGamma);
+    p.coatGamma = clampf(m.coat, 0.0f, 1.0f) * m.coatAffectColor + 1.0f;
     p.mainRough = roughAniso(car, m.specularAnisotropy);
-    p.mainTangent = rotateVector(sh.tangent, m.specularRotation * 360.0f, sh.normal);
-    p.coatTangent = rotateVector(sh.tangent, m.coatRotation * 360.0f, sh.normal);
-    p.diffuseColor = pow3(sh.baseColor, coatGamma);
+    p.mainTangent = tangentRotation(sh, m.specularRotation);
+    p.coatTangent = tangentRotation(sh, m.coatRotation);
+    p.diffuseColor = coatGammaPow(sh.baseColor, p.coatGamma);
     return p;
 }
 
@@ -590,7 +598,9 @@ __device__ f3 evalMaterial(const DevMaterial& m, const Shading& sh, f3 Vw, f3 Lw
     f3 N = sh.normal;
     f3 diffuse = diffuseBrdf(Lw, Vw, m.base, p.diffuseColor, m.diffuseRoughness, N);
     f3 ssMix = diffuse;
-    if (m.subsurface > 0.0f) ssMix = mxMix(diffuse, subsurfaceBrdf(Lw, 1.0f, p.subsurfaceColor, N), m.subsurface);
+    if (m.subsurface > 0.0f)
+        ssMix = mxMix(diffuse, subsurfaceBrdf(Lw, 1.0f, coatGammaPow(m.subsurfaceColor, p.coatGamma), N),
+                      m.subsurface);
     if (dot(N, Vw) * dot(N, Lw) < 0.0f) return ssMix;
     f3 sheenDiff = ssMix;
     if (m.sheen > 0.0f) sheenDiff = sheenBrdf(Lw, Vw, m.sheen, m.sheenColor, m.sheenRoughness, N, ssMix);

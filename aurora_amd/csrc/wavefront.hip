@@ -301,7 +301,7 @@ __global__ __launch_bounds__(kWB) void kExtend(const DevReq* reqs, const SegInfo
     addCounter(&ctr->overflow, ts.overflow);
 }
 
-__device__ __forceinline__ void emitShadow(Queues& Q, bool emit, f3 org, f3 dir, uint32_t slotId)
+__device__ __forceinline__ void emitShadow(const Queues& Q, bool emit, f3 org, f3 dir, uint32_t slotId)
 {
     uint32_t at = waveAppend(emit, &Q.counts[2]);
     if (emit) {
@@ -311,175 +311,233 @@ __device__ __forceinline__ void emitShadow(Queues& Q, bool emit, f3 org, f3 dir,
 }
 
 // ---- shade: closest-hit / miss programs without the shadow traces ----
-// Four waves per SIMD (128 VGPRs, a few spills) hide the gathers of shading
-// data, materials and environment texels better than three without spills.
-__global__ __launch_bounds__(kWB) __attribute__((amdgpu_waves_per_eu(4, 8))) void kShade(const DevReq* reqs, const SegInfo* segs, Paths P, Queues Q, int cur)
+
+// Miss program of queue entry i (the path ends; ShadeFunctions.slang via
+// MainEntryPoints.slang:385-405): background for primary, transmission and
+// ground-plane reflection rays.  Contribution and RNG state are unchanged.
+__device__ __forceinline__ void shadeMiss(const DevReq* reqs, const SegInfo* segs, const Paths& P, const Queues& Q,
+                                          int cur, uint32_t i)
+{
+    const uint32_t p = __float_as_uint(Q.rayB[cur][i].w);
+    Flags f{P.flags[p]};
+    const bool bg = f.primary() || f.rayType() == (uint32_t)kTransmissionLobe ||
+                    f.rayType() == (uint32_t)kGroundPlaneReflectionRay;
+    f3 d = splat(0.0f);
+    if (bg) {
+        const DevReq& rq = reqs[segs[P.seg[p]].req];
+        const uint32_t pix = P.pix[p];
+        d = evalEnvironment(rq.rs.env, xyz(Q.rayB[cur][i]), true, pix % rq.W, pix / rq.W, rq.W, rq.H) *
+            xyz(P.contrib[p]);
+    }
+    P.pendDirect[p] = f4(d, 0.0f);
+    f.event(kEvMiss);
+    f.ended(true);
+    f.terminal(false);
+    P.shMask[p] = 0;
+    P.flags[p] = f.v;
+}
+
+// Closest-hit program of queue entry i (live lanes), then the wave-uniform
+// append of the continuing rays (every lane of the wave).
+__device__ __forceinline__ void shadeHit(const DevReq* reqs, const SegInfo* segs, const Paths& P, const Queues& Q,
+                                         int cur, bool live, uint32_t i)
 {
     const int nxt = cur ^ 1;
-    uint32_t n = Q.counts[cur];
-    // Every lane of a wave runs the same number of iterations so that the
-    // ballot-based appends see whole waves.
-    uint32_t stride = gridDim.x * kWB;
-    uint32_t iters = (n + stride - 1) / stride;
-    for (uint32_t it = 0; it < iters; it++) {
-        uint32_t i = it * stride + blockIdx.x * kWB + threadIdx.x;
-        bool live = i < n;
-        bool contPath = false;
-        f3 nOrg = splat(0.0f), nDir = splat(0.0f);
-        float nTmin = kRayTMin;
-        uint32_t p = 0;
-        if (live) {
-            float4 ra = Q.rayA[cur][i], rb = Q.rayB[cur][i];
-            p = __float_as_uint(rb.w);
-            const DevReq& rq = reqs[segs[P.seg[p]].req];
-            const aph_frame& fp = rq.fp;
-            const DevEnv& env = rq.rs.env;
-            uint32_t pix = P.pix[p], px = pix % rq.W, py = pix / rq.W;
-            Flags f{P.flags[p]};
-            f3 dir = xyz(rb);
-            f3 contrib = xyz(P.contrib[p]);
-            uint2 rs = P.rng[p];
-            Rng rng{rs.x, rs.y};
-            float4 ha = Q.hitA[i];
-            uint32_t inst = __float_as_uint(ha.y);
-            uint32_t depth = f.depth();
-            uint8_t mask = 0;
-            // A shadow request is queued where it is made (the appends work on
-            // whatever lanes are active), so no slot state stays live.
-            auto shadow = [&](int slot, f3 org, f3 sdir, f3 color) {
-                P.shColor[(size_t)p * kSlots + slot] = f4(color, 0.0f);
-                mask |= (uint8_t)(1u << slot);
-                emitShadow(Q, true, org, sdir, p * kSlots + (uint32_t)slot);
-            };
-            if (inst == 0xFFFFFFFFu) {
-                bool bg = f.primary() || f.rayType() == (uint32_t)kTransmissionLobe ||
-                          f.rayType() == (uint32_t)kGroundPlaneReflectionRay;
-                f3 d = bg ? evalEnvironment(env, dir, true, px, py, rq.W, rq.H) * contrib : splat(0.0f);
-                P.pendDirect[p] = f4(d, 0.0f);
-                f.event(kEvMiss);
-                f.ended(true);
+    bool contPath = false;
+    f3 nOrg = splat(0.0f), nDir = splat(0.0f);
+    float nTmin = kRayTMin;
+    uint32_t p = 0;
+    if (live) {
+        float4 rb = Q.rayB[cur][i];
+        p = __float_as_uint(rb.w);
+        const DevReq& rq = reqs[segs[P.seg[p]].req];
+        const aph_frame& fp = rq.fp;
+        const DevEnv& env = rq.rs.env;
+        uint32_t pix = P.pix[p], px = pix % rq.W, py = pix / rq.W;
+        Flags f{P.flags[p]};
+        f3 dir = xyz(rb);
+        f3 contrib = xyz(P.contrib[p]);
+        uint2 rs = P.rng[p];
+        Rng rng{rs.x, rs.y};
+        float4 ha = Q.hitA[i];
+        uint32_t inst = __float_as_uint(ha.y);
+        uint32_t depth = f.depth();
+        uint8_t mask = 0;
+        // A shadow request is queued where it is made (the appends work on
+        // whatever lanes are active), so no slot state stays live.
+        auto shadow = [&](int slot, f3 org, f3 sdir, f3 color) {
+            P.shColor[(size_t)p * kSlots + slot] = f4(color, 0.0f);
+            mask |= (uint8_t)(1u << slot);
+            emitShadow(Q, true, org, sdir, p * kSlots + (uint32_t)slot);
+        };
+        {
+            f3 V = -dir;
+            const InstShade& ins = rq.rs.ishade[inst];
+            Shading shd;
+            const uint32_t prim = __float_as_uint(ha.z);
+            const float hv = Q.hitV[i];
+            computeShading(ins, prim, ha.w, hv, shd);
+            HitUV huv(ins, prim, ha.w, hv);
+            float Pt;
+            f3 opacity;
+            const uint32_t mi = selectLayer(rq.rs, ins, huv, rng, contrib, Pt, opacity);
+            if (mi == kNoMaterial) {
+                nOrg = shd.geomPosition;
+                nDir = dir;
+                f.event(kEvSkip);
+                contPath = true;
             } else {
-                f3 V = -dir;
-                const InstShade& ins = rq.rs.ishade[inst];
-                Shading shd;
-                const uint32_t prim = __float_as_uint(ha.z);
-                const float hv = Q.hitV[i];
-                computeShading(ins, prim, ha.w, hv, shd);
-                HitUV huv(ins, prim, ha.w, hv);
-                float Pt;
-                f3 opacity;
-                const uint32_t mi = selectLayer(rq.rs, ins, huv, rng, contrib, Pt, opacity);
-                if (mi == kNoMaterial) {
-                    nOrg = shd.geomPosition;
-                    nDir = dir;
-                    f.event(kEvSkip);
-                    contPath = true;
-                } else {
-                    const DevMaterial& m = rq.rs.mats[mi];
-                    f3 emissionColor;
-                    materialTextures(rq.rs, mi, m, huv, shd, emissionColor);
-                    f3 direct = emissionColor * m.emission * lerp(splat(1.0f), m.coatColor, m.coat);
-                    f3 so = dot(V, shd.normal) < 0.0f ? shd.geomPosition : shd.position;
-                    f.primary(false);
-                    if (fp.light_count > 1) {
-                        for (int li = 0; li < fp.light_count; li++) {
-                            if (fp.lights[li].color_intensity[3] <= 0.0f) continue;
-                            f3 sd;
-                            f3 lr = shadeDirectional(fp.lights[li], m, shd, V, rng, sd);
-                            if (isBlack(lr)) continue;
-                            shadow(li, so, sd, lr);
-                        }
-                    } else if (fp.light_count > 0) {
-                        const aph_distant_light& l0 = fp.lights[0];
-                        float w0 = luminance(f3{l0.color_intensity[0], l0.color_intensity[1], l0.color_intensity[2]}) *
-                                   l0.color_intensity[3];
-                        float selPdf = 1.0f;
-                        random2D(rng);
-                        if (w0 > 0.0f) selPdf = w0 / w0;
-                        if (l0.color_intensity[3] > 0.0f) {
-                            f3 sd;
-                            f3 dc = shadeDirectional(l0, m, shd, V, rng, sd) / fmaxf(selPdf, 1e-4f);
-                            shadow(0, so, sd, dc);
-                        }
+                const DevMaterial& m = rq.rs.mats[mi];
+                f3 emissionColor;
+                materialTextures(rq.rs, mi, m, huv, shd, emissionColor);
+                f3 direct = emissionColor * m.emission * lerp(splat(1.0f), m.coatColor, m.coat);
+                f3 so = dot(V, shd.normal) < 0.0f ? shd.geomPosition : shd.position;
+                f.primary(false);
+                if (fp.light_count > 1) {
+                    for (int li = 0; li < fp.light_count; li++) {
+                        if (fp.lights[li].color_intensity[3] <= 0.0f) continue;
+                        f3 sd;
+                        f3 lr = shadeDirectional(fp.lights[li], m, shd, V, rng, sd);
+                        if (isBlack(lr)) continue;
+                        shadow(li, so, sd, lr);
                     }
-                    // environment MIS (ShadeFunctions.slang:135-259)
-                    {
-                        f2 r = random2D(rng);
-                        f3 L;
-                        float matPdf;
-                        int lobe;
-                        f3 fs = sampleMaterial(m, shd, V, r, L, matPdf, lobe);
-                        if (lobe != kTransmissionLobe) {
-                            f3 Le = evalEnvironment(env, L, false, px, py, rq.W, rq.H);
-                            if (!isBlack(Le) && !isBlack(fs)) {
-                                float lightPdf = env.hasLightTex ? luminance(Le) / env.lightIntegral : 0.25f * kPiInv;
-                                shadow(kSlotMisMat, so, L, Le * misWeight(matPdf, lightPdf) * fs / matPdf);
-                            }
-                            float lightPdf;
-                            Le = sampleEnvironment(env, r, L, lightPdf);
-                            int evLobe;
-                            fs = evalMaterialPdf(m, shd, V, L, r, matPdf, evLobe);
-                            if (!isBlack(Le) && !isBlack(fs))
-                                shadow(kSlotMisLight, so, L, Le * misWeight(lightPdf, matPdf) * fs / lightPdf);
-                        }
-                        if (mask & ((1u << kSlotMisMat) | (1u << kSlotMisLight))) f.misLobe((uint32_t)lobe);
+                } else if (fp.light_count > 0) {
+                    const aph_distant_light& l0 = fp.lights[0];
+                    float w0 = luminance(f3{l0.color_intensity[0], l0.color_intensity[1], l0.color_intensity[2]}) *
+                               l0.color_intensity[3];
+                    float selPdf = 1.0f;
+                    random2D(rng);
+                    if (w0 > 0.0f) selPdf = w0 / w0;
+                    if (l0.color_intensity[3] > 0.0f) {
+                        f3 sd;
+                        f3 dc = shadeDirectional(l0, m, shd, V, rng, sd) / fmaxf(selPdf, 1e-4f);
+                        shadow(0, so, sd, dc);
                     }
-                    contrib = contrib * (opacity / (1.0f - Pt));
-                    f.alpha((f.alpha() == 0.0f && f.rayType() == (uint32_t)kTransmissionLobe) ? 0.0f : 1.0f);
-                    f3 nextContrib = splat(1.0f);
-                    uint32_t nextType;
-                    {
-                        f2 r = random2D(rng);
-                        f3 L;
-                        float pdf;
-                        int lobe;
-                        f3 fs = sampleMaterial(m, shd, V, r, L, pdf, lobe);
-                        if (isBlack(fs)) {
-                            nextType = kNoSecondaryRay;
-                        } else {
-                            bool trans = lobe == kTransmissionLobe;
-                            nOrg = (dot(V, shd.normal) < 0.0f || trans) ? shd.geomPosition : shd.position;
-                            nDir = L;
-                            nextContrib = fs / pdf;
-                            nextType = (uint32_t)lobe;
-                        }
-                    }
-                    bool ended = nextType == (uint32_t)kNoSecondaryRay;
-                    if (nextType == (uint32_t)kTransmissionLobe) f.inside(!f.inside());
-                    P.pendDirect[p] = f4(direct, 0.0f);
-                    P.pendContrib[p] = f4(contrib, 0.0f);
-                    contrib = contrib * nextContrib;
-                    if (fp.rr_enabled && (int)depth + 1 >= fp.rr_start_depth) {
-                        float surv = clampf(maxc(contrib), 0.05f, 1.0f);
-                        if (random2D(rng).x > surv) ended = true;
-                        else contrib = contrib / surv;
-                    }
-                    f.nextRayType(nextType);
-                    f.event(kEvHit);
-                    f.ended(ended);
-                    contPath = !ended;
                 }
-            }
-            f.terminal(false);
-            if (contPath) {
-                uint32_t nd = depth + 1;
-                if ((int)nd >= fp.trace_depth) {
-                    f.terminal(true);
-                    contPath = false;
-                    P.nextDir[p] = f4(nDir, 0.0f);
+                // environment MIS (ShadeFunctions.slang:135-259)
+                {
+                    f2 r = random2D(rng);
+                    f3 L;
+                    float matPdf;
+                    int lobe;
+                    f3 fs = sampleMaterial(m, shd, V, r, L, matPdf, lobe);
+                    if (lobe != kTransmissionLobe) {
+                        f3 Le = evalEnvironment(env, L, false, px, py, rq.W, rq.H);
+                        if (!isBlack(Le) && !isBlack(fs)) {
+                            float lightPdf = env.hasLightTex ? luminance(Le) / env.lightIntegral : 0.25f * kPiInv;
+                            shadow(kSlotMisMat, so, L, Le * misWeight(matPdf, lightPdf) * fs / matPdf);
+                        }
+                        float lightPdf;
+                        Le = sampleEnvironment(env, r, L, lightPdf);
+                        int evLobe;
+                        fs = evalMaterialPdf(m, shd, V, L, r, matPdf, evLobe);
+                        if (!isBlack(Le) && !isBlack(fs))
+                            shadow(kSlotMisLight, so, L, Le * misWeight(lightPdf, matPdf) * fs / lightPdf);
+                    }
+                    if (mask & ((1u << kSlotMisMat) | (1u << kSlotMisLight))) f.misLobe((uint32_t)lobe);
                 }
+                contrib = contrib * (opacity / (1.0f - Pt));
+                f.alpha((f.alpha() == 0.0f && f.rayType() == (uint32_t)kTransmissionLobe) ? 0.0f : 1.0f);
+                f3 nextContrib = splat(1.0f);
+                uint32_t nextType;
+                {
+                    f2 r = random2D(rng);
+                    f3 L;
+                    float pdf;
+                    int lobe;
+                    f3 fs = sampleMaterial(m, shd, V, r, L, pdf, lobe);
+                    if (isBlack(fs)) {
+                        nextType = kNoSecondaryRay;
+                    } else {
+                        bool trans = lobe == kTransmissionLobe;
+                        nOrg = (dot(V, shd.normal) < 0.0f || trans) ? shd.geomPosition : shd.position;
+                        nDir = L;
+                        nextContrib = fs / pdf;
+                        nextType = (uint32_t)lobe;
+                    }
+                }
+                bool ended = nextType == (uint32_t)kNoSecondaryRay;
+                if (nextType == (uint32_t)kTransmissionLobe) f.inside(!f.inside());
+                P.pendDirect[p] = f4(direct, 0.0f);
+                P.pendContrib[p] = f4(contrib, 0.0f);
+                contrib = contrib * nextContrib;
+                if (fp.rr_enabled && (int)depth + 1 >= fp.rr_start_depth) {
+                    float surv = clampf(maxc(contrib), 0.05f, 1.0f);
+                    if (random2D(rng).x > surv) ended = true;
+                    else contrib = contrib / surv;
+                }
+                f.nextRayType(nextType);
+                f.event(kEvHit);
+                f.ended(ended);
+                contPath = !ended;
             }
-            P.shMask[p] = mask;
-            P.flags[p] = f.v;
-            P.contrib[p] = f4(contrib, 0.0f);
-            P.rng[p] = make_uint2(rng.x, rng.y);
         }
-        // wave-uniform appends: next ray queue, then one shadow slot at a time
-        uint32_t at = waveAppend(contPath, &Q.counts[nxt]);
+        f.terminal(false);
         if (contPath) {
-            Q.rayA[nxt][at] = f4(nOrg, nTmin);
-            Q.rayB[nxt][at] = f4(nDir, __uint_as_float(p));
+            uint32_t nd = depth + 1;
+            if ((int)nd >= fp.trace_depth) {
+                f.terminal(true);
+                contPath = false;
+                P.nextDir[p] = f4(nDir, 0.0f);
+            }
         }
+        P.shMask[p] = mask;
+        P.flags[p] = f.v;
+        P.contrib[p] = f4(contrib, 0.0f);
+        P.rng[p] = make_uint2(rng.x, rng.y);
+    }
+    uint32_t at = waveAppend(contPath, &Q.counts[nxt]);
+    if (contPath) {
+        Q.rayA[nxt][at] = f4(nOrg, nTmin);
+        Q.rayB[nxt][at] = f4(nDir, __uint_as_float(p));
+    }
+}
+
+// The block walks its grid-stride slices of the queue: misses are shaded where
+// they are found, hits are queued in LDS and shaded kWB at a time, so the long
+// closest-hit program runs on full waves whatever the miss rate.  Results are
+// per path, so the order of shading does not change them.  Four waves per SIMD
+// (128 VGPRs, a few spills) hide the gathers of shading data, materials and
+// environment texels better than three without spills.
+__global__ __launch_bounds__(kWB) __attribute__((amdgpu_waves_per_eu(4, 8))) void kShade(const DevReq* reqs, const SegInfo* segs, Paths P, Queues Q, int cur)
+{
+    __shared__ uint32_t hitq[2 * kWB];
+    __shared__ uint32_t hitCount;
+    const uint32_t n = Q.counts[cur];
+    const uint32_t stride = gridDim.x * kWB;
+    uint32_t next = blockIdx.x * kWB;  // start of the block's next slice (block-uniform)
+    uint32_t pending = 0;              // hits queued in LDS (block-uniform)
+    if (threadIdx.x == 0) hitCount = 0;
+    __syncthreads();
+    for (;;) {
+        // Refill to at least one block of hits (at most 2 kWB - 1 queued).
+        while (pending < kWB && next < n) {
+            const uint32_t i = next + threadIdx.x;
+            next += stride;
+            bool hit = false;
+            if (i < n) {
+                hit = __float_as_uint(Q.hitA[i].y) != 0xFFFFFFFFu;
+                if (!hit) shadeMiss(reqs, segs, P, Q, cur, i);
+            }
+            const uint64_t m = __ballot(hit);
+            const int lane = __lane_id();
+            uint32_t base = 0;
+            if (lane == 0 && m) base = atomicAdd(&hitCount, (uint32_t)__popcll(m));
+            base = __shfl(base, 0, 64);
+            if (hit) hitq[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = i;
+            __syncthreads();
+            pending = hitCount;
+            __syncthreads();
+        }
+        if (pending == 0) break;
+        const uint32_t take = pending < (uint32_t)kWB ? pending : (uint32_t)kWB;
+        const uint32_t rest = pending - take;
+        const bool live = threadIdx.x < take;
+        const uint32_t i = live ? hitq[rest + threadIdx.x] : 0u;
+        if (threadIdx.x == 0) hitCount = rest;
+        pending = rest;
+        __syncthreads();
+        shadeHit(reqs, segs, P, Q, cur, live, i);
     }
 }
 
