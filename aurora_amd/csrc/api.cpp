@@ -245,6 +245,7 @@ int aph_blas_build(aph_ctx_t c, uint64_t mesh, int builder)
     uint32_t n = m->nt;
     TimedScope ts(c, "blas_build");
     HIPCALL(c, m->nodes.alloc((size_t)(n > 1 ? n - 1 : 1) * sizeof(BvhNode)), "alloc BLAS nodes");
+    HIPCALL(c, m->nodes4.alloc((size_t)(n > 1 ? n - 1 : 1) * sizeof(Bvh4Node)), "alloc BLAS nodes");
     HIPCALL(c, m->tris.alloc((size_t)(n ? n : 1) * sizeof(Tri)), "alloc BLAS triangles");
     if (n == 0) {
         m->built = true;
@@ -269,6 +270,7 @@ int aph_blas_build(aph_ctx_t c, uint64_t mesh, int builder)
     HIPCALL(c, reorderTris(m->pos.as<float>(), m->idx.as<uint32_t>(), sorted.as<uint32_t>(), n, c->stream,
                            m->tris.as<Tri>()),
             "reorder triangles");
+    HIPCALL(c, collapse4(m->nodes.as<BvhNode>(), info.nodes, c->stream, m->nodes4.as<Bvh4Node>()), "BLAS nodes");
     HIPCALL(c, hipStreamSynchronize(c->stream), "BLAS build");
     m->info = info;
     m->built = true;
@@ -331,7 +333,7 @@ int aph_tlas_build(aph_ctx_t c, const aph_instance* inst, uint32_t n, uint64_t* 
         }
         InstTrav& tv = trav[i];
         std::memcpy(tv.w2o, w2o, sizeof(w2o));
-        tv.nodes = m->nodes.as<BvhNode>();
+        tv.nodes = m->nodes4.as<Bvh4Node>();
         tv.tris = m->tris.as<Tri>();
         InstShade& sh = shade[i];
         std::memcpy(sh.o2w, in.o2w, sizeof(sh.o2w));
@@ -380,6 +382,7 @@ int aph_tlas_build(aph_ctx_t c, const aph_instance* inst, uint32_t n, uint64_t* 
                               c->stream),
             "copy instances");
     HIPCALL(c, t->nodes.alloc(sizeof(BvhNode) * (nv > 1 ? nv - 1 : 1)), "alloc TLAS nodes");
+    HIPCALL(c, t->nodes4.alloc(sizeof(Bvh4Node) * (nv > 1 ? nv - 1 : 1)), "alloc TLAS nodes");
     if (nv) {
         TimedScope ts(c, "tlas_build");
         DevBuf lo, hi, sorted, rm;
@@ -395,22 +398,28 @@ int aph_tlas_build(aph_ctx_t c, const aph_instance* inst, uint32_t n, uint64_t* 
         HIPCALL(c, sahBuild(lo.as<float4>(), hi.as<float4>(), nv, 1, true, rm.as<uint32_t>(), c->stream,
                             t->nodes.as<BvhNode>(), sorted.as<uint32_t>(), &info),
                 "TLAS build");
+        HIPCALL(c, collapse4(t->nodes.as<BvhNode>(), info.nodes, c->stream, t->nodes4.as<Bvh4Node>()), "TLAS nodes");
         HIPCALL(c, hipStreamSynchronize(c->stream), "TLAS build");
         t->depth = info.depth;
-        // Every pending sibling needs a stack slot: TLAS path + exit marker + BLAS path.
-        if (info.depth + 1 + maxBlasDepth >= (uint32_t)kStackCap)
-            return fail(c, APH_E_OVERFLOW, "BVH too deep for the traversal stack (TLAS %u + BLAS %u >= %d)",
-                        info.depth, maxBlasDepth, kStackCap - 1);
+        // Stack bound: a four-child level (two levels of the two-child tree) leaves
+        // at most 3 siblings pending; entering an instance adds the pending TLAS
+        // entry and the exit marker.
+        const uint32_t need = 3 * ((info.depth + 1) / 2) + 2 + 3 * ((maxBlasDepth + 1) / 2);
+        if (need > (uint32_t)kStackCap)
+            return fail(c, APH_E_OVERFLOW,
+                        "BVH too deep for the traversal stack (TLAS depth %u, BLAS depth %u: %u > %d entries)",
+                        info.depth, maxBlasDepth, need, kStackCap);
     }
     HIPCALL(c, hipStreamSynchronize(c->stream), "TLAS upload");
-    t->sc.tlas = t->nodes.as<BvhNode>();
+    t->sc.tlas = t->nodes4.as<Bvh4Node>();
     t->sc.inst = t->trav.as<InstTrav>();
     t->sc.extent = (float)(3.0 * extent);
     t->sc.padScale = (float)(1e-5 * kappa);
     t->sc.ninst = nv;
-    t->bytes = t->nodes.bytes + t->trav.bytes + t->shade.bytes;
+    t->bytes = t->nodes.bytes + t->nodes4.bytes + t->trav.bytes + t->shade.bytes;
     for (const Mesh* m : usedMeshes)
-        t->bytes += m->pos.bytes + m->nrm.bytes + m->uv.bytes + m->idx.bytes + m->nodes.bytes + m->tris.bytes;
+        t->bytes += m->pos.bytes + m->nrm.bytes + m->uv.bytes + m->idx.bytes + m->nodes.bytes + m->nodes4.bytes +
+                    m->tris.bytes;
     uint64_t h = c->nextHandle++;
     c->tlases[h] = std::move(t);
     *tlas = h;

@@ -246,6 +246,48 @@ __global__ void kReorderTris(const float* pos, const uint32_t* idx, const uint32
     tris[i] = t;
 }
 
+// Four-child copy of node i: each internal child is replaced by its own two
+// children (boxes and references copied bit for bit), leaves stay; unused slots
+// get an inverted box, which every slab test rejects, and kEmptyChild.
+__global__ void kCollapse4(const BvhNode* bin, uint32_t n, Bvh4Node* out)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float e = 3.0e38f;
+    float lo[3][4], hi[3][4];
+    int32_t ref[4];
+    int k = 0;
+    auto add = [&](float4 l, float4 h, int32_t r) {
+        lo[0][k] = l.x; lo[1][k] = l.y; lo[2][k] = l.z;
+        hi[0][k] = h.x; hi[1][k] = h.y; hi[2][k] = h.z;
+        ref[k] = r;
+        k++;
+    };
+    const BvhNode nd = bin[i];
+    const float4 cl[2] = {nd.a, nd.c}, ch[2] = {nd.b, nd.d};
+    const int32_t cr[2] = {__float_as_int(nd.a.w), __float_as_int(nd.b.w)};
+    for (int j = 0; j < 2; j++) {
+        if (cr[j] >= 0) {
+            const BvhNode g = bin[cr[j]];
+            add(g.a, g.b, __float_as_int(g.a.w));
+            add(g.c, g.d, __float_as_int(g.b.w));
+        } else {
+            add(cl[j], ch[j], cr[j]);
+        }
+    }
+    while (k < 4) add(make_float4(e, e, e, 0.0f), make_float4(-e, -e, -e, 0.0f), kEmptyChild);
+    Bvh4Node o;
+    o.lox = make_float4(lo[0][0], lo[0][1], lo[0][2], lo[0][3]);
+    o.loy = make_float4(lo[1][0], lo[1][1], lo[1][2], lo[1][3]);
+    o.loz = make_float4(lo[2][0], lo[2][1], lo[2][2], lo[2][3]);
+    o.hix = make_float4(hi[0][0], hi[0][1], hi[0][2], hi[0][3]);
+    o.hiy = make_float4(hi[1][0], hi[1][1], hi[1][2], hi[1][3]);
+    o.hiz = make_float4(hi[2][0], hi[2][1], hi[2][2], hi[2][3]);
+    o.ref = make_int4(ref[0], ref[1], ref[2], ref[3]);
+    o.pad = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    out[i] = o;
+}
+
 inline unsigned blocks(uint32_t n) { return (n + kB - 1) / kB; }
 
 #define APH_CHECK(x)                                  \
@@ -338,6 +380,13 @@ hipError_t lbvhBuild(const float4* primLo, const float4* primHi, uint32_t n, boo
     hipFree(bounds6); hipFree(keys); hipFree(keys2); hipFree(vals); hipFree(parent); hipFree(visits);
     hipFree(depth); hipFree(child); hipFree(boxLo); hipFree(boxHi); hipFree(sortTmp);
     return err;
+}
+
+hipError_t collapse4(const BvhNode* nodes, uint32_t n, hipStream_t st, Bvh4Node* out)
+{
+    if (n == 0) return hipSuccess;
+    kCollapse4<<<blocks(n), kB, 0, st>>>(nodes, n, out);
+    return hipGetLastError();
 }
 
 hipError_t triBoxes(const float* pos, const uint32_t* idx, uint32_t n, hipStream_t st, float4* lo, float4* hi)

@@ -25,6 +25,8 @@ hipError_t lbvhBuild(const float4* primLo, const float4* primHi, uint32_t n, boo
 hipError_t sahBuild(const float4* primLo, const float4* primHi, uint32_t n, uint32_t maxLeaf, bool leafOriginal,
                     const uint32_t* leafRemap, hipStream_t st, BvhNode* nodes, uint32_t* sortedOut,
                     BuildInfo* info);
+// Writes the four-child traversal copy of an n-node two-child tree (n >= 1).
+hipError_t collapse4(const BvhNode* nodes, uint32_t n, hipStream_t st, Bvh4Node* out);
 hipError_t triBoxes(const float* pos, const uint32_t* idx, uint32_t n, hipStream_t st, float4* lo, float4* hi);
 hipError_t reorderTris(const float* pos, const uint32_t* idx, const uint32_t* sorted, uint32_t n, hipStream_t st,
                        Tri* tris);

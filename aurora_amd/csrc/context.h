@@ -54,14 +54,14 @@ struct DevBuf {
 struct Mesh {
     uint32_t nv = 0, nt = 0;
     bool hasNrm = false;
-    DevBuf pos, nrm, uv, idx, nodes, tris;
+    DevBuf pos, nrm, uv, idx, nodes, nodes4, tris;
     bool built = false;
     BuildInfo info{};
 };
 
 struct Tlas {
     uint32_t n = 0, nvalid = 0;
-    DevBuf nodes, trav, shade;
+    DevBuf nodes, nodes4, trav, shade;
     TraceScene sc{};
     DevBuf layerMats;  // material indices of the instances' layers (aph_tlas_set_layers)
     std::vector<InstShade> shadeHost;

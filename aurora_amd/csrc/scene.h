@@ -2,10 +2,11 @@
 //
 // HBM layout (all arrays 16-byte aligned):
 //   BLAS per mesh   BvhNode[n-1]  64 B/node, both children's boxes in the parent
+//                   Bvh4Node[n-1] 128 B/node, the traversal copy (four children)
 //                   Tri[n]        48 B/triangle, vertices in BVH leaf order,
 //                                 original primitive index in v0.w
 //   mesh shading    pos float[3V], nrm float[3V] (optional), idx uint[3T]
-//   TLAS            BvhNode[m-1]  over instance world boxes
+//   TLAS            BvhNode[m-1]  over instance world boxes, and its Bvh4Node copy
 //                   InstTrav[m]   64 B: world->object 3x4 + BLAS pointers
 //                   InstShade[m]  object->world 3x4 + shading pointers
 //   materials       DevMaterial[k]
@@ -26,6 +27,18 @@ constexpr uint32_t kLeafBit = 0x80000000u;
 constexpr uint32_t kLeafFirstMask = 0x07FFFFFFu;
 constexpr int32_t kExitMarker = 0x7FFFFFFF;  // stack marker: leave the BLAS, back to the TLAS
 constexpr uint32_t kMaxLeafPrims = 16;
+constexpr int32_t kEmptyChild = 0x7FFFFFFD;  // unused slot of a Bvh4Node (its box is inverted)
+
+// Four-child node, the one traversal reads: node i holds the children of
+// two-child node i with every internal child replaced by that child's own two
+// children, so indices carry over and the nodes a traversal reaches are those
+// at even depths of the two-child tree.  Boxes in SoA form (one float4 per
+// plane coordinate, child k in component k); 128 B, one L2 line.
+struct alignas(16) Bvh4Node {
+    float4 lox, loy, loz, hix, hiy, hiz;
+    int4 ref;
+    float4 pad;
+};
 APH_HD int32_t makeLeaf(uint32_t first, uint32_t count)
 {
     return (int32_t)(kLeafBit | ((count - 1u) << 27) | first);
@@ -39,7 +52,7 @@ struct alignas(16) Tri {
 
 struct alignas(16) InstTrav {
     float w2o[12];
-    const BvhNode* nodes;
+    const Bvh4Node* nodes;
     const Tri* tris;
 };
 
@@ -56,7 +69,7 @@ struct alignas(16) InstShade {
 };
 
 struct TraceScene {
-    const BvhNode* tlas;
+    const Bvh4Node* tlas;
     const InstTrav* inst;
     float extent;      // 3 x max |coordinate| of all instance world boxes
     float padScale;    // relative pad of instance boxes (times condition number bound)

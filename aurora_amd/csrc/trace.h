@@ -23,7 +23,7 @@
 namespace aph {
 
 constexpr int kLdsStack = 16;    // stack slots per lane held in LDS
-constexpr int kSpillStack = 48;  // deeper slots spill to private (scratch) memory
+constexpr int kSpillStack = 112; // deeper slots spill to private (scratch) memory
 constexpr int kStackCap = kLdsStack + kSpillStack;
 // Far-distance widening of the slab test.  Each distance (b - o) * inv carries a
 // subtraction and a product rounding (u each, u = 2^-24) and the reciprocal's
@@ -80,7 +80,7 @@ __device__ __forceinline__ InstTrav gloadInst(const InstTrav* p)
     InstTrav t;
     float m[12] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w};
     for (int i = 0; i < 12; i++) t.w2o[i] = m[i];
-    t.nodes = reinterpret_cast<const BvhNode*>(((uint64_t)__float_as_uint(d.y) << 32) | __float_as_uint(d.x));
+    t.nodes = reinterpret_cast<const Bvh4Node*>(((uint64_t)__float_as_uint(d.y) << 32) | __float_as_uint(d.x));
     t.tris = reinterpret_cast<const Tri*>(((uint64_t)__float_as_uint(d.w) << 32) | __float_as_uint(d.z));
     return t;
 }
@@ -254,7 +254,7 @@ struct TraceState {
     float tlim;           // box cull distance: the closest hit's t rounded up to float
     BoxRay br;            // box-test ray of the current level (world or object space)
     WRay w;               // watertight ray of the current instance
-    const BvhNode* nodes; // node array of the current level
+    const Bvh4Node* nodes; // node array of the current level
     const Tri* tris;      // triangles of the current instance
     uint32_t curInst;     // 0xFFFFFFFF in the TLAS
     int32_t cur, leaf;    // next entry; parked leaf (0 = none)
@@ -333,21 +333,50 @@ __device__ __forceinline__ bool traceRound(TraceState& s, const TraceScene& sc, 
     while (isInner(s.cur)) {
         if (++s.steps > kMaxSteps) { st.overflow = 1; return true; }
         const float4* np = reinterpret_cast<const float4*>(s.nodes + s.cur);
-        float4 a = gload4(np), b = gload4(np + 1), c = gload4(np + 2), d = gload4(np + 3);
+        const float4 lx = gload4(np), ly = gload4(np + 1), lz = gload4(np + 2);
+        const float4 hx = gload4(np + 3), hy = gload4(np + 4), hz = gload4(np + 5), rf = gload4(np + 6);
         if (STATS) st.nodes++;
-        float t0, t1;
-        bool h0 = slab(s.br, a, b, s.tmin, s.tlim, t0);
-        bool h1 = slab(s.br, c, d, s.tmin, s.tlim, t1);
-        int32_t c0 = __float_as_int(a.w), c1 = __float_as_int(b.w);
-        if (h0 && h1) {
-            bool swap = t1 < t0;
-            if (!stackPush(s, m, swap ? c0 : c1)) { st.overflow = 1; return true; }
-            s.cur = swap ? c1 : c0;
-        } else if (h0 || h1) {
-            s.cur = h0 ? c0 : c1;
-        } else {
-            s.cur = stackPop(s, m);
+        // Four box tests; a missed child becomes kEmptyChild with key +inf.
+        float k0, k1, k2, k3;
+        int32_t r0 = slab(s.br, make_float4(lx.x, ly.x, lz.x, 0.0f), make_float4(hx.x, hy.x, hz.x, 0.0f), s.tmin,
+                          s.tlim, k0) ? __float_as_int(rf.x) : kEmptyChild;
+        int32_t r1 = slab(s.br, make_float4(lx.y, ly.y, lz.y, 0.0f), make_float4(hx.y, hy.y, hz.y, 0.0f), s.tmin,
+                          s.tlim, k1) ? __float_as_int(rf.y) : kEmptyChild;
+        int32_t r2 = slab(s.br, make_float4(lx.z, ly.z, lz.z, 0.0f), make_float4(hx.z, hy.z, hz.z, 0.0f), s.tmin,
+                          s.tlim, k2) ? __float_as_int(rf.z) : kEmptyChild;
+        int32_t r3 = slab(s.br, make_float4(lx.w, ly.w, lz.w, 0.0f), make_float4(hx.w, hy.w, hz.w, 0.0f), s.tmin,
+                          s.tlim, k3) ? __float_as_int(rf.w) : kEmptyChild;
+        if (r0 == kEmptyChild) k0 = INFINITY;
+        if (r1 == kEmptyChild) k1 = INFINITY;
+        if (r2 == kEmptyChild) k2 = INFINITY;
+        if (r3 == kEmptyChild) k3 = INFINITY;
+        // Nearest first: a five-comparator sorting network on (key, child).
+        auto cswap = [](float& ka, int32_t& ra, float& kb, int32_t& rb) {
+            const bool sw = kb < ka;
+            const float kt = sw ? kb : ka;
+            const int32_t rt = sw ? rb : ra;
+            kb = sw ? ka : kb;
+            rb = sw ? ra : rb;
+            ka = kt;
+            ra = rt;
+        };
+        cswap(k0, r0, k1, r1);
+        cswap(k2, r2, k3, r3);
+        cswap(k0, r0, k2, r2);
+        cswap(k1, r1, k3, r3);
+        cswap(k1, r1, k2, r2);
+        // Far to near: every hit child but the nearest is pushed, the nearest goes on.
+        int32_t next = kEmptyChild;
+        const int32_t rs[4] = {r3, r2, r1, r0};
+        bool ok = true;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            if (rs[j] == kEmptyChild) continue;
+            if (next != kEmptyChild) ok = ok && stackPush(s, m, next);
+            next = rs[j];
         }
+        if (!ok) { st.overflow = 1; return true; }
+        s.cur = next != kEmptyChild ? next : stackPop(s, m);
         if (s.cur < 0 && s.leaf == 0) {
             s.leaf = s.cur;
             s.cur = stackPop(s, m);

@@ -62,7 +62,7 @@ struct Flags {
 
 struct Paths {
     uint32_t* pix;
-    uint32_t* seg;
+    uint32_t* req;         // request index of the path
     uint2* rng;
     float4* contrib;
     uint32_t* flags;
@@ -80,11 +80,11 @@ struct Paths {
 };
 
 struct Queues {
-    float4* rayA[2];       // org.xyz, tmin
+    float4* rayA[2];       // org.xyz, ray tag (request index; camera bit)
     float4* rayB[2];       // dir.xyz, path id bits
     float4* hitA;          // t, inst, prim, u
     float* hitV;
-    float4* shA;           // org.xyz, tmin
+    float4* shA;           // org.xyz, request index bits (traced from kRayTMin)
     float4* shB;           // dir.xyz, (path * kSlots + slot) bits
     uint32_t* counts;      // [0],[1] ray queues, [2] shadow queue, [3]/[4] extend/shadow fetch cursors
 };
@@ -109,6 +109,18 @@ struct DevReq {
     uint32_t W, H, seedOffset, pad;
     float4* accum;
 };
+
+// The w word of a ray queue entry: the path's request index, with bit 31 set
+// for camera rays, which are traced from t = 0 (every other ray from kRayTMin).
+// Carrying the request in the entry lets a kernel fetch its request record
+// alongside the ray instead of after a load of path state.
+constexpr uint32_t kCameraTag = 0x80000000u;
+__device__ __forceinline__ float rayTag(uint32_t req, bool camera)
+{
+    return __uint_as_float(req | (camera ? kCameraTag : 0u));
+}
+__device__ __forceinline__ uint32_t tagReq(float w) { return __float_as_uint(w) & ~kCameraTag; }
+__device__ __forceinline__ float tagTmin(float w) { return (__float_as_uint(w) & kCameraTag) ? 0.0f : kRayTMin; }
 
 __device__ __forceinline__ uint32_t findSeg(const SegInfo* segs, uint32_t nseg, uint32_t p)
 {
@@ -186,7 +198,7 @@ __global__ __launch_bounds__(kWB) void kGen(const DevReq* reqs, const SegInfo* s
         f3 org, dir;
         cameraRay(rq.fp, px, py, rq.W, rq.H, rng, org, dir);
         P.pix[p] = pix;
-        P.seg[p] = s;
+        P.req[p] = sg.req;
         P.rng[p] = make_uint2(rng.x, rng.y);
         P.contrib[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
         Flags f{0};
@@ -200,7 +212,7 @@ __global__ __launch_bounds__(kWB) void kGen(const DevReq* reqs, const SegInfo* s
         P.outDiffuse[p] = z;
         P.outGlossy[p] = z;
         P.lastIndirect[p] = z;
-        Q.rayA[0][p] = f4(org, 0.0f);
+        Q.rayA[0][p] = f4(org, rayTag(sg.req, true));
         Q.rayB[0][p] = f4(dir, __uint_as_float(p));
         started++;
     }
@@ -278,8 +290,8 @@ __global__ __launch_bounds__(kWB) void kExtend(const DevReq* reqs, const SegInfo
         float4 a = rayA[i], b = rayB[i];
         org = xyz(a);
         dir = xyz(b);
-        tmin = a.w;
-        return &reqs[segs[P.seg[__float_as_uint(b.w)]].req].rs.sc;
+        tmin = tagTmin(a.w);
+        return &reqs[tagReq(a.w)].rs.sc;
     };
     auto store = [&](uint32_t i, const TraceState& s) {
         if (s.found) {
@@ -301,11 +313,11 @@ __global__ __launch_bounds__(kWB) void kExtend(const DevReq* reqs, const SegInfo
     addCounter(&ctr->overflow, ts.overflow);
 }
 
-__device__ __forceinline__ void emitShadow(const Queues& Q, bool emit, f3 org, f3 dir, uint32_t slotId)
+__device__ __forceinline__ void emitShadow(const Queues& Q, bool emit, f3 org, f3 dir, uint32_t slotId, uint32_t req)
 {
     uint32_t at = waveAppend(emit, &Q.counts[2]);
     if (emit) {
-        Q.shA[at] = f4(org, kRayTMin);
+        Q.shA[at] = f4(org, __uint_as_float(req));
         Q.shB[at] = f4(dir, __uint_as_float(slotId));
     }
 }
@@ -324,7 +336,7 @@ __device__ __forceinline__ void shadeMiss(const DevReq* reqs, const SegInfo* seg
                     f.rayType() == (uint32_t)kGroundPlaneReflectionRay;
     f3 d = splat(0.0f);
     if (bg) {
-        const DevReq& rq = reqs[segs[P.seg[p]].req];
+        const DevReq& rq = reqs[tagReq(Q.rayA[cur][i].w)];
         const uint32_t pix = P.pix[p];
         d = evalEnvironment(rq.rs.env, xyz(Q.rayB[cur][i]), true, pix % rq.W, pix / rq.W, rq.W, rq.H) *
             xyz(P.contrib[p]);
@@ -345,12 +357,12 @@ __device__ __forceinline__ void shadeHit(const DevReq* reqs, const SegInfo* segs
     const int nxt = cur ^ 1;
     bool contPath = false;
     f3 nOrg = splat(0.0f), nDir = splat(0.0f);
-    float nTmin = kRayTMin;
-    uint32_t p = 0;
+    uint32_t p = 0, ri = 0;
     if (live) {
         float4 rb = Q.rayB[cur][i];
         p = __float_as_uint(rb.w);
-        const DevReq& rq = reqs[segs[P.seg[p]].req];
+        ri = tagReq(Q.rayA[cur][i].w);
+        const DevReq& rq = reqs[ri];
         const aph_frame& fp = rq.fp;
         const DevEnv& env = rq.rs.env;
         uint32_t pix = P.pix[p], px = pix % rq.W, py = pix / rq.W;
@@ -368,7 +380,7 @@ __device__ __forceinline__ void shadeHit(const DevReq* reqs, const SegInfo* segs
         auto shadow = [&](int slot, f3 org, f3 sdir, f3 color) {
             P.shColor[(size_t)p * kSlots + slot] = f4(color, 0.0f);
             mask |= (uint8_t)(1u << slot);
-            emitShadow(Q, true, org, sdir, p * kSlots + (uint32_t)slot);
+            emitShadow(Q, true, org, sdir, p * kSlots + (uint32_t)slot, ri);
         };
         {
             f3 V = -dir;
@@ -488,7 +500,7 @@ __device__ __forceinline__ void shadeHit(const DevReq* reqs, const SegInfo* segs
     }
     uint32_t at = waveAppend(contPath, &Q.counts[nxt]);
     if (contPath) {
-        Q.rayA[nxt][at] = f4(nOrg, nTmin);
+        Q.rayA[nxt][at] = f4(nOrg, rayTag(ri, false));
         Q.rayB[nxt][at] = f4(nDir, __uint_as_float(p));
     }
 }
@@ -555,8 +567,8 @@ __global__ __launch_bounds__(kWB) void kShadow(const DevReq* reqs, const SegInfo
         float4 a = Q.shA[i], b = Q.shB[i];
         org = xyz(a);
         dir = xyz(b);
-        tmin = a.w;
-        return &reqs[segs[P.seg[__float_as_uint(b.w) / kSlots]].req].rs.sc;
+        tmin = kRayTMin;
+        return &reqs[__float_as_uint(a.w)].rs.sc;
     };
     auto store = [&](uint32_t i, const TraceState& s) { P.vis[__float_as_uint(Q.shB[i].w)] = s.found ? 0 : 1; };
     uint32_t done = traceQueue<true, STATS>(n, &Q.counts[4], mem, ts, load, store);
@@ -585,7 +597,7 @@ __global__ __launch_bounds__(kWB) void kResolve(const DevReq* reqs, const SegInf
     uint32_t n = Q.counts[cur];
     for (uint32_t i = blockIdx.x * kWB + threadIdx.x; i < n; i += gridDim.x * kWB) {
         uint32_t p = __float_as_uint(Q.rayB[cur][i].w);
-        const DevReq& rq = reqs[segs[P.seg[p]].req];
+        const DevReq& rq = reqs[P.req[p]];
         const aph_frame& fp = rq.fp;
         Flags f{P.flags[p]};
         uint32_t depth = f.depth();
@@ -737,6 +749,7 @@ hipError_t wavefrontRender(WavefrontState& st, const WavefrontRequest* reqs, uin
                            bool stats, hipStream_t stream, std::vector<PendingEvent>* timing)
 {
     if (nreq == 0) return hipSuccess;
+    if (nreq >= kCameraTag) return hipErrorInvalidValue;  // request indices travel in 31 bits
     const WavefrontTuning& tn = st.tuning;
     uint32_t maxPix = 0;
     for (uint32_t i = 0; i < nreq; i++) maxPix = std::max(maxPix, reqs[i].width * reqs[i].height);
@@ -765,7 +778,7 @@ hipError_t wavefrontRender(WavefrontState& st, const WavefrontRequest* reqs, uin
     };
     Paths P;
     P.pix = (uint32_t*)take((size_t)cap * 4);
-    P.seg = (uint32_t*)take((size_t)cap * 4);
+    P.req = (uint32_t*)take((size_t)cap * 4);
     P.rng = (uint2*)take((size_t)cap * 8);
     P.contrib = (float4*)take((size_t)cap * 16);
     P.flags = (uint32_t*)take((size_t)cap * 4);

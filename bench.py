@@ -37,12 +37,12 @@ from aurora_amd import denoiser as dnmod  # noqa: E402
 from aurora_amd.session import Renderer  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# Algorithmic bytes per unit of work: a BVH node (two child boxes + refs), a
-# triangle record, an instance traversal record, a ray (origin/tmin, direction/id),
-# a hit record (wavefront: t/inst/prim/u + v; megakernel: kept in registers, the
-# 16 B of the RGBA32F sample is charged to accumulation), the path's segment id
-# and one RGBA32F accumulation read+write per pixel.
-NODE_B, TRI_B, INST_B, RAY_B, HIT_B, SEG_B, ACC_B = 64, 48, 64, 32, 20, 4, 32
+# Algorithmic bytes per unit of work: a four-child BVH node (four child boxes +
+# refs, one 128 B line), a triangle record, an instance traversal record, a ray
+# (origin + request tag, direction + path id), a hit record (wavefront:
+# t/inst/prim/u + v; megakernel: kept in registers, the 16 B of the RGBA32F sample
+# is charged to accumulation) and one RGBA32F accumulation read+write per pixel.
+NODE_B, TRI_B, INST_B, RAY_B, HIT_B, ACC_B = 128, 48, 64, 32, 20, 32
 
 
 def log(msg):
@@ -583,7 +583,7 @@ def main():
         frame_bytes = ((st["node_visits"] - st["shadow_node_visits"]) * NODE_B
                        + (st["tri_tests"] - st["shadow_tri_tests"]) * TRI_B
                        + (st["inst_visits"] - st["shadow_inst_visits"]) * INST_B
-                       + st["extension_rays"] * (RAY_B + SEG_B + HIT_B))
+                       + st["extension_rays"] * (RAY_B + HIT_B))
     else:
         # the megakernel does the whole frame: every query, and accumulation
         frame_bytes = (st["node_visits"] * NODE_B + st["tri_tests"] * TRI_B + st["inst_visits"] * INST_B
