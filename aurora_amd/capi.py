@@ -215,6 +215,9 @@ SIGNATURES = {
     "aph_materials_upload": (_i, [_P, _P, _u32]),
     "aph_material_defaults": (None, [C.POINTER(Material)]),
     "aph_environment_upload": (_i, [_P, C.POINTER(Environment)]),
+    "aph_environment_create": (_i, [_P, C.POINTER(Environment), C.POINTER(C.c_uint64)]),
+    "aph_environment_destroy": (_i, [_P, C.c_uint64]),
+    "aph_tlas_set_environment": (_i, [_P, C.c_uint64, C.c_uint64]),
     "aph_alias_build": (_i, [_P, _u32, _u32, _P, C.POINTER(C.c_float)]),
     "aph_trace_closest": (_i, [_P, _u64, _P, _u32, _P]),
     "aph_trace_any": (_i, [_P, _u64, _P, _u32, _P]),

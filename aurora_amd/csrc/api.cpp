@@ -78,7 +78,8 @@ void identity4(float* m)
 void setDefaultEnv(aph_ctx* c)
 {
     // EnvironmentBase.cpp:33-45 defaults.
-    DevEnv& e = c->env;
+    c->env.reset(new Environment());
+    DevEnv& e = c->env->env;
     std::memset(&e, 0, sizeof(e));
     e.lightTop = f3{1.0f, 1.0f, 1.0f};
     e.lightBottom = f3{0.0f, 0.0f, 0.0f};
@@ -642,14 +643,13 @@ int aph_alias_build(const float* rgb, uint32_t w, uint32_t h, void* entries, flo
     return APH_OK;
 }
 
-int aph_environment_upload(aph_ctx_t c, const aph_environment* e)
+// Builds the device copy of an environment (EnvironmentBase::updateGPUStruct,
+// EnvironmentBase.cpp:52-71, and the alias map of AliasMap.cpp:31-166) into `out`.
+static int buildEnvironment(aph_ctx_t c, const aph_environment* e, Environment& out)
 {
-    int rc = enter(c);
-    if (rc) return rc;
     if (!e) return fail(c, APH_E_INVALID, "environment is null");
     if (e->filter != 0 && e->filter != 1) return fail(c, APH_E_INVALID, "filter must be 0 or 1");
-    HIPCALL(c, hipStreamSynchronize(c->stream), "environment");
-    DevEnv d;
+    DevEnv& d = out.env;
     std::memset(&d, 0, sizeof(d));
     d.lightTop = f3{e->light_top[0], e->light_top[1], e->light_top[2]};
     d.lightBottom = f3{e->light_bottom[0], e->light_bottom[1], e->light_bottom[2]};
@@ -670,13 +670,13 @@ int aph_environment_upload(aph_ctx_t c, const aph_environment* e)
         std::vector<AliasEntry> alias(n);
         float integral = 0.0f;
         aliasBuild(e->light_rgb, w, h, alias.data(), &integral);
-        HIPCALL(c, c->envLight.alloc(n * 16), "alloc light image");
-        HIPCALL(c, c->envAlias.alloc(n * 16), "alloc alias map");
-        HIPCALL(c, hipMemcpy(c->envLight.p, tex.data(), n * 16, hipMemcpyHostToDevice), "copy light image");
-        HIPCALL(c, hipMemcpy(c->envAlias.p, alias.data(), n * 16, hipMemcpyHostToDevice), "copy alias map");
+        HIPCALL(c, out.light.alloc(n * 16), "alloc light image");
+        HIPCALL(c, out.alias.alloc(n * 16), "alloc alias map");
+        HIPCALL(c, hipMemcpy(out.light.p, tex.data(), n * 16, hipMemcpyHostToDevice), "copy light image");
+        HIPCALL(c, hipMemcpy(out.alias.p, alias.data(), n * 16, hipMemcpyHostToDevice), "copy alias map");
         d.hasLightTex = 1;
-        d.lightTex = c->envLight.as<float4>();
-        d.alias = c->envAlias.as<AliasEntry>();
+        d.lightTex = out.light.as<float4>();
+        d.alias = out.alias.as<AliasEntry>();
         d.lw = w;
         d.lh = h;
         d.lightIntegral = integral;
@@ -689,14 +689,68 @@ int aph_environment_upload(aph_ctx_t c, const aph_environment* e)
         for (size_t i = 0; i < n; i++)
             tex[i] = make_float4(e->background_rgb[3 * i], e->background_rgb[3 * i + 1], e->background_rgb[3 * i + 2],
                                  1.0f);
-        HIPCALL(c, c->envBg.alloc(n * 16), "alloc background image");
-        HIPCALL(c, hipMemcpy(c->envBg.p, tex.data(), n * 16, hipMemcpyHostToDevice), "copy background image");
+        HIPCALL(c, out.bg.alloc(n * 16), "alloc background image");
+        HIPCALL(c, hipMemcpy(out.bg.p, tex.data(), n * 16, hipMemcpyHostToDevice), "copy background image");
         d.hasBgTex = 1;
-        d.bgTex = c->envBg.as<float4>();
+        d.bgTex = out.bg.as<float4>();
         d.bw = w;
         d.bh = h;
     }
-    c->env = d;
+    return APH_OK;
+}
+
+int aph_environment_upload(aph_ctx_t c, const aph_environment* e)
+{
+    int rc = enter(c);
+    if (rc) return rc;
+    std::unique_ptr<Environment> env(new Environment());
+    if ((rc = buildEnvironment(c, e, *env))) return rc;
+    HIPCALL(c, hipStreamSynchronize(c->stream), "environment");  // renders in flight read the old one
+    c->env = std::move(env);
+    return APH_OK;
+}
+
+int aph_environment_create(aph_ctx_t c, const aph_environment* e, uint64_t* env)
+{
+    int rc = enter(c);
+    if (rc) return rc;
+    if (!env) return fail(c, APH_E_INVALID, "environment output pointer is null");
+    *env = 0;
+    std::unique_ptr<Environment> en(new Environment());
+    if ((rc = buildEnvironment(c, e, *en))) return rc;
+    uint64_t h = c->nextHandle++;
+    c->environments[h] = std::move(en);
+    *env = h;
+    return APH_OK;
+}
+
+int aph_environment_destroy(aph_ctx_t c, uint64_t env)
+{
+    int rc = enter(c);
+    if (rc) return rc;
+    Environment* en = lookup(c->environments, env);
+    if (!en) return fail(c, APH_E_INVALID, "unknown environment handle %llu", (unsigned long long)env);
+    for (const auto& kv : c->tlases)
+        if (kv.second->environment == en)
+            return fail(c, APH_E_STATE, "environment %llu is set on TLAS %llu", (unsigned long long)env,
+                        (unsigned long long)kv.first);
+    HIPCALL(c, hipStreamSynchronize(c->stream), "environment");
+    c->environments.erase(env);
+    return APH_OK;
+}
+
+int aph_tlas_set_environment(aph_ctx_t c, uint64_t tlas, uint64_t env)
+{
+    int rc = enter(c);
+    if (rc) return rc;
+    Tlas* tl = lookup(c->tlases, tlas);
+    if (!tl) return fail(c, APH_E_INVALID, "unknown tlas handle %llu", (unsigned long long)tlas);
+    const Environment* en = nullptr;
+    if (env) {
+        en = lookup(c->environments, env);
+        if (!en) return fail(c, APH_E_INVALID, "unknown environment handle %llu", (unsigned long long)env);
+    }
+    tl->environment = en;
     return APH_OK;
 }
 

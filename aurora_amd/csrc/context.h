@@ -59,8 +59,17 @@ struct Mesh {
     BuildInfo info{};
 };
 
+// A lat-long environment resident on the device: light image + alias map,
+// background image, and the DevEnv record that points at them.
+struct Environment {
+    DevBuf light, alias, bg;
+    DevEnv env{};
+    size_t bytes() const { return light.bytes + alias.bytes + bg.bytes; }
+};
+
 struct Tlas {
     uint32_t n = 0, nvalid = 0;
+    const Environment* environment = nullptr;  // aph_tlas_set_environment; null: the context's
     DevBuf nodes, nodes4, trav, shade;
     TraceScene sc{};
     DevBuf layerMats;  // material indices of the instances' layers (aph_tlas_set_layers)
@@ -126,8 +135,8 @@ struct aph_ctx {
     std::vector<uint32_t> texturesMissing;         // material * APH_TEX_SLOTS + slot: flagged, unbound
     DevBuf matTex, textures;                       // DevMatTex per material, DevTexture table
     std::map<uint64_t, std::unique_ptr<Image>> images;
-    DevBuf envLight, envAlias, envBg;
-    aph::DevEnv env{};
+    std::unique_ptr<aph::Environment> env;  // aph_environment_upload: scenes without their own
+    std::map<uint64_t, std::unique_ptr<aph::Environment>> environments;
     DevBuf counters;
     int statsLevel = 0;
     DevBuf scratchA, scratchB;
@@ -190,7 +199,7 @@ inline RenderScene renderScene(const aph_ctx* c, const Tlas* tl)
     rs.nmat = c->nmat;
     rs.layerMats = tl->layerMats.as<uint32_t>();
     rs.pad_ = 0;
-    rs.env = c->env;
+    rs.env = tl->environment ? tl->environment->env : c->env->env;
     return rs;
 }
 

@@ -216,10 +216,12 @@ int aph_server_submit(aph_server_t s, const aph_job* job, double tArrival, uint6
     if (job->frame.light_count < 0 || job->frame.light_count > 4)
         return fail(c, APH_E_INVALID, "job: light_count must be in [0,4]");
     if (!s->resident.count(tl)) {
-        if (s->cfg.hbm_budget && s->residentBytes + tl->bytes > s->cfg.hbm_budget)
-            return fail(c, APH_E_MEMORY, "scene of %zu bytes does not fit the budget", tl->bytes);
+        // BVHs, instance and shading records, and the scene's own environment images
+        const size_t bytes = tl->bytes + (tl->environment ? tl->environment->bytes() : 0);
+        if (s->cfg.hbm_budget && s->residentBytes + bytes > s->cfg.hbm_budget)
+            return fail(c, APH_E_MEMORY, "scene of %zu bytes does not fit the budget", bytes);
         s->resident.insert(tl);
-        s->residentBytes += tl->bytes;
+        s->residentBytes += bytes;
     }
     std::unique_ptr<Job> j(new Job());
     j->ticket = s->nextTicket++;

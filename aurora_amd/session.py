@@ -182,13 +182,28 @@ class Renderer:
         return {(mi + mat_base, k): (handles[ii], au, av) for (mi, k), (ii, au, av) in textures.items()}
 
     def environment(self, env):
+        """Sets the context's environment (scenes without one of their own)."""
         e, keep = scenes.environment_struct(env)
         self._check(self.lib.aph_environment_upload(self.ctx, C.byref(e)))
 
+    def create_environment(self, env):
+        """A resident environment object; returns its handle."""
+        e, keep = scenes.environment_struct(env)
+        h = C.c_uint64()
+        self._check(self.lib.aph_environment_create(self.ctx, C.byref(e), C.byref(h)))
+        return h.value
+
+    def destroy_environment(self, handle):
+        self._check(self.lib.aph_environment_destroy(self.ctx, handle))
+
+    def set_environment(self, tlas, handle):
+        """Renders of `tlas` use environment `handle` (0: the context's)."""
+        self._check(self.lib.aph_tlas_set_environment(self.ctx, tlas, handle))
+
     def load_scenes(self, scs, builder=capi.APH_BUILDER_LBVH):
         """Uploads several scenes into this context, resident together: one material
-        table (each scene's instance material indices offset into it) and the first
-        scene's environment.  Returns one TLAS handle per scene."""
+        table (each scene's instance material indices offset into it) and one
+        environment object per scene, set on its TLAS.  Returns one TLAS handle per scene."""
         mats, tl, slots = [], [], {}
         for sc in scs:
             handles = []
@@ -207,7 +222,8 @@ class Renderer:
         self.materials(mats)
         if slots:
             self.bind_textures(len(mats), slots)
-        self.environment(scs[0].env or {})
+        for sc, t in zip(scs, tl):
+            self.set_environment(t, self.create_environment(sc.env or {}))
         return tl
 
     def load_scene(self, sc, builder=capi.APH_BUILDER_LBVH):

@@ -269,6 +269,16 @@ int aph_materials_bind_textures(aph_ctx_t ctx, const aph_texture_binding* bindin
 /* ---- environment: EnvironmentBase::updateGPUStruct (EnvironmentBase.cpp:52-71)
  *      + AliasMap::build (AliasMap.cpp:31-166) ---- */
 int aph_environment_upload(aph_ctx_t ctx, const aph_environment* env);
+/* The environment as a resident object of its own, for scenes that differ in it
+ * (IScene::setEnvironment, Aurora.h:974, SceneBase.h:81: each Aurora scene
+ * holds one).
+ * aph_tlas_set_environment(ctx, tlas, env) makes renders of that TLAS use it;
+ * env 0 goes back to the context's (aph_environment_upload).  A request batch
+ * may mix TLASes with different environments.  An environment set on a TLAS
+ * cannot be destroyed (APH_E_STATE). */
+int aph_environment_create(aph_ctx_t ctx, const aph_environment* env, uint64_t* handle);
+int aph_environment_destroy(aph_ctx_t ctx, uint64_t handle);
+int aph_tlas_set_environment(aph_ctx_t ctx, uint64_t tlas, uint64_t env);
 /* Alias table + luminance integral for an RGB32F lat-long image (host build,
  * the AliasMap::build algorithm); entries are 16 B {alias, prob, pdf, pad}. */
 int aph_alias_build(const float* rgb, uint32_t w, uint32_t h, void* entries, float* integral);

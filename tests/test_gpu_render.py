@@ -15,6 +15,7 @@ import pytest
 
 import pyoracle
 from aurora_amd import capi, scenes
+from aurora_amd.session import AphError
 
 pytestmark = pytest.mark.gpu
 
@@ -227,6 +228,39 @@ def test_graph_replays_equal_eager_launches(renderer):
     _, tuned = render()
     np.testing.assert_array_equal(tuned, eager)
     renderer.set_tuning(4 << 20, 16, 8)
+
+
+def test_scenes_keep_their_own_environments(renderer):
+    """Two resident scenes with different lat-long environments, each set on its TLAS
+    (aph_environment_create / aph_tlas_set_environment), rendered together in one call:
+    each image equals that scene rendered alone with its environment as the context's,
+    bit for bit, in both modes.  An environment set on a TLAS cannot be destroyed."""
+    r = renderer
+    w, h = 64, 48
+    a = scenes.config1(seed=31, width=w, height=h, trace_depth=3)
+    b = scenes.config1(seed=32, width=w, height=h, trace_depth=3)
+    assert not np.array_equal(a.env["light_rgb"], b.env["light_rgb"])
+    ta, tb = r.load_scenes([a, b], capi.APH_BUILDER_SAH)
+    mixed = {}
+    for mode in (capi.APH_MODE_WAVEFRONT, capi.APH_MODE_MEGAKERNEL):
+        ga, gb = r.target(w, h), r.target(w, h)
+        r.render([r.request(ga, ta, a.frame, 0, 8), r.request(gb, tb, b.frame, 0, 8)], mode)
+        r.wait()
+        mixed[mode] = (r.readback(ga, w, h), r.readback(gb, w, h))
+    assert not np.array_equal(mixed[capi.APH_MODE_WAVEFRONT][0], mixed[capi.APH_MODE_WAVEFRONT][1])
+    e = r.create_environment(b.env)
+    r.set_environment(ta, e)
+    with pytest.raises(AphError):
+        r.destroy_environment(e)
+    r.set_environment(ta, 0)
+    r.destroy_environment(e)
+    for k, sc in enumerate((a, b)):
+        t = r.load_scene(sc, capi.APH_BUILDER_SAH)  # the scene's environment as the context's
+        for mode, imgs in mixed.items():
+            g = r.target(w, h)
+            r.render([r.request(g, t, sc.frame, 0, 8)], mode)
+            r.wait()
+            np.testing.assert_array_equal(r.readback(g, w, h), imgs[k])
 
 
 def test_autotune_keeps_results(renderer, teapot_path):
