@@ -35,6 +35,34 @@ __device__ __forceinline__ bf16x8 load8(const __bf16* p)
     return *reinterpret_cast<const bf16x8*>(p);
 }
 
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+
+// Output channels co..co+3 of pixel q (bias b, optional ReLU): one 8-byte bf16
+// or 16-byte fp32 store when all four are stored (channel offsets and strides
+// are multiples of 4), single stores for a partial group.
+template <bool RELU, bool OUT_F32>
+__device__ __forceinline__ void storeQuad(const ConvArgs& a, int64_t q, int co, f32x4 acc, const float* b)
+{
+    float v[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        v[e] = acc[e] + b[e];
+        if (RELU) v[e] = fmaxf(v[e], 0.0f);
+    }
+    const int64_t at = q * a.outStride + a.outOffset + co;
+    if (co + 4 <= a.coutStore) {
+        if (OUT_F32) *reinterpret_cast<float4*>(a.outF + at) = make_float4(v[0], v[1], v[2], v[3]);
+        else *reinterpret_cast<bf16x4*>(a.out + at) = bf16x4{toBf(v[0]), toBf(v[1]), toBf(v[2]), toBf(v[3])};
+    } else {
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            if (co + e >= a.coutStore) continue;
+            if (OUT_F32) a.outF[at + e] = v[e];
+            else a.out[at + e] = toBf(v[e]);
+        }
+    }
+}
+
 // ---- convolution (forward, and data gradient with flipped weights) ----
 // One wave: P subtiles of 16 consecutive pixels x T subtiles of 16 output
 // channels (P * T accumulators); each A fragment (8 input channels of one
@@ -94,28 +122,25 @@ __global__ __launch_bounds__(256) void kConv3x3(ConvArgs a)
                 if (j < nTiles) {
 #pragma unroll
                     for (int i = 0; i < P; i++)
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bv[j], acc[i][j], 0, 0, 0);
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bv[j], av[i], acc[i][j], 0, 0, 0);
                 }
             }
         }
     }
+    // D of (j, i), operands swapped: rows = 16 output channels, columns = the
+    // 16 pixels of subtile i; lane (row, kg) holds channels 4kg..4kg+3 of pixel row.
 #pragma unroll
     for (int j = 0; j < T; j++) {
         if (j >= nTiles) continue;
-        const int co = coBase + j * 16 + row;
+        const int co = coBase + j * 16 + kg * 4;
         if (co >= a.coutStore) continue;
-        const float b = a.bias ? a.bias[co] : 0.0f;
+        float b[4];
+#pragma unroll
+        for (int e = 0; e < 4; e++) b[e] = a.bias ? a.bias[co + e] : 0.0f;
 #pragma unroll
         for (int i = 0; i < P; i++) {
-#pragma unroll
-            for (int e = 0; e < 4; e++) {
-                const int64_t q = p0 + i * 16 + kg * 4 + e;
-                if (q >= npix) continue;
-                float v = acc[i][j][e] + b;
-                if (RELU) v = fmaxf(v, 0.0f);
-                if (OUT_F32) a.outF[q * a.outStride + a.outOffset + co] = v;
-                else a.out[q * a.outStride + a.outOffset + co] = toBf(v);
-            }
+            const int64_t q = p0 + i * 16 + row;
+            if (q < npix) storeQuad<RELU, OUT_F32>(a, q, co, acc[i][j], b);
         }
     }
 }
@@ -147,26 +172,49 @@ __global__ __launch_bounds__(256) void kConvLds(ConvArgs a)
     for (int i = 0; i < P; i++)
 #pragma unroll
         for (int j = 0; j < T; j++) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    for (int c0 = 0; c0 < a.cinP; c0 += 32) {
-        // stage the haloed input tile (4 segments of 8 channels per pixel)
-        for (int sgi = tid; sgi < HR * HC * 4; sgi += 256) {
+    // Staging of one 32-channel chunk: every thread's segments of the haloed
+    // input tile (4 x 8 channels per pixel) and of the 9-tap weights are loaded
+    // into registers in one batch, then written to LDS, so a block pays one
+    // global-memory latency per chunk; the next chunk's loads are issued before
+    // the current chunk's MFMAs.
+    constexpr int NSEG = HR * HC * 4, NIT = (NSEG + 255) / 256;
+    constexpr int NWSEG = 9 * 16 * T * 4, NWIT = (NWSEG + 255) / 256;
+    bf16x8 sv[NIT], sw[NWIT];
+    auto fetch = [&](int c0) {
+#pragma unroll
+        for (int k = 0; k < NIT; k++) {
+            const int sgi = tid + k * 256;
             const int px = sgi >> 2, part = sgi & 3;
             const int hy = px / HC, hx = px - hy * HC;
             const int gy = ty0 + hy - 1, gx = tx0 + hx - 1;
-            bf16x8 v{};
-            if (gy >= 0 && gy < a.H && gx >= 0 && gx < a.W)
-                v = load8(a.in + (((int64_t)n * a.H + gy) * a.W + gx) * a.inStride + c0 + part * 8);
-            *reinterpret_cast<bf16x8*>(&tile[px * kLdsPix + part * 8]) = v;
+            sv[k] = bf16x8{};
+            if (sgi < NSEG && gy >= 0 && gy < a.H && gx >= 0 && gx < a.W)
+                sv[k] = load8(a.in + (((int64_t)n * a.H + gy) * a.W + gx) * a.inStride + c0 + part * 8);
         }
-        // stage the chunk's weights: 9 taps x 16*T output channels x 32 inputs
-        for (int sgi = tid; sgi < 9 * 16 * T * 4; sgi += 256) {
+#pragma unroll
+        for (int k = 0; k < NWIT; k++) {
+            const int sgi = tid + k * 256;
             const int r = sgi >> 2, part = sgi & 3;
             const int t = r / (16 * T), co = r - t * (16 * T);
-            bf16x8 v{};
-            if (co < nTiles * 16) v = load8(a.w + ((int64_t)t * a.coutP + coBase + co) * a.cinP + c0 + part * 8);
-            *reinterpret_cast<bf16x8*>(&wts[r * kLdsPix + part * 8]) = v;
+            sw[k] = bf16x8{};
+            if (sgi < NWSEG && co < nTiles * 16)
+                sw[k] = load8(a.w + ((int64_t)t * a.coutP + coBase + co) * a.cinP + c0 + part * 8);
+        }
+    };
+    fetch(0);
+    for (int c0 = 0; c0 < a.cinP; c0 += 32) {
+#pragma unroll
+        for (int k = 0; k < NIT; k++) {
+            const int sgi = tid + k * 256;
+            if (sgi < NSEG) *reinterpret_cast<bf16x8*>(&tile[(sgi >> 2) * kLdsPix + (sgi & 3) * 8]) = sv[k];
+        }
+#pragma unroll
+        for (int k = 0; k < NWIT; k++) {
+            const int sgi = tid + k * 256;
+            if (sgi < NWSEG) *reinterpret_cast<bf16x8*>(&wts[(sgi >> 2) * kLdsPix + (sgi & 3) * 8]) = sw[k];
         }
         __syncthreads();
+        if (c0 + 32 < a.cinP) fetch(c0 + 32);
 #pragma unroll 1
         for (int t = 0; t < 9; t++) {
             const int dy = t / 3, dx = t % 3;  // halo offsets of the tap
@@ -183,33 +231,28 @@ __global__ __launch_bounds__(256) void kConvLds(ConvArgs a)
                 if (j < nTiles) {
 #pragma unroll
                     for (int i = 0; i < P; i++)
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bv[j], acc[i][j], 0, 0, 0);
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bv[j], av[i], acc[i][j], 0, 0, 0);
                 }
             }
         }
         __syncthreads();
     }
-    // D of subtile i: rows = the 16 pixels of tile row wave*P + i, columns = channels
+    // D of (j, i), operands swapped: rows = 16 output channels, columns = the 16
+    // pixels of tile row wave*P + i; lane (row, kg) holds channels 4kg..4kg+3 of
+    // pixel tx0 + row.
+    const int x = tx0 + row;
 #pragma unroll
     for (int j = 0; j < T; j++) {
         if (j >= nTiles) continue;
-        const int co = coBase + j * 16 + row;
-        if (co >= a.coutStore) continue;
-        const float b = a.bias ? a.bias[co] : 0.0f;
+        const int co = coBase + j * 16 + kg * 4;
+        if (co >= a.coutStore || x >= a.W) continue;
+        float b[4];
+#pragma unroll
+        for (int e = 0; e < 4; e++) b[e] = a.bias ? a.bias[co + e] : 0.0f;
 #pragma unroll
         for (int i = 0; i < P; i++) {
             const int y = ty0 + wave * P + i;
-            if (y >= a.H) continue;
-#pragma unroll
-            for (int e = 0; e < 4; e++) {
-                const int x = tx0 + kg * 4 + e;
-                if (x >= a.W) continue;
-                const int64_t q = ((int64_t)n * a.H + y) * a.W + x;
-                float v = acc[i][j][e] + b;
-                if (RELU) v = fmaxf(v, 0.0f);
-                if (OUT_F32) a.outF[q * a.outStride + a.outOffset + co] = v;
-                else a.out[q * a.outStride + a.outOffset + co] = toBf(v);
-            }
+            if (y < a.H) storeQuad<RELU, OUT_F32>(a, ((int64_t)n * a.H + y) * a.W + x, co, acc[i][j], b);
         }
     }
 }
