@@ -99,6 +99,8 @@ def evaluate(r, dn, examples, size):
 
 def train(r, args, train_set, held, loss_name, out_path, log):
     dn = Denoiser(r, seed=args.init_seed)
+    if args.resume:
+        dn.load(args.resume)  # fine-tuning: start from saved weights (fresh Adam moments)
     dn.set_loss(LOSSES[loss_name])
     rng = np.random.Generator(np.random.PCG64(args.seed))
     warm = max(1, args.steps // 40)
@@ -118,12 +120,21 @@ def train(r, args, train_set, held, loss_name, out_path, log):
             log(f"[{loss_name}] step {step + 1}/{args.steps} lr {lr:.2e} loss {np.mean(losses[-500:]):.6f} "
                 f"({time.perf_counter() - t0:.1f} s)")
     t_train = time.perf_counter() - t0
+    resumed = None
+    if args.resume:
+        prior = os.path.splitext(args.resume)[0] + ".json"
+        resumed = {"weights": os.path.basename(args.resume)}
+        if os.path.exists(prior):
+            with open(prior) as f:
+                p = json.load(f)
+            resumed.update({k: p[k] for k in ("steps", "batch", "lr", "resumed_from") if k in p})
     ev = evaluate(r, dn, held, args.size)
     log(f"[{loss_name}] held-out: " + json.dumps({k: round(v, 5) for k, v in ev.items()}))
     dn.save(out_path)
     summary = dict(scenes=args.scenes, holdout=args.holdout, size=args.size, ref_spp=args.ref_spp,
                    noisy_spp=[4, 4, 1], loss=loss_name, augment=bool(args.augment), steps=args.steps,
                    batch=args.batch, lr=args.lr, seed=args.seed, init_seed=args.init_seed,
+                   resumed_from=resumed,
                    train_seconds=round(t_train, 1),
                    loss_curve=[float(np.mean(losses[i:i + 500])) for i in range(0, len(losses), 500)],
                    held_out={k: round(v, 6) for k, v in ev.items()})
@@ -150,6 +161,9 @@ def main():
     ap.add_argument("--no-augment", dest="augment", action="store_false",
                     help="no random mirroring / transposition of the training pairs")
     ap.add_argument("--out", default=DEFAULT_WEIGHTS)
+    ap.add_argument("--resume", default=None,
+                    help="weights (.npy) to start from instead of a random initialisation; the schedule "
+                         "(warm-up, cosine decay from --lr) runs again over --steps")
     args = ap.parse_args()
 
     def log(msg):
