@@ -336,21 +336,26 @@ __device__ __forceinline__ bool traceRound(TraceState& s, const TraceScene& sc, 
         const float4 lx = gload4(np), ly = gload4(np + 1), lz = gload4(np + 2);
         const float4 hx = gload4(np + 3), hy = gload4(np + 4), hz = gload4(np + 5), rf = gload4(np + 6);
         if (STATS) st.nodes++;
-        // Four box tests; a missed child becomes kEmptyChild with key +inf.
+        // Four box tests.  A missed child gets key +inf and a hit one its entry
+        // distance capped at the largest float, so after sorting the h hit children come
+        // first, nearest first.
         float k0, k1, k2, k3;
-        int32_t r0 = slab(s.br, make_float4(lx.x, ly.x, lz.x, 0.0f), make_float4(hx.x, hy.x, hz.x, 0.0f), s.tmin,
-                          s.tlim, k0) ? __float_as_int(rf.x) : kEmptyChild;
-        int32_t r1 = slab(s.br, make_float4(lx.y, ly.y, lz.y, 0.0f), make_float4(hx.y, hy.y, hz.y, 0.0f), s.tmin,
-                          s.tlim, k1) ? __float_as_int(rf.y) : kEmptyChild;
-        int32_t r2 = slab(s.br, make_float4(lx.z, ly.z, lz.z, 0.0f), make_float4(hx.z, hy.z, hz.z, 0.0f), s.tmin,
-                          s.tlim, k2) ? __float_as_int(rf.z) : kEmptyChild;
-        int32_t r3 = slab(s.br, make_float4(lx.w, ly.w, lz.w, 0.0f), make_float4(hx.w, hy.w, hz.w, 0.0f), s.tmin,
-                          s.tlim, k3) ? __float_as_int(rf.w) : kEmptyChild;
-        if (r0 == kEmptyChild) k0 = INFINITY;
-        if (r1 == kEmptyChild) k1 = INFINITY;
-        if (r2 == kEmptyChild) k2 = INFINITY;
-        if (r3 == kEmptyChild) k3 = INFINITY;
-        // Nearest first: a five-comparator sorting network on (key, child).
+        const bool b0 = slab(s.br, make_float4(lx.x, ly.x, lz.x, 0.0f), make_float4(hx.x, hy.x, hz.x, 0.0f), s.tmin,
+                             s.tlim, k0);
+        const bool b1 = slab(s.br, make_float4(lx.y, ly.y, lz.y, 0.0f), make_float4(hx.y, hy.y, hz.y, 0.0f), s.tmin,
+                             s.tlim, k1);
+        const bool b2 = slab(s.br, make_float4(lx.z, ly.z, lz.z, 0.0f), make_float4(hx.z, hy.z, hz.z, 0.0f), s.tmin,
+                             s.tlim, k2);
+        const bool b3 = slab(s.br, make_float4(lx.w, ly.w, lz.w, 0.0f), make_float4(hx.w, hy.w, hz.w, 0.0f), s.tmin,
+                             s.tlim, k3);
+        int32_t r0 = __float_as_int(rf.x), r1 = __float_as_int(rf.y), r2 = __float_as_int(rf.z),
+                r3 = __float_as_int(rf.w);
+        k0 = b0 ? fminf(k0, 3.40282347e38f) : INFINITY;
+        k1 = b1 ? fminf(k1, 3.40282347e38f) : INFINITY;
+        k2 = b2 ? fminf(k2, 3.40282347e38f) : INFINITY;
+        k3 = b3 ? fminf(k3, 3.40282347e38f) : INFINITY;
+        const int h = (int)b0 + (int)b1 + (int)b2 + (int)b3;
+        // A five-comparator sorting network on (key, child).
         auto cswap = [](float& ka, int32_t& ra, float& kb, int32_t& rb) {
             const bool sw = kb < ka;
             const float kt = sw ? kb : ka;
@@ -365,18 +370,13 @@ __device__ __forceinline__ bool traceRound(TraceState& s, const TraceScene& sc, 
         cswap(k0, r0, k2, r2);
         cswap(k1, r1, k3, r3);
         cswap(k1, r1, k2, r2);
-        // Far to near: every hit child but the nearest is pushed, the nearest goes on.
-        int32_t next = kEmptyChild;
-        const int32_t rs[4] = {r3, r2, r1, r0};
+        // Far to near: the h - 1 farther hit children are pushed, the nearest goes on.
         bool ok = true;
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            if (rs[j] == kEmptyChild) continue;
-            if (next != kEmptyChild) ok = ok && stackPush(s, m, next);
-            next = rs[j];
-        }
+        if (h > 3) ok = stackPush(s, m, r3);
+        if (h > 2) ok = ok && stackPush(s, m, r2);
+        if (h > 1) ok = ok && stackPush(s, m, r1);
         if (!ok) { st.overflow = 1; return true; }
-        s.cur = next != kEmptyChild ? next : stackPop(s, m);
+        s.cur = h > 0 ? r0 : stackPop(s, m);
         if (s.cur < 0 && s.leaf == 0) {
             s.leaf = s.cur;
             s.cur = stackPop(s, m);
