@@ -393,6 +393,48 @@ def config5_sequence(r, builder, frames=120, spp=4, width=1920, height=1080):
             "fps": round(1e3 / float(np.mean(lat)), 1)}
 
 
+def config5_quality(r, builder, frames=120, spp=4, width=1920, height=1080, score=(29, 59, 89, 119), ref_spp=1024):
+    """Image quality of the config-5 sequence: the same per-frame pipeline as
+    config5_sequence, and at the frames in `score` the 4-spp render, the denoised
+    frame and the temporally resolved output are compared with a `ref_spp` render of
+    the same camera (disjoint seeds).  RMSE of linear radiance and of x / (1 + x)."""
+    sc = scenes.config5(width=width, height=height)
+    tlas = r.load_scene(sc, builder)
+    cams = scenes.camera_spline(sc.seed, frames, width, height)
+    dn = dnmod.Denoiser(r)
+    dn.load()
+    noisy, den, aov = r.target(width, height), r.target(width, height), r.aov(width, height)
+    hist = [r.target(width, height), r.target(width, height)]
+    kept = {}
+    prev = cams[0]
+    for i, fr in enumerate(cams):
+        r.render([r.request(noisy, tlas, fr, 0, spp, i)], capi.APH_MODE_WAVEFRONT)
+        r.render_aovs(aov, tlas, fr, scenes.view_proj(fr), scenes.view_proj(prev),
+                      np.array(prev.inv_view, dtype=np.float32))
+        dn.denoise([(noisy, aov, den)])
+        r.temporal_resolve(den, aov, hist[i & 1], hist[(i + 1) & 1], is_reset=(i == 0))
+        if i in score:
+            r.wait()
+            kept[i] = [r.readback(t, width, height)[..., :3].astype(np.float64) for t in (noisy, den, hist[(i + 1) & 1])]
+        prev = fr
+    dn.close()
+    tm = lambda x: np.maximum(x, 0) / (1.0 + np.maximum(x, 0))  # noqa: E731
+    ref = r.target(width, height)
+    per = {}
+    for i, imgs in kept.items():
+        r.render([r.request(ref, tlas, cams[i], 0, ref_spp, 1_000_003 + i)], capi.APH_MODE_WAVEFRONT)
+        r.wait()
+        g = r.readback(ref, width, height)[..., :3].astype(np.float64)
+        per[i] = {name: [round(float(np.sqrt(np.mean((x - g) ** 2))), 5),
+                         round(float(np.sqrt(np.mean((tm(x) - tm(g)) ** 2))), 5)]
+                  for name, x in zip(("noisy", "denoised", "resolved"), imgs)}
+        log(f"config5 quality frame {i}: {per[i]}")
+    mean = {name: [round(float(np.mean([per[i][name][k] for i in per])), 5) for k in (0, 1)]
+            for name in ("noisy", "denoised", "resolved")}
+    return {"frames_scored": list(score), "reference_spp": ref_spp, "spp": spp, "resolution": f"{width}x{height}",
+            "rmse": "[linear, tonemapped x/(1+x)]", "mean": mean, "per_frame": {str(i): v for i, v in per.items()}}
+
+
 def denoiser_flops(width, height):
     """Algorithmic work of one denoise: 2 * 9 * cin * cout multiply-adds per output
     pixel of every convolution, over the real (unpadded) channels."""
@@ -511,7 +553,8 @@ def main():
                     help="extra measurements after the headline line's frame: literal (config2 at subdiv 5), "
                          "c3 (64 cameras), serving (Poisson replay), rmse (equal-time RMSE), denoise (denoised "
                          "4 spp vs plain 256 spp at 1080p), config5 (24.7M-triangle 120-frame sequence with "
-                         "denoiser and temporal resolve); '' for none")
+                         "denoiser and temporal resolve); config5q (not by default: that sequence's frames "
+                         "against 1024-spp references); '' for none")
     ap.add_argument("--tuning", default="",
                     help="wavefront launch configuration 'chunk_paths,trace_blocks_per_cu,shade_blocks_per_cu'; "
                          "default: the autotuner's choice on the headline frame (16 spp, child process)")
@@ -686,6 +729,9 @@ def main():
     if "config5" in sections:
         out["config5_sequence"] = config5_sequence(r, builder)
         log(f"config5: {out['config5_sequence']}")
+    if "config5q" in sections:
+        out["config5_quality"] = config5_quality(r, builder)
+        log(f"config5 quality: {out['config5_quality']['mean']}")
     if "denoise" in sections:
         out["denoise"] = denoise_vs_plain(r, builder,
                                           weights=[x for x in args.denoiser_weights.split(",") if x] or None)
