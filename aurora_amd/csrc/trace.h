@@ -123,7 +123,10 @@ __device__ __forceinline__ BoxRay makeBoxRayPadded(f3 o, f3 d, float pad)
 
 // Robust slab test.  A zero direction component gives +-inf distances, or NaN
 // when the origin lies on the plane; fmaxf/fminf ignore NaN, which treats that
-// plane as not limiting - the correct answer for a ray inside the slab.
+// plane as not limiting - the correct answer for a ray inside the slab.  The
+// inverted box of an unused four-child slot (lo = 3e38 > hi = -3e38) gives a
+// near distance of +huge or +inf on every axis for any finite origin, so it is
+// never accepted and its kEmptyChild reference never followed.
 __device__ __forceinline__ bool slab(const BoxRay& r, float4 lo, float4 hi, float tmin, float tmax, float& tnear)
 {
     float tlx = (lo.x - r.lx) * r.ix, thx = (hi.x - r.hx) * r.ix;
@@ -284,9 +287,11 @@ __device__ __forceinline__ int32_t stackPop(TraceState& s, const StackMem& m)
     return v;
 }
 
-// Starts a ray.  Returns false when it is finished at once: an empty scene, or
-// a ray with a non-finite component, which misses (every triangle test rejects
-// it, but the NaN-ignoring slab test would accept every box).
+// Starts a ray.  Returns false when it is finished at once: an empty scene, a
+// ray with a non-finite component (every triangle test rejects it, but the
+// NaN-ignoring slab test would accept every box), or a zero direction (NaN shear
+// constants: every triangle test rejects it; a distant light whose BSDF value
+// is black queues such a shadow ray).
 __device__ __forceinline__ bool traceBegin(TraceState& s, const TraceScene& sc, f3 org, f3 dir, float tmin,
                                            float tmax)
 {
@@ -298,6 +303,7 @@ __device__ __forceinline__ bool traceBegin(TraceState& s, const TraceScene& sc, 
     if (!(isfinite(org.x) && isfinite(org.y) && isfinite(org.z) && isfinite(dir.x) && isfinite(dir.y) &&
           isfinite(dir.z)))
         return false;
+    if (dir.x == 0.0f && dir.y == 0.0f && dir.z == 0.0f) return false;
     s.org = org;
     s.dir = dir;
     s.tmin = tmin;

@@ -187,6 +187,28 @@ def test_degenerate_inputs(renderer, builder):
 
 
 @pytest.mark.parametrize("builder", BUILDERS)
+def test_zero_direction_rays_miss(renderer, builder):
+    """Rays with a zero direction (a distant light whose BSDF value is black leaves one
+    in the shadow queue) hit nothing and are not occluded, as in the oracle, whose
+    watertight test gets NaN shear constants for them; the scene's four-child nodes
+    have unused slots, which such a ray must never follow."""
+    sc = scenes.config1(seed=3, width=8, height=8)
+    tlas = renderer.load_scene(sc, builder)
+    ora = pyoracle.OracleScene(sc)
+    rng = np.random.default_rng(5)
+    rays = np.zeros(24, dtype=capi.RAY_DTYPE)
+    rays["org"] = rng.uniform(-5.0, 5.0, (24, 3)).astype(np.float32)
+    rays["org"][:8, 1] = 0.0  # on the ground plane
+    signs = np.where(rng.random((24, 3)) < 0.5, -0.0, 0.0).astype(np.float32)
+    rays["dir"] = signs
+    rays["tmax"] = np.inf
+    g = renderer.trace_closest(tlas, rays)
+    assert (g["inst"] == 0xFFFFFFFF).all()
+    assert_hits_equal(g, ora.trace_closest(rays))
+    np.testing.assert_array_equal(renderer.trace_any(tlas, rays), ora.trace_any(rays))
+
+
+@pytest.mark.parametrize("builder", BUILDERS)
 def test_transformed_instances_scaled_rotated(renderer, builder):
     """Non-uniform scale + rotation + translation; instance IDs and t must match the oracle."""
     rng = np.random.default_rng(11)
