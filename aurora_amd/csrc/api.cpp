@@ -911,6 +911,7 @@ int aph_get_tuning(aph_ctx_t c, aph_tuning* t)
     t->chunk_paths = c->wf.tuning.chunkPaths;
     t->trace_blocks_per_cu = c->wf.tuning.tracePerCU;
     t->shade_blocks_per_cu = c->wf.tuning.shadePerCU;
+    t->trace_refill_lanes = c->wf.tuning.refillLanes;
     t->ms = 0.0f;
     return APH_OK;
 }
@@ -920,12 +921,14 @@ int aph_set_tuning(aph_ctx_t c, const aph_tuning* t)
     int rc = enter(c);
     if (rc) return rc;
     if (!t || t->chunk_paths < (1u << 16) || t->chunk_paths > (1u << 26) || t->trace_blocks_per_cu == 0 ||
-        t->trace_blocks_per_cu > 64 || t->shade_blocks_per_cu == 0 || t->shade_blocks_per_cu > 64)
+        t->trace_blocks_per_cu > 64 || t->shade_blocks_per_cu == 0 || t->shade_blocks_per_cu > 64 ||
+        t->trace_refill_lanes == 0 || t->trace_refill_lanes > 64)
         return fail(c, APH_E_INVALID, "tuning out of range");
     HIPCALL(c, hipStreamSynchronize(c->stream), "tuning");
     c->wf.tuning.chunkPaths = t->chunk_paths;
     c->wf.tuning.tracePerCU = t->trace_blocks_per_cu;
     c->wf.tuning.shadePerCU = t->shade_blocks_per_cu;
+    c->wf.tuning.refillLanes = t->trace_refill_lanes;
     return APH_OK;
 }
 
@@ -938,6 +941,7 @@ int aph_wavefront_autotune(aph_ctx_t c, const aph_request* reqs, uint32_t nreq, 
     const uint32_t chunks[] = {4u << 20, 8u << 20, 16u << 20, 32u << 20};
     const uint32_t traces[] = {4, 8, 16};
     const uint32_t shades[] = {4, 8, 16};
+    const uint32_t refills[] = {16, 24, 32, 40, 48, 56, 64};
     aph_tuning keep{};
     aph_get_tuning(c, &keep);
     aph_tuning win = keep;
@@ -945,31 +949,34 @@ int aph_wavefront_autotune(aph_ctx_t c, const aph_request* reqs, uint32_t nreq, 
     hipEvent_t a, b;
     HIPCALL(c, hipEventCreate(&a), "autotune");
     HIPCALL(c, hipEventCreate(&b), "autotune");
-    int status = APH_OK;
-    for (uint32_t ch : chunks) {
-        for (uint32_t tr : traces) {
-            for (uint32_t sh : shades) {
-                aph_tuning t{ch, tr, sh, 0.0f};
-                if ((status = aph_set_tuning(c, &t)) != APH_OK) break;
-                // first run captures the graph for this launch shape; the second is timed
-                if ((status = aph_render(c, reqs, nreq, APH_MODE_WAVEFRONT)) != APH_OK) break;
-                if (hipEventRecord(a, c->stream) != hipSuccess) { status = APH_E_DEVICE; break; }
-                if ((status = aph_render(c, reqs, nreq, APH_MODE_WAVEFRONT)) != APH_OK) break;
-                if (hipEventRecord(b, c->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess) {
-                    status = APH_E_DEVICE;
-                    break;
-                }
-                float ms = 0.0f;
-                hipEventElapsedTime(&ms, a, b);
-                if (ms < win.ms) {
-                    win = t;
-                    win.ms = ms;
-                }
-            }
-            if (status) break;
+    // Renders the requests twice under t (the first run captures the graph for
+    // this launch shape) and times the second; keeps t if it is the fastest.
+    auto trial = [&](aph_tuning t) -> int {
+        int st;
+        if ((st = aph_set_tuning(c, &t)) != APH_OK) return st;
+        if ((st = aph_render(c, reqs, nreq, APH_MODE_WAVEFRONT)) != APH_OK) return st;
+        if (hipEventRecord(a, c->stream) != hipSuccess) return APH_E_DEVICE;
+        if ((st = aph_render(c, reqs, nreq, APH_MODE_WAVEFRONT)) != APH_OK) return st;
+        if (hipEventRecord(b, c->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess) return APH_E_DEVICE;
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, a, b);
+        if (ms < win.ms) {
+            win = t;
+            win.ms = ms;
         }
-        if (status) break;
-    }
+        return APH_OK;
+    };
+    int status = APH_OK;
+    // Stage 1: chunk size x traversal grid x shade grid at the current refill
+    // threshold; stage 2: the refill threshold at the stage-1 winner.
+    for (uint32_t ch : chunks)
+        for (uint32_t tr : traces)
+            for (uint32_t sh : shades)
+                if (!status) status = trial(aph_tuning{ch, tr, sh, keep.trace_refill_lanes, 0.0f});
+    const aph_tuning grid = win;
+    for (uint32_t rf : refills)
+        if (!status && rf != grid.trace_refill_lanes)
+            status = trial(aph_tuning{grid.chunk_paths, grid.trace_blocks_per_cu, grid.shade_blocks_per_cu, rf, 0.0f});
     hipEventDestroy(a);
     hipEventDestroy(b);
     if (status) {

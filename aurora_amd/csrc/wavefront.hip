@@ -221,17 +221,20 @@ __global__ __launch_bounds__(kWB) void kGen(const DevReq* reqs, const SegInfo* s
 
 // ---- persistent traversal of a ray queue ----
 constexpr uint32_t kNoRay = 0xFFFFFFFFu;
-constexpr uint32_t kRefillMin = 16;  // idle lanes of a wave that trigger a refill
 
 // Each lane keeps one ray in flight and advances it one traversal round per
-// iteration.  When at least kRefillMin lanes of the wave are idle (or all are),
+// iteration.  When at least `refill` lanes of the wave are idle (or all are),
 // the wave claims that many queue entries with one atomic on the fetch cursor,
-// so lanes do not wait for the slowest ray of their wave.  The loop is
-// wave-uniform: it ends once the cursor has passed n and no lane has a ray.
+// so lanes do not wait for the slowest ray of their wave.  A refill runs the
+// ray set-up on a few lanes while the rest wait, so refilling on every idle
+// lane costs more than it saves; the threshold is a launch parameter (32 by
+// default, searched by aph_wavefront_autotune: it was 16 until measured; 32 gave
+// +14 % on the 1080p headline frame, 64 +24 % on 64 small cameras).  The loop
+// is wave-uniform: it ends once the cursor has passed n and no lane has a ray.
 // load(i, org, dir, tmin) returns entry i's scene; store(i, state) writes its result.
 template <bool ANY, bool STATS, class Load, class Store>
-__device__ __forceinline__ uint32_t traceQueue(uint32_t n, uint32_t* cursor, const StackMem& mem, TraceStats& ts,
-                                               Load load, Store store)
+__device__ __forceinline__ uint32_t traceQueue(uint32_t n, uint32_t* cursor, uint32_t refill, const StackMem& mem,
+                                               TraceStats& ts, Load load, Store store)
 {
     const int lane = __lane_id();
     const uint64_t below = (1ull << lane) - 1ull;
@@ -243,7 +246,7 @@ __device__ __forceinline__ uint32_t traceQueue(uint32_t n, uint32_t* cursor, con
     while (true) {
         const uint64_t idle = __ballot(ray == kNoRay);
         const uint32_t nIdle = (uint32_t)__popcll(idle);
-        if (!drained && (nIdle >= kRefillMin || nIdle == width)) {
+        if (!drained && (nIdle >= refill || nIdle == width)) {
             const int leader = __ffsll((unsigned long long)idle) - 1;
             uint32_t base = 0;
             if (lane == leader) base = atomicAdd(cursor, nIdle);
@@ -277,7 +280,7 @@ __device__ __forceinline__ uint32_t traceQueue(uint32_t n, uint32_t* cursor, con
 // ---- extend: closest hit of the live queue ----
 template <bool STATS>
 __global__ __launch_bounds__(kWB) void kExtend(const DevReq* reqs, const SegInfo* segs, Paths P, Queues Q, int cur,
-                                              DevCounters* ctr)
+                                              uint32_t refill, DevCounters* ctr)
 {
     __shared__ uint32_t sStack[kLdsStack * kWB];
     uint32_t spill[kSpillStack];
@@ -303,7 +306,7 @@ __global__ __launch_bounds__(kWB) void kExtend(const DevReq* reqs, const SegInfo
             Q.hitV[i] = 0.0f;
         }
     };
-    uint32_t done = traceQueue<false, STATS>(n, &Q.counts[3], mem, ts, load, store);
+    uint32_t done = traceQueue<false, STATS>(n, &Q.counts[3], refill, mem, ts, load, store);
     addCounter(&ctr->ext, done);
     if (STATS) {
         addCounter(&ctr->nodes, ts.nodes);
@@ -556,7 +559,7 @@ __global__ __launch_bounds__(kWB) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
 // ---- shadow: any-hit of the shadow queue (traceShadowRay, forced opaque) ----
 template <bool STATS>
 __global__ __launch_bounds__(kWB) void kShadow(const DevReq* reqs, const SegInfo* segs, Paths P, Queues Q,
-                                              DevCounters* ctr)
+                                              uint32_t refill, DevCounters* ctr)
 {
     __shared__ uint32_t sStack[kLdsStack * kWB];
     uint32_t spill[kSpillStack];
@@ -571,7 +574,7 @@ __global__ __launch_bounds__(kWB) void kShadow(const DevReq* reqs, const SegInfo
         return &reqs[__float_as_uint(a.w)].rs.sc;
     };
     auto store = [&](uint32_t i, const TraceState& s) { P.vis[__float_as_uint(Q.shB[i].w)] = s.found ? 0 : 1; };
-    uint32_t done = traceQueue<true, STATS>(n, &Q.counts[4], mem, ts, load, store);
+    uint32_t done = traceQueue<true, STATS>(n, &Q.counts[4], refill, mem, ts, load, store);
     addCounter(&ctr->shadow, done);
     if (STATS) {
         addCounter(&ctr->nodes, ts.nodes);
@@ -700,6 +703,7 @@ namespace {
 // and the same captured graph - serve every chunk.
 struct Launch {
     unsigned gen, trace, shade, light, shadow, accum;
+    uint32_t refill;  // idle lanes that trigger a traversal refill
 };
 
 // Everything one chunk runs after its tables are in place: generation, then per
@@ -719,8 +723,8 @@ void enqueueChunk(hipStream_t stream, const Launch& L, uint32_t maxDepth, bool s
         kResetNext<<<1, 64, 0, stream>>>(Q.counts, cur ^ 1);
         {
             EventScope ev(timing, "wf_extend", stream);
-            if (stats) kExtend<true><<<L.trace, kWB, 0, stream>>>(dreq, gseg, P, Q, cur, ctr);
-            else kExtend<false><<<L.trace, kWB, 0, stream>>>(dreq, gseg, P, Q, cur, ctr);
+            if (stats) kExtend<true><<<L.trace, kWB, 0, stream>>>(dreq, gseg, P, Q, cur, L.refill, ctr);
+            else kExtend<false><<<L.trace, kWB, 0, stream>>>(dreq, gseg, P, Q, cur, L.refill, ctr);
         }
         {
             EventScope ev(timing, "wf_shade", stream);
@@ -728,8 +732,8 @@ void enqueueChunk(hipStream_t stream, const Launch& L, uint32_t maxDepth, bool s
         }
         {
             EventScope ev(timing, "wf_shadow", stream);
-            if (stats) kShadow<true><<<L.shadow, kWB, 0, stream>>>(dreq, gseg, P, Q, ctr);
-            else kShadow<false><<<L.shadow, kWB, 0, stream>>>(dreq, gseg, P, Q, ctr);
+            if (stats) kShadow<true><<<L.shadow, kWB, 0, stream>>>(dreq, gseg, P, Q, L.refill, ctr);
+            else kShadow<false><<<L.shadow, kWB, 0, stream>>>(dreq, gseg, P, Q, L.refill, ctr);
         }
         {
             EventScope ev(timing, "wf_resolve", stream);
@@ -839,13 +843,15 @@ hipError_t wavefrontRender(WavefrontState& st, const WavefrontRequest* reqs, uin
     L.light = grid(cap, 8);
     L.shadow = grid((uint64_t)cap * 2, tn.tracePerCU);
     L.accum = std::max(1u, std::min<unsigned>((cap + 255) / 256, (unsigned)cus * 8));
+    L.refill = tn.refillLanes;
 
     // One graph per (buffers, depth, stats, launch sizes): captured on first use,
     // replayed for every chunk.  Timed runs launch eagerly so each stage gets its
     // own HIP events.
     const bool graph = st.useGraph && timing == nullptr;
     if (graph) {
-        WavefrontState::GraphKey key{st.buf, ctr, maxDepth, stats, L.gen, L.trace, L.shade, L.shadow, L.accum};
+        WavefrontState::GraphKey key{st.buf, ctr, maxDepth, stats, L.gen, L.trace, L.shade, L.shadow, L.accum,
+                                     L.refill};
         if (!st.graphExec || !(key == st.graphKey)) {
             if (st.graphExec) hipGraphExecDestroy(st.graphExec);
             st.graphExec = nullptr;

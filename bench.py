@@ -123,8 +123,7 @@ def measure_traffic(args, kname):
 
 def apply_tuning(r, tuning):
     if tuning:
-        c, t, sh = (int(v) for v in tuning.split(","))
-        r.set_tuning(c, t, sh)
+        r.set_tuning(*(int(v) for v in tuning.split(",")))
 
 
 def cpu_baseline(sc, budget_s=12.0):
@@ -345,10 +344,11 @@ def autotune_section(r, sc, tlas, spp=16):
     base = frame_ms()
     best = r.autotune(req)
     tuned = frame_ms()
-    out = {"spp": spp, "default": [default.chunk_paths, default.trace_blocks_per_cu, default.shade_blocks_per_cu],
+    out = {"spp": spp, "default": [default.chunk_paths, default.trace_blocks_per_cu, default.shade_blocks_per_cu,
+                                   default.trace_refill_lanes],
            "default_ms": round(base, 2),
-           "chosen": [best.chunk_paths, best.trace_blocks_per_cu, best.shade_blocks_per_cu],
-           "chosen_ms": round(tuned, 2), "candidates": 36}
+           "chosen": [best.chunk_paths, best.trace_blocks_per_cu, best.shade_blocks_per_cu, best.trace_refill_lanes],
+           "chosen_ms": round(tuned, 2), "candidates": 42}
     return out
 
 
@@ -556,7 +556,8 @@ def main():
                          "denoiser and temporal resolve); config5q (not by default: that sequence's frames "
                          "against 1024-spp references); '' for none")
     ap.add_argument("--tuning", default="",
-                    help="wavefront launch configuration 'chunk_paths,trace_blocks_per_cu,shade_blocks_per_cu'; "
+                    help="wavefront launch configuration 'chunk_paths,trace_blocks_per_cu,shade_blocks_per_cu"
+                         "[,trace_refill_lanes]'; "
                          "default: the autotuner's choice on the headline frame (16 spp, child process)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the library's default launch configuration")
     ap.add_argument("--autotune-child", action="store_true",
@@ -683,6 +684,7 @@ def main():
             "launch_config": {"chunk_paths": r.tuning().chunk_paths,
                               "trace_blocks_per_cu": r.tuning().trace_blocks_per_cu,
                               "shade_blocks_per_cu": r.tuning().shade_blocks_per_cu,
+                              "trace_refill_lanes": r.tuning().trace_refill_lanes,
                               "source": "autotuner" if tune_info else ("--tuning" if args.tuning else "default")},
             "samples_per_s": round(sc.width * sc.height * sc.spp * args.steps / wall, 1),
             "extension_rays_per_frame": tst["extension_rays"] // args.steps,

@@ -336,13 +336,15 @@ typedef struct {
     uint32_t chunk_paths;          /* paths in flight per chunk */
     uint32_t trace_blocks_per_cu;  /* persistent traversal kernels: 128-thread blocks per CU */
     uint32_t shade_blocks_per_cu;  /* shade kernel */
+    uint32_t trace_refill_lanes;   /* idle lanes (1..64) of a traversal wave that trigger a refill; default 32 */
     float ms;                      /* measured time of the winning configuration (autotune) */
 } aph_tuning;
 int aph_get_tuning(aph_ctx_t ctx, aph_tuning* t);
 int aph_set_tuning(aph_ctx_t ctx, const aph_tuning* t);
 /* Renders the given requests (into their targets) under each candidate launch
  * configuration - chunk of 4M/8M/16M/32M paths x 4/8/16 traversal blocks per
- * CU x 4/8/16 shade blocks per CU - and keeps the fastest for this context.
+ * CU x 4/8/16 shade blocks per CU, then refill thresholds 16..64 (step 8) at
+ * the best of those - and keeps the fastest for this context.
  * A chunk holds about 0.55 KB of device memory per path (17 GB at 32M paths),
  * allocated at the tuned size even for smaller frames. */
 int aph_wavefront_autotune(aph_ctx_t ctx, const aph_request* reqs, uint32_t nreq, aph_tuning* best);
