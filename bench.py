@@ -217,9 +217,11 @@ def literal_config2(r, builder, spp):
             "mrays_s": round(rays_of(r.stats()) / dt / 1e6, 2), "ms_per_frame": round(dt * 1e3, 1)}
 
 
-def c3_cameras(r, builder, seed=3):
+def c3_cameras(r, builder, seed=3, autotune=True):
     """64 simultaneous cameras on the config1 scene, resolutions from RESOLUTIONS, spp U{1..64},
-    5 bounces: all in one batched wavefront call vs 64 sequential plain megakernel launches."""
+    5 bounces: all in one batched wavefront call vs 64 sequential plain megakernel launches.
+    The batched call runs with the autotuner's choice for this request set (tuned on the
+    same 64 cameras at 1/8 of their spp); the context's previous tuning is restored after."""
     sc = scenes.config1()
     tlas = r.load_scene(sc, builder)
     rng = np.random.default_rng(seed)
@@ -229,6 +231,13 @@ def c3_cameras(r, builder, seed=3):
         cams.append((w, h, int(rng.integers(1, 65)), scenes.orbit_frame(rng, w, h)))
     tg = [r.target(w, h) for w, h, _, _ in cams]
     reqs = [r.request(t, tlas, fr, 0, spp) for t, (w, h, spp, fr) in zip(tg, cams)]
+    keep = r.tuning()
+    tuned = None
+    if autotune:
+        small = [r.request(t, tlas, fr, 0, max(1, spp // 8)) for t, (w, h, spp, fr) in zip(tg, cams)]
+        best = r.autotune(small)
+        tuned = [best.chunk_paths, best.trace_blocks_per_cu, best.shade_blocks_per_cu, best.trace_refill_lanes]
+        log(f"c3 autotune: {tuned}")
     r.render(reqs[:4], capi.APH_MODE_WAVEFRONT)  # warm
     r.wait()
     r.reset_stats()
@@ -236,6 +245,7 @@ def c3_cameras(r, builder, seed=3):
     r.render(reqs, capi.APH_MODE_WAVEFRONT)
     r.wait()
     dt_b = time.perf_counter() - t
+    r.set_tuning(keep.chunk_paths, keep.trace_blocks_per_cu, keep.shade_blocks_per_cu, keep.trace_refill_lanes)
     rays_b = rays_of(r.stats())
     samples = sum(w * h * spp for w, h, spp, _ in cams)
     r.render(reqs[:1], capi.APH_MODE_MEGAKERNEL)  # warm
@@ -251,7 +261,10 @@ def c3_cameras(r, builder, seed=3):
     return {"cameras": 64, "samples": samples, "batched_mrays_s": round(b, 2), "plain_mrays_s": round(p, 2),
             "batched_vs_plain": round(b / p, 2), "batched_s": round(dt_b, 3), "plain_s": round(dt_p, 3),
             "batched_samples_per_s": round(samples / dt_b, 1), "plain_samples_per_s": round(samples / dt_p, 1),
-            "plain": "64 sequential megakernel launches, one per camera"}
+            "plain": "64 sequential megakernel launches, one per camera",
+            "launch_config": tuned or [keep.chunk_paths, keep.trace_blocks_per_cu, keep.shade_blocks_per_cu,
+                                       keep.trace_refill_lanes],
+            "launch_config_source": "autotuner on these 64 cameras at 1/8 spp" if tuned else "context"}
 
 
 def serving(r, builder, jobs=24, rates=(4.0, 16.0), max_spp=32, seed=5, budget_gib=24.0):
