@@ -224,8 +224,9 @@ constexpr uint32_t kNoRay = 0xFFFFFFFFu;
 
 // Each lane keeps one ray in flight and advances it one traversal round per
 // iteration.  When at least `refill` lanes of the wave are idle (or all are),
-// the wave claims that many queue entries with one atomic on the fetch cursor,
-// so lanes do not wait for the slowest ray of their wave.  A refill runs the
+// they start that many queue entries - from the wave's claimed reserve, topped
+// up with one atomic on the fetch cursor when it runs short - so lanes do not
+// wait for the slowest ray of their wave.  A refill runs the
 // ray set-up on a few lanes while the rest wait, so refilling on every idle
 // lane costs more than it saves; the threshold is a launch parameter (32 by
 // default, searched by aph_wavefront_autotune: it was 16 until measured; 32 gave
@@ -242,19 +243,28 @@ __device__ __forceinline__ uint32_t traceQueue(uint32_t n, uint32_t* cursor, uin
     uint32_t ray = kNoRay, done = 0;
     const TraceScene* sc = nullptr;
     TraceState s;
-    bool drained = false;
+    // Queue entries the wave has claimed but not started (wave-uniform): a claim
+    // takes what the refill lacks plus a wave's worth in reserve, so most refills
+    // start rays without waiting for an atomic.
+    uint32_t poolBase = 0, poolLeft = 0;
+    bool exhausted = false;  // the shared cursor has passed n
     while (true) {
         const uint64_t idle = __ballot(ray == kNoRay);
         const uint32_t nIdle = (uint32_t)__popcll(idle);
-        if (!drained && (nIdle >= refill || nIdle == width)) {
-            const int leader = __ffsll((unsigned long long)idle) - 1;
-            uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(cursor, nIdle);
-            base = __shfl(base, leader, 64);
-            if (base + nIdle >= n) drained = true;
+        if ((poolLeft > 0 || !exhausted) && (nIdle >= refill || nIdle == width)) {
+            uint32_t newBase = 0, fresh = 0;
+            if (poolLeft < nIdle && !exhausted) {
+                fresh = nIdle - poolLeft + width;
+                const int leader = __ffsll((unsigned long long)idle) - 1;
+                if (lane == leader) newBase = atomicAdd(cursor, fresh);
+                newBase = __shfl(newBase, leader, 64);
+                if (newBase + fresh >= n) exhausted = true;
+            }
+            // idle lane of rank r: the pool's entry r, then the fresh claim's
             if (ray == kNoRay) {
-                const uint32_t i = base + (uint32_t)__popcll(idle & below);
-                if (i < n) {
+                const uint32_t r = (uint32_t)__popcll(idle & below);
+                const uint32_t i = r < poolLeft ? poolBase + r : newBase + (r - poolLeft);
+                if ((r < poolLeft || r - poolLeft < fresh) && i < n) {
                     f3 org, dir;
                     float tmin;
                     sc = load(i, org, dir, tmin);
@@ -266,13 +276,22 @@ __device__ __forceinline__ uint32_t traceQueue(uint32_t n, uint32_t* cursor, uin
                     }
                 }
             }
+            if (poolLeft >= nIdle) {
+                poolBase += nIdle;
+                poolLeft -= nIdle;
+            } else {
+                const uint32_t used = nIdle - poolLeft;  // taken from the fresh claim
+                poolBase = newBase + used;
+                poolLeft = fresh >= used ? fresh - used : 0u;
+            }
+            poolLeft = poolBase >= n ? 0u : min(poolLeft, n - poolBase);
         }
         if (ray != kNoRay && traceRound<ANY, STATS>(s, *sc, mem, ts)) {
             store(ray, s);
             ray = kNoRay;
             done++;
         }
-        if (drained && !__any(ray != kNoRay)) break;
+        if (exhausted && poolLeft == 0 && !__any(ray != kNoRay)) break;
     }
     return done;
 }
