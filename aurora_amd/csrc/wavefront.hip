@@ -244,8 +244,9 @@ __device__ __forceinline__ uint32_t traceQueue(uint32_t n, uint32_t* cursor, uin
     const TraceScene* sc = nullptr;
     TraceState s;
     // Queue entries the wave has claimed but not started (wave-uniform): a claim
-    // takes what the refill lacks plus a wave's worth in reserve, so most refills
-    // start rays without waiting for an atomic.
+    // takes what the refill lacks plus two waves' worth in reserve, so most
+    // refills start rays without waiting for an atomic (reserve 64: -8 % on 64
+    // small cameras, 256: -1 % on the 1080p frame against 128).
     uint32_t poolBase = 0, poolLeft = 0;
     bool exhausted = false;  // the shared cursor has passed n
     while (true) {
@@ -254,7 +255,7 @@ __device__ __forceinline__ uint32_t traceQueue(uint32_t n, uint32_t* cursor, uin
         if ((poolLeft > 0 || !exhausted) && (nIdle >= refill || nIdle == width)) {
             uint32_t newBase = 0, fresh = 0;
             if (poolLeft < nIdle && !exhausted) {
-                fresh = nIdle - poolLeft + width;
+                fresh = nIdle - poolLeft + 2u * width;
                 const int leader = __ffsll((unsigned long long)idle) - 1;
                 if (lane == leader) newBase = atomicAdd(cursor, fresh);
                 newBase = __shfl(newBase, leader, 64);
