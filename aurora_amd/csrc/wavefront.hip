@@ -233,9 +233,19 @@ constexpr uint32_t kNoRay = 0xFFFFFFFFu;
 // +14 % on the 1080p headline frame, 64 +24 % on 64 small cameras).  The loop
 // is wave-uniform: it ends once the cursor has passed n and no lane has a ray.
 // load(i, org, dir, tmin) returns entry i's scene; store(i, state) writes its result.
-template <bool ANY, bool STATS, class Load, class Store>
+constexpr uint32_t kReserve = 128;  // claimed, not yet started queue entries a wave keeps (two waves' worth)
+
+// Queue entry k of a ray queue (A, B records) copied by the DMA path of
+// global_load_lds_dwordx4: lane l of the wave writes its 16 bytes to dst + 16 l.
+__device__ __forceinline__ void prefetchToLds(const float4* src, float4* dst)
+{
+    __builtin_amdgcn_global_load_lds((const void*)src, (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
+}
+
+template <bool ANY, bool STATS, class Decode, class Store>
 __device__ __forceinline__ uint32_t traceQueue(uint32_t n, uint32_t* cursor, uint32_t refill, const StackMem& mem,
-                                               TraceStats& ts, Load load, Store store)
+                                               TraceStats& ts, const float4* qa, const float4* qb, float4* ringA,
+                                               float4* ringB, Decode decode, Store store)
 {
     const int lane = __lane_id();
     const uint64_t below = (1ull << lane) - 1ull;
@@ -244,31 +254,47 @@ __device__ __forceinline__ uint32_t traceQueue(uint32_t n, uint32_t* cursor, uin
     const TraceScene* sc = nullptr;
     TraceState s;
     // Queue entries the wave has claimed but not started (wave-uniform): a claim
-    // takes what the refill lacks plus two waves' worth in reserve, so most
-    // refills start rays without waiting for an atomic (reserve 64: -8 % on 64
-    // small cameras, 256: -1 % on the 1080p frame against 128).
-    uint32_t poolBase = 0, poolLeft = 0;
-    bool exhausted = false;  // the shared cursor has passed n
+    // takes what the refill lacks plus kReserve entries, so most refills start
+    // rays without waiting for an atomic (reserve 64: -8 % on 64 small cameras,
+    // 256: -1 % on the 1080p frame against 128).  The reserve's rays are copied
+    // into the wave's LDS ring (slot = entry - poolStart) while traversal goes
+    // on, so a refill from it reads LDS instead of waiting on global memory.
+    uint32_t poolBase = 0, poolLeft = 0, poolStart = 0;
+    bool exhausted = false;    // the shared cursor has passed n
+    bool ringPending = false;  // a copy into the ring may still be in flight
     while (true) {
         const uint64_t idle = __ballot(ray == kNoRay);
         const uint32_t nIdle = (uint32_t)__popcll(idle);
         if ((poolLeft > 0 || !exhausted) && (nIdle >= refill || nIdle == width)) {
             uint32_t newBase = 0, fresh = 0;
             if (poolLeft < nIdle && !exhausted) {
-                fresh = nIdle - poolLeft + 2u * width;
+                fresh = nIdle - poolLeft + kReserve;
                 const int leader = __ffsll((unsigned long long)idle) - 1;
                 if (lane == leader) newBase = atomicAdd(cursor, fresh);
                 newBase = __shfl(newBase, leader, 64);
                 if (newBase + fresh >= n) exhausted = true;
             }
-            // idle lane of rank r: the pool's entry r, then the fresh claim's
+            if (poolLeft > 0 && ringPending) {
+                __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the ring's copies have landed
+                ringPending = false;
+            }
+            // idle lane of rank r: the reserve's entry r (from the ring), then the fresh claim's
             if (ray == kNoRay) {
                 const uint32_t r = (uint32_t)__popcll(idle & below);
-                const uint32_t i = r < poolLeft ? poolBase + r : newBase + (r - poolLeft);
-                if ((r < poolLeft || r - poolLeft < fresh) && i < n) {
+                const bool fromPool = r < poolLeft;
+                const uint32_t i = fromPool ? poolBase + r : newBase + (r - poolLeft);
+                if ((fromPool || r - poolLeft < fresh) && i < n) {
+                    float4 a, b;
+                    if (fromPool) {
+                        a = ringA[i - poolStart];
+                        b = ringB[i - poolStart];
+                    } else {
+                        a = qa[i];
+                        b = qb[i];
+                    }
                     f3 org, dir;
                     float tmin;
-                    sc = load(i, org, dir, tmin);
+                    sc = decode(a, b, org, dir, tmin);
                     ray = i;
                     if (!traceBegin(s, *sc, org, dir, tmin, INFINITY)) {
                         store(i, s);
@@ -286,6 +312,19 @@ __device__ __forceinline__ uint32_t traceQueue(uint32_t n, uint32_t* cursor, uin
                 poolLeft = fresh >= used ? fresh - used : 0u;
             }
             poolLeft = poolBase >= n ? 0u : min(poolLeft, n - poolBase);
+            if (fresh > 0 && poolLeft > 0) {
+                // a new reserve (at most kReserve entries from poolBase): copy its rays into the ring
+                poolStart = poolBase;
+#pragma unroll
+                for (uint32_t h = 0; h < kReserve / 64; h++) {
+                    const uint32_t k = h * 64 + (uint32_t)lane;
+                    if (k < poolLeft) {
+                        prefetchToLds(qa + poolStart + k, ringA + h * 64);
+                        prefetchToLds(qb + poolStart + k, ringB + h * 64);
+                    }
+                }
+                ringPending = true;
+            }
         }
         if (ray != kNoRay && traceRound<ANY, STATS>(s, *sc, mem, ts)) {
             store(ray, s);
@@ -303,14 +342,12 @@ __global__ __launch_bounds__(kWB) void kExtend(const DevReq* reqs, const SegInfo
                                               uint32_t refill, DevCounters* ctr)
 {
     __shared__ uint32_t sStack[kLdsStack * kWB];
+    __shared__ float4 sRing[kWB / 64][2][kReserve];
     uint32_t spill[kSpillStack];
     const StackMem mem{sStack + threadIdx.x, spill};
     TraceStats ts = {};
     const uint32_t n = Q.counts[cur];
-    const float4* rayA = Q.rayA[cur];
-    const float4* rayB = Q.rayB[cur];
-    auto load = [&](uint32_t i, f3& org, f3& dir, float& tmin) -> const TraceScene* {
-        float4 a = rayA[i], b = rayB[i];
+    auto decode = [&](float4 a, float4 b, f3& org, f3& dir, float& tmin) -> const TraceScene* {
         org = xyz(a);
         dir = xyz(b);
         tmin = tagTmin(a.w);
@@ -326,7 +363,9 @@ __global__ __launch_bounds__(kWB) void kExtend(const DevReq* reqs, const SegInfo
             Q.hitV[i] = 0.0f;
         }
     };
-    uint32_t done = traceQueue<false, STATS>(n, &Q.counts[3], refill, mem, ts, load, store);
+    float4(*ring)[kReserve] = sRing[threadIdx.x >> 6];
+    uint32_t done = traceQueue<false, STATS>(n, &Q.counts[3], refill, mem, ts, Q.rayA[cur], Q.rayB[cur], ring[0],
+                                             ring[1], decode, store);
     addCounter(&ctr->ext, done);
     if (STATS) {
         addCounter(&ctr->nodes, ts.nodes);
@@ -582,19 +621,21 @@ __global__ __launch_bounds__(kWB) void kShadow(const DevReq* reqs, const SegInfo
                                               uint32_t refill, DevCounters* ctr)
 {
     __shared__ uint32_t sStack[kLdsStack * kWB];
+    __shared__ float4 sRing[kWB / 64][2][kReserve];
     uint32_t spill[kSpillStack];
     const StackMem mem{sStack + threadIdx.x, spill};
     TraceStats ts = {};
     const uint32_t n = Q.counts[2];
-    auto load = [&](uint32_t i, f3& org, f3& dir, float& tmin) -> const TraceScene* {
-        float4 a = Q.shA[i], b = Q.shB[i];
+    auto decode = [&](float4 a, float4 b, f3& org, f3& dir, float& tmin) -> const TraceScene* {
         org = xyz(a);
         dir = xyz(b);
         tmin = kRayTMin;
         return &reqs[__float_as_uint(a.w)].rs.sc;
     };
     auto store = [&](uint32_t i, const TraceState& s) { P.vis[__float_as_uint(Q.shB[i].w)] = s.found ? 0 : 1; };
-    uint32_t done = traceQueue<true, STATS>(n, &Q.counts[4], refill, mem, ts, load, store);
+    float4(*ring)[kReserve] = sRing[threadIdx.x >> 6];
+    uint32_t done = traceQueue<true, STATS>(n, &Q.counts[4], refill, mem, ts, Q.shA, Q.shB, ring[0], ring[1], decode,
+                                            store);
     addCounter(&ctr->shadow, done);
     if (STATS) {
         addCounter(&ctr->nodes, ts.nodes);
