@@ -641,6 +641,8 @@ APH_HD f3 ggxDirPdf(const Shading& sh, f3 V, f2 a, f2 r, f3& Lw, float& pdf)  //
 }
 
 // evaluateMaterialAndPDF, :862-1088 (progressive path: lobe probabilities unadjusted).
+// As in sampleMaterial, the lobe is chosen first and the GGX lobes share one pdf
+// and (coat, specular) one dielectric evaluation.
 __device__ f3 evalMaterialPdf(const DevMaterial& m, const Shading& sh, f3 Vw, f3 Lw, f2 r, float& pdf, int& lobe)
 {
     f3 N = sh.normal;
@@ -651,56 +653,70 @@ __device__ f3 evalMaterialPdf(const DevMaterial& m, const Shading& sh, f3 Vw, f3
     Prologue p = prologue(m, sh);
     f3 res = splat(0.0f);
     float coatLum = luminance(m.coatColor * (fresnelSchlick(NdotV, iorToF0(m.coatIOR)) * m.coat));
+    float specLum = 0.0f, sheenAlpha = 0.0f;
+    f2 ga = f2{0.0f, 0.0f};
     if (r.x < coatLum) {
         r.x = r.x / coatLum;
-        f2 ca = roughAniso(m.coatRoughness, m.coatAnisotropy);
-        pdf = ggxPdf(V, ca, L);
-        res = dielectricSpecular(Lw, Vw, m.coatColor * m.coat, m.coatIOR, ca, N, p.coatTangent);
+        ga = roughAniso(m.coatRoughness, m.coatAnisotropy);
         lobe = kCoatLobe;
     } else {
         r.x = (r.x - coatLum) / (1.0f - coatLum);
         f2 sa = roughAniso(sh.specularRoughness, m.specularAnisotropy);
         if (r.x < m.metalness) {
             r.x = r.x / m.metalness;
-            pdf = ggxPdf(V, sa, L);
-            res = conductorBrdf(Lw, Vw, 1.0f, sh.baseColor * m.base, m.specularColor * m.specular, p.mainRough, N,
-                                p.mainTangent) /
-                  fabsf(L.z);
+            ga = sa;
             lobe = kMetalLobe;
         } else {
             r.x = (r.x - m.metalness) / (1.0f - m.metalness);
-            float specLum = luminance(m.specularColor * (fresnelSchlick(NdotV, iorToF0(m.specularIOR)) * m.specular));
+            specLum = luminance(m.specularColor * (fresnelSchlick(NdotV, iorToF0(m.specularIOR)) * m.specular));
             if (r.x < specLum) {
                 r.x = r.x / specLum;
-                pdf = ggxPdf(V, sa, L);
-                res = dielectricSpecular(Lw, Vw, m.specularColor * m.specular, m.specularIOR, p.mainRough, N,
-                                         p.mainTangent) /
-                      specLum;
+                ga = sa;
                 lobe = kGlossyLobe;
             } else {
                 r.x = (r.x - specLum) / (1.0f - specLum);
-                float sheenAlpha = clampf(m.sheenRoughness, kFloatEps, 1.0f);
+                sheenAlpha = clampf(m.sheenRoughness, kFloatEps, 1.0f);
                 if (r.x < m.sheen) {
                     r.x = r.x / m.sheen;
-                    f3 H = normalize(L + V);
-                    res = sheenSpecular(Lw, Vw, m.sheen, m.sheenColor, m.sheenRoughness, N);
-                    pdf = sheenNdf(H.z, sheenAlpha);
                     lobe = kSheenLobe;
                 } else {
                     r.x = (r.x - m.sheen) / (1.0f - m.sheen);
-                    pdf = dot(N, Lw) * kPiInv;
-                    res = dielectricDiffuse(Lw, Vw, m.base, m.specularIOR, p.mainRough, N, p.diffuseColor);
-                    res = res * (1.0f - sheenAlbedo(NdotV, sheenAlpha) * m.sheen);
                     lobe = kDiffuseLobe;
                 }
             }
         }
-        res = res * lerp(splat(1.0f), m.coatColor, m.coat);
     }
+    if (lobe == kCoatLobe || lobe == kMetalLobe || lobe == kGlossyLobe) {
+        pdf = ggxPdf(V, ga, L);
+        if (lobe == kMetalLobe) {
+            res = conductorBrdf(Lw, Vw, 1.0f, sh.baseColor * m.base, m.specularColor * m.specular, p.mainRough, N,
+                                p.mainTangent) /
+                  fabsf(L.z);
+        } else {
+            const bool coat = lobe == kCoatLobe;
+            res = dielectricSpecular(Lw, Vw, coat ? m.coatColor * m.coat : m.specularColor * m.specular,
+                                     coat ? m.coatIOR : m.specularIOR, coat ? ga : p.mainRough, N,
+                                     coat ? p.coatTangent : p.mainTangent);
+            if (!coat) res = res / specLum;
+        }
+    } else if (lobe == kSheenLobe) {
+        f3 H = normalize(L + V);
+        res = sheenSpecular(Lw, Vw, m.sheen, m.sheenColor, m.sheenRoughness, N);
+        pdf = sheenNdf(H.z, sheenAlpha);
+    } else {
+        pdf = dot(N, Lw) * kPiInv;
+        res = dielectricDiffuse(Lw, Vw, m.base, m.specularIOR, p.mainRough, N, p.diffuseColor);
+        res = res * (1.0f - sheenAlbedo(NdotV, sheenAlpha) * m.sheen);
+    }
+    if (lobe != kCoatLobe) res = res * lerp(splat(1.0f), m.coatColor, m.coat);
     return res * fabsf(L.z);
 }
 
-// sampleMaterial, :1093-1372.
+// sampleMaterial, :1093-1372.  The lobe is chosen first; the three GGX lobes
+// (coat, metal, specular) then share one VNDF sample with their own roughness,
+// and coat and specular one dielectric evaluation with their own parameters - the
+// same calls on the same arguments as lobe-by-lobe code, but a wave whose lanes
+// chose different GGX lobes runs the sampling once instead of up to three times.
 __device__ f3 sampleMaterial(const DevMaterial& m, const Shading& sh, f3 Vw, f2 r, f3& Lw, float& pdf, int& lobe)
 {
     f3 N = sh.normal;
@@ -713,67 +729,79 @@ __device__ f3 sampleMaterial(const DevMaterial& m, const Shading& sh, f3 Vw, f2 
     Prologue p = prologue(m, sh);
     f3 res = splat(0.0f);
     float coatLum = luminance(m.coatColor * (fresnelSchlick(NdotV, iorToF0(m.coatIOR)) * m.coat));
+    float specLum = 0.0f, sheenAlpha = 0.0f;
+    f2 ga = f2{0.0f, 0.0f};  // roughness of the chosen GGX lobe
     if (r.x < coatLum) {
         r.x = r.x / coatLum;
-        f2 ca = roughAniso(m.coatRoughness, m.coatAnisotropy);
-        L = ggxDirPdf(sh, V, ca, r, Lw, pdf);
-        res = dielectricSpecular(Lw, Vw, m.coatColor * m.coat, m.coatIOR, ca, N, p.coatTangent) / coatLum;
+        ga = roughAniso(m.coatRoughness, m.coatAnisotropy);
         lobe = kCoatLobe;
     } else {
         r.x = (r.x - coatLum) / (1.0f - coatLum);
         f2 sa = roughAniso(sh.specularRoughness, m.specularAnisotropy);
         if (r.x < m.metalness) {
             r.x = r.x / m.metalness;
-            L = ggxDirPdf(sh, V, sa, r, Lw, pdf);
-            res = conductorBrdf(Lw, Vw, 1.0f, sh.baseColor * m.base, m.specularColor * m.specular, p.mainRough, N,
-                                p.mainTangent) /
-                  fabsf(L.z);
+            ga = sa;
             lobe = kMetalLobe;
         } else {
             r.x = (r.x - m.metalness) / (1.0f - m.metalness);
-            float specLum = luminance(m.specularColor * (fresnelSchlick(NdotV, iorToF0(m.specularIOR)) * m.specular));
+            specLum = luminance(m.specularColor * (fresnelSchlick(NdotV, iorToF0(m.specularIOR)) * m.specular));
             if (r.x < specLum) {
                 r.x = r.x / specLum;
-                L = ggxDirPdf(sh, V, sa, r, Lw, pdf);
-                res = dielectricSpecular(Lw, Vw, m.specularColor * m.specular, m.specularIOR, p.mainRough, N,
-                                         p.mainTangent) /
-                      specLum;
+                ga = sa;
                 lobe = kGlossyLobe;
             } else {
                 r.x = (r.x - specLum) / (1.0f - specLum);
-                float sheenAlpha = clampf(m.sheenRoughness, kFloatEps, 1.0f);
+                sheenAlpha = clampf(m.sheenRoughness, kFloatEps, 1.0f);
                 if (r.x < m.sheen) {
                     r.x = r.x / m.sheen;
-                    L = sampleHemisphere(r, f3{0.0f, 0.0f, V.z > 0.0f ? 1.0f : -1.0f}, pdf);
-                    Lw = toWorld(L, sh);
-                    f3 H = normalize(L + V);
-                    res = sheenSpecular(Lw, Vw, m.sheen, m.sheenColor, m.sheenRoughness, N);
-                    pdf = sheenNdf(H.z, sheenAlpha);
                     lobe = kSheenLobe;
                 } else {
                     r.x = (r.x - m.sheen) / (1.0f - m.sheen);
                     if (r.x < m.transmission) {
                         r.x = r.x / m.transmission;
-                        bool reflected = false;
-                        L = -V;
-                        if (!m.thinWalled) L = computeRefraction(V, m.specularIOR, reflected);
-                        Lw = toWorld(L, sh);
-                        res = m.transmissionColor / fabsf(L.z);
-                        pdf = 1.0f;
-                        lobe = reflected ? kGlossyLobe : kTransmissionLobe;
+                        lobe = kTransmissionLobe;
                     } else {
                         r.x = (r.x - m.transmission) / (1.0f - m.transmission);
-                        L = sampleHemisphere(r, f3{0.0f, 0.0f, V.z > 0.0f ? 1.0f : -1.0f}, pdf);
-                        Lw = toWorld(L, sh);
-                        res = dielectricDiffuse(Lw, Vw, m.base, m.specularIOR, p.mainRough, N, p.diffuseColor);
-                        res = res * (1.0f - sheenAlbedo(NdotV, sheenAlpha) * m.sheen);
                         lobe = kDiffuseLobe;
                     }
                 }
             }
         }
-        res = res * lerp(splat(1.0f), m.coatColor, m.coat);
     }
+    if (lobe == kCoatLobe || lobe == kMetalLobe || lobe == kGlossyLobe) {
+        L = ggxDirPdf(sh, V, ga, r, Lw, pdf);
+        if (lobe == kMetalLobe) {
+            res = conductorBrdf(Lw, Vw, 1.0f, sh.baseColor * m.base, m.specularColor * m.specular, p.mainRough, N,
+                                p.mainTangent) /
+                  fabsf(L.z);
+        } else {
+            const bool coat = lobe == kCoatLobe;
+            res = dielectricSpecular(Lw, Vw, coat ? m.coatColor * m.coat : m.specularColor * m.specular,
+                                     coat ? m.coatIOR : m.specularIOR, coat ? ga : p.mainRough, N,
+                                     coat ? p.coatTangent : p.mainTangent) /
+                  (coat ? coatLum : specLum);
+        }
+    } else if (lobe == kSheenLobe) {
+        L = sampleHemisphere(r, f3{0.0f, 0.0f, V.z > 0.0f ? 1.0f : -1.0f}, pdf);
+        Lw = toWorld(L, sh);
+        f3 H = normalize(L + V);
+        res = sheenSpecular(Lw, Vw, m.sheen, m.sheenColor, m.sheenRoughness, N);
+        pdf = sheenNdf(H.z, sheenAlpha);
+    } else if (lobe == kTransmissionLobe) {
+        bool reflected = false;
+        L = -V;
+        if (!m.thinWalled) L = computeRefraction(V, m.specularIOR, reflected);
+        Lw = toWorld(L, sh);
+        res = m.transmissionColor / fabsf(L.z);
+        pdf = 1.0f;
+        lobe = reflected ? kGlossyLobe : kTransmissionLobe;
+    } else {
+        L = sampleHemisphere(r, f3{0.0f, 0.0f, V.z > 0.0f ? 1.0f : -1.0f}, pdf);
+        Lw = toWorld(L, sh);
+        res = dielectricDiffuse(Lw, Vw, m.base, m.specularIOR, p.mainRough, N, p.diffuseColor);
+        res = res * (1.0f - sheenAlbedo(NdotV, sheenAlpha) * m.sheen);
+    }
+    if (lobe != kCoatLobe) res = res * lerp(splat(1.0f), m.coatColor, m.coat);
     return res * fabsf(L.z);
 }
 
