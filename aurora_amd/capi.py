@@ -95,7 +95,8 @@ class Stats(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("chunk_paths", C.c_uint32), ("trace_blocks_per_cu", C.c_uint32),
-                ("shade_blocks_per_cu", C.c_uint32), ("trace_refill_lanes", C.c_uint32), ("ms", C.c_float)]
+                ("shade_blocks_per_cu", C.c_uint32), ("trace_refill_lanes", C.c_uint32),
+                ("trace_leaf_wait_lanes", C.c_uint32), ("ms", C.c_float)]
 
 
 APH_LAUNCH_GRAPH, APH_LAUNCH_EAGER = 0, 1

@@ -912,6 +912,7 @@ int aph_get_tuning(aph_ctx_t c, aph_tuning* t)
     t->trace_blocks_per_cu = c->wf.tuning.tracePerCU;
     t->shade_blocks_per_cu = c->wf.tuning.shadePerCU;
     t->trace_refill_lanes = c->wf.tuning.refillLanes;
+    t->trace_leaf_wait_lanes = c->wf.tuning.leafWaitLanes;
     t->ms = 0.0f;
     return APH_OK;
 }
@@ -922,13 +923,14 @@ int aph_set_tuning(aph_ctx_t c, const aph_tuning* t)
     if (rc) return rc;
     if (!t || t->chunk_paths < (1u << 16) || t->chunk_paths > (1u << 26) || t->trace_blocks_per_cu == 0 ||
         t->trace_blocks_per_cu > 64 || t->shade_blocks_per_cu == 0 || t->shade_blocks_per_cu > 64 ||
-        t->trace_refill_lanes == 0 || t->trace_refill_lanes > 64)
+        t->trace_refill_lanes == 0 || t->trace_refill_lanes > 64 || t->trace_leaf_wait_lanes > 63)
         return fail(c, APH_E_INVALID, "tuning out of range");
     HIPCALL(c, hipStreamSynchronize(c->stream), "tuning");
     c->wf.tuning.chunkPaths = t->chunk_paths;
     c->wf.tuning.tracePerCU = t->trace_blocks_per_cu;
     c->wf.tuning.shadePerCU = t->shade_blocks_per_cu;
     c->wf.tuning.refillLanes = t->trace_refill_lanes;
+    c->wf.tuning.leafWaitLanes = t->trace_leaf_wait_lanes;
     return APH_OK;
 }
 
@@ -942,6 +944,7 @@ int aph_wavefront_autotune(aph_ctx_t c, const aph_request* reqs, uint32_t nreq, 
     const uint32_t traces[] = {4, 8, 16};
     const uint32_t shades[] = {4, 8, 16};
     const uint32_t refills[] = {16, 24, 32, 40, 48, 56, 64};
+    const uint32_t leafWaits[] = {0, 8, 16, 24, 32};
     aph_tuning keep{};
     aph_get_tuning(c, &keep);
     aph_tuning win = keep;
@@ -968,15 +971,23 @@ int aph_wavefront_autotune(aph_ctx_t c, const aph_request* reqs, uint32_t nreq, 
     };
     int status = APH_OK;
     // Stage 1: chunk size x traversal grid x shade grid at the current refill
-    // threshold; stage 2: the refill threshold at the stage-1 winner.
+    // threshold and leaf wait; stage 2: the refill threshold at the stage-1
+    // winner; stage 3: the leaf wait at the stage-2 winner.
     for (uint32_t ch : chunks)
         for (uint32_t tr : traces)
             for (uint32_t sh : shades)
-                if (!status) status = trial(aph_tuning{ch, tr, sh, keep.trace_refill_lanes, 0.0f});
+                if (!status)
+                    status = trial(aph_tuning{ch, tr, sh, keep.trace_refill_lanes, keep.trace_leaf_wait_lanes, 0.0f});
     const aph_tuning grid = win;
     for (uint32_t rf : refills)
         if (!status && rf != grid.trace_refill_lanes)
-            status = trial(aph_tuning{grid.chunk_paths, grid.trace_blocks_per_cu, grid.shade_blocks_per_cu, rf, 0.0f});
+            status = trial(aph_tuning{grid.chunk_paths, grid.trace_blocks_per_cu, grid.shade_blocks_per_cu, rf,
+                                      grid.trace_leaf_wait_lanes, 0.0f});
+    const aph_tuning withRefill = win;
+    for (uint32_t lw : leafWaits)
+        if (!status && lw != withRefill.trace_leaf_wait_lanes)
+            status = trial(aph_tuning{withRefill.chunk_paths, withRefill.trace_blocks_per_cu,
+                                      withRefill.shade_blocks_per_cu, withRefill.trace_refill_lanes, lw, 0.0f});
     hipEventDestroy(a);
     hipEventDestroy(b);
     if (status) {

@@ -25,6 +25,7 @@ struct WavefrontTuning {
     uint32_t tracePerCU = 16;        // blocks per CU of the persistent traversal kernels
     uint32_t shadePerCU = 8;         // blocks per CU of the shade kernel
     uint32_t refillLanes = 32;       // idle lanes of a traversal wave that trigger a refill from the queue
+    uint32_t leafWaitLanes = 16;     // lanes still looking for a leaf when a traversal wave processes leaves
 };
 
 // Device buffers kept across calls (grown on demand) and the captured chunk graph.
@@ -39,12 +40,12 @@ struct WavefrontState {
         void* ctr;
         uint32_t depth;
         bool stats;
-        unsigned gen, trace, shade, shadow, accum, refill;
+        unsigned gen, trace, shade, shadow, accum, refill, leafWait;
         bool operator==(const GraphKey& o) const
         {
             return buf == o.buf && ctr == o.ctr && depth == o.depth && stats == o.stats && gen == o.gen &&
                    trace == o.trace && shade == o.shade && shadow == o.shadow && accum == o.accum &&
-                   refill == o.refill;
+                   refill == o.refill && leafWait == o.leafWait;
         }
     };
     GraphKey graphKey{};

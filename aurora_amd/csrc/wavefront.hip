@@ -243,7 +243,8 @@ __device__ __forceinline__ void prefetchToLds(const float4* src, float4* dst)
 }
 
 template <bool ANY, bool STATS, class Decode, class Store>
-__device__ __forceinline__ uint32_t traceQueue(uint32_t n, uint32_t* cursor, uint32_t refill, const StackMem& mem,
+__device__ __forceinline__ uint32_t traceQueue(uint32_t n, uint32_t* cursor, uint32_t refill, uint32_t leafWait,
+                                               const StackMem& mem,
                                                TraceStats& ts, const float4* qa, const float4* qb, float4* ringA,
                                                float4* ringB, Decode decode, Store store)
 {
@@ -326,7 +327,7 @@ __device__ __forceinline__ uint32_t traceQueue(uint32_t n, uint32_t* cursor, uin
                 ringPending = true;
             }
         }
-        if (ray != kNoRay && traceRound<ANY, STATS>(s, *sc, mem, ts)) {
+        if (ray != kNoRay && traceRound<ANY, STATS>(s, *sc, mem, ts, leafWait)) {
             store(ray, s);
             ray = kNoRay;
             done++;
@@ -339,7 +340,7 @@ __device__ __forceinline__ uint32_t traceQueue(uint32_t n, uint32_t* cursor, uin
 // ---- extend: closest hit of the live queue ----
 template <bool STATS>
 __global__ __launch_bounds__(kWB) void kExtend(const DevReq* reqs, const SegInfo* segs, Paths P, Queues Q, int cur,
-                                              uint32_t refill, DevCounters* ctr)
+                                              uint32_t refill, uint32_t leafWait, DevCounters* ctr)
 {
     __shared__ uint32_t sStack[kLdsStack * kWB];
     __shared__ float4 sRing[kWB / 64][2][kReserve];
@@ -364,7 +365,7 @@ __global__ __launch_bounds__(kWB) void kExtend(const DevReq* reqs, const SegInfo
         }
     };
     float4(*ring)[kReserve] = sRing[threadIdx.x >> 6];
-    uint32_t done = traceQueue<false, STATS>(n, &Q.counts[3], refill, mem, ts, Q.rayA[cur], Q.rayB[cur], ring[0],
+    uint32_t done = traceQueue<false, STATS>(n, &Q.counts[3], refill, leafWait, mem, ts, Q.rayA[cur], Q.rayB[cur], ring[0],
                                              ring[1], decode, store);
     addCounter(&ctr->ext, done);
     if (STATS) {
@@ -618,7 +619,7 @@ __global__ __launch_bounds__(kWB) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
 // ---- shadow: any-hit of the shadow queue (traceShadowRay, forced opaque) ----
 template <bool STATS>
 __global__ __launch_bounds__(kWB) void kShadow(const DevReq* reqs, const SegInfo* segs, Paths P, Queues Q,
-                                              uint32_t refill, DevCounters* ctr)
+                                              uint32_t refill, uint32_t leafWait, DevCounters* ctr)
 {
     __shared__ uint32_t sStack[kLdsStack * kWB];
     __shared__ float4 sRing[kWB / 64][2][kReserve];
@@ -634,7 +635,7 @@ __global__ __launch_bounds__(kWB) void kShadow(const DevReq* reqs, const SegInfo
     };
     auto store = [&](uint32_t i, const TraceState& s) { P.vis[__float_as_uint(Q.shB[i].w)] = s.found ? 0 : 1; };
     float4(*ring)[kReserve] = sRing[threadIdx.x >> 6];
-    uint32_t done = traceQueue<true, STATS>(n, &Q.counts[4], refill, mem, ts, Q.shA, Q.shB, ring[0], ring[1], decode,
+    uint32_t done = traceQueue<true, STATS>(n, &Q.counts[4], refill, leafWait, mem, ts, Q.shA, Q.shB, ring[0], ring[1], decode,
                                             store);
     addCounter(&ctr->shadow, done);
     if (STATS) {
@@ -764,7 +765,8 @@ namespace {
 // and the same captured graph - serve every chunk.
 struct Launch {
     unsigned gen, trace, shade, light, shadow, accum;
-    uint32_t refill;  // idle lanes that trigger a traversal refill
+    uint32_t refill;    // idle lanes that trigger a traversal refill
+    uint32_t leafWait;  // lanes still looking for a leaf when a traversal wave processes leaves
 };
 
 // Everything one chunk runs after its tables are in place: generation, then per
@@ -784,8 +786,8 @@ void enqueueChunk(hipStream_t stream, const Launch& L, uint32_t maxDepth, bool s
         kResetNext<<<1, 64, 0, stream>>>(Q.counts, cur ^ 1);
         {
             EventScope ev(timing, "wf_extend", stream);
-            if (stats) kExtend<true><<<L.trace, kWB, 0, stream>>>(dreq, gseg, P, Q, cur, L.refill, ctr);
-            else kExtend<false><<<L.trace, kWB, 0, stream>>>(dreq, gseg, P, Q, cur, L.refill, ctr);
+            if (stats) kExtend<true><<<L.trace, kWB, 0, stream>>>(dreq, gseg, P, Q, cur, L.refill, L.leafWait, ctr);
+            else kExtend<false><<<L.trace, kWB, 0, stream>>>(dreq, gseg, P, Q, cur, L.refill, L.leafWait, ctr);
         }
         {
             EventScope ev(timing, "wf_shade", stream);
@@ -793,8 +795,8 @@ void enqueueChunk(hipStream_t stream, const Launch& L, uint32_t maxDepth, bool s
         }
         {
             EventScope ev(timing, "wf_shadow", stream);
-            if (stats) kShadow<true><<<L.shadow, kWB, 0, stream>>>(dreq, gseg, P, Q, L.refill, ctr);
-            else kShadow<false><<<L.shadow, kWB, 0, stream>>>(dreq, gseg, P, Q, L.refill, ctr);
+            if (stats) kShadow<true><<<L.shadow, kWB, 0, stream>>>(dreq, gseg, P, Q, L.refill, L.leafWait, ctr);
+            else kShadow<false><<<L.shadow, kWB, 0, stream>>>(dreq, gseg, P, Q, L.refill, L.leafWait, ctr);
         }
         {
             EventScope ev(timing, "wf_resolve", stream);
@@ -905,6 +907,7 @@ hipError_t wavefrontRender(WavefrontState& st, const WavefrontRequest* reqs, uin
     L.shadow = grid((uint64_t)cap * 2, tn.tracePerCU);
     L.accum = std::max(1u, std::min<unsigned>((cap + 255) / 256, (unsigned)cus * 8));
     L.refill = tn.refillLanes;
+    L.leafWait = tn.leafWaitLanes;
 
     // One graph per (buffers, depth, stats, launch sizes): captured on first use,
     // replayed for every chunk.  Timed runs launch eagerly so each stage gets its
@@ -912,7 +915,7 @@ hipError_t wavefrontRender(WavefrontState& st, const WavefrontRequest* reqs, uin
     const bool graph = st.useGraph && timing == nullptr;
     if (graph) {
         WavefrontState::GraphKey key{st.buf, ctr, maxDepth, stats, L.gen, L.trace, L.shade, L.shadow, L.accum,
-                                     L.refill};
+                                     L.refill, L.leafWait};
         if (!st.graphExec || !(key == st.graphKey)) {
             if (st.graphExec) hipGraphExecDestroy(st.graphExec);
             st.graphExec = nullptr;

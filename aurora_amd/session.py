@@ -70,8 +70,10 @@ class Renderer:
         self._check(self.lib.aph_get_tuning(self.ctx, C.byref(t)))
         return t
 
-    def set_tuning(self, chunk_paths, trace_blocks_per_cu, shade_blocks_per_cu, trace_refill_lanes=32):
-        t = capi.Tuning(chunk_paths, trace_blocks_per_cu, shade_blocks_per_cu, trace_refill_lanes, 0.0)
+    def set_tuning(self, chunk_paths, trace_blocks_per_cu, shade_blocks_per_cu, trace_refill_lanes=32,
+                   trace_leaf_wait_lanes=16):
+        t = capi.Tuning(chunk_paths, trace_blocks_per_cu, shade_blocks_per_cu, trace_refill_lanes,
+                        trace_leaf_wait_lanes, 0.0)
         self._check(self.lib.aph_set_tuning(self.ctx, C.byref(t)))
 
     def autotune(self, requests):

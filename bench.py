@@ -236,7 +236,8 @@ def c3_cameras(r, builder, seed=3, autotune=True):
     if autotune:
         small = [r.request(t, tlas, fr, 0, max(1, spp // 8)) for t, (w, h, spp, fr) in zip(tg, cams)]
         best = r.autotune(small)
-        tuned = [best.chunk_paths, best.trace_blocks_per_cu, best.shade_blocks_per_cu, best.trace_refill_lanes]
+        tuned = [best.chunk_paths, best.trace_blocks_per_cu, best.shade_blocks_per_cu, best.trace_refill_lanes,
+                 best.trace_leaf_wait_lanes]
         log(f"c3 autotune: {tuned}")
     r.render(reqs[:4], capi.APH_MODE_WAVEFRONT)  # warm
     r.wait()
@@ -245,7 +246,8 @@ def c3_cameras(r, builder, seed=3, autotune=True):
     r.render(reqs, capi.APH_MODE_WAVEFRONT)
     r.wait()
     dt_b = time.perf_counter() - t
-    r.set_tuning(keep.chunk_paths, keep.trace_blocks_per_cu, keep.shade_blocks_per_cu, keep.trace_refill_lanes)
+    r.set_tuning(keep.chunk_paths, keep.trace_blocks_per_cu, keep.shade_blocks_per_cu, keep.trace_refill_lanes,
+                 keep.trace_leaf_wait_lanes)
     rays_b = rays_of(r.stats())
     samples = sum(w * h * spp for w, h, spp, _ in cams)
     r.render(reqs[:1], capi.APH_MODE_MEGAKERNEL)  # warm
@@ -263,7 +265,7 @@ def c3_cameras(r, builder, seed=3, autotune=True):
             "batched_samples_per_s": round(samples / dt_b, 1), "plain_samples_per_s": round(samples / dt_p, 1),
             "plain": "64 sequential megakernel launches, one per camera",
             "launch_config": tuned or [keep.chunk_paths, keep.trace_blocks_per_cu, keep.shade_blocks_per_cu,
-                                       keep.trace_refill_lanes],
+                                       keep.trace_refill_lanes, keep.trace_leaf_wait_lanes],
             "launch_config_source": "autotuner on these 64 cameras at 1/8 spp" if tuned else "context"}
 
 
@@ -358,10 +360,11 @@ def autotune_section(r, sc, tlas, spp=16):
     best = r.autotune(req)
     tuned = frame_ms()
     out = {"spp": spp, "default": [default.chunk_paths, default.trace_blocks_per_cu, default.shade_blocks_per_cu,
-                                   default.trace_refill_lanes],
+                                   default.trace_refill_lanes, default.trace_leaf_wait_lanes],
            "default_ms": round(base, 2),
-           "chosen": [best.chunk_paths, best.trace_blocks_per_cu, best.shade_blocks_per_cu, best.trace_refill_lanes],
-           "chosen_ms": round(tuned, 2), "candidates": 42}
+           "chosen": [best.chunk_paths, best.trace_blocks_per_cu, best.shade_blocks_per_cu, best.trace_refill_lanes,
+                      best.trace_leaf_wait_lanes],
+           "chosen_ms": round(tuned, 2), "candidates": 46}
     return out
 
 
@@ -570,7 +573,7 @@ def main():
                          "against 1024-spp references); '' for none")
     ap.add_argument("--tuning", default="",
                     help="wavefront launch configuration 'chunk_paths,trace_blocks_per_cu,shade_blocks_per_cu"
-                         "[,trace_refill_lanes]'; "
+                         "[,trace_refill_lanes[,trace_leaf_wait_lanes]]'; "
                          "default: the autotuner's choice on the headline frame (16 spp, child process)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the library's default launch configuration")
     ap.add_argument("--autotune-child", action="store_true",
@@ -698,6 +701,7 @@ def main():
                               "trace_blocks_per_cu": r.tuning().trace_blocks_per_cu,
                               "shade_blocks_per_cu": r.tuning().shade_blocks_per_cu,
                               "trace_refill_lanes": r.tuning().trace_refill_lanes,
+                              "trace_leaf_wait_lanes": r.tuning().trace_leaf_wait_lanes,
                               "source": "autotuner" if tune_info else ("--tuning" if args.tuning else "default")},
             "samples_per_s": round(sc.width * sc.height * sc.spp * args.steps / wall, 1),
             "extension_rays_per_frame": tst["extension_rays"] // args.steps,

@@ -337,6 +337,8 @@ typedef struct {
     uint32_t trace_blocks_per_cu;  /* persistent traversal kernels: 128-thread blocks per CU */
     uint32_t shade_blocks_per_cu;  /* shade kernel */
     uint32_t trace_refill_lanes;   /* idle lanes (1..64) of a traversal wave that trigger a refill; default 32 */
+    uint32_t trace_leaf_wait_lanes;/* lanes (0..63) still looking for a leaf when a traversal wave processes
+                                      leaves; default 16 */
     float ms;                      /* measured time of the winning configuration (autotune) */
 } aph_tuning;
 int aph_get_tuning(aph_ctx_t ctx, aph_tuning* t);
@@ -344,7 +346,8 @@ int aph_set_tuning(aph_ctx_t ctx, const aph_tuning* t);
 /* Renders the given requests (into their targets) under each candidate launch
  * configuration - chunk of 4M/8M/16M/32M paths x 4/8/16 traversal blocks per
  * CU x 4/8/16 shade blocks per CU, then refill thresholds 16..64 (step 8) at
- * the best of those - and keeps the fastest for this context.
+ * the best of those, then leaf-wait lanes 0/8/16/24/32 at the best of those -
+ * and keeps the fastest for this context.
  * A chunk holds about 0.55 KB of device memory per path (17 GB at 32M paths),
  * allocated at the tuned size even for smaller frames. */
 int aph_wavefront_autotune(aph_ctx_t ctx, const aph_request* reqs, uint32_t nreq, aph_tuning* best);

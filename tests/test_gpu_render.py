@@ -227,8 +227,8 @@ def test_graph_replays_equal_eager_launches(renderer):
     renderer.set_tuning(1 << 17, 8, 16)
     _, tuned = render()
     np.testing.assert_array_equal(tuned, eager)
-    for refill in (1, 64):  # traversal refill threshold: every idle lane, or only whole waves
-        renderer.set_tuning(1 << 17, 8, 16, refill)
+    for refill, wait in ((1, 0), (64, 0), (16, 63)):  # refill threshold; lanes still searching at leaf time
+        renderer.set_tuning(1 << 17, 8, 16, refill, wait)
         _, tuned = render()
         np.testing.assert_array_equal(tuned, eager)
     renderer.set_tuning(4 << 20, 16, 8)
@@ -280,9 +280,11 @@ def test_autotune_keeps_results(renderer, teapot_path):
     assert best.ms > 0 and best.chunk_paths in (4 << 20, 8 << 20, 16 << 20, 32 << 20)
     assert best.trace_blocks_per_cu in (4, 8, 16) and best.shade_blocks_per_cu in (4, 8, 16)
     assert best.trace_refill_lanes in (16, 24, 32, 40, 48, 56, 64)
+    assert best.trace_leaf_wait_lanes in (0, 8, 16, 24, 32)
     t = renderer.tuning()
-    assert (t.chunk_paths, t.trace_blocks_per_cu, t.shade_blocks_per_cu, t.trace_refill_lanes) == \
-        (best.chunk_paths, best.trace_blocks_per_cu, best.shade_blocks_per_cu, best.trace_refill_lanes)
+    assert (t.chunk_paths, t.trace_blocks_per_cu, t.shade_blocks_per_cu, t.trace_refill_lanes,
+            t.trace_leaf_wait_lanes) == (best.chunk_paths, best.trace_blocks_per_cu, best.shade_blocks_per_cu,
+                                         best.trace_refill_lanes, best.trace_leaf_wait_lanes)
     after = renderer.target(64, 48)
     renderer.render([renderer.request(after, tlas, sc.frame, 0, 4)], capi.APH_MODE_WAVEFRONT)
     renderer.wait()
