@@ -589,8 +589,9 @@ def main():
                     help="internal: calibration copies + one frame, run under rocprofv3 --pmc")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
-              "the path shards by sample range across GPUs, see DESIGN.md.", file=sys.stderr)
+        print(f"bench.py: measures one GPU only (asked for --gpus {args.gpus}); "
+              "the path would shard by sample range across GPUs (per-pixel sums reduced once per frame).",
+              file=sys.stderr)
         sys.exit(2)
 
     sc = scenes.config2(subdiv=args.subdiv, spp=args.spp)
