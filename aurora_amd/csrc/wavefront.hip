@@ -8,10 +8,11 @@
 //   gen      camera rays for all (request, sample, pixel) paths of the chunk
 //   extend   closest-hit traversal of the live ray queue          (lean, high occupancy)
 //   shade    material, opacity coin, lights, environment MIS, next ray, roulette;
-//            appends shadow rays and continuing paths with wave-ballot compaction
+//            appends shadow rays with wave-ballot compaction, marks continuing paths
 //   shadow   any-hit traversal of the shadow queue
 //   resolve  adds shadow-weighted lighting in the reference's order, routes it
-//            to the direct/diffuse/glossy outputs, applies the terminal bounce
+//            to the direct/diffuse/glossy outputs, applies the terminal bounce,
+//            and moves the continuing rays to the next queue in queue order
 //   accum    per pixel, folds the chunk's samples into the target in sample order
 //
 // Arithmetic per path is the megakernel's (pathtrace.h helpers, same operation
@@ -87,7 +88,15 @@ struct Queues {
     float4* shA;           // org.xyz, request index bits (traced from kRayTMin)
     float4* shB;           // dir.xyz, (path * kSlots + slot) bits
     uint32_t* counts;      // [0],[1] ray queues, [2] shadow queue, [3]/[4] extend/shadow fetch cursors
+    uint8_t* cont;         // per entry of the current ray queue: 1 if its path continues
+    uint32_t* tileCnt;     // continuing entries per kTile entries of the current queue
+    uint32_t* tileOff;     // their exclusive prefix sum: where each tile's survivors go
 };
+
+// The next ray queue is written in the order of the current one (a stable
+// compaction per bounce), so it stays sorted by path id and the path state
+// gathered by path id is read in nearly contiguous runs.
+constexpr uint32_t kTile = 1024;
 
 struct SegInfo {
     uint32_t req, sample, pathOffset, count;
@@ -410,14 +419,15 @@ __device__ __forceinline__ void shadeMiss(const DevReq* reqs, const SegInfo* seg
     f.terminal(false);
     P.shMask[p] = 0;
     P.flags[p] = f.v;
+    Q.cont[i] = 0;
 }
 
-// Closest-hit program of queue entry i (live lanes), then the wave-uniform
-// append of the continuing rays (every lane of the wave).
+// Closest-hit program of queue entry i (live lanes).  A continuing path's next
+// ray replaces entry i of the current queue (the path id stays in rayB.w), and
+// kResolve moves it to the next queue in queue order.
 __device__ __forceinline__ void shadeHit(const DevReq* reqs, const SegInfo* segs, const Paths& P, const Queues& Q,
                                          int cur, bool live, uint32_t i)
 {
-    const int nxt = cur ^ 1;
     bool contPath = false;
     f3 nOrg = splat(0.0f), nDir = splat(0.0f);
     uint32_t p = 0, ri = 0;
@@ -560,11 +570,11 @@ __device__ __forceinline__ void shadeHit(const DevReq* reqs, const SegInfo* segs
         P.flags[p] = f.v;
         P.contrib[p] = f4(contrib, 0.0f);
         P.rng[p] = make_uint2(rng.x, rng.y);
-    }
-    uint32_t at = waveAppend(contPath, &Q.counts[nxt]);
-    if (contPath) {
-        Q.rayA[nxt][at] = f4(nOrg, rayTag(ri, false));
-        Q.rayB[nxt][at] = f4(nDir, __uint_as_float(p));
+        Q.cont[i] = contPath ? 1 : 0;
+        if (contPath) {
+            Q.rayA[cur][i] = f4(nOrg, rayTag(ri, false));
+            Q.rayB[cur][i] = f4(nDir, __uint_as_float(p));
+        }
     }
 }
 
@@ -656,11 +666,67 @@ __device__ __forceinline__ void finalizePath(const Paths& P, uint32_t p, const a
     P.sampleOut[p] = f4(fin, f.alpha());
 }
 
-// ---- resolve: the accumulation part of the RayGenShader loop (MainEntryPoints.slang:124-247) ----
-__global__ __launch_bounds__(kWB) void kResolve(const DevReq* reqs, const SegInfo* segs, Paths P, Queues Q, int cur)
+// ---- stable compaction of the continuing rays: tile counts, then their prefix sum ----
+
+// One wave per kTile entries of the current queue (16 flags per lane).
+__global__ __launch_bounds__(kWB) void kCount(Queues Q, int cur)
 {
-    uint32_t n = Q.counts[cur];
-    for (uint32_t i = blockIdx.x * kWB + threadIdx.x; i < n; i += gridDim.x * kWB) {
+    const uint32_t n = Q.counts[cur];
+    const uint32_t nt = (n + kTile - 1) / kTile;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t waves = gridDim.x * (kWB / 64);
+    for (uint32_t t = blockIdx.x * (kWB / 64) + (threadIdx.x >> 6); t < nt; t += waves) {
+        const uint32_t i0 = t * kTile + lane * 16;
+        uint32_t s = 0;
+        if (i0 + 16 <= n) {
+            const uint4 v = *(const uint4*)(Q.cont + i0);
+            // flags are 0 or 1, so the byte sums of a word fit its top byte
+            s = ((v.x * 0x01010101u) >> 24) + ((v.y * 0x01010101u) >> 24) + ((v.z * 0x01010101u) >> 24) +
+                ((v.w * 0x01010101u) >> 24);
+        } else {
+            for (uint32_t k = 0; k < 16; k++)
+                if (i0 + k < n) s += Q.cont[i0 + k];
+        }
+        s = waveSum(s);
+        if (lane == 0) Q.tileCnt[t] = s;
+    }
+}
+
+// One block of kScanThreads: exclusive prefix sum of the tile counts into
+// tileOff, and the total as the next queue's length.
+constexpr int kScanThreads = 1024;
+__global__ __launch_bounds__(kScanThreads) void kScan(Queues Q, int cur)
+{
+    __shared__ uint32_t part[kScanThreads];
+    const uint32_t n = Q.counts[cur];
+    const uint32_t nt = (n + kTile - 1) / kTile;
+    const uint32_t per = (nt + kScanThreads - 1) / kScanThreads;
+    const uint32_t b = threadIdx.x * per, e = min(nt, b + per);
+    uint32_t s = 0;
+    for (uint32_t t = b; t < e; t++) s += Q.tileCnt[t];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    // Hillis-Steele inclusive scan over the per-thread sums
+    for (int off = 1; off < kScanThreads; off <<= 1) {
+        const uint32_t add = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += add;
+        __syncthreads();
+    }
+    uint32_t run = part[threadIdx.x] - s;
+    for (uint32_t t = b; t < e; t++) {
+        const uint32_t c = Q.tileCnt[t];
+        Q.tileOff[t] = run;
+        run += c;
+    }
+    if (threadIdx.x == kScanThreads - 1) Q.counts[cur ^ 1] = part[kScanThreads - 1];
+}
+
+// The accumulation part of the RayGenShader loop (MainEntryPoints.slang:124-247)
+// for queue entry i, after its shadow rays are traced.
+__device__ __forceinline__ void resolvePath(const DevReq* reqs, const Paths& P, const Queues& Q, int cur, uint32_t i)
+{
+    {
         uint32_t p = __float_as_uint(Q.rayB[cur][i].w);
         const DevReq& rq = reqs[P.req[p]];
         const aph_frame& fp = rq.fp;
@@ -720,6 +786,33 @@ __global__ __launch_bounds__(kWB) void kResolve(const DevReq* reqs, const SegInf
         f.depth(depth + 1);
         P.flags[p] = f.v;
         if (done) finalizePath(P, p, fp, f);
+    }
+}
+
+// ---- resolve: the accumulation part of the RayGenShader loop (MainEntryPoints.slang:124-247) ----
+// Each wave walks whole tiles of the queue in order and copies the entries of
+// continuing paths (their next rays, written by kShade) to the next queue at
+// the tile's offset, so the next queue keeps this queue's order.
+__global__ __launch_bounds__(kWB) void kResolve(const DevReq* reqs, const SegInfo* segs, Paths P, Queues Q, int cur)
+{
+    const uint32_t n = Q.counts[cur];
+    const uint32_t nt = (n + kTile - 1) / kTile;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t waves = gridDim.x * (kWB / 64);
+    for (uint32_t t = blockIdx.x * (kWB / 64) + (threadIdx.x >> 6); t < nt; t += waves) {
+        uint32_t at = Q.tileOff[t];
+        for (uint32_t j = 0; j < kTile; j += 64) {
+            const uint32_t i = t * kTile + j + lane;
+            const bool cont = i < n && Q.cont[i] != 0;
+            const uint64_t m = __ballot(cont);
+            if (cont) {
+                const uint32_t o = at + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+                Q.rayA[cur ^ 1][o] = Q.rayA[cur][i];
+                Q.rayB[cur ^ 1][o] = Q.rayB[cur][i];
+            }
+            at += (uint32_t)__popcll(m);
+            if (i < n) resolvePath(reqs, P, Q, cur, i);
+        }
     }
 }
 
@@ -800,6 +893,8 @@ void enqueueChunk(hipStream_t stream, const Launch& L, uint32_t maxDepth, bool s
         }
         {
             EventScope ev(timing, "wf_resolve", stream);
+            kCount<<<L.light, kWB, 0, stream>>>(Q, cur);
+            kScan<<<1, kScanThreads, 0, stream>>>(Q, cur);
             kResolve<<<L.light, kWB, 0, stream>>>(dreq, gseg, P, Q, cur);
         }
         cur ^= 1;
@@ -823,7 +918,7 @@ hipError_t wavefrontRender(WavefrontState& st, const WavefrontRequest* reqs, uin
     // Chunk capacity: at least one full frame of the largest request, the tuned path count otherwise.
     uint32_t cap = std::max<uint32_t>(maxPix, tn.chunkPaths);
     size_t perPath = 4 + 4 + 8 + 16 + 4 + 16 * 4 + 16 * 3 + 16 * kSlots + 1 + kSlots + 16 +  // path state
-                     16 * 4 + 16 + 4 + 32 * kSlots;                                              // queues
+                     16 * 4 + 16 + 4 + 32 * kSlots + 1 + 1;                                      // queues
     const size_t kTableBytes = 1u << 20;  // chunk header + segment + chunk-request tables of one chunk
     const size_t kHdrBytes = 256;
     const uint32_t kMaxSegs = (uint32_t)((kTableBytes / 2 - kHdrBytes) / sizeof(SegInfo));
@@ -870,6 +965,9 @@ hipError_t wavefrontRender(WavefrontState& st, const WavefrontRequest* reqs, uin
     Q.shA = (float4*)take((size_t)cap * kSlots * 16);
     Q.shB = (float4*)take((size_t)cap * kSlots * 16);
     Q.counts = (uint32_t*)take(64);
+    Q.cont = (uint8_t*)take((size_t)cap);
+    Q.tileCnt = (uint32_t*)take(((size_t)cap + kTile - 1) / kTile * 4);
+    Q.tileOff = (uint32_t*)take(((size_t)cap + kTile - 1) / kTile * 4);
     char* tbl = (char*)take(kTableBytes);
     DevReq* dreq = (DevReq*)take(sizeof(DevReq) * nreq);  // last: its size varies, nothing after it moves
     if (off > need) return hipErrorOutOfMemory;
