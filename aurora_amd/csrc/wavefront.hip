@@ -668,7 +668,9 @@ __device__ __forceinline__ void finalizePath(const Paths& P, uint32_t p, const a
 
 // ---- stable compaction of the continuing rays: tile counts, then their prefix sum ----
 
-// One wave per kTile entries of the current queue (16 flags per lane).
+// One wave per kTile entries of the current queue (kTile / 64 flags per lane).
+constexpr uint32_t kFlagsPerLane = kTile / 64;
+static_assert(kTile % 256 == 0, "each lane counts whole 32-bit words of flags");
 __global__ __launch_bounds__(kWB) void kCount(Queues Q, int cur)
 {
     const uint32_t n = Q.counts[cur];
@@ -676,15 +678,15 @@ __global__ __launch_bounds__(kWB) void kCount(Queues Q, int cur)
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t waves = gridDim.x * (kWB / 64);
     for (uint32_t t = blockIdx.x * (kWB / 64) + (threadIdx.x >> 6); t < nt; t += waves) {
-        const uint32_t i0 = t * kTile + lane * 16;
+        const uint32_t i0 = t * kTile + lane * kFlagsPerLane;
         uint32_t s = 0;
-        if (i0 + 16 <= n) {
-            const uint4 v = *(const uint4*)(Q.cont + i0);
-            // flags are 0 or 1, so the byte sums of a word fit its top byte
-            s = ((v.x * 0x01010101u) >> 24) + ((v.y * 0x01010101u) >> 24) + ((v.z * 0x01010101u) >> 24) +
-                ((v.w * 0x01010101u) >> 24);
+        if (i0 + kFlagsPerLane <= n) {
+            const uint32_t* w = (const uint32_t*)(Q.cont + i0);
+            // flags are 0 or 1, so the byte sum of a word fits its top byte
+#pragma unroll
+            for (uint32_t k = 0; k < kFlagsPerLane / 4; k++) s += (w[k] * 0x01010101u) >> 24;
         } else {
-            for (uint32_t k = 0; k < 16; k++)
+            for (uint32_t k = 0; k < kFlagsPerLane; k++)
                 if (i0 + k < n) s += Q.cont[i0 + k];
         }
         s = waveSum(s);
