@@ -940,7 +940,7 @@ int aph_wavefront_autotune(aph_ctx_t c, const aph_request* reqs, uint32_t nreq, 
     if (rc) return rc;
     if ((rc = checkTextures(c))) return rc;
     if (!reqs || nreq == 0) return fail(c, APH_E_INVALID, "autotune needs at least one request");
-    const uint32_t chunks[] = {4u << 20, 8u << 20, 16u << 20, 32u << 20};
+    const uint32_t chunks[] = {4u << 20, 8u << 20, 16u << 20, 32u << 20, 64u << 20};
     const uint32_t traces[] = {4, 8, 16};
     const uint32_t shades[] = {4, 8, 16};
     const uint32_t refills[] = {16, 24, 32, 40, 48, 56, 64};

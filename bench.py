@@ -340,7 +340,7 @@ def equal_time_rmse(r, builder, width=480, height=270, ref_spp=4096, plain_spp=6
             "rmse_ratio": round(a / b, 3)}
 
 
-def autotune_section(r, sc, tlas, spp=16):
+def autotune_section(r, sc, tlas, spp=32):
     """aph_wavefront_autotune on the headline frame at reduced spp: the chosen launch
     configuration and its frame time against the default one (the context keeps
     the chosen one)."""
@@ -364,7 +364,7 @@ def autotune_section(r, sc, tlas, spp=16):
            "default_ms": round(base, 2),
            "chosen": [best.chunk_paths, best.trace_blocks_per_cu, best.shade_blocks_per_cu, best.trace_refill_lanes,
                       best.trace_leaf_wait_lanes],
-           "chosen_ms": round(tuned, 2), "candidates": 46}
+           "chosen_ms": round(tuned, 2), "candidates": 57}
     return out
 
 
@@ -574,7 +574,7 @@ def main():
     ap.add_argument("--tuning", default="",
                     help="wavefront launch configuration 'chunk_paths,trace_blocks_per_cu,shade_blocks_per_cu"
                          "[,trace_refill_lanes[,trace_leaf_wait_lanes]]'; "
-                         "default: the autotuner's choice on the headline frame (16 spp, child process)")
+                         "default: the autotuner's choice on the headline frame (32 spp, child process)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the library's default launch configuration")
     ap.add_argument("--autotune-child", action="store_true",
                     help="internal: run the autotuner on the headline frame and print its choice")

@@ -344,11 +344,11 @@ typedef struct {
 int aph_get_tuning(aph_ctx_t ctx, aph_tuning* t);
 int aph_set_tuning(aph_ctx_t ctx, const aph_tuning* t);
 /* Renders the given requests (into their targets) under each candidate launch
- * configuration - chunk of 4M/8M/16M/32M paths x 4/8/16 traversal blocks per
+ * configuration - chunk of 4M/8M/16M/32M/64M paths x 4/8/16 traversal blocks per
  * CU x 4/8/16 shade blocks per CU, then refill thresholds 16..64 (step 8) at
  * the best of those, then leaf-wait lanes 0/8/16/24/32 at the best of those -
  * and keeps the fastest for this context.
- * A chunk holds about 0.55 KB of device memory per path (17 GB at 32M paths),
+ * A chunk holds about 0.55 KB of device memory per path (35 GB at 64M paths),
  * allocated at the tuned size even for smaller frames. */
 int aph_wavefront_autotune(aph_ctx_t ctx, const aph_request* reqs, uint32_t nreq, aph_tuning* best);
 

@@ -277,7 +277,7 @@ def test_autotune_keeps_results(renderer, teapot_path):
     tgt = renderer.target(64, 48)
     best = renderer.autotune([renderer.request(tgt, tlas, sc.frame, 0, 4)])
     # the candidate set of aph_wavefront_autotune
-    assert best.ms > 0 and best.chunk_paths in (4 << 20, 8 << 20, 16 << 20, 32 << 20)
+    assert best.ms > 0 and best.chunk_paths in (4 << 20, 8 << 20, 16 << 20, 32 << 20, 64 << 20)
     assert best.trace_blocks_per_cu in (4, 8, 16) and best.shade_blocks_per_cu in (4, 8, 16)
     assert best.trace_refill_lanes in (16, 24, 32, 40, 48, 56, 64)
     assert best.trace_leaf_wait_lanes in (0, 8, 16, 24, 32)
