@@ -728,67 +728,65 @@ __global__ __launch_bounds__(kScanThreads) void kScan(Queues Q, int cur)
 // for queue entry i, after its shadow rays are traced.
 __device__ __forceinline__ void resolvePath(const DevReq* reqs, const Paths& P, const Queues& Q, int cur, uint32_t i)
 {
-    {
-        uint32_t p = __float_as_uint(Q.rayB[cur][i].w);
-        const DevReq& rq = reqs[P.req[p]];
-        const aph_frame& fp = rq.fp;
-        Flags f{P.flags[p]};
-        uint32_t depth = f.depth();
-        uint32_t ev = f.event();
-        if (ev != kEvSkip) {
-            f3 direct = xyz(P.pendDirect[p]);
-            f3 indirect = splat(0.0f);
-            if (ev == kEvHit) {
-                uint8_t mask = P.shMask[p];
-                const size_t b = (size_t)p * kSlots;
+    uint32_t p = __float_as_uint(Q.rayB[cur][i].w);
+    const DevReq& rq = reqs[P.req[p]];
+    const aph_frame& fp = rq.fp;
+    Flags f{P.flags[p]};
+    uint32_t depth = f.depth();
+    uint32_t ev = f.event();
+    if (ev != kEvSkip) {
+        f3 direct = xyz(P.pendDirect[p]);
+        f3 indirect = splat(0.0f);
+        if (ev == kEvHit) {
+            uint8_t mask = P.shMask[p];
+            const size_t b = (size_t)p * kSlots;
 #pragma unroll
-                for (int k = 0; k < 4; k++)
-                    if (mask & (1u << k)) direct = direct + xyz(P.shColor[b + k]) * (P.vis[b + k] ? 1.0f : 0.0f);
-                if (mask & ((1u << kSlotMisMat) | (1u << kSlotMisLight))) {
-                    f3 mis = splat(0.0f);
-                    if (mask & (1u << kSlotMisMat))
-                        mis = mis + xyz(P.shColor[b + kSlotMisMat]) * (P.vis[b + kSlotMisMat] ? 1.0f : 0.0f);
-                    if (mask & (1u << kSlotMisLight))
-                        mis = mis + xyz(P.shColor[b + kSlotMisLight]) * (P.vis[b + kSlotMisLight] ? 1.0f : 0.0f);
-                    indirect = mis;
-                }
-                f3 c = xyz(P.pendContrib[p]);
-                direct = direct * c;
-                indirect = indirect * c;
+            for (int k = 0; k < 4; k++)
+                if (mask & (1u << k)) direct = direct + xyz(P.shColor[b + k]) * (P.vis[b + k] ? 1.0f : 0.0f);
+            if (mask & ((1u << kSlotMisMat) | (1u << kSlotMisLight))) {
+                f3 mis = splat(0.0f);
+                if (mask & (1u << kSlotMisMat))
+                    mis = mis + xyz(P.shColor[b + kSlotMisMat]) * (P.vis[b + kSlotMisMat] ? 1.0f : 0.0f);
+                if (mask & (1u << kSlotMisLight))
+                    mis = mis + xyz(P.shColor[b + kSlotMisLight]) * (P.vis[b + kSlotMisLight] ? 1.0f : 0.0f);
+                indirect = mis;
             }
-            if (depth == 0) {
-                f.primary(false);
-                P.outDirect[p] = f4(direct, 0.0f);
-                if (f.misLobe() == (uint32_t)kDiffuseLobe) P.outDiffuse[p] = f4(indirect, 0.0f);
-                else P.outGlossy[p] = f4(indirect, 0.0f);
-            } else {
-                f3 radiance = direct + indirect;
-                if (f.rayType() == (uint32_t)kDiffuseLobe) P.outDiffuse[p] = f4(xyz(P.outDiffuse[p]) + radiance, 0.0f);
-                else P.outGlossy[p] = f4(xyz(P.outGlossy[p]) + radiance, 0.0f);
-            }
-            if (ev == kEvHit) f.rayType(f.nextRayType());
-            P.lastIndirect[p] = f4(indirect, 0.0f);
-        } else {
-            P.lastIndirect[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            f3 c = xyz(P.pendContrib[p]);
+            direct = direct * c;
+            indirect = indirect * c;
         }
-        bool done = f.ended();
-        if (!done && f.terminal()) {
-            // rayDepth == traceDepth: unoccluded environment in the next direction plus the
-            // previous segment's indirect, which the reference does not clear (:127-138, :198)
-            uint32_t pix = P.pix[p];
-            bool bg = f.primary() || f.rayType() == (uint32_t)kTransmissionLobe ||
-                      f.rayType() == (uint32_t)kGroundPlaneReflectionRay;
-            f3 dterm = evalEnvironment(rq.rs.env, xyz(P.nextDir[p]), bg, pix % rq.W, pix / rq.W, rq.W, rq.H) *
-                       xyz(P.contrib[p]);
-            f3 radiance = dterm + xyz(P.lastIndirect[p]);
+        if (depth == 0) {
+            f.primary(false);
+            P.outDirect[p] = f4(direct, 0.0f);
+            if (f.misLobe() == (uint32_t)kDiffuseLobe) P.outDiffuse[p] = f4(indirect, 0.0f);
+            else P.outGlossy[p] = f4(indirect, 0.0f);
+        } else {
+            f3 radiance = direct + indirect;
             if (f.rayType() == (uint32_t)kDiffuseLobe) P.outDiffuse[p] = f4(xyz(P.outDiffuse[p]) + radiance, 0.0f);
             else P.outGlossy[p] = f4(xyz(P.outGlossy[p]) + radiance, 0.0f);
-            done = true;
         }
-        f.depth(depth + 1);
-        P.flags[p] = f.v;
-        if (done) finalizePath(P, p, fp, f);
+        if (ev == kEvHit) f.rayType(f.nextRayType());
+        P.lastIndirect[p] = f4(indirect, 0.0f);
+    } else {
+        P.lastIndirect[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
+    bool done = f.ended();
+    if (!done && f.terminal()) {
+        // rayDepth == traceDepth: unoccluded environment in the next direction plus the
+        // previous segment's indirect, which the reference does not clear (:127-138, :198)
+        uint32_t pix = P.pix[p];
+        bool bg = f.primary() || f.rayType() == (uint32_t)kTransmissionLobe ||
+                  f.rayType() == (uint32_t)kGroundPlaneReflectionRay;
+        f3 dterm = evalEnvironment(rq.rs.env, xyz(P.nextDir[p]), bg, pix % rq.W, pix / rq.W, rq.W, rq.H) *
+                   xyz(P.contrib[p]);
+        f3 radiance = dterm + xyz(P.lastIndirect[p]);
+        if (f.rayType() == (uint32_t)kDiffuseLobe) P.outDiffuse[p] = f4(xyz(P.outDiffuse[p]) + radiance, 0.0f);
+        else P.outGlossy[p] = f4(xyz(P.outGlossy[p]) + radiance, 0.0f);
+        done = true;
+    }
+    f.depth(depth + 1);
+    P.flags[p] = f.v;
+    if (done) finalizePath(P, p, fp, f);
 }
 
 // ---- resolve: the accumulation part of the RayGenShader loop (MainEntryPoints.slang:124-247) ----
