@@ -109,6 +109,36 @@ DevMaterial toDev(const aph_material& a)
     return m;
 }
 
+
+// Validated device records of render requests (targets, TLASes, sample ranges, lights).
+int buildRequests(aph_ctx* c, const aph_request* reqs, uint32_t nreq, std::vector<WavefrontRequest>& out)
+{
+    out.clear();
+    out.reserve(nreq);
+    for (uint32_t i = 0; i < nreq; i++) {
+        const aph_request& r = reqs[i];
+        Target* tg = lookup(c->targets, r.target);
+        Tlas* tl = lookup(c->tlases, r.tlas);
+        if (!tg) return fail(c, APH_E_INVALID, "request %u: unknown target", i);
+        if (!tl) return fail(c, APH_E_INVALID, "request %u: unknown tlas", i);
+        if ((uint64_t)r.sample_start + r.sample_count > 0xFFFFFFFFull)
+            return fail(c, APH_E_INVALID, "request %u: sample range overflows", i);
+        if (r.frame.light_count < 0 || r.frame.light_count > 4)
+            return fail(c, APH_E_INVALID, "request %u: light_count must be in [0,4]", i);
+        WavefrontRequest w;
+        w.rs = renderScene(c, tl);
+        w.frame = clampFrame(r.frame);
+        w.width = tg->w;
+        w.height = tg->h;
+        w.sampleStart = r.sample_start;
+        w.sampleCount = r.sample_count;
+        w.seedOffset = r.seed_offset;
+        w.accum = tg->accum.as<float4>();
+        out.push_back(w);
+    }
+    return APH_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -149,7 +179,7 @@ void aph_destroy(aph_ctx_t c)
     c->meshes.clear();
     c->tlases.clear();
     c->targets.clear();
-    wavefrontRelease(c->wf);
+    wavefrontRelease(c->wf, c->stream);
     hipStreamDestroy(c->stream);
     delete c;
 }
@@ -856,43 +886,28 @@ int aph_render(aph_ctx_t c, const aph_request* reqs, uint32_t nreq, int mode)
     if ((rc = checkTextures(c))) return rc;
     if (!reqs) return fail(c, APH_E_INVALID, "requests pointer is null");
     if (mode != APH_MODE_MEGAKERNEL && mode != APH_MODE_WAVEFRONT) return fail(c, APH_E_INVALID, "unknown mode %d", mode);
+    // Every request is checked before any is launched, so a bad request renders nothing.
     std::vector<WavefrontRequest> wreq;
-    for (uint32_t i = 0; i < nreq; i++) {
-        const aph_request& r = reqs[i];
-        Target* tg = lookup(c->targets, r.target);
-        Tlas* tl = lookup(c->tlases, r.tlas);
-        if (!tg) return fail(c, APH_E_INVALID, "request %u: unknown target", i);
-        if (!tl) return fail(c, APH_E_INVALID, "request %u: unknown tlas", i);
-        if ((uint64_t)r.sample_start + r.sample_count > 0xFFFFFFFFull)
-            return fail(c, APH_E_INVALID, "request %u: sample range overflows", i);
-        if (r.frame.light_count < 0 || r.frame.light_count > 4)
-            return fail(c, APH_E_INVALID, "request %u: light_count must be in [0,4]", i);
-        aph_frame fp = clampFrame(r.frame);
-        RenderScene rs = renderScene(c, tl);
-        if (mode == APH_MODE_MEGAKERNEL) {
+    if ((rc = buildRequests(c, reqs, nreq, wreq))) return rc;
+    if (mode == APH_MODE_MEGAKERNEL) {
+        for (const WavefrontRequest& w : wreq) {
             TimedScope ts(c, "render_megakernel");
-            HIPCALL(c, launchRenderMega(rs, fp, tg->w, tg->h, r.sample_start, r.sample_count, r.seed_offset,
-                                        tg->accum.as<float4>(), c->counters.as<DevCounters>(), c->statsLevel > 0,
-                                        c->stream),
+            HIPCALL(c, launchRenderMega(w.rs, w.frame, w.width, w.height, w.sampleStart, w.sampleCount, w.seedOffset,
+                                        w.accum, c->counters.as<DevCounters>(), c->statsLevel > 0, c->stream),
                     "megakernel launch");
-        } else {
-            WavefrontRequest w;
-            w.rs = rs;
-            w.frame = fp;
-            w.width = tg->w;
-            w.height = tg->h;
-            w.sampleStart = r.sample_start;
-            w.sampleCount = r.sample_count;
-            w.seedOffset = r.seed_offset;
-            w.accum = tg->accum.as<float4>();
-            wreq.push_back(w);
         }
+        return APH_OK;
     }
-    if (mode == APH_MODE_WAVEFRONT) {
-        hipError_t e = wavefrontRender(c->wf, wreq.data(), (uint32_t)wreq.size(), c->counters.as<DevCounters>(),
-                                       c->statsLevel > 0, c->stream, c->timing ? &c->pending : nullptr);
-        if (e != hipSuccess) return hipFail(c, e, "wavefront render");
-    }
+    // The wavefront folds each chunk's samples into every target in one pass, so
+    // two requests on one target would race: one target per request of a call.
+    for (uint32_t i = 0; i < nreq; i++)
+        for (uint32_t k = 0; k < i; k++)
+            if (reqs[k].target == reqs[i].target)
+                return fail(c, APH_E_INVALID, "requests %u and %u share target %llu (one request per target per call)",
+                            k, i, (unsigned long long)reqs[i].target);
+    hipError_t e = wavefrontRender(c->wf, wreq.data(), (uint32_t)wreq.size(), c->counters.as<DevCounters>(),
+                                   c->statsLevel > 0, c->stream, c->timing ? &c->pending : nullptr);
+    if (e != hipSuccess) return hipFail(c, e, "wavefront render");
     return APH_OK;
 }
 
@@ -940,26 +955,44 @@ int aph_wavefront_autotune(aph_ctx_t c, const aph_request* reqs, uint32_t nreq, 
     if (rc) return rc;
     if ((rc = checkTextures(c))) return rc;
     if (!reqs || nreq == 0) return fail(c, APH_E_INVALID, "autotune needs at least one request");
+    std::vector<WavefrontRequest> wreq;
+    if ((rc = buildRequests(c, reqs, nreq, wreq))) return rc;
+    // Trials render into scratch targets of the requests' sizes, so the caller's
+    // targets are untouched (a request with sample_start > 0 would blend into them).
+    std::vector<DevBuf> scratch(nreq);
+    for (uint32_t i = 0; i < nreq; i++) {
+        const size_t bytes = (size_t)wreq[i].width * wreq[i].height * sizeof(float4);
+        HIPCALL(c, scratch[i].alloc(bytes), "autotune scratch target");
+        HIPCALL(c, hipMemsetAsync(scratch[i].p, 0, bytes, c->stream), "autotune scratch target");
+        wreq[i].accum = scratch[i].as<float4>();
+    }
     const uint32_t chunks[] = {4u << 20, 8u << 20, 16u << 20, 32u << 20, 64u << 20};
     const uint32_t traces[] = {4, 8, 16};
     const uint32_t shades[] = {4, 8, 16};
     const uint32_t refills[] = {16, 24, 32, 40, 48, 56, 64};
     const uint32_t leafWaits[] = {0, 8, 16, 24, 32};
-    aph_tuning keep{};
-    aph_get_tuning(c, &keep);
-    aph_tuning win = keep;
-    win.ms = INFINITY;
+    const WavefrontTuning keep = c->wf.tuning;
+    aph_tuning win{keep.chunkPaths, keep.tracePerCU, keep.shadePerCU, keep.refillLanes, keep.leafWaitLanes, INFINITY};
     hipEvent_t a, b;
     HIPCALL(c, hipEventCreate(&a), "autotune");
     HIPCALL(c, hipEventCreate(&b), "autotune");
     // Renders the requests twice under t (the first run captures the graph for
-    // this launch shape) and times the second; keeps t if it is the fastest.
+    // this launch shape) and times the second; keeps t if it is the fastest.  A
+    // candidate whose buffers do not fit in device memory is skipped.
     auto trial = [&](aph_tuning t) -> int {
-        int st;
-        if ((st = aph_set_tuning(c, &t)) != APH_OK) return st;
-        if ((st = aph_render(c, reqs, nreq, APH_MODE_WAVEFRONT)) != APH_OK) return st;
+        if (hipStreamSynchronize(c->stream) != hipSuccess) return APH_E_DEVICE;
+        c->wf.tuning = WavefrontTuning{t.chunk_paths, t.trace_blocks_per_cu, t.shade_blocks_per_cu,
+                                       t.trace_refill_lanes, t.trace_leaf_wait_lanes};
+        DevCounters* ctr = c->counters.as<DevCounters>();
+        hipError_t e = wavefrontRender(c->wf, wreq.data(), nreq, ctr, c->statsLevel > 0, c->stream, nullptr);
+        if (e == hipErrorOutOfMemory) {
+            (void)hipGetLastError();
+            return APH_OK;
+        }
+        if (e != hipSuccess) return hipFail(c, e, "autotune render");
         if (hipEventRecord(a, c->stream) != hipSuccess) return APH_E_DEVICE;
-        if ((st = aph_render(c, reqs, nreq, APH_MODE_WAVEFRONT)) != APH_OK) return st;
+        e = wavefrontRender(c->wf, wreq.data(), nreq, ctr, c->statsLevel > 0, c->stream, nullptr);
+        if (e != hipSuccess) return hipFail(c, e, "autotune render");
         if (hipEventRecord(b, c->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess) return APH_E_DEVICE;
         float ms = 0.0f;
         hipEventElapsedTime(&ms, a, b);
@@ -976,8 +1009,7 @@ int aph_wavefront_autotune(aph_ctx_t c, const aph_request* reqs, uint32_t nreq, 
     for (uint32_t ch : chunks)
         for (uint32_t tr : traces)
             for (uint32_t sh : shades)
-                if (!status)
-                    status = trial(aph_tuning{ch, tr, sh, keep.trace_refill_lanes, keep.trace_leaf_wait_lanes, 0.0f});
+                if (!status) status = trial(aph_tuning{ch, tr, sh, keep.refillLanes, keep.leafWaitLanes, 0.0f});
     const aph_tuning grid = win;
     for (uint32_t rf : refills)
         if (!status && rf != grid.trace_refill_lanes)
@@ -990,12 +1022,14 @@ int aph_wavefront_autotune(aph_ctx_t c, const aph_request* reqs, uint32_t nreq, 
                                       withRefill.shade_blocks_per_cu, withRefill.trace_refill_lanes, lw, 0.0f});
     hipEventDestroy(a);
     hipEventDestroy(b);
+    hipStreamSynchronize(c->stream);  // the scratch targets are freed on return
+    if (!status && !std::isfinite(win.ms)) status = fail(c, APH_E_MEMORY, "no candidate fits in device memory");
     if (status) {
-        aph_set_tuning(c, &keep);
+        c->wf.tuning = keep;
         return status;
     }
-    rc = aph_set_tuning(c, &win);
-    if (rc) return rc;
+    c->wf.tuning = WavefrontTuning{win.chunk_paths, win.trace_blocks_per_cu, win.shade_blocks_per_cu,
+                                   win.trace_refill_lanes, win.trace_leaf_wait_lanes};
     if (best) *best = win;
     return APH_OK;
 }

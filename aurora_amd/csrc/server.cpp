@@ -6,8 +6,12 @@
 // Here jobs of uneven resolution and spp arrive against TLASes already built
 // in the context.  Each step
 //   1. admits waiting jobs in arrival order while a slot is free and the HBM
-//      budget (resident scenes + every admitted job's accumulation buffer)
-//      allows, allocating the job's RGBA32F accumulation buffer;
+//      budget allows, allocating the job's RGBA32F accumulation buffer.  The
+//      budget covers the resident scenes (BVHs, meshes, instance records, their
+//      environments), the images bound to the material table, every admitted
+//      job's accumulation buffer and the wavefront working set (path state,
+//      queues and tables of the paths in flight), whose chunk is lowered from
+//      the context's tuned one until it fits;
 //   2. gives every admitted job its next run of sample indices - an equal
 //      number per job, sized so the whole batch is about step_samples
 //      pixel-samples - and renders them all in one shared wavefront;
@@ -54,7 +58,10 @@ struct aph_server {
     std::deque<Job*> pending;
     std::vector<Job*> active;
     std::set<const Tlas*> resident;  // scenes charged to the budget
-    size_t residentBytes = 0, jobBytes = 0;
+    size_t sceneBytes = 0, jobBytes = 0;
+    size_t residentBytes = 0;        // scenes + material images, refreshed at each admission
+    size_t wfBytes = 0;              // wavefront working set of the last batched step
+    uint32_t wfPaths = 0;            // its paths in flight
     uint64_t samplesRendered = 0;
 
     double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
@@ -63,8 +70,48 @@ struct aph_server {
 namespace {
 
 constexpr uint32_t kDefaultStepSamples = 8u << 20;
+constexpr uint32_t kMaxQuantum = 1024;  // samples of one job per step (bounds the step's tables)
 
 size_t accumBytes(const aph_job& j) { return (size_t)j.width * j.height * sizeof(float4); }
+
+uint32_t pixels(const Job* j) { return j->desc.width * j->desc.height; }
+
+// Device bytes of the images bound to the context's material table, each counted once.
+size_t textureBytes(const aph_ctx* c)
+{
+    std::set<uint64_t> seen;
+    size_t bytes = 0;
+    for (const aph_texture_binding& b : c->texBindings) {
+        if (!b.image || !seen.insert(b.image).second) continue;
+        auto it = c->images.find(b.image);
+        if (it != c->images.end()) bytes += it->second->texels.bytes;
+    }
+    return bytes;
+}
+
+// Upper bound of a step's call table: one request record and up to kMaxQuantum
+// sample segments (and chunk records) per active job.
+size_t tableReserve(size_t jobs)
+{
+    return ((size_t)1 << 20) + jobs * (sizeof(WavefrontRequest) + (size_t)kMaxQuantum * 48);
+}
+
+// Wavefront working set for `paths` paths in flight and `jobs` requests.
+size_t workingSet(uint32_t paths, size_t jobs) { return wavefrontBufferBytes(paths) + tableReserve(jobs); }
+
+// Paths in flight of a batched step: the context's tuned chunk, halved (down to
+// the largest active frame, which one chunk must hold) until the working set
+// fits what the budget leaves after scenes and job buffers.
+uint32_t stepPaths(const aph_server* s, uint32_t maxPix)
+{
+    uint32_t cap = std::max(s->ctx->wf.tuning.chunkPaths, maxPix);
+    const size_t budget = s->cfg.hbm_budget;
+    if (!budget) return cap;
+    const size_t used = s->residentBytes + s->jobBytes;
+    const size_t avail = budget > used ? budget - used : 0;
+    while (cap > maxPix && workingSet(cap, s->active.size()) > avail) cap = std::max(maxPix, cap / 2);
+    return cap;
+}
 
 // Admits waiting jobs in arrival order.  Returns APH_OK or an error for a job
 // that can never fit.
@@ -72,16 +119,23 @@ int admit(aph_server* s)
 {
     const bool plain = s->cfg.mode == APH_MODE_MEGAKERNEL;
     const size_t slots = plain ? 1 : (s->cfg.max_active ? s->cfg.max_active : (size_t)-1);
+    s->residentBytes = s->sceneBytes + textureBytes(s->ctx);
+    uint32_t maxPix = 0;
+    for (const Job* j : s->active) maxPix = std::max(maxPix, pixels(j));
     while (!s->pending.empty() && s->active.size() < slots) {
         Job* j = s->pending.front();
         size_t need = accumBytes(j->desc);
+        // batched jobs also need the smallest working set that holds one frame of every active job
+        const uint32_t pix = std::max(maxPix, pixels(j));
+        const size_t wf = plain ? 0 : workingSet(pix, s->active.size() + 1);
         size_t budget = s->cfg.hbm_budget;
-        if (budget && s->residentBytes + s->jobBytes + need > budget) {
+        if (budget && s->residentBytes + s->jobBytes + need + wf > budget) {
             if (s->active.empty() && s->jobBytes == 0) {
                 s->pending.pop_front();
                 j->state = APH_JOB_FAILED;
-                return fail(s->ctx, APH_E_MEMORY, "job %llu needs %zu bytes beyond the resident scenes; budget %zu",
-                            (unsigned long long)j->ticket, need, budget);
+                return fail(s->ctx, APH_E_MEMORY,
+                            "job %llu needs %zu bytes (image) + %zu (working set) beyond the resident %zu; budget %zu",
+                            (unsigned long long)j->ticket, need, wf, s->residentBytes, budget);
             }
             break;  // wait for running jobs to return memory
         }
@@ -94,6 +148,7 @@ int admit(aph_server* s)
         j->tAdmit = s->now();
         s->pending.pop_front();
         s->active.push_back(j);
+        maxPix = pix;
     }
     return APH_OK;
 }
@@ -131,9 +186,14 @@ int stepOnce(aph_server* s)
         return APH_OK;
     }
     uint64_t pix = 0;
-    for (Job* j : s->active) pix += (uint64_t)j->desc.width * j->desc.height;
+    uint32_t maxPix = 0;
+    for (Job* j : s->active) {
+        pix += pixels(j);
+        maxPix = std::max(maxPix, pixels(j));
+    }
     uint64_t stepSamples = s->cfg.step_samples ? s->cfg.step_samples : kDefaultStepSamples;
-    uint32_t quantum = (uint32_t)std::max<uint64_t>(1, stepSamples / std::max<uint64_t>(pix, 1));
+    uint32_t quantum = (uint32_t)std::min<uint64_t>(
+        kMaxQuantum, std::max<uint64_t>(1, stepSamples / std::max<uint64_t>(pix, 1)));
     std::vector<WavefrontRequest> batch;
     std::vector<uint32_t> counts;
     batch.reserve(s->active.size());
@@ -151,9 +211,14 @@ int stepOnce(aph_server* s)
         batch.push_back(w);
         counts.push_back(n);
     }
+    const uint32_t paths = stepPaths(s, maxPix);
+    const size_t wf = wavefrontBufferBytes(paths) +
+                      std::max(wavefrontCallTableBytes(batch.data(), (uint32_t)batch.size(), paths), (size_t)1 << 20);
     hipError_t e = wavefrontRender(c->wf, batch.data(), (uint32_t)batch.size(), c->counters.as<DevCounters>(),
-                                   c->statsLevel > 0, c->stream, c->timing ? &c->pending : nullptr);
+                                   c->statsLevel > 0, c->stream, c->timing ? &c->pending : nullptr, paths);
     if (e != hipSuccess) return hipFail(c, e, "serving step");
+    s->wfPaths = paths;
+    s->wfBytes = std::max(wf, c->wf.bytes + c->wf.tblBytes);
     HIPCALL(c, hipStreamSynchronize(c->stream), "serving step");
     std::vector<Job*> still;
     for (size_t k = 0; k < s->active.size(); k++) {
@@ -218,10 +283,11 @@ int aph_server_submit(aph_server_t s, const aph_job* job, double tArrival, uint6
     if (!s->resident.count(tl)) {
         // BVHs, instance and shading records, and the scene's own environment images
         const size_t bytes = tl->bytes + (tl->environment ? tl->environment->bytes() : 0);
-        if (s->cfg.hbm_budget && s->residentBytes + bytes > s->cfg.hbm_budget)
+        if (s->cfg.hbm_budget && s->sceneBytes + textureBytes(c) + bytes > s->cfg.hbm_budget)
             return fail(c, APH_E_MEMORY, "scene of %zu bytes does not fit the budget", bytes);
         s->resident.insert(tl);
-        s->residentBytes += bytes;
+        s->sceneBytes += bytes;
+        s->residentBytes = s->sceneBytes + textureBytes(c);
     }
     std::unique_ptr<Job> j(new Job());
     j->ticket = s->nextTicket++;
@@ -284,6 +350,14 @@ int aph_server_usage(aph_server_t s, uint64_t* residentBytes, uint64_t* jobBytes
     if (jobBytes) *jobBytes = s->jobBytes;
     if (active) *active = (uint32_t)s->active.size();
     if (pending) *pending = (uint32_t)s->pending.size();
+    return APH_OK;
+}
+
+int aph_server_working_set(aph_server_t s, uint64_t* wavefront_bytes, uint32_t* paths_in_flight)
+{
+    if (!s) return APH_E_INVALID;
+    if (wavefront_bytes) *wavefront_bytes = s->wfBytes;
+    if (paths_in_flight) *paths_in_flight = s->wfPaths;
     return APH_OK;
 }
 

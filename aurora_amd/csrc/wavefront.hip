@@ -849,6 +849,77 @@ __global__ void kAccum(const DevReq* reqs, const SegInfo* segs, const ChunkReq* 
 
 size_t alignUp(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// ---- call table: request records and every chunk's tables, uploaded once per call ----
+// Layout (byte offsets in CallHdr): CallHdr | DevReq[nreq] | ChunkDesc[nchunk] |
+// SegInfo[all segments] | ChunkReq[all chunk requests].  kChunkNext, the first
+// node of every chunk's launch sequence, copies chunk `cursor`'s tables to the
+// fixed per-chunk addresses the stages read; kChunkBump advances the cursor.
+struct CallHdr {
+    uint32_t cursor, nchunk, nreq, pad;
+    uint32_t descOff, segOff, creqOff, pad2;
+};
+struct ChunkDesc {
+    ChunkHdr hdr;
+    uint32_t segBase, creqBase, pad[2];
+};
+constexpr size_t kCallHdrBytes = 256;  // DevReq records start here
+
+__global__ void kChunkNext(const uint8_t* tbl, ChunkHdr* ghdr, SegInfo* gseg, ChunkReq* gcr)
+{
+    const CallHdr* call = (const CallHdr*)tbl;
+    const uint32_t c = call->cursor;
+    const ChunkDesc d = ((const ChunkDesc*)(tbl + call->descOff))[c];
+    const SegInfo* segs = (const SegInfo*)(tbl + call->segOff) + d.segBase;
+    const ChunkReq* creqs = (const ChunkReq*)(tbl + call->creqOff) + d.creqBase;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) *ghdr = d.hdr;
+    for (uint32_t i = t; i < d.hdr.nseg; i += stride) gseg[i] = segs[i];
+    for (uint32_t i = t; i < d.hdr.ncreq; i += stride) gcr[i] = creqs[i];
+}
+
+__global__ void kChunkBump(uint8_t* tbl)
+{
+    if (threadIdx.x == 0) ((CallHdr*)tbl)->cursor += 1;
+}
+
+// Host -> device upload through kernel arguments: the runtime copies a launch's
+// arguments when the launch is enqueued, so the host image may be freed as soon
+// as the call returns and no host memory stays referenced by queued work.
+constexpr uint32_t kUploadWords = 504;  // 2 KB of payload per launch
+struct UploadArgs {
+    uint32_t n, pad[3];
+    uint32_t w[kUploadWords];
+};
+
+__global__ void kUpload(uint32_t* dst, UploadArgs a)
+{
+    for (uint32_t i = threadIdx.x; i < a.n; i += blockDim.x) dst[i] = a.w[i];
+}
+
+hipError_t upload(void* dst, const std::vector<uint32_t>& words, hipStream_t stream)
+{
+    UploadArgs a;
+    std::memset(&a, 0, sizeof(a));
+    for (size_t k = 0; k < words.size(); k += kUploadWords) {
+        a.n = (uint32_t)std::min<size_t>(kUploadWords, words.size() - k);
+        std::memcpy(a.w, words.data() + k, a.n * 4);
+        kUpload<<<1, 256, 0, stream>>>((uint32_t*)dst + k, a);
+    }
+    return hipGetLastError();
+}
+
+constexpr uint32_t kChunkBlocks = 16;
+
+// Bytes per path of the path state and queues (the Paths and Queues arrays).
+constexpr size_t kPerPath = 4 + 4 + 8 + 16 + 4 + 16 * 4 + 16 * 3 + 16 * kSlots + 1 + kSlots + 16 +  // path state
+                            16 * 4 + 16 + 4 + 32 * kSlots + 1 + 1;                                      // queues
+constexpr size_t kTableBytes = 1u << 20;  // the chunk in flight: header + segment + chunk-request tables
+constexpr size_t kHdrBytes = 256;
+constexpr uint32_t kMaxSegs = (uint32_t)((kTableBytes / 2 - kHdrBytes) / sizeof(SegInfo));
+constexpr uint32_t kMaxCreqs = (uint32_t)(kTableBytes / 2 / sizeof(ChunkReq));
+constexpr size_t kFixedBytes = 64 * 1024 + kTableBytes;  // alignment padding, tile tables, counters; chunk tables
+
 }  // namespace
 
 namespace {
@@ -862,15 +933,17 @@ struct Launch {
     uint32_t leafWait;  // lanes still looking for a leaf when a traversal wave processes leaves
 };
 
-// Everything one chunk runs after its tables are in place: generation, then per
-// bounce the reset, extension, shading, shadow and resolve stages, then
-// accumulation.  Timed per stage with HIP events when timing is given.
-void enqueueChunk(hipStream_t stream, const Launch& L, uint32_t maxDepth, bool stats, const DevReq* dreq,
-                  const SegInfo* gseg, const ChunkReq* gcr, const ChunkHdr* hdr, const Paths& P, const Queues& Q,
+// Everything one chunk runs: its tables moved into place from the call table,
+// generation, then per bounce the reset, extension, shading, shadow and resolve
+// stages, then accumulation.  Timed per stage with HIP events when timing is given.
+void enqueueChunk(hipStream_t stream, const Launch& L, uint32_t maxDepth, bool stats, uint8_t* tbl,
+                  const DevReq* dreq, SegInfo* gseg, ChunkReq* gcr, ChunkHdr* hdr, const Paths& P, const Queues& Q,
                   DevCounters* ctr, std::vector<PendingEvent>* timing)
 {
     {
         EventScope ev(timing, "wf_gen", stream);
+        kChunkNext<<<kChunkBlocks, 256, 0, stream>>>(tbl, hdr, gseg, gcr);
+        kChunkBump<<<1, 64, 0, stream>>>(tbl);
         kReset<<<1, 64, 0, stream>>>(Q.counts, hdr);
         kGen<<<L.gen, kWB, 0, stream>>>(dreq, gseg, hdr, P, Q, ctr);
     }
@@ -905,30 +978,140 @@ void enqueueChunk(hipStream_t stream, const Launch& L, uint32_t maxDepth, bool s
     }
 }
 
+// Segments of a call: (request, sample) pairs in request-major, sample-ascending
+// order, cut into chunks of at most `cap` paths and the per-chunk table limits.
+struct CallPlan {
+    std::vector<ChunkDesc> desc;
+    std::vector<SegInfo> segs;
+    std::vector<ChunkReq> creqs;
+};
+
+bool planCall(const WavefrontRequest* reqs, uint32_t nreq, uint32_t cap, CallPlan& plan)
+{
+    uint32_t total = 0, totalPix = 0, nseg = 0, ncreq = 0;
+    auto open = [&]() {
+        plan.desc.push_back(ChunkDesc{});
+        plan.desc.back().segBase = (uint32_t)plan.segs.size();
+        plan.desc.back().creqBase = (uint32_t)plan.creqs.size();
+        total = totalPix = nseg = ncreq = 0;
+    };
+    auto close = [&]() { plan.desc.back().hdr = ChunkHdr{total, nseg, ncreq, totalPix}; };
+    open();
+    for (uint32_t i = 0; i < nreq; i++) {
+        const uint32_t count = reqs[i].width * reqs[i].height;
+        if (count > cap) return false;
+        for (uint32_t s = 0; s < reqs[i].sampleCount; s++) {
+            const bool sameReq = ncreq > 0 && plan.creqs.back().req == i;
+            if (total + count > cap || nseg == kMaxSegs || (!sameReq && ncreq == kMaxCreqs)) {
+                close();
+                open();
+            }
+            if (ncreq == 0 || plan.creqs.back().req != i) {
+                plan.creqs.push_back(ChunkReq{i, nseg, 0, totalPix});
+                ncreq++;
+                totalPix += count;
+            }
+            plan.creqs.back().nSeg++;
+            plan.segs.push_back(SegInfo{i, reqs[i].sampleStart + s, total, count});
+            nseg++;
+            total += count;
+        }
+    }
+    close();
+    if (plan.desc.back().hdr.nseg == 0) plan.desc.pop_back();
+    return true;
+}
+
+size_t callTableBytes(uint32_t nreq, size_t nchunk, size_t nseg, size_t ncreq)
+{
+    return kCallHdrBytes + alignUp(sizeof(DevReq) * nreq) + alignUp(sizeof(ChunkDesc) * nchunk) +
+           alignUp(sizeof(SegInfo) * nseg) + alignUp(sizeof(ChunkReq) * ncreq);
+}
+
 }  // namespace
 
+size_t wavefrontBufferBytes(uint32_t paths) { return alignUp((size_t)paths * kPerPath) + kFixedBytes; }
+
+size_t wavefrontCallTableBytes(const WavefrontRequest* reqs, uint32_t nreq, uint32_t paths)
+{
+    size_t nseg = 0;
+    uint64_t work = 0;
+    for (uint32_t i = 0; i < nreq; i++) {
+        nseg += reqs[i].sampleCount;
+        work += (uint64_t)reqs[i].width * reqs[i].height * reqs[i].sampleCount;
+    }
+    // every chunk but the last holds more than half the capacity or hits a table limit
+    size_t nchunk = 1 + (size_t)(2 * work / std::max<uint32_t>(paths, 1)) + nseg / kMaxSegs + nseg / kMaxCreqs;
+    nchunk = std::min(nchunk, nseg + 1);
+    return callTableBytes(nreq, nchunk, nseg, nseg);
+}
+
+uint32_t wavefrontCapacity(const WavefrontState& st, const WavefrontRequest* reqs, uint32_t nreq, uint32_t chunkPaths)
+{
+    uint32_t maxPix = 0;
+    for (uint32_t i = 0; i < nreq; i++) maxPix = std::max(maxPix, reqs[i].width * reqs[i].height);
+    return std::max<uint32_t>(maxPix, chunkPaths ? chunkPaths : st.tuning.chunkPaths);
+}
+
 hipError_t wavefrontRender(WavefrontState& st, const WavefrontRequest* reqs, uint32_t nreq, DevCounters* ctr,
-                           bool stats, hipStream_t stream, std::vector<PendingEvent>* timing)
+                           bool stats, hipStream_t stream, std::vector<PendingEvent>* timing, uint32_t chunkPaths)
 {
     if (nreq == 0) return hipSuccess;
     if (nreq >= kCameraTag) return hipErrorInvalidValue;  // request indices travel in 31 bits
     const WavefrontTuning& tn = st.tuning;
-    uint32_t maxPix = 0;
-    for (uint32_t i = 0; i < nreq; i++) maxPix = std::max(maxPix, reqs[i].width * reqs[i].height);
-    // Chunk capacity: at least one full frame of the largest request, the tuned path count otherwise.
-    uint32_t cap = std::max<uint32_t>(maxPix, tn.chunkPaths);
-    size_t perPath = 4 + 4 + 8 + 16 + 4 + 16 * 4 + 16 * 3 + 16 * kSlots + 1 + kSlots + 16 +  // path state
-                     16 * 4 + 16 + 4 + 32 * kSlots + 1 + 1;                                      // queues
-    const size_t kTableBytes = 1u << 20;  // chunk header + segment + chunk-request tables of one chunk
-    const size_t kHdrBytes = 256;
-    const uint32_t kMaxSegs = (uint32_t)((kTableBytes / 2 - kHdrBytes) / sizeof(SegInfo));
-    const uint32_t kMaxCreqs = (uint32_t)(kTableBytes / 2 / sizeof(ChunkReq));
-    size_t need = alignUp((size_t)cap * perPath) + 64 * 1024 + kTableBytes + alignUp(sizeof(DevReq) * nreq);
-    if (st.bytes < need || st.capacity != cap) {
-        wavefrontRelease(st);
-        hipError_t e = hipMalloc(&st.buf, need);
-        if (e != hipSuccess) return e;
-        st.bytes = need;
+    uint32_t cap = wavefrontCapacity(st, reqs, nreq, chunkPaths);
+    if (st.external && wavefrontBufferBytes(cap) > st.bytes) {
+        // the largest capacity the given memory holds, if it still holds one frame of every request
+        const size_t fit = st.bytes > kFixedBytes + 256 ? (st.bytes - kFixedBytes - 256) / kPerPath : 0;
+        uint32_t maxPix = 0;
+        for (uint32_t i = 0; i < nreq; i++) maxPix = std::max(maxPix, reqs[i].width * reqs[i].height);
+        if (fit < maxPix) return hipErrorOutOfMemory;
+        cap = (uint32_t)std::min<size_t>(fit, 0xFFFFFFFFu);
+    }
+
+    CallPlan plan;
+    if (!planCall(reqs, nreq, cap, plan) || plan.desc.empty()) return hipErrorInvalidValue;
+    const size_t tblNeed = callTableBytes(nreq, plan.desc.size(), plan.segs.size(), plan.creqs.size());
+
+    // device memory: path state + queues + chunk tables, and the call table
+    const size_t need = wavefrontBufferBytes(cap);
+    if (st.external) {
+        if (need > st.bytes || tblNeed > st.tblBytes) return hipErrorOutOfMemory;
+    } else {
+        if (st.bytes < need || st.capacity != cap) {
+            if (st.buf) {
+                hipStreamSynchronize(stream);  // queued chunks still use the old buffers
+                if (st.graphExec) hipGraphExecDestroy(st.graphExec);
+                st.graphExec = nullptr;
+                hipFree(st.buf);
+                st.buf = nullptr;
+                st.bytes = 0;
+            }
+            hipError_t e = hipMalloc(&st.buf, need);
+            if (e != hipSuccess) {
+                st.buf = nullptr;
+                st.capacity = 0;
+                return e;
+            }
+            st.bytes = need;
+        }
+        if (st.tblBytes < tblNeed) {
+            if (st.tbl) {
+                hipStreamSynchronize(stream);
+                if (st.graphExec) hipGraphExecDestroy(st.graphExec);
+                st.graphExec = nullptr;
+                hipFree(st.tbl);
+                st.tbl = nullptr;
+                st.tblBytes = 0;
+            }
+            const size_t grow = std::max(tblNeed, (size_t)1 << 20);
+            hipError_t e = hipMalloc(&st.tbl, grow);
+            if (e != hipSuccess) {
+                st.tbl = nullptr;
+                return e;
+            }
+            st.tblBytes = grow;
+        }
     }
     st.capacity = cap;
     char* base = (char*)st.buf;
@@ -968,27 +1151,41 @@ hipError_t wavefrontRender(WavefrontState& st, const WavefrontRequest* reqs, uin
     Q.cont = (uint8_t*)take((size_t)cap);
     Q.tileCnt = (uint32_t*)take(((size_t)cap + kTile - 1) / kTile * 4);
     Q.tileOff = (uint32_t*)take(((size_t)cap + kTile - 1) / kTile * 4);
-    char* tbl = (char*)take(kTableBytes);
-    DevReq* dreq = (DevReq*)take(sizeof(DevReq) * nreq);  // last: its size varies, nothing after it moves
+    char* ctbl = (char*)take(kTableBytes);
     if (off > need) return hipErrorOutOfMemory;
-    ChunkHdr* ghdr = (ChunkHdr*)tbl;
-    SegInfo* gseg = (SegInfo*)(tbl + kHdrBytes);
-    ChunkReq* gcr = (ChunkReq*)(tbl + kTableBytes / 2);
+    ChunkHdr* ghdr = (ChunkHdr*)ctbl;
+    SegInfo* gseg = (SegInfo*)(ctbl + kHdrBytes);
+    ChunkReq* gcr = (ChunkReq*)(ctbl + kTableBytes / 2);
+    uint8_t* tbl = (uint8_t*)st.tbl;
+    const DevReq* dreq = (const DevReq*)(tbl + kCallHdrBytes);
 
-    // request table
-    std::vector<DevReq> hreq(nreq);
+    // the call table's host image, uploaded through kernel arguments
     uint32_t maxDepth = 1;
+    std::vector<uint32_t> image(tblNeed / 4, 0u);
+    uint8_t* img = (uint8_t*)image.data();
+    CallHdr ch{};
+    ch.nchunk = (uint32_t)plan.desc.size();
+    ch.nreq = nreq;
+    ch.descOff = (uint32_t)(kCallHdrBytes + alignUp(sizeof(DevReq) * nreq));
+    ch.segOff = (uint32_t)(ch.descOff + alignUp(sizeof(ChunkDesc) * plan.desc.size()));
+    ch.creqOff = (uint32_t)(ch.segOff + alignUp(sizeof(SegInfo) * plan.segs.size()));
+    std::memcpy(img, &ch, sizeof(ch));
     for (uint32_t i = 0; i < nreq; i++) {
-        std::memset(&hreq[i], 0, sizeof(DevReq));
-        hreq[i].rs = reqs[i].rs;
-        hreq[i].fp = reqs[i].frame;
-        hreq[i].W = reqs[i].width;
-        hreq[i].H = reqs[i].height;
-        hreq[i].seedOffset = reqs[i].seedOffset;
-        hreq[i].accum = reqs[i].accum;
+        DevReq r;
+        std::memset(&r, 0, sizeof(DevReq));
+        r.rs = reqs[i].rs;
+        r.fp = reqs[i].frame;
+        r.W = reqs[i].width;
+        r.H = reqs[i].height;
+        r.seedOffset = reqs[i].seedOffset;
+        r.accum = reqs[i].accum;
+        std::memcpy(img + kCallHdrBytes + sizeof(DevReq) * i, &r, sizeof(DevReq));
         maxDepth = std::max<uint32_t>(maxDepth, (uint32_t)reqs[i].frame.trace_depth);
     }
-    hipError_t e = hipMemcpyAsync(dreq, hreq.data(), sizeof(DevReq) * nreq, hipMemcpyHostToDevice, stream);
+    std::memcpy(img + ch.descOff, plan.desc.data(), sizeof(ChunkDesc) * plan.desc.size());
+    std::memcpy(img + ch.segOff, plan.segs.data(), sizeof(SegInfo) * plan.segs.size());
+    std::memcpy(img + ch.creqOff, plan.creqs.data(), sizeof(ChunkReq) * plan.creqs.size());
+    hipError_t e = upload(tbl, image, stream);
     if (e != hipSuccess) return e;
 
     int dev = 0, cus = 256;
@@ -1007,19 +1204,19 @@ hipError_t wavefrontRender(WavefrontState& st, const WavefrontRequest* reqs, uin
     L.refill = tn.refillLanes;
     L.leafWait = tn.leafWaitLanes;
 
-    // One graph per (buffers, depth, stats, launch sizes): captured on first use,
-    // replayed for every chunk.  Timed runs launch eagerly so each stage gets its
-    // own HIP events.
-    const bool graph = st.useGraph && timing == nullptr;
+    // One graph per (buffers, capacity, depth, stats, launch sizes): captured on
+    // first use, replayed for every chunk.  Timed runs and external states launch
+    // eagerly (stage by stage, with HIP events when timed).
+    const bool graph = st.useGraph && timing == nullptr && !st.external;
     if (graph) {
-        WavefrontState::GraphKey key{st.buf, ctr, maxDepth, stats, L.gen, L.trace, L.shade, L.shadow, L.accum,
-                                     L.refill, L.leafWait};
+        WavefrontState::GraphKey key{st.buf, st.tbl, ctr, cap, maxDepth, stats, L.gen, L.trace, L.shade, L.shadow,
+                                     L.accum, L.refill, L.leafWait};
         if (!st.graphExec || !(key == st.graphKey)) {
             if (st.graphExec) hipGraphExecDestroy(st.graphExec);
             st.graphExec = nullptr;
             hipGraph_t g = nullptr;
             if ((e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal)) != hipSuccess) return e;
-            enqueueChunk(stream, L, maxDepth, stats, dreq, gseg, gcr, ghdr, P, Q, ctr, nullptr);
+            enqueueChunk(stream, L, maxDepth, stats, tbl, dreq, gseg, gcr, ghdr, P, Q, ctr, nullptr);
             if ((e = hipStreamEndCapture(stream, &g)) != hipSuccess) return e;
             e = hipGraphInstantiate(&st.graphExec, g, nullptr, nullptr, 0);
             hipGraphDestroy(g);
@@ -1030,62 +1227,31 @@ hipError_t wavefrontRender(WavefrontState& st, const WavefrontRequest* reqs, uin
             st.graphKey = key;
         }
     }
-
-    // Segments: (request, sample) pairs in request-major, sample-ascending order, cut into chunks.
-    struct PendingSeg {
-        uint32_t req, sample, count;
-    };
-    std::vector<PendingSeg> all;
-    for (uint32_t i = 0; i < nreq; i++)
-        for (uint32_t s = 0; s < reqs[i].sampleCount; s++)
-            all.push_back(PendingSeg{i, reqs[i].sampleStart + s, reqs[i].width * reqs[i].height});
-    size_t k = 0;
-    while (k < all.size()) {
-        std::vector<SegInfo> segs;
-        std::vector<ChunkReq> creqs;
-        uint32_t total = 0, totalPix = 0;
-        while (k < all.size() && total + all[k].count <= cap && segs.size() < kMaxSegs) {
-            const PendingSeg& s = all[k];
-            if (creqs.empty() || creqs.back().req != s.req) {
-                if (creqs.size() == kMaxCreqs) break;
-                creqs.push_back(ChunkReq{s.req, (uint32_t)segs.size(), 0, totalPix});
-                totalPix += s.count;
-            }
-            creqs.back().nSeg++;
-            segs.push_back(SegInfo{s.req, s.sample, total, s.count});
-            total += s.count;
-            k++;
-        }
-        if (segs.empty()) return hipErrorInvalidValue;
-        ChunkHdr hdr{total, (uint32_t)segs.size(), (uint32_t)creqs.size(), totalPix};
-        if ((e = hipMemcpyAsync(ghdr, &hdr, sizeof(hdr), hipMemcpyHostToDevice, stream)) != hipSuccess) return e;
-        if ((e = hipMemcpyAsync(gseg, segs.data(), sizeof(SegInfo) * segs.size(), hipMemcpyHostToDevice, stream)) !=
-            hipSuccess)
-            return e;
-        if ((e = hipMemcpyAsync(gcr, creqs.data(), sizeof(ChunkReq) * creqs.size(), hipMemcpyHostToDevice, stream)) !=
-            hipSuccess)
-            return e;
+    // Chunks follow each other on the stream: kChunkNext of chunk k+1 runs after
+    // chunk k's last stage, so no host synchronization is needed between them.
+    for (size_t k = 0; k < plan.desc.size(); k++) {
         if (graph) {
             if ((e = hipGraphLaunch(st.graphExec, stream)) != hipSuccess) return e;
         } else {
-            enqueueChunk(stream, L, maxDepth, stats, dreq, gseg, gcr, ghdr, P, Q, ctr, timing);
+            enqueueChunk(stream, L, maxDepth, stats, tbl, dreq, gseg, gcr, ghdr, P, Q, ctr, timing);
         }
         if ((e = hipGetLastError()) != hipSuccess) return e;
-        // the tables are overwritten by the next chunk
-        if (k < all.size()) {
-            if ((e = hipStreamSynchronize(stream)) != hipSuccess) return e;
-        }
     }
     return hipSuccess;
 }
 
-void wavefrontRelease(WavefrontState& st)
+void wavefrontRelease(WavefrontState& st, hipStream_t stream)
 {
+    if (st.external) return;
+    if (st.buf || st.tbl) hipStreamSynchronize(stream);
     if (st.graphExec) hipGraphExecDestroy(st.graphExec);
     st.graphExec = nullptr;
     if (st.buf) hipFree(st.buf);
+    if (st.tbl) hipFree(st.tbl);
     st.buf = nullptr;
+    st.tbl = nullptr;
     st.bytes = 0;
+    st.tblBytes = 0;
     st.capacity = 0;
 }
 

@@ -376,6 +376,12 @@ class Server:
         self.r._check(self.lib.aph_server_usage(self.h, C.byref(r), C.byref(j), C.byref(a), C.byref(p)))
         return r.value, j.value, a.value, p.value
 
+    def working_set(self):
+        """(wavefront working-set bytes, paths in flight) of the last batched step."""
+        b, n = C.c_uint64(), C.c_uint32()
+        self.r._check(self.lib.aph_server_working_set(self.h, C.byref(b), C.byref(n)))
+        return b.value, n.value
+
     def release(self, ticket):
         self.r._check(self.lib.aph_server_release(self.h, ticket))
 

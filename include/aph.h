@@ -299,11 +299,15 @@ int aph_target_destroy(aph_ctx_t ctx, uint64_t target);
 int aph_target_device_ptr(aph_ctx_t ctx, uint64_t target, void** ptr);
 
 /* ---- render: IRenderer::render(sampleStart, sampleCount) (Aurora.h:1196;
- *      HGIRenderer::render, HGIRenderer.cpp:477-589).  Asynchronous.
- *      mode APH_MODE_MEGAKERNEL: one launch per request and sample, the
- *      reference's one-dispatch-per-sample structure;
+ *      HGIRenderer::render, HGIRenderer.cpp:477-589).  Asynchronous; no host
+ *      memory is referenced after the call returns.  Every request is checked
+ *      before any is launched.
+ *      mode APH_MODE_MEGAKERNEL: one megakernel launch per request, one lane per
+ *      pixel looping over all of the request's samples in order (the reference
+ *      dispatches once per sample, HGIRenderer.cpp:526-566);
  *      APH_MODE_WAVEFRONT: all requests and samples share one wavefront
- *      pipeline (raygen, extend, shade, shadow, accumulate). ---- */
+ *      pipeline (raygen, extend, shade, shadow, accumulate); the requests of
+ *      one call must name distinct targets (APH_E_INVALID otherwise). ---- */
 int aph_render(aph_ctx_t ctx, const aph_request* reqs, uint32_t nreq, int mode);
 /* IRenderer::waitForTask (HGIRenderer.cpp:591-594 is a no-op because HGI is synchronous). */
 int aph_wait(aph_ctx_t ctx);
@@ -343,11 +347,13 @@ typedef struct {
 } aph_tuning;
 int aph_get_tuning(aph_ctx_t ctx, aph_tuning* t);
 int aph_set_tuning(aph_ctx_t ctx, const aph_tuning* t);
-/* Renders the given requests (into their targets) under each candidate launch
+/* Renders the given requests - into scratch targets of their sizes, so the
+ * requests' own targets are left as they were - under each candidate launch
  * configuration - chunk of 4M/8M/16M/32M/64M paths x 4/8/16 traversal blocks per
  * CU x 4/8/16 shade blocks per CU, then refill thresholds 16..64 (step 8) at
  * the best of those, then leaf-wait lanes 0/8/16/24/32 at the best of those -
- * and keeps the fastest for this context.
+ * and keeps the fastest for this context.  A candidate whose buffers do not
+ * fit in device memory is skipped (APH_E_MEMORY if none fits).
  * A chunk holds about 0.55 KB of device memory per path (35 GB at 64M paths),
  * allocated at the tuned size even for smaller frames. */
 int aph_wavefront_autotune(aph_ctx_t ctx, const aph_request* reqs, uint32_t nreq, aph_tuning* best);
@@ -403,14 +409,18 @@ void aph_halton_jitter(uint32_t frame_index, uint32_t phase_count, float out[2])
  * (HGIRenderer::render, HGIRenderer.cpp:477-589, progressive
  * render(sampleStart, sampleCount) driven by SampleCounter, Timer.h:97-286).
  * Each aph_server_step admits waiting jobs in arrival order while a slot is
- * free and the HBM budget (resident scenes + admitted jobs' RGBA32F
- * accumulation buffers) allows, then renders the next run of samples of every
- * admitted job in one shared wavefront.  A job's image is the same alone and in
- * any mix.  Errors are reported through aph_last_error of the context. ---- */
+ * free and the HBM budget allows, then renders the next run of samples of every
+ * admitted job in one shared wavefront.  The budget covers the resident scenes
+ * (BVHs, meshes, instance records, their environments), the images bound to
+ * the material table, the admitted jobs' RGBA32F accumulation buffers and the
+ * wavefront working set; a step's chunk of paths in flight is the context's
+ * tuned one, halved until the working set fits (never below the largest
+ * active frame).  A job's image is the same alone and in any mix.  Errors are
+ * reported through aph_last_error of the context. ---- */
 typedef struct aph_server* aph_server_t;
 
 typedef struct {
-    uint64_t hbm_budget;    /* bytes: resident scenes + job buffers; 0 = unlimited */
+    uint64_t hbm_budget;    /* bytes: resident scenes + images + job buffers + working set; 0 = unlimited */
     uint32_t max_active;    /* jobs sharing the wavefront at once; 0 = unlimited */
     uint32_t step_samples;  /* pixel-samples per step over all active jobs; 0 = 8M */
     int mode;               /* APH_MODE_WAVEFRONT: batched; APH_MODE_MEGAKERNEL: one job at a time */
@@ -444,10 +454,13 @@ int aph_server_step(aph_server_t s, uint32_t* active, uint32_t* pending);
 int aph_server_status(aph_server_t s, uint64_t ticket, aph_job_status* st);
 /* RGBA32F image of a finished job kept with keep_results. */
 int aph_server_result(aph_server_t s, uint64_t ticket, float* dst, size_t bytes);
-/* Device memory charged to the budget (resident scenes, admitted job buffers)
- * and the jobs running / waiting. */
+/* Device memory charged to the budget (resident scenes and material images,
+ * admitted job buffers) and the jobs running / waiting. */
 int aph_server_usage(aph_server_t s, uint64_t* resident_bytes, uint64_t* job_bytes, uint32_t* active,
                      uint32_t* pending);
+/* The wavefront working set of the last batched step (path state, queues and
+ * tables, charged to the budget) and its paths in flight. */
+int aph_server_working_set(aph_server_t s, uint64_t* wavefront_bytes, uint32_t* paths_in_flight);
 /* Forgets a finished or waiting job and frees its buffer. */
 int aph_server_release(aph_server_t s, uint64_t ticket);
 /* Replays an arrival trace in real time: job i is submitted arrival[i] seconds

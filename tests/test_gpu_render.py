@@ -275,7 +275,10 @@ def test_autotune_keeps_results(renderer, teapot_path):
     renderer.wait()
     before = renderer.readback(ref, 64, 48)
     tgt = renderer.target(64, 48)
-    best = renderer.autotune([renderer.request(tgt, tlas, sc.frame, 0, 4)])
+    untouched = renderer.readback(tgt, 64, 48)
+    best = renderer.autotune([renderer.request(tgt, tlas, sc.frame, 4, 4)])
+    # trials render into scratch targets: the request's own target is left as it was
+    np.testing.assert_array_equal(renderer.readback(tgt, 64, 48), untouched)
     # the candidate set of aph_wavefront_autotune
     assert best.ms > 0 and best.chunk_paths in (4 << 20, 8 << 20, 16 << 20, 32 << 20, 64 << 20)
     assert best.trace_blocks_per_cu in (4, 8, 16) and best.shade_blocks_per_cu in (4, 8, 16)
@@ -380,3 +383,54 @@ def test_texture_errors(renderer):
     import ctypes as C
     h = C.c_uint64()
     assert r.lib.aph_image_create(r.ctx, C.byref(d), C.byref(h)) == capi.APH_E_INVALID
+
+
+def test_duplicate_targets_are_rejected_before_any_launch(renderer, teapot_path):
+    """The wavefront folds a chunk's samples into each target in one pass, so two
+    requests of one call may not share a target; a bad request in either mode leaves
+    every target of the call untouched (all requests are checked before any launch)."""
+    r = renderer
+    sc = scenes.teapot_scene(teapot_path, 16, 16)
+    tlas = r.load_scene(sc)
+    t = r.target(16, 16)
+    before = r.readback(t, 16, 16)
+    with pytest.raises(AphError) as e:
+        r.render([r.request(t, tlas, sc.frame, 0, 2), r.request(t, tlas, sc.frame, 2, 2)], capi.APH_MODE_WAVEFRONT)
+    assert e.value.code == capi.APH_E_INVALID
+    for mode in (capi.APH_MODE_WAVEFRONT, capi.APH_MODE_MEGAKERNEL):
+        with pytest.raises(AphError) as e:
+            r.render([r.request(t, tlas, sc.frame, 0, 2), r.request(t + 10 ** 6, tlas, sc.frame, 0, 2)], mode)
+        assert e.value.code == capi.APH_E_INVALID
+    r.wait()
+    np.testing.assert_array_equal(r.readback(t, 16, 16), before)
+    # megakernel requests on one target run in order: the same as one progressive call
+    u = r.target(16, 16)
+    r.render([r.request(u, tlas, sc.frame, 0, 2), r.request(u, tlas, sc.frame, 2, 2)], capi.APH_MODE_MEGAKERNEL)
+    v = r.target(16, 16)
+    r.render([r.request(v, tlas, sc.frame, 0, 4)], capi.APH_MODE_MEGAKERNEL)
+    r.wait()
+    np.testing.assert_array_equal(r.readback(u, 16, 16), r.readback(v, 16, 16))
+
+
+def test_production_chunk_shape_equals_small_chunks(renderer):
+    """The autotuned launch shape of the headline frame: 64M-path chunks (about 35 GB of
+    path state and queues), 1920x1080 at 33 spp = 68.4M paths, so one call runs two
+    chunks of the captured graph back to back.  The image must equal, bit for bit, the
+    same render cut into 4M-path chunks (17 chunks), and the eager launches."""
+    sc = scenes.config2(subdiv=4)
+    tlas = renderer.load_scene(sc, capi.APH_BUILDER_SAH)
+    w, h, spp = sc.width, sc.height, 33
+    imgs = []
+    for chunk, tr, sh, mode in ((64 << 20, 16, 8, capi.APH_LAUNCH_GRAPH), (4 << 20, 16, 8, capi.APH_LAUNCH_GRAPH),
+                                (64 << 20, 16, 8, capi.APH_LAUNCH_EAGER)):
+        renderer.set_tuning(chunk, tr, sh, 16, 16)
+        renderer.set_launch_mode(mode)
+        tgt = renderer.target(w, h)
+        renderer.render([renderer.request(tgt, tlas, sc.frame, 0, spp)], capi.APH_MODE_WAVEFRONT)
+        renderer.wait()
+        imgs.append(renderer.readback(tgt, w, h))
+    renderer.set_launch_mode(capi.APH_LAUNCH_GRAPH)
+    renderer.set_tuning(4 << 20, 16, 8)
+    assert np.isfinite(imgs[0]).all() and imgs[0][..., 3].mean() > 0.05
+    np.testing.assert_array_equal(imgs[0], imgs[1])
+    np.testing.assert_array_equal(imgs[0], imgs[2])

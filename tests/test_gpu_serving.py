@@ -132,3 +132,48 @@ def test_replay_of_an_arrival_trace(renderer, two_scenes):
         j = jobs[k]
         ref = plain_render(renderer, j.tlas, j.frame, j.width, j.height, j.spp)
         np.testing.assert_array_equal(srv.result(tickets[k], j.width, j.height), ref)
+
+
+def _device_free():
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    free, total = C.c_size_t(), C.c_size_t()
+    assert hip.hipMemGetInfo(C.byref(free), C.byref(total)) == 0
+    return free.value
+
+
+def test_budget_covers_the_wavefront_working_set(renderer, two_scenes):
+    """The context is tuned for 64M-path chunks (about 35 GB of path state and queues).
+    A server whose budget is a few GB beyond the resident scene serves its jobs with a
+    smaller chunk, charges that working set, and the device memory it takes - measured
+    with hipMemGetInfo around every step - stays within the budget.  Images are the
+    same as plain renders (chunking never changes results)."""
+    (a, ta), (b, tb) = two_scenes
+    r = renderer
+    r.set_tuning(1 << 16, 16, 8)
+    plain_render(r, ta, a.frame, 40, 32, 1)  # the context's buffers shrink to a 64K-path chunk
+    r.set_tuning(64 << 20, 16, 8)
+    budget = 3 << 30
+    srv = Server(r, hbm_budget=budget, step_samples=1 << 22, keep_results=True)
+    jobs = [Server.job(ta, a.frame, 40, 32, 6), Server.job(tb, b.frame, 48, 40, 5), Server.job(ta, a.frame, 40, 32, 3)]
+    tickets = [srv.submit(j) for j in jobs]
+    free0 = _device_free()
+    resident = srv.usage()[0]
+    assert 0 < resident < budget
+    peak = 0
+    while True:
+        act, pend = srv.step()
+        used = free0 - _device_free()
+        peak = max(peak, used)
+        wf, paths = srv.working_set()
+        res, jb, _, _ = srv.usage()
+        assert res + jb + wf <= budget
+        if act == 0 and pend == 0:
+            break
+    assert 0 < paths < (64 << 20) and wf <= budget - resident
+    assert peak <= budget - resident, (peak, budget, resident)
+    r.set_tuning(4 << 20, 16, 8)
+    for k, j in enumerate(jobs):
+        ref = plain_render(r, j.tlas, j.frame, j.width, j.height, j.spp)
+        np.testing.assert_array_equal(srv.result(tickets[k], j.width, j.height), ref)
+    srv.close()
