@@ -550,6 +550,21 @@ def orbit_frame(rng, width, height, radius=14.0, target=(0.0, 0.5, 0.0), trace_d
     return look_at_frame(eye, target, (0, 1, 0), 45.0, width / height, trace_depth)
 
 
+C3_RESOLUTIONS = [(256, 256), (512, 512), (1024, 768), (1920, 1080)]
+
+
+def c3_cameras(seed=3, count=64):
+    """The many-camera workload on the config1 scene: `count` cameras on random orbits,
+    resolutions drawn from C3_RESOLUTIONS, spp U{1..64}, 5 bounces.  Returns
+    [(width, height, spp, frame)]."""
+    rng = np.random.default_rng(seed)
+    cams = []
+    for _ in range(count):
+        w, h = C3_RESOLUTIONS[int(rng.integers(0, 4))]
+        cams.append((w, h, int(rng.integers(1, 65)), orbit_frame(rng, w, h)))
+    return cams
+
+
 def teapot_scene(path, width=128, height=128, spp=4, trace_depth=5):
     """The reference fixture: teapot, 45 degree camera at (0,0,-5) (AuroraTestHelpers.cpp:617-622)."""
     sc = Scene()
@@ -561,6 +576,63 @@ def teapot_scene(path, width=128, height=128, spp=4, trace_depth=5):
     sc.env = dict()  # default gradient environment (EnvironmentBase.cpp:33-45)
     sc.width, sc.height, sc.spp = width, height, spp
     sc.frame = look_at_frame((0.0, 0.0, -5.0), (0.0, 0.0, 0.0), (0, 1, 0), 45.0, width / height, trace_depth)
+    return sc
+
+
+SWEEP_KINDS = ("subsurface", "sheen", "coat", "coat_affect", "transmission", "anisotropy", "thin_walled")
+
+
+def material_sweep(path, kind, width=96, height=96, spp=4, trace_depth=5):
+    """A 5 x 5 grid of teapots, one material each, varied along u = column / 4 and
+    v = row / 4: the grid and sweeps of the reference's TestMaterialAdvancedMaterialProperties
+    (Tests/Aurora/Tests/TestMaterial.cpp:613-830; grid at :271-306: gap 3.1, centred at
+    z = 17) seen by the fixture camera (AuroraTestHelpers.cpp:617-622).
+      subsurface   grey base, subsurface 0.75, colour bc0->bc1 (u), radius bc2->bc3 (v)
+      sheen        grey base, sheen 1, colour bc0->bc1 (u), roughness v
+      coat         coat 1 at IOR 0.5 over nothing (base 0, specular 0), colour (u), roughness (v)
+      coat_affect  coat 1, colour bc0, coat_affect_color u, coat_affect_roughness v - over a
+                   grey base with specular 1 (the reference sweeps it over nothing, where
+                   neither parameter shows)
+      transmission base 0.75, specular u, transmission v
+    and two sweeps of parameters that test does not vary:
+      anisotropy   metal, roughness 0.4, specular_anisotropy u, specular_rotation v / 2, with a
+                   clear coat of anisotropy u and rotation v
+      thin_walled  transmission 1, thin_walled, transmission colour bc0->bc1 (u), IOR 1 + v"""
+    if kind not in SWEEP_KINDS:
+        raise ValueError(kind)
+    sc = teapot_scene(path, width, height, spp, trace_depth)
+    sc.name = f"sweep_{kind}"
+    bc0, bc1 = np.array([1, 0.4, 0.25]), np.array([0.35, 1, 0.3])
+    bc2, bc3 = np.array([0.3, 0.2, 0.95]), np.array([0.7, 0.8, 0.05])
+    grey = (0.25, 0.25, 0.25)
+    mats, inst = [], []
+    for i in range(5):
+        v = i / 4.0
+        for j in range(5):
+            u = j / 4.0
+            mix = tuple(bc0 + (bc1 - bc0) * u)
+            if kind == "subsurface":
+                m = _material(grey, subsurface=0.75, subsurface_color=mix,
+                              subsurface_radius=tuple(bc2 + (bc3 - bc2) * v))
+            elif kind == "sheen":
+                m = _material(grey, sheen=1.0, sheen_color=mix, sheen_roughness=v)
+            elif kind == "coat":
+                m = _material(coat=1.0, coat_ior=0.5, base=0.0, specular=0.0, coat_color=mix, coat_roughness=v)
+            elif kind == "coat_affect":
+                m = _material(grey, coat=1.0, coat_ior=0.5, base=0.8, specular=1.0, coat_color=tuple(bc0),
+                              coat_affect_color=u, coat_affect_roughness=v)
+            elif kind == "transmission":
+                m = _material(base=0.75, specular=u, transmission=v)
+            elif kind == "anisotropy":
+                m = _material((0.9, 0.7, 0.5), 1.0, 0.4, specular_anisotropy=u, specular_rotation=v * 0.5,
+                              coat=0.5, coat_roughness=0.3, coat_anisotropy=u, coat_rotation=v)
+            else:
+                m = _material(transmission=1.0, thin_walled=1, transmission_color=mix, specular_ior=1.0 + v)
+            mats.append(m)
+            pos = np.array([j * -3.1, i * 3.1, 0.0]) + np.array([0.0, 0.0, 17.0]) - np.array([-3.1, 3.1, 0.0]) * 2
+            inst.append((0, len(mats) - 1, affine(trans=pos)))
+    sc.materials = mats
+    sc.instances = _instances(inst)
     return sc
 
 

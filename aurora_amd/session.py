@@ -266,6 +266,9 @@ class Renderer:
         self._check(self.lib.aph_target_create(self.ctx, w, h, C.byref(t)))
         return t.value
 
+    def destroy_target(self, target):
+        self._check(self.lib.aph_target_destroy(self.ctx, target))
+
     def render(self, requests, mode=capi.APH_MODE_MEGAKERNEL):
         arr = (capi.Request * len(requests))(*requests)
         self._check(self.lib.aph_render(self.ctx, C.cast(arr, C.c_void_p), len(requests), mode))

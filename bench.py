@@ -193,7 +193,7 @@ def cpu_baseline_child(args):
     return json.loads(p.stdout.strip().splitlines()[-1])
 
 
-RESOLUTIONS = [(256, 256), (512, 512), (1024, 768), (1920, 1080)]
+RESOLUTIONS = scenes.C3_RESOLUTIONS
 
 
 def rays_of(st):
@@ -224,11 +224,7 @@ def c3_cameras(r, builder, seed=3, autotune=True):
     same 64 cameras at 1/8 of their spp); the context's previous tuning is restored after."""
     sc = scenes.config1()
     tlas = r.load_scene(sc, builder)
-    rng = np.random.default_rng(seed)
-    cams = []
-    for _ in range(64):
-        w, h = RESOLUTIONS[int(rng.integers(0, 4))]
-        cams.append((w, h, int(rng.integers(1, 65)), scenes.orbit_frame(rng, w, h)))
+    cams = scenes.c3_cameras(seed)
     tg = [r.target(w, h) for w, h, _, _ in cams]
     reqs = [r.request(t, tlas, fr, 0, spp) for t, (w, h, spp, fr) in zip(tg, cams)]
     keep = r.tuning()

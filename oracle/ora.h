@@ -206,6 +206,10 @@ typedef struct {
     uint64_t shadow_rays;
 } ora_counters;
 
+/* computeCameraRay (RayTrace.slang:130-182) of pixel (x, y) at sample_index. */
+void ora_camera_ray(const ora_frame_params* fp, uint32_t W, uint32_t H, uint32_t x, uint32_t y,
+                    uint32_t sample_index, float org[3], float dir[3]);
+
 /* Renders sample_count samples into accum (RGBA float, W*H*4) starting at
  * sample_start, with HGI running-mean accumulation (Accumulation.glsl:13-34).
  * pixels: optional list of pixel indices (y*W+x) to render (others untouched),

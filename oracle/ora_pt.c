@@ -421,6 +421,19 @@ static void camera_ray(const ora_frame_params* fp, uint32_t x, uint32_t y, uint3
     }
 }
 
+/* The camera ray of one pixel-sample, with the path's RNG state (sample_index
+ * includes the seed offset), for unit tests of computeCameraRay. */
+void ora_camera_ray(const ora_frame_params* fp, uint32_t W, uint32_t H, uint32_t x, uint32_t y,
+                    uint32_t sample_index, float org[3], float dir[3])
+{
+    uint32_t rng[2];
+    ora_pcg2d_init(sample_index, W, x, y, rng);
+    v3 o, d;
+    camera_ray(fp, x, y, W, H, rng, &o, &d);
+    v3store(org, o);
+    v3store(dir, d);
+}
+
 /* RayGenShader, MainEntryPoints.slang:70-334 (HGI: no AOVs, no denoising). */
 void ora_pixel_sample(const ora_scene* s, const ora_material_constants* mats, uint32_t nmat, const ora_env* env,
                       const ora_frame_params* fp, uint32_t W, uint32_t H, uint32_t x, uint32_t y,

@@ -159,3 +159,54 @@ def test_opaque_top_layer_equals_swapped_base_material(teapot_path):
     np.testing.assert_array_equal(layered, swapped)
     base, _ = pyoracle.OracleScene(scenes.teapot_scene(teapot_path, 24, 24)).render(sc.frame, 24, 24, 0, 2)
     assert not np.array_equal(layered, base)
+
+
+@pytest.mark.parametrize("sky", ["broad", "hotspot"])
+def test_environment_mis_is_unbiased_under_textured_skies(sky):
+    """Environment MIS (shadeEnvironmentLightMIS, ShadeFunctions.slang:135-259) under a
+    textured sky: the alias-table light sample (pdf = luminance / integral, a texel-centre
+    direction, point-loaded radiance) and the BSDF sample, each power-heuristic weighted and
+    sharing one random pair (quirk Q1), must together estimate the lighting integral without
+    bias.  At traceDepth 1 the pixel mean is 3x that integral (see the diffuse-plane
+    accounting above); the integral is a midpoint quadrature over the texels: texel solid
+    angle x radiance x the sampled diffuse layer at the texel-centre direction, seen from
+    straight above.  The broad sky (radiance 0.2 + 3 cos^3 about one direction) keeps the
+    two pdfs within a small factor of each other over most directions, where weights that
+    do not sum to one (e.g. the balance heuristic on one side) bias the mean by about 10
+    standard errors; the hotspot sky (a 40x lobe over a gradient) exercises the regime
+    where one strategy dominates."""
+    rho = 0.6
+    m = capi.default_material()
+    m.base = 1.0
+    for i in range(3):
+        m.base_color[i] = rho
+    m.specular = 0.0
+    sc = plane_scene(m)
+    w, h = 128, 64
+    u, v = (np.arange(w) + 0.5) / w, (np.arange(h) + 0.5) / h
+    dirs = np.zeros((h, w, 3), np.float32)
+    lib = pyoracle.load()
+    for y in range(h):
+        for x in range(w):
+            lib.ora_latlong_to_dir(pyoracle._p(np.array([u[x], v[y]], np.float32)), pyoracle._p(dirs[y, x]))
+    sd = np.array([0.6, 0.5, 0.2])
+    sd /= np.linalg.norm(sd)
+    if sky == "broad":
+        img = (0.2 + 3.0 * np.clip(dirs @ sd, 0, None) ** 3)[..., None] * np.array([1.0, 0.8, 0.6])
+    else:
+        img = scenes.sky_gradient(w, h, np.random.default_rng(3)) + (40.0 * np.exp(-(1 - dirs @ sd) / 0.01))[..., None]
+    img = np.ascontiguousarray(img.astype(np.float32))
+    sc.env = dict(light_rgb=img)
+    lat = math.pi / 2 - np.arange(h) * (math.pi / h)
+    solid = (np.sin(lat) - np.sin(lat - math.pi / h)) * 2 * math.pi / w
+    cos = np.clip(dirs[..., 1], 0, None).astype(np.float64)
+    a, f0 = 0.2 * 0.2, 0.04
+    ess, da = karis_albedo(1.0, a, 1, 1), karis_albedo(1.0, a, f0, 1)
+    F = f0 + (1 - f0) * np.clip(1 - np.sqrt((1 + cos) / 2), 0, 1) ** 5
+    layer = (1 - da * (1 + F * (1 - ess) / ess)) * rho * cos / math.pi
+    integral = (img * (layer * solid[:, None])[..., None]).sum(axis=(0, 1))
+    W = H = 48
+    acc, _ = pyoracle.OracleScene(sc).render(down_camera(W, H, 1), W, H, 0, 64)
+    px = acc[..., :3].reshape(-1, 3).astype(np.float64)
+    se = px.std(axis=0, ddof=1) / math.sqrt(len(px))
+    assert np.all(np.abs(px.mean(axis=0) - 3 * integral) <= 4 * se + 2e-3 * 3 * integral), (px.mean(0), 3 * integral)

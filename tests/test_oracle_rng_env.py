@@ -226,3 +226,69 @@ def test_nan_direction_reads_texel_zero():
             # u = v = 0 sits on the texel corner: the average of the four wrapped corners
             exp = (rgb[0, 0] + rgb[0, -1] + rgb[-1, 0] + rgb[-1, -1]) / 4
             np.testing.assert_allclose(out, exp, rtol=1e-6)
+
+
+def rot4(ax_deg, ay_deg):
+    """Row-major 4x4 rotation Ry(ay) Rx(ax), applied to directions as M (v, 0)."""
+    a, b = math.radians(ax_deg), math.radians(ay_deg)
+    rx = np.array([[1, 0, 0], [0, math.cos(a), -math.sin(a)], [0, math.sin(a), math.cos(a)]])
+    ry = np.array([[math.cos(b), 0, math.sin(b)], [0, 1, 0], [-math.sin(b), 0, math.cos(b)]])
+    m = np.eye(4)
+    m[:3, :3] = ry @ rx
+    return m
+
+
+def env_eval(env, d, background=0, px=0, py=0, W=1, H=1):
+    out = np.zeros(3, dtype=np.float32)
+    pyoracle.load().ora_env_evaluate(C.byref(env), pyoracle._p(np.asarray(d, dtype=np.float32)), background,
+                                     px, py, W, H, pyoracle._p(out))
+    return out.astype(np.float64)
+
+
+def test_environment_transforms_rotate_lookups_and_samples():
+    """EnvironmentBase::updateGPUStruct's light/background transforms (EnvironmentBase.cpp:52-71)
+    used by computeEnvironmentUV (Environment.slang:62-81): a lookup in direction d reads the
+    untransformed environment at M d; a light sample is the untransformed texel-centre sample
+    taken back through M^-1 (:137-186), with the same pdf and radiance."""
+    rng = np.random.default_rng(21)
+    sky = scenes.sky_gradient(64, 32, rng)
+    bg = scenes.sky_gradient(48, 24, np.random.default_rng(22))
+    M, B = rot4(25.0, 70.0), rot4(-40.0, 15.0)
+    plain, k1 = pyoracle.make_env(dict(light_rgb=sky, background_rgb=bg, filter=1))
+    rotated, k2 = pyoracle.make_env(dict(light_rgb=sky, background_rgb=bg, filter=1,
+                                         light_transform=list(M.ravel()), background_transform=list(B.ravel())))
+    for d in rng.normal(size=(200, 3)):
+        d /= np.linalg.norm(d)
+        np.testing.assert_allclose(env_eval(rotated, d), env_eval(plain, M[:3, :3] @ d), rtol=2e-4, atol=1e-6)
+        np.testing.assert_allclose(env_eval(rotated, d, 1), env_eval(plain, B[:3, :3] @ d, 1), rtol=2e-4,
+                                   atol=1e-6)
+    lib = pyoracle.load()
+    for r in rng.uniform(0, 1, (200, 2)).astype(np.float32):
+        out = []
+        for env in (plain, rotated):
+            L, pdf, rad = np.zeros(3, np.float32), C.c_float(), np.zeros(3, np.float32)
+            lib.ora_env_sample(C.byref(env), pyoracle._p(r), pyoracle._p(L), C.byref(pdf), pyoracle._p(rad))
+            out.append((L.astype(np.float64), pdf.value, rad.copy()))
+        np.testing.assert_allclose(out[1][0], M[:3, :3].T @ out[0][0], atol=2e-6)
+        assert out[1][1] == out[0][1]
+        np.testing.assert_array_equal(out[1][2], out[0][2])
+
+
+def test_screen_space_background_reads_the_pixel_position():
+    """background_use_screen: the background uv is the pixel position over the image size,
+    whatever the direction (computeEnvironmentUV, Environment.slang:68-74)."""
+    rng = np.random.default_rng(23)
+    W, H = 24, 16
+    bg = rng.uniform(0, 1, (H, W, 3)).astype(np.float32)
+    img_env, k1 = pyoracle.make_env(dict(background_rgb=bg, background_use_screen=1))
+    grad_env, k2 = pyoracle.make_env(dict(background_top=(1.0, 0.5, 0.0), background_bottom=(0.0, 0.5, 1.0),
+                                          background_use_screen=1))
+    for px, py in ((0, 0), (23, 15), (5, 9), (12, 3)):
+        for d in rng.normal(size=(3, 3)):
+            d /= np.linalg.norm(d)
+            np.testing.assert_array_equal(env_eval(img_env, d, 1, px, py, W, H), bg[py, px])
+            v = py / H
+            np.testing.assert_allclose(env_eval(grad_env, d, 1, px, py, W, H),
+                                       np.array([1.0, 0.5, 0.0]) * (1 - v) + np.array([0.0, 0.5, 1.0]) * v, atol=1e-6)
+        # the light lookup ignores the screen flag
+        np.testing.assert_array_equal(env_eval(img_env, (0.0, 1.0, 0.0), 0, px, py, W, H), [1.0, 1.0, 1.0])
