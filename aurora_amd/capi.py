@@ -90,7 +90,7 @@ class Stats(C.Structure):
     _fields_ = [("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("node_visits", C.c_uint64),
                 ("tri_tests", C.c_uint64), ("inst_visits", C.c_uint64), ("paths", C.c_uint64),
                 ("shadow_node_visits", C.c_uint64), ("shadow_tri_tests", C.c_uint64),
-                ("shadow_inst_visits", C.c_uint64)]
+                ("shadow_inst_visits", C.c_uint64), ("hits", C.c_uint64)]
 
 
 class Tuning(C.Structure):

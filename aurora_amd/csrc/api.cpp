@@ -1103,6 +1103,7 @@ int aph_stats_get(aph_ctx_t c, aph_stats* out)
     out->shadow_node_visits = k.snodes;
     out->shadow_tri_tests = k.stris;
     out->shadow_inst_visits = k.sinsts;
+    out->hits = k.hits;
     return APH_OK;
 }
 

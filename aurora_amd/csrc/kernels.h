@@ -11,6 +11,7 @@ namespace aph {
 struct DevCounters {
     unsigned long long ext, shadow, paths, nodes, tris, insts, overflow;
     unsigned long long snodes, stris, sinsts;  // the any-hit (shadow) share of nodes/tris/insts
+    unsigned long long hits;                   // closest-hit queries that found a triangle (wavefront, stats mode)
 };
 
 hipError_t launchRenderMega(const RenderScene& rs, const aph_frame& fp, uint32_t W, uint32_t H, uint32_t start,

@@ -357,6 +357,7 @@ __global__ __launch_bounds__(kWB) void kExtend(const DevReq* reqs, const SegInfo
     const StackMem mem{sStack + threadIdx.x, spill};
     TraceStats ts = {};
     const uint32_t n = Q.counts[cur];
+    uint32_t found = 0;
     auto decode = [&](float4 a, float4 b, f3& org, f3& dir, float& tmin) -> const TraceScene* {
         org = xyz(a);
         dir = xyz(b);
@@ -364,6 +365,7 @@ __global__ __launch_bounds__(kWB) void kExtend(const DevReq* reqs, const SegInfo
         return &reqs[tagReq(a.w)].rs.sc;
     };
     auto store = [&](uint32_t i, const TraceState& s) {
+        if (STATS) found += s.found ? 1u : 0u;
         if (s.found) {
             Q.hitA[i] = make_float4((float)s.best.t, __uint_as_float(s.best.inst), __uint_as_float(s.best.prim),
                                     s.best.u);
@@ -381,6 +383,7 @@ __global__ __launch_bounds__(kWB) void kExtend(const DevReq* reqs, const SegInfo
         addCounter(&ctr->nodes, ts.nodes);
         addCounter(&ctr->tris, ts.tris);
         addCounter(&ctr->insts, ts.insts);
+        addCounter(&ctr->hits, found);
     }
     addCounter(&ctr->overflow, ts.overflow);
 }
@@ -1040,8 +1043,9 @@ size_t wavefrontCallTableBytes(const WavefrontRequest* reqs, uint32_t nreq, uint
         nseg += reqs[i].sampleCount;
         work += (uint64_t)reqs[i].width * reqs[i].height * reqs[i].sampleCount;
     }
-    // every chunk but the last holds more than half the capacity or hits a table limit
-    size_t nchunk = 1 + (size_t)(2 * work / std::max<uint32_t>(paths, 1)) + nseg / kMaxSegs + nseg / kMaxCreqs;
+    // a chunk closes when the next segment does not fit (so two consecutive chunks hold more
+    // than the capacity) or at a table limit
+    size_t nchunk = 2 + (size_t)(2 * work / std::max<uint32_t>(paths, 1)) + nseg / kMaxSegs + nseg / kMaxCreqs;
     nchunk = std::min(nchunk, nseg + 1);
     return callTableBytes(nreq, nchunk, nseg, nseg);
 }

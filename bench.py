@@ -42,7 +42,15 @@ HBM_PEAK_GBS = 8000.0
 # (origin + request tag, direction + path id), a hit record (wavefront:
 # t/inst/prim/u + v; megakernel: kept in registers, the 16 B of the RGBA32F sample
 # is charged to accumulation) and one RGBA32F accumulation read+write per pixel.
-NODE_B, TRI_B, INST_B, RAY_B, HIT_B, ACC_B = 128, 48, 64, 32, 20, 32
+NODE_B, TRI_B, INST_B, RAY_B, HIT_B, ACC_B, VIS_B = 128, 48, 64, 32, 20, 32, 1
+# kShade per queue entry: ray (32) + hit (20) + path state read (pixel, request, flags, contribution,
+# rng: 36) and written (flags, contribution, rng, pending emission, continue and shadow-mask bytes: 46);
+# per hit: shading data (indices + 3 x position, normal, uv: 108), instance shading record (96),
+# material (192), environment (2 alias entries + 2 texels: 64), written next ray (32), pending
+# contribution (16) and two shadow rays with their colours (2 x 48); per miss: one environment texel.
+SHADE_ENTRY_B = 32 + 20 + 36 + 46
+SHADE_HIT_B = 108 + 96 + 192 + 64 + 32 + 16 + 2 * 48
+SHADE_MISS_B = 16
 
 
 def log(msg):
@@ -54,8 +62,17 @@ def device_sync():
     hip.hipDeviceSynchronize()
 
 
-KERNEL_SYMBOL = {"render_megakernel": "kRenderMega<false>", "wf_extend": "kExtend<false>"}
+# Kernels priced by the measurement, by their timing name: the symbol substring that
+# picks their dispatches out of a rocprofv3 database.
+KERNEL_SYMBOL = {"render_megakernel": "kRenderMega<false>", "wf_extend": "kExtend<false>",
+                 "wf_shadow": "kShadow<false>", "wf_shade": "kShade("}
 COPY_BYTES, COPIES = 1 << 30, 3
+# Issue-side counters of one pass (at most 8 SQ and 2 GRBM counters per pass).
+ISSUE_COUNTERS = ["SQ_INSTS_VALU", "SQ_ACTIVE_INST_VALU", "SQ_THREAD_CYCLES_VALU", "SQ_WAVE_CYCLES",
+                  "SQ_BUSY_CYCLES", "GRBM_GUI_ACTIVE"]
+# MI355X: 256 CUs x 4 SIMDs at 2.4 GHz; a wave64 VALU instruction occupies a 16-lane SIMD for 4
+# cycles, so the chip issues at most 256 * 4 * 2.4e9 / 4 wave-level VALU instructions per second.
+PEAK_VALU_WAVE_INSTS = 256 * 4 * 2.4e9 / 4
 
 
 def copy_calibration():
@@ -74,7 +91,7 @@ def copy_calibration():
 
 
 def pmc_dispatches(db_dir, counter):
-    """(kernel name, counter value in KiB) per dispatch from a rocprofv3 database."""
+    """(kernel name, counter value) per dispatch from a rocprofv3 database."""
     rows = []
     for db in glob.glob(os.path.join(db_dir, "**", "*.db"), recursive=True):
         con = sqlite3.connect(db)
@@ -84,46 +101,87 @@ def pmc_dispatches(db_dir, counter):
     return rows
 
 
-def measure_traffic(args, kname):
-    """HBM bytes per launch of the dominant kernel: FETCH_SIZE and WRITE_SIZE per
-    dispatch (two rocprofv3 passes over one frame), each scaled by actual bytes /
-    counter reading of the calibration copies in the same pass."""
+def pmc_pass(args, counters):
+    """One rocprofv3 --pmc pass over the pmc child (calibration copies + one frame):
+    {counter: [(kernel name, value) per dispatch]}, or None."""
     prof = shutil.which("rocprofv3")
     if prof is None:
-        return None, "rocprofv3 not found"
-    sym = KERNEL_SYMBOL[kname]
-    res = {}
+        return None
+    d = tempfile.mkdtemp(prefix="aph_pmc_", dir="/tmp")
+    cmd = ["timeout", "-s", "KILL", "240", prof, "--pmc", *counters, "-d", d, "-o", "pmc", "--",
+           sys.executable, "-u", os.path.abspath(__file__), "--pmc-child", "--spp", str(args.spp),
+           "--subdiv", str(args.subdiv), "--mode", args.mode, "--builder", args.builder, "--tuning", args.tuning]
+    log(f"pmc pass {' '.join(counters)}: {' '.join(cmd)}")
+    env = dict(os.environ, TMPDIR="/tmp")
+    p = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = {c: pmc_dispatches(d, c) for c in counters}
+    shutil.rmtree(d, ignore_errors=True)
+    if p.returncode != 0 or not all(out.values()):
+        log(p.stdout[-2000:])
+        return None
+    return out
+
+
+def measure_counters(args, kernels):
+    """Per kernel (timing name): HBM bytes per launch from FETCH_SIZE and WRITE_SIZE (two
+    passes, each scaled by actual bytes / counter reading of 1 GiB calibration copies made in
+    the same pass) and the VALU issue counters per launch (a third pass).  Returns
+    ({kernel: {...}}, detail) or (None, reason)."""
+    res = {k: {} for k in kernels}
+    factors = {}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
-        d = tempfile.mkdtemp(prefix="aph_pmc_", dir="/tmp")
-        cmd = ["timeout", "-s", "KILL", "240", prof, "--pmc", counter, "-d", d, "-o", "pmc", "--",
-               sys.executable, "-u", os.path.abspath(__file__), "--pmc-child", "--spp", str(args.spp),
-               "--subdiv", str(args.subdiv), "--mode", args.mode, "--builder", args.builder,
-               "--tuning", args.tuning]
-        log(f"pmc pass {counter}: {' '.join(cmd)}")
-        env = dict(os.environ, TMPDIR="/tmp")
-        p = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
-        rows = pmc_dispatches(d, counter)
-        shutil.rmtree(d, ignore_errors=True)
-        if p.returncode != 0 or not rows:
-            log(p.stdout[-2000:])
-            return None, f"{counter} pass failed (rc {p.returncode})"
-        kern = [v for n, v in rows if sym in n]
+        rows = pmc_pass(args, [counter])
+        if rows is None:
+            return None, f"{counter} pass failed"
+        rows = rows[counter]
         copies = sorted((v for n, v in rows if "copyBuffer" in n), reverse=True)[:COPIES]
-        if not kern or len(copies) < COPIES:
-            return None, f"{counter} pass: {len(kern)} {sym} dispatches, {len(copies)} copies"
-        factor = COPY_BYTES * COPIES / (sum(copies) * 1024.0)
-        res[counter] = (sum(kern) / len(kern), factor, len(kern))
-        log(f"{counter}: {res[counter][0]:.1f} KiB per {sym} dispatch over {len(kern)}, copy factor {factor:.4f}")
-    f_kib, f_fac, nd = res["FETCH_SIZE"]
-    w_kib, w_fac, _ = res["WRITE_SIZE"]
-    detail = {"fetch_kib_per_dispatch": round(f_kib, 1), "fetch_factor": round(f_fac, 4),
-              "write_kib_per_dispatch": round(w_kib, 1), "write_factor": round(w_fac, 4), "dispatches": nd}
-    return int((f_kib * f_fac + w_kib * w_fac) * 1024), detail
+        if len(copies) < COPIES:
+            return None, f"{counter} pass: {len(copies)} calibration copies"
+        factors[counter] = COPY_BYTES * COPIES / (sum(copies) * 1024.0)
+        for k in kernels:
+            vals = [v for n, v in rows if KERNEL_SYMBOL[k] in n]
+            if vals:
+                res[k][counter] = sum(vals) / len(vals)
+                res[k]["dispatches"] = len(vals)
+        log(f"{counter}: copy factor {factors[counter]:.4f}; " +
+            ", ".join(f"{k} {res[k].get(counter, 0):.1f} KiB" for k in kernels))
+    rows = pmc_pass(args, ISSUE_COUNTERS)
+    for k in kernels:
+        r = res[k]
+        if "FETCH_SIZE" in r and "WRITE_SIZE" in r:
+            r["traffic_bytes"] = int((r["FETCH_SIZE"] * factors["FETCH_SIZE"] + r["WRITE_SIZE"] * factors["WRITE_SIZE"])
+                                     * 1024)
+        if rows is not None:
+            for c in ISSUE_COUNTERS:
+                vals = [v for n, v in rows[c] if KERNEL_SYMBOL[k] in n]
+                if vals:
+                    r[c] = sum(vals) / len(vals)
+    detail = {"fetch_factor": round(factors["FETCH_SIZE"], 4), "write_factor": round(factors["WRITE_SIZE"], 4),
+              "issue_pass": rows is not None}
+    return res, detail
 
 
 def apply_tuning(r, tuning):
     if tuning:
         r.set_tuning(*(int(v) for v in tuning.split(",")))
+
+
+def host_cpu():
+    """(CPU model, logical CPUs of the host, CPUs this process may run on)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count() or 1, len(os.sched_getaffinity(0))
+
+
+# A one-GPU job on the GPU host is allotted 16 host cores; worker pools are sized to that share.
+HOST_SHARE = 16
 
 
 def cpu_baseline(sc, budget_s=12.0):
@@ -132,7 +190,8 @@ def cpu_baseline(sc, budget_s=12.0):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
 
-    threads = max(1, min(16, os.cpu_count() or 1))
+    model, nproc, allowed = host_cpu()
+    threads = max(1, min(HOST_SHARE, allowed))
     ora = pyoracle.OracleScene(sc)
     ora.env()  # alias-table build outside the timed sample
     w, h = sc.width, sc.height
@@ -155,6 +214,10 @@ def cpu_baseline(sc, budget_s=12.0):
         "value": round(total_rays / total_time / 1e6, 3),
         "unit": "Mrays/s",
         "cores": threads,
+        "cpu_model": model,
+        "host_logical_cpus": nproc,
+        "cpus_allowed": allowed,
+        "cores_note": f"all {threads} cores of the one-GPU job's host share ({HOST_SHARE}); the host has {nproc}",
         "kind": "port",
         "sample": f"{total_pix} random pixel-samples of the 1920x1080 frame ({sample} sample indices), 5 bounces: "
                   f"{total_rays} rays in {total_time:.1f} s on {threads} host threads",
@@ -265,22 +328,27 @@ def c3_cameras(r, builder, seed=3, autotune=True):
             "launch_config_source": "autotuner on these 64 cameras at 1/8 spp" if tuned else "context"}
 
 
-def serving(r, builder, jobs=24, rates=(4.0, 16.0), max_spp=32, seed=5, budget_gib=24.0):
-    """Poisson arrivals of render jobs over the four resident serving scenes (about 10k, 100k,
-    1M and 5M triangles); resolution from RESOLUTIONS, spp U{1..max_spp}; HBM-budget admission.
-    Batched (continuous batching in shared wavefronts) vs plain (one job at a time, megakernel)."""
+def serving(r, builder, jobs=32, rates=(10.0, 40.0), max_spp=64, seed=5, budget_gib=48.0, width=1920, height=1080):
+    """Config 4 at reduced length: Poisson arrivals of 1080p jobs with spp U{1..max_spp} over the
+    four resident serving scenes (about 10k, 100k, 1M and 5M triangles), HBM-budget admission
+    (scenes, job buffers and the wavefront working set).  Batched (continuous batching in
+    shared wavefronts) vs plain (one job at a time, megakernel).  Both rates saturate the plain
+    form; the batched jobs/s at the higher rate is its saturation throughput."""
     from aurora_amd.session import Server
     scs = scenes.serving_scenes()
     tl = r.load_scenes(scs, builder)
     rng = np.random.default_rng(seed)
-    out = {"scenes_triangles": [s.instanced_triangles for s in scs], "jobs": jobs, "max_spp": max_spp,
-           "hbm_budget_gib": budget_gib, "rates": {}}
+    out = {"scenes_triangles": [s.instanced_triangles for s in scs], "jobs": jobs, "resolution": f"{width}x{height}",
+           "spp": f"U{{1..{max_spp}}}", "hbm_budget_gib": budget_gib, "rates": {},
+           "differs_from_config4": f"{jobs} jobs per rate (config 4: 1,024) at rates {list(rates)} /s (config 4: "
+                                   "10-500 /s); every rate above about 2 jobs/s saturates one GPU, so longer traces "
+                                   "only lengthen the queue"}
     js = []
     for _ in range(jobs):
         k = int(rng.integers(0, 4))
-        w, h = RESOLUTIONS[int(rng.integers(0, 4))]
-        js.append(Server.job(tl[k], scenes.orbit_frame(rng, w, h, radius=30.0 if k else 14.0), w, h,
-                             int(rng.integers(1, max_spp + 1))))
+        js.append(Server.job(tl[k], scenes.orbit_frame(rng, width, height, radius=30.0 if k else 14.0), width,
+                             height, int(rng.integers(1, max_spp + 1))))
+    samples = sum(j.width * j.height * j.spp for j in js)
     for rate in rates:
         arr = np.cumsum(rng.exponential(1.0 / rate, jobs))
         res = {}
@@ -292,27 +360,51 @@ def serving(r, builder, jobs=24, rates=(4.0, 16.0), max_spp=32, seed=5, budget_g
             dt = time.perf_counter() - t
             lat = np.array([srv.status(tk).t_done - srv.status(tk).t_arrival for tk in tickets])
             resident = srv.usage()[0]
+            wf_bytes, wf_paths = srv.working_set()
             srv.close()
             res[name] = {"p50_s": round(float(np.percentile(lat, 50)), 3),
                          "p99_s": round(float(np.percentile(lat, 99)), 3),
                          "jobs_per_s": round(jobs / dt, 2), "mrays_s": round(rays_of(r.stats()) / dt / 1e6, 2),
-                         "wall_s": round(dt, 2)}
+                         "samples_per_s": round(samples / dt, 1), "wall_s": round(dt, 2),
+                         "last_arrival_s": round(float(arr[-1]), 2)}
+            if mode == capi.APH_MODE_WAVEFRONT:
+                res[name]["working_set_gib"] = round(wf_bytes / (1 << 30), 2)
+                res[name]["paths_in_flight"] = wf_paths
             log(f"serving rate {rate}/s {name}: {res[name]}")
+        res["batched_vs_plain_throughput"] = round(res["batched"]["jobs_per_s"] / res["plain"]["jobs_per_s"], 2)
         out["rates"][str(rate)] = res
     out["resident_scene_bytes"] = int(resident)
     return out
 
 
-def equal_time_rmse(r, builder, width=480, height=270, ref_spp=4096, plain_spp=64):
-    """Equal-time RMSE vs a 4096-spp reference (config2 at reduced resolution): the plain
-    megakernel's 64-spp image against the wavefront image with as many samples as it renders
-    in the same time (same sampler, so images differ only by sample count)."""
+def config2_reference(r, builder, width=1920, height=1080, ref_spp=4096):
+    """The 1080p config2 frame (the headline scene and camera) and its 4096-spp wavefront render
+    with seeds disjoint from every measured render (seed offset 1,000,003); rendered once per
+    renderer and shared by the sections that score against it."""
+    key = (builder, width, height, ref_spp)
+    cache = r.__dict__.setdefault("_reference_cache", {})
     sc = scenes.config2(width=width, height=height)
-    tlas = r.load_scene(sc, builder)
-    ref = r.target(width, height)
-    r.render([r.request(ref, tlas, sc.frame, 0, ref_spp)], capi.APH_MODE_WAVEFRONT)
-    r.wait()
-    ref_img = r.readback(ref, width, height)[..., :3].astype(np.float64)
+    tlas = r.load_scene(sc, builder)  # loaded again each time: other sections replace the material table
+    if key not in cache:
+        ref = r.target(width, height)
+        step = 512
+        for s0 in range(0, ref_spp, step):
+            r.render([r.request(ref, tlas, sc.frame, s0, min(step, ref_spp - s0), 1_000_003)],
+                     capi.APH_MODE_WAVEFRONT)
+            r.wait()
+            log(f"reference: {s0 + step}/{ref_spp} spp")
+        cache[key] = r.readback(ref, width, height)[..., :3].astype(np.float64)
+        r.destroy_target(ref)
+    return sc, tlas, cache[key]
+
+
+def equal_time_rmse(r, builder, plain_spp=64):
+    """Equal-time RMSE on the 1080p headline frame against its 4096-spp reference: the plain
+    megakernel's 64-spp image against the wavefront image with as many samples as the
+    wavefront renders in the same wall time (same sampler, so the images differ only by
+    sample count).  Linear radiance and tonemapped x / (1 + x)."""
+    sc, tlas, ref_img = config2_reference(r, builder)
+    width, height = sc.width, sc.height
     tp = r.target(width, height)
     t = time.perf_counter()
     r.render([r.request(tp, tlas, sc.frame, 0, plain_spp)], capi.APH_MODE_MEGAKERNEL)
@@ -324,16 +416,22 @@ def equal_time_rmse(r, builder, width=480, height=270, ref_spp=4096, plain_spp=6
     r.wait()
     t_wf = time.perf_counter() - t
     spp_eq = max(plain_spp, int(plain_spp * t_plain / t_wf))
+    t = time.perf_counter()
     if spp_eq > plain_spp:
         r.render([r.request(tw, tlas, sc.frame, plain_spp, spp_eq - plain_spp)], capi.APH_MODE_WAVEFRONT)
         r.wait()
+    t_wf += time.perf_counter() - t
+    tm = lambda x: np.maximum(x, 0) / (1.0 + np.maximum(x, 0))  # noqa: E731
 
     def rmse(tgt):
-        return float(np.sqrt(np.mean((r.readback(tgt, width, height)[..., :3] - ref_img) ** 2)))
+        img = r.readback(tgt, width, height)[..., :3].astype(np.float64)
+        return (float(np.sqrt(np.mean((img - ref_img) ** 2))), float(np.sqrt(np.mean((tm(img) - tm(ref_img)) ** 2))))
     a, b = rmse(tp), rmse(tw)
-    return {"resolution": f"{width}x{height}", "reference_spp": ref_spp, "time_s": round(t_plain, 3),
-            "plain_spp": plain_spp, "plain_rmse": round(a, 5), "wavefront_spp": spp_eq, "wavefront_rmse": round(b, 5),
-            "rmse_ratio": round(a / b, 3)}
+    return {"resolution": f"{width}x{height}", "reference_spp": 4096, "plain_s": round(t_plain, 3),
+            "wavefront_s": round(t_wf, 3), "plain_spp": plain_spp, "wavefront_spp": spp_eq,
+            "plain_rmse": round(a[0], 5), "wavefront_rmse": round(b[0], 5), "rmse_ratio": round(a[0] / b[0], 3),
+            "plain_rmse_tonemapped": round(a[1], 5), "wavefront_rmse_tonemapped": round(b[1], 5),
+            "rmse_ratio_tonemapped": round(a[1] / b[1], 3)}
 
 
 def autotune_section(r, sc, tlas, spp=32):
@@ -464,17 +562,9 @@ def denoise_vs_plain(r, builder, width=1920, height=1080, ref_spp=4096, plain_sp
     and the U-Net with the committed weights; the plain path is one megakernel launch.
     Further weight files, if given, are scored on the same images under "variants"."""
     weights = list(weights or [dnmod.DEFAULT_WEIGHTS])
-    sc = scenes.config2(width=width, height=height)
-    tlas = r.load_scene(sc, builder)
+    sc, tlas, g = config2_reference(r, builder, width, height, ref_spp)
     dn = dnmod.Denoiser(r)
     w, h = width, height
-    ref = r.target(w, h)
-    step = 512
-    for s0 in range(0, ref_spp, step):
-        r.render([r.request(ref, tlas, sc.frame, s0, min(step, ref_spp - s0), 1_000_003)], capi.APH_MODE_WAVEFRONT)
-        r.wait()
-        log(f"denoise: reference {s0 + step}/{ref_spp} spp")
-    g = r.readback(ref, w, h)[..., :3].astype(np.float64)
 
     def errors(tgt):
         img = r.readback(tgt, w, h)[..., :3].astype(np.float64)
@@ -618,7 +708,8 @@ def main():
         if tune_info:
             args.tuning = ",".join(str(v) for v in tune_info["chosen"])
             log(f"autotune: {tune_info}")
-    traffic, traffic_detail = (None, "skipped (--no-traffic)") if args.no_traffic else measure_traffic(args, kname)
+    priced = [kname] if args.mode == "megakernel" else ["wf_extend", "wf_shadow", "wf_shade"]
+    counters, counter_detail = (None, "skipped (--no-traffic)") if args.no_traffic else measure_counters(args, priced)
     log(f"scene: {sc.instanced_triangles} instanced / {sc.unique_triangles} unique triangles, "
         f"{len(sc.instances)} instances, {sc.width}x{sc.height} x {sc.spp} spp, mode {args.mode}")
     r = Renderer(0)
@@ -636,15 +727,24 @@ def main():
     r.set_stats_level(0)
     queries = st["extension_rays"] + st["shadow_rays"]
     if args.mode == "wavefront":
-        # kExtend alone: closest-hit traversal of the extension queue
-        frame_bytes = ((st["node_visits"] - st["shadow_node_visits"]) * NODE_B
-                       + (st["tri_tests"] - st["shadow_tri_tests"]) * TRI_B
-                       + (st["inst_visits"] - st["shadow_inst_visits"]) * INST_B
-                       + st["extension_rays"] * (RAY_B + HIT_B))
+        ext, hits = st["extension_rays"], st["hits"]
+        frame_bytes = {
+            # kExtend: closest-hit traversal of the extension queue
+            "wf_extend": ((st["node_visits"] - st["shadow_node_visits"]) * NODE_B
+                          + (st["tri_tests"] - st["shadow_tri_tests"]) * TRI_B
+                          + (st["inst_visits"] - st["shadow_inst_visits"]) * INST_B + ext * (RAY_B + HIT_B)),
+            # kShadow: any-hit traversal of the shadow queue, one visibility byte per query
+            "wf_shadow": (st["shadow_node_visits"] * NODE_B + st["shadow_tri_tests"] * TRI_B
+                          + st["shadow_inst_visits"] * INST_B + st["shadow_rays"] * (RAY_B + VIS_B)),
+            # kShade: every queue entry's ray, hit and path state; per hit the shading data,
+            # instance record, material, environment lookups and the rays and shadow records written;
+            # per miss one environment texel
+            "wf_shade": ext * SHADE_ENTRY_B + hits * SHADE_HIT_B + (ext - hits) * SHADE_MISS_B,
+        }
     else:
         # the megakernel does the whole frame: every query, and accumulation
-        frame_bytes = (st["node_visits"] * NODE_B + st["tri_tests"] * TRI_B + st["inst_visits"] * INST_B
-                       + queries * RAY_B + sc.width * sc.height * ACC_B)
+        frame_bytes = {kname: st["node_visits"] * NODE_B + st["tri_tests"] * TRI_B + st["inst_visits"] * INST_B
+                       + queries * RAY_B + sc.width * sc.height * ACC_B}
     log(f"stats: {st}")
 
     for i in range(args.warmup):
@@ -669,11 +769,34 @@ def main():
     r.render(req, mode)
     r.wait()
     r.enable_timing(False)
-    kms, klaunch = r.timing(kname)
-    avg_launch_ms = kms / max(klaunch, 1)
-    launches_per_frame = klaunch
-    bytes_per_launch = frame_bytes / max(launches_per_frame, 1)
-    achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
+    kernels = {}
+    for k in priced:
+        kms, klaunch = r.timing(k)
+        avg_ms = kms / max(klaunch, 1)
+        per_launch = frame_bytes[k] / max(klaunch, 1)
+        e = {"launches_per_frame": klaunch, "avg_launch_ms": round(avg_ms, 3), "ms_per_frame": round(kms, 2),
+             "algorithmic_bytes_per_launch": int(per_launch),
+             "algorithmic_gbs": round(per_launch / (avg_ms * 1e-3) / 1e9, 1)}
+        c = (counters or {}).get(k, {})
+        if "traffic_bytes" in c:
+            e["hbm_bytes_per_launch"] = c["traffic_bytes"]
+            e["hbm_gbs"] = round(c["traffic_bytes"] / (avg_ms * 1e-3) / 1e9, 1)
+            e["hbm_frac"] = round(e["hbm_gbs"] / HBM_PEAK_GBS, 4)
+            e["algorithmic_over_hbm_bytes"] = round(per_launch / max(c["traffic_bytes"], 1), 2)
+        if "SQ_INSTS_VALU" in c and "SQ_ACTIVE_INST_VALU" in c:
+            e["valu_wave_insts_per_launch"] = int(c["SQ_INSTS_VALU"])
+            e["valu_issue_frac"] = round(c["SQ_INSTS_VALU"] / (avg_ms * 1e-3) / PEAK_VALU_WAVE_INSTS, 4)
+            if c.get("SQ_THREAD_CYCLES_VALU") and c["SQ_ACTIVE_INST_VALU"]:
+                e["valu_lanes_active"] = round(c["SQ_THREAD_CYCLES_VALU"] / c["SQ_ACTIVE_INST_VALU"], 1)
+            e["counters_per_launch"] = {n: int(c[n]) for n in ISSUE_COUNTERS if n in c}
+        if "hbm_frac" in e and "valu_issue_frac" in e:
+            e["limiter"] = ("VALU issue (divergent waves)" if e["valu_issue_frac"] > e["hbm_frac"]
+                            else "HBM bandwidth")
+        kernels[k] = e
+    dom = kernels[kname]
+    avg_launch_ms = dom["avg_launch_ms"]
+    bytes_per_launch = dom["algorithmic_bytes_per_launch"]
+    achieved = dom["algorithmic_gbs"]
     mrays = rays / wall / 1e6
     out = {
         "metric": "Mrays/s (1M tri, 1080p)",
@@ -711,10 +834,15 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic,
-            "traffic_detail": traffic_detail,
+            "traffic": dom.get("hbm_bytes_per_launch"),
+            "traffic_detail": counter_detail,
             "bytes_per_launch": int(bytes_per_launch),
             "avg_launch_ms": round(avg_launch_ms, 3),
+            "achieved_meaning": "algorithmic bytes (BVH nodes, triangles, instance records, rays, hits) per launch "
+                                "over the launch time: effective bandwidth on L2/MALL-resident data",
+            "frac_of_hbm_by_counters": dom.get("hbm_frac"),
+            "limiter": dom.get("limiter"),
+            "kernels": kernels,
         },
     }
     if args.mode == "wavefront":

@@ -176,6 +176,7 @@ typedef struct {
     uint64_t shadow_node_visits; /* the any-hit queries' share of node_visits */
     uint64_t shadow_tri_tests;   /* ... of tri_tests */
     uint64_t shadow_inst_visits; /* ... of inst_visits */
+    uint64_t hits;               /* extension rays that hit a triangle (wavefront, stats mode only) */
 } aph_stats;
 
 /* ---- context: createRenderer(Backend, taskCount), Aurora.h:1233-1234;
