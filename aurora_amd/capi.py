@@ -257,6 +257,7 @@ SIGNATURES = {
     "aph_server_release": (_i, [_P, _u64]),
     "aph_server_usage": (_i, [_P, C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u32), C.POINTER(_u32)]),
     "aph_server_working_set": (_i, [_P, C.POINTER(_u64), C.POINTER(_u32)]),
+    "aph_render_noalloc": (_i, [_P, _P, _u32]),
     "aph_server_replay": (_i, [_P, _P, _P, _u32, _P]),
     "aph_image_create": (_i, [_P, C.POINTER(ImageDesc), C.POINTER(_u64)]),
     "aph_image_destroy": (_i, [_P, _u64]),

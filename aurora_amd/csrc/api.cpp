@@ -166,6 +166,8 @@ int aph_create(int device, aph_ctx_t* out)
     c->nmat = 1;
     c->matsHost.assign(1, dm);
     if (uploadMaterialTextures(c.get()) != APH_OK) return APH_E_DEVICE;
+    // the allocation-free render's memory on this device (a failure only disables that entry point)
+    c->wfStaticStatus = (int)wavefrontScratchState(c->wfStatic);
     *out = c.release();
     return APH_OK;
 }
@@ -908,6 +910,38 @@ int aph_render(aph_ctx_t c, const aph_request* reqs, uint32_t nreq, int mode)
     hipError_t e = wavefrontRender(c->wf, wreq.data(), (uint32_t)wreq.size(), c->counters.as<DevCounters>(),
                                    c->statsLevel > 0, c->stream, c->timing ? &c->pending : nullptr);
     if (e != hipSuccess) return hipFail(c, e, "wavefront render");
+    return APH_OK;
+}
+
+int aph_render_noalloc(aph_ctx_t c, const aph_request* reqs, uint32_t nreq)
+{
+    int rc = enter(c);
+    if (rc) return rc;
+    if (nreq == 0) return APH_OK;
+    if ((rc = checkTextures(c))) return rc;
+    if (!reqs) return fail(c, APH_E_INVALID, "requests pointer is null");
+    if (c->wfStaticStatus != (int)hipSuccess)
+        return fail(c, APH_E_STATE, "allocation-free render memory unavailable on device %d: %s", c->device,
+                    hipGetErrorString((hipError_t)c->wfStaticStatus));
+    std::vector<WavefrontRequest> wreq;
+    if ((rc = buildRequests(c, reqs, nreq, wreq))) return rc;
+    for (uint32_t i = 0; i < nreq; i++) {
+        for (uint32_t k = 0; k < i; k++)
+            if (reqs[k].target == reqs[i].target)
+                return fail(c, APH_E_INVALID, "requests %u and %u share target %llu (one request per target per call)",
+                            k, i, (unsigned long long)reqs[i].target);
+        if ((uint64_t)wreq[i].width * wreq[i].height > kNoAllocPaths)
+            return fail(c, APH_E_INVALID, "request %u: %ux%u pixels exceed the allocation-free render's %u paths",
+                        i, wreq[i].width, wreq[i].height, kNoAllocPaths);
+    }
+    if (wavefrontCallTableBytes(wreq.data(), nreq, kNoAllocPaths) > kNoAllocTableBytes)
+        return fail(c, APH_E_INVALID, "%u requests and their sample segments exceed the allocation-free render's "
+                    "%zu-byte call table", nreq, kNoAllocTableBytes);
+    c->wfStatic.tuning = c->wf.tuning;
+    hipError_t e = wavefrontRender(c->wfStatic, wreq.data(), nreq, c->counters.as<DevCounters>(), c->statsLevel > 0,
+                                   c->stream, nullptr, kNoAllocPaths);
+    if (e == hipErrorOutOfMemory) return fail(c, APH_E_INVALID, "the call does not fit the allocation-free render");
+    if (e != hipSuccess) return hipFail(c, e, "allocation-free render");
     return APH_OK;
 }
 

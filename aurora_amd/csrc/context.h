@@ -141,6 +141,8 @@ struct aph_ctx {
     int statsLevel = 0;
     DevBuf scratchA, scratchB;
     aph::WavefrontState wf;
+    aph::WavefrontState wfStatic;   // aph_render_noalloc: the device's scratch array, set up by aph_create
+    int wfStaticStatus = -1;        // hipError_t of that set-up (hipSuccess = 0)
     bool timing = false;
     std::map<std::string, Timing> timings;
     std::vector<aph::PendingEvent> pending;

@@ -80,6 +80,12 @@ uint32_t wavefrontCapacity(const WavefrontState& st, const WavefrontRequest* req
 hipError_t wavefrontRender(WavefrontState& st, const WavefrontRequest* reqs, uint32_t nreq, DevCounters* ctr,
                            bool stats, hipStream_t stream, std::vector<PendingEvent>* timing,
                            uint32_t chunkPaths = 0);
+// The allocation-free render's limits: paths in flight and call-table bytes.
+constexpr uint32_t kNoAllocPaths = 1u << 17;
+constexpr size_t kNoAllocTableBytes = 1u << 20;
+// Points an external state at this device's allocation-free render memory (looked up
+// once per device; the device must be current).
+hipError_t wavefrontScratchState(WavefrontState& st);
 // Frees the buffers of a state that owns them (waits for `stream` first).
 void wavefrontRelease(WavefrontState& st, hipStream_t stream = nullptr);
 

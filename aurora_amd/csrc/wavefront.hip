@@ -21,6 +21,8 @@
 
 #include <algorithm>
 #include <cstring>
+#include <map>
+#include <mutex>
 
 namespace aph {
 
@@ -1034,6 +1036,49 @@ size_t callTableBytes(uint32_t nreq, size_t nchunk, size_t nseg, size_t ncreq)
 }  // namespace
 
 size_t wavefrontBufferBytes(uint32_t paths) { return alignUp((size_t)paths * kPerPath) + kFixedBytes; }
+
+namespace {
+
+// Device memory of the allocation-free render (aph_render_noalloc): the path state,
+// queues and chunk tables of kNoAllocPaths paths in flight, then a call table of
+// kNoAllocTableBytes.  Each GPU holds its own copy from the library's first use on it
+// until the process ends; every allocation-free call on a GPU works in that GPU's copy.
+constexpr size_t kNoAllocPathBytes = ((size_t)kNoAllocPaths * kPerPath + 255) / 256 * 256 + kFixedBytes;
+constexpr size_t kNoAllocBytes = kNoAllocPathBytes + kNoAllocTableBytes;
+static __device__ __align__(256) uint8_t scratch[kNoAllocBytes];
+
+}  // namespace
+
+hipError_t wavefrontScratchState(WavefrontState& st)
+{
+    // The array's address on the current device, looked up once per device: in host
+    // code `scratch` names a host-side stand-in, never the device memory itself.
+    static std::mutex mu;
+    static std::map<int, void*> perDevice;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    void* ptr = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        auto it = perDevice.find(dev);
+        if (it == perDevice.end()) {
+            e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+            if (e != hipSuccess) return e;
+            perDevice[dev] = ptr;
+        } else {
+            ptr = it->second;
+        }
+    }
+    st.external = true;
+    st.useGraph = false;
+    st.buf = ptr;
+    st.bytes = kNoAllocPathBytes;
+    st.tbl = (uint8_t*)ptr + kNoAllocPathBytes;
+    st.tblBytes = kNoAllocTableBytes;
+    st.capacity = 0;
+    return hipSuccess;
+}
 
 size_t wavefrontCallTableBytes(const WavefrontRequest* reqs, uint32_t nreq, uint32_t paths)
 {

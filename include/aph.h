@@ -310,6 +310,18 @@ int aph_target_device_ptr(aph_ctx_t ctx, uint64_t target, void** ptr);
  *      pipeline (raygen, extend, shade, shadow, accumulate); the requests of
  *      one call must name distinct targets (APH_E_INVALID otherwise). ---- */
 int aph_render(aph_ctx_t ctx, const aph_request* reqs, uint32_t nreq, int mode);
+/* Allocation-free render.  No counterpart in the reference: for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, hosts that
+ * forbid hipMalloc).  The same wavefront pipeline as aph_render(..., APH_MODE_WAVEFRONT)
+ * and the same images, but its path state, queues and tables live in one fixed array of
+ * about 74 MB per GPU (held from the library's first use on that GPU until the process
+ * ends), and the call makes no hipMalloc, hipFree or other allocating call; its chunks
+ * launch eagerly (no graph).  Limits: each request at most 131072 pixels (the paths in
+ * flight), and the call's request and sample-segment tables at most 1 MiB (about 1,000
+ * requests, or some 60,000 requested samples); beyond them APH_E_INVALID.  Every call on
+ * one GPU shares that GPU's array: calls on one context are serialized by its stream, but
+ * allocation-free calls of two contexts on the same GPU that overlap in time race. */
+int aph_render_noalloc(aph_ctx_t ctx, const aph_request* reqs, uint32_t nreq);
 /* IRenderer::waitForTask (HGIRenderer.cpp:591-594 is a no-op because HGI is synchronous). */
 int aph_wait(aph_ctx_t ctx);
 /* IRenderBuffer::data (HGIRenderBuffer.cpp:87-136): blocking copy of the
