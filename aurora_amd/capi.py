@@ -23,6 +23,7 @@ APH_E_OVERFLOW = -6
 APH_BUILDER_LBVH = 0
 APH_BUILDER_SAH = 1
 APH_MODE_MEGAKERNEL = 0
+APH_TLAS_REBUILD, APH_TLAS_REFIT = 0, 1
 APH_MODE_WAVEFRONT = 1
 
 RAY_DTYPE = np.dtype([("org", "<f4", 3), ("tmin", "<f4"), ("dir", "<f4", 3), ("tmax", "<f4")])
@@ -258,6 +259,8 @@ SIGNATURES = {
     "aph_server_usage": (_i, [_P, C.POINTER(_u64), C.POINTER(_u64), C.POINTER(_u32), C.POINTER(_u32)]),
     "aph_server_working_set": (_i, [_P, C.POINTER(_u64), C.POINTER(_u32)]),
     "aph_render_noalloc": (_i, [_P, _P, _u32]),
+    "aph_tlas_update": (_i, [_P, _u64, _P, _u32, _i]),
+    "aph_materials_update": (_i, [_P, _u32, _P, _u32]),
     "aph_server_replay": (_i, [_P, _P, _P, _u32, _P]),
     "aph_image_create": (_i, [_P, C.POINTER(ImageDesc), C.POINTER(_u64)]),
     "aph_image_destroy": (_i, [_P, _u64]),

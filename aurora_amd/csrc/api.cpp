@@ -110,6 +110,37 @@ DevMaterial toDev(const aph_material& a)
 }
 
 
+// Recomputes the child boxes of two-child node `idx` and its subtree from the
+// instance boxes (leaf references carry instance indices); returns the node's box.
+// Children with an inverted (empty) box are left as they are.
+void refitTree(std::vector<BvhNode>& nodes, uint32_t idx, const std::vector<float4>& lo,
+               const std::vector<float4>& hi, float4* outLo = nullptr, float4* outHi = nullptr)
+{
+    BvhNode& nd = nodes[idx];
+    float4* clo[2] = {&nd.a, &nd.c};
+    float4* chi[2] = {&nd.b, &nd.d};
+    int32_t ref[2];
+    std::memcpy(&ref[0], &nd.a.w, 4);
+    std::memcpy(&ref[1], &nd.b.w, 4);
+    float4 ulo = make_float4(INFINITY, INFINITY, INFINITY, 0.0f), uhi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.0f);
+    for (int k = 0; k < 2; k++) {
+        if (clo[k]->x > chi[k]->x) continue;  // empty slot (a single-instance tree's second child)
+        float4 l, h;
+        if ((uint32_t)ref[k] & kLeafBit) {
+            l = lo[leafFirst(ref[k])];
+            h = hi[leafFirst(ref[k])];
+        } else {
+            refitTree(nodes, (uint32_t)ref[k], lo, hi, &l, &h);
+        }
+        clo[k]->x = l.x; clo[k]->y = l.y; clo[k]->z = l.z;
+        chi[k]->x = h.x; chi[k]->y = h.y; chi[k]->z = h.z;
+        ulo = make_float4(std::fmin(ulo.x, l.x), std::fmin(ulo.y, l.y), std::fmin(ulo.z, l.z), 0.0f);
+        uhi = make_float4(std::fmax(uhi.x, h.x), std::fmax(uhi.y, h.y), std::fmax(uhi.z, h.z), 0.0f);
+    }
+    if (outLo) *outLo = ulo;
+    if (outHi) *outHi = uhi;
+}
+
 // Validated device records of render requests (targets, TLASes, sample ranges, lights).
 int buildRequests(aph_ctx* c, const aph_request* reqs, uint32_t nreq, std::vector<WavefrontRequest>& out)
 {
@@ -334,14 +365,13 @@ int aph_blas_export(aph_ctx_t c, uint64_t mesh, void* nodes, uint64_t nodesBytes
     return APH_OK;
 }
 
-int aph_tlas_build(aph_ctx_t c, const aph_instance* inst, uint32_t n, uint64_t* tlas)
+// Fills `t` (a fresh Tlas) from the instances: the traversal and shading records and
+// the tree over the instance world boxes - a new binned-SAH build, or, with refitFrom,
+// the tree of that TLAS with every box recomputed bottom-up for the new instances
+// (its topology kept: boxes only cull, so any grouping gives the same hits).
+static int buildTlas(aph_ctx_t c, const aph_instance* inst, uint32_t n, Tlas* t, const Tlas* refitFrom)
 {
-    int rc = enter(c);
-    if (rc) return rc;
-    if (!tlas) return fail(c, APH_E_INVALID, "tlas output pointer is null");
-    *tlas = 0;
     if (n && !inst) return fail(c, APH_E_INVALID, "instances pointer is null");
-    std::unique_ptr<Tlas> t(new Tlas());
     t->n = n;
     std::vector<InstTrav> trav(n ? n : 1);
     std::vector<InstShade> shade(n ? n : 1);
@@ -404,7 +434,11 @@ int aph_tlas_build(aph_ctx_t c, const aph_instance* inst, uint32_t n, uint64_t* 
         maxBlasDepth = std::max(maxBlasDepth, m->info.depth);
     }
     uint32_t nv = (uint32_t)blo.size();
+    if (refitFrom && (refitFrom->n != n || refitFrom->remap != remap))
+        return fail(c, APH_E_INVALID, "refit needs the same instances with geometry as the TLAS was built with "
+                    "(%u of %u before, %u of %u now); rebuild instead", refitFrom->nvalid, refitFrom->n, nv, n);
     t->nvalid = nv;
+    t->remap = remap;
     HIPCALL(c, t->trav.alloc(sizeof(InstTrav) * (n ? n : 1)), "alloc instances");
     t->shadeHost = shade;
     HIPCALL(c, t->layerMats.alloc(16), "alloc layers");
@@ -417,23 +451,46 @@ int aph_tlas_build(aph_ctx_t c, const aph_instance* inst, uint32_t n, uint64_t* 
     HIPCALL(c, t->nodes.alloc(sizeof(BvhNode) * (nv > 1 ? nv - 1 : 1)), "alloc TLAS nodes");
     HIPCALL(c, t->nodes4.alloc(sizeof(Bvh4Node) * (nv > 1 ? nv - 1 : 1)), "alloc TLAS nodes");
     if (nv) {
-        TimedScope ts(c, "tlas_build");
-        DevBuf lo, hi, sorted, rm;
-        HIPCALL(c, lo.alloc((size_t)nv * 16), "alloc");
-        HIPCALL(c, hi.alloc((size_t)nv * 16), "alloc");
-        HIPCALL(c, sorted.alloc((size_t)nv * 4), "alloc");
-        HIPCALL(c, rm.alloc((size_t)nv * 4), "alloc");
-        HIPCALL(c, hipMemcpyAsync(lo.p, blo.data(), (size_t)nv * 16, hipMemcpyHostToDevice, c->stream), "copy");
-        HIPCALL(c, hipMemcpyAsync(hi.p, bhi.data(), (size_t)nv * 16, hipMemcpyHostToDevice, c->stream), "copy");
-        HIPCALL(c, hipMemcpyAsync(rm.p, remap.data(), (size_t)nv * 4, hipMemcpyHostToDevice, c->stream), "copy");
+        TimedScope ts(c, refitFrom ? "tlas_refit" : "tlas_build");
         BuildInfo info{};
-        // binned SAH over the instance world boxes, one instance per leaf
-        HIPCALL(c, sahBuild(lo.as<float4>(), hi.as<float4>(), nv, 1, true, rm.as<uint32_t>(), c->stream,
-                            t->nodes.as<BvhNode>(), sorted.as<uint32_t>(), &info),
-                "TLAS build");
+        if (refitFrom) {
+            // the old tree, every child box recomputed from its subtree's new instance boxes
+            const uint32_t nn = refitFrom->nodeCount;
+            std::vector<BvhNode> nodes(nn);
+            HIPCALL(c, hipMemcpyAsync(nodes.data(), refitFrom->nodes.p, sizeof(BvhNode) * nn, hipMemcpyDeviceToHost,
+                                      c->stream),
+                    "refit download");
+            HIPCALL(c, hipStreamSynchronize(c->stream), "refit download");
+            std::vector<float4> boxLo(n), boxHi(n);
+            for (uint32_t p = 0; p < nv; p++) {
+                boxLo[remap[p]] = blo[p];
+                boxHi[remap[p]] = bhi[p];
+            }
+            refitTree(nodes, 0, boxLo, boxHi);
+            HIPCALL(c, hipMemcpyAsync(t->nodes.p, nodes.data(), sizeof(BvhNode) * nn, hipMemcpyHostToDevice, c->stream),
+                    "refit upload");
+            HIPCALL(c, hipStreamSynchronize(c->stream), "refit upload");
+            info.nodes = nn;
+            info.depth = refitFrom->depth;
+        } else {
+            DevBuf lo, hi, sorted, rm;
+            HIPCALL(c, lo.alloc((size_t)nv * 16), "alloc");
+            HIPCALL(c, hi.alloc((size_t)nv * 16), "alloc");
+            HIPCALL(c, sorted.alloc((size_t)nv * 4), "alloc");
+            HIPCALL(c, rm.alloc((size_t)nv * 4), "alloc");
+            HIPCALL(c, hipMemcpyAsync(lo.p, blo.data(), (size_t)nv * 16, hipMemcpyHostToDevice, c->stream), "copy");
+            HIPCALL(c, hipMemcpyAsync(hi.p, bhi.data(), (size_t)nv * 16, hipMemcpyHostToDevice, c->stream), "copy");
+            HIPCALL(c, hipMemcpyAsync(rm.p, remap.data(), (size_t)nv * 4, hipMemcpyHostToDevice, c->stream), "copy");
+            // binned SAH over the instance world boxes, one instance per leaf
+            HIPCALL(c, sahBuild(lo.as<float4>(), hi.as<float4>(), nv, 1, true, rm.as<uint32_t>(), c->stream,
+                                t->nodes.as<BvhNode>(), sorted.as<uint32_t>(), &info),
+                    "TLAS build");
+            HIPCALL(c, hipStreamSynchronize(c->stream), "TLAS build");  // the scratch buffers are freed on return
+        }
         HIPCALL(c, collapse4(t->nodes.as<BvhNode>(), info.nodes, c->stream, t->nodes4.as<Bvh4Node>()), "TLAS nodes");
         HIPCALL(c, hipStreamSynchronize(c->stream), "TLAS build");
         t->depth = info.depth;
+        t->nodeCount = info.nodes;
         // Stack bound: a four-child level (two levels of the two-child tree) leaves
         // at most 3 siblings pending; entering an instance adds the pending TLAS
         // entry and the exit marker.
@@ -453,9 +510,46 @@ int aph_tlas_build(aph_ctx_t c, const aph_instance* inst, uint32_t n, uint64_t* 
     for (const Mesh* m : usedMeshes)
         t->bytes += m->pos.bytes + m->nrm.bytes + m->uv.bytes + m->idx.bytes + m->nodes.bytes + m->nodes4.bytes +
                     m->tris.bytes;
+    return APH_OK;
+}
+
+int aph_tlas_build(aph_ctx_t c, const aph_instance* inst, uint32_t n, uint64_t* tlas)
+{
+    int rc = enter(c);
+    if (rc) return rc;
+    if (!tlas) return fail(c, APH_E_INVALID, "tlas output pointer is null");
+    *tlas = 0;
+    std::unique_ptr<Tlas> t(new Tlas());
+    if ((rc = buildTlas(c, inst, n, t.get(), nullptr))) return rc;
     uint64_t h = c->nextHandle++;
     c->tlases[h] = std::move(t);
     *tlas = h;
+    return APH_OK;
+}
+
+int aph_tlas_update(aph_ctx_t c, uint64_t tlas, const aph_instance* inst, uint32_t n, int mode)
+{
+    int rc = enter(c);
+    if (rc) return rc;
+    Tlas* old = lookup(c->tlases, tlas);
+    if (!old) return fail(c, APH_E_INVALID, "unknown tlas handle %llu", (unsigned long long)tlas);
+    if (mode != APH_TLAS_REBUILD && mode != APH_TLAS_REFIT) return fail(c, APH_E_INVALID, "unknown update mode %d", mode);
+    Tlas fresh;
+    if ((rc = buildTlas(c, inst, n, &fresh, mode == APH_TLAS_REFIT ? old : nullptr))) return rc;
+    // The handle keeps its environment, and its layers while the instance count is unchanged.
+    fresh.environment = old->environment;
+    if (n == old->n) {
+        fresh.layerMats = std::move(old->layerMats);
+        for (uint32_t k = 0; k < n && k < old->shadeHost.size(); k++) {
+            fresh.shadeHost[k].layerFirst = old->shadeHost[k].layerFirst;
+            fresh.shadeHost[k].layerCount = old->shadeHost[k].layerCount;
+        }
+        HIPCALL(c, hipMemcpy(fresh.shade.p, fresh.shadeHost.data(), sizeof(InstShade) * fresh.shadeHost.size(),
+                             hipMemcpyHostToDevice),
+                "instance layers");
+    }
+    HIPCALL(c, hipStreamSynchronize(c->stream), "TLAS update");  // renders in flight read the old records
+    *old = std::move(fresh);  // same object: servers and pending jobs keep their pointer
     return APH_OK;
 }
 
@@ -591,6 +685,27 @@ int aph_materials_upload(aph_ctx_t c, const aph_material* mats, uint32_t n)
     c->nmat = n;
     c->matsHost.assign(mats, mats + n);
     c->texBindings.clear();
+    return uploadMaterialTextures(c);
+}
+
+int aph_materials_update(aph_ctx_t c, uint32_t first, const aph_material* mats, uint32_t n)
+{
+    int rc = enter(c);
+    if (rc) return rc;
+    if (!mats || n == 0) return fail(c, APH_E_INVALID, "need at least one material");
+    if ((uint64_t)first + n > c->nmat)
+        return fail(c, APH_E_INVALID, "materials %u..%u beyond the table of %u", first, first + n - 1, c->nmat);
+    std::vector<DevMaterial> d(n);
+    for (uint32_t i = 0; i < n; i++) {
+        d[i] = toDev(mats[i]);
+        for (int k = 0; k < kTexSlots; k++)
+            if (mats[i].texture_block[kTexWord[k][0]]) d[i].texMask |= 1 << k;
+    }
+    HIPCALL(c, hipStreamSynchronize(c->stream), "materials");  // renders in flight read the table
+    HIPCALL(c, hipMemcpy(c->mats.as<DevMaterial>() + first, d.data(), sizeof(DevMaterial) * n, hipMemcpyHostToDevice),
+            "patch materials");
+    std::copy(mats, mats + n, c->matsHost.begin() + first);
+    // texture flags and transforms may have changed; the bindings stay
     return uploadMaterialTextures(c);
 }
 

@@ -26,6 +26,25 @@ namespace aph {
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes)
+    {
+        o.p = nullptr;
+        o.bytes = 0;
+    }
+    DevBuf& operator=(DevBuf&& o) noexcept
+    {
+        if (this != &o) {
+            reset();
+            p = o.p;
+            bytes = o.bytes;
+            o.p = nullptr;
+            o.bytes = 0;
+        }
+        return *this;
+    }
     ~DevBuf() { reset(); }
     void reset()
     {
@@ -74,7 +93,9 @@ struct Tlas {
     TraceScene sc{};
     DevBuf layerMats;  // material indices of the instances' layers (aph_tlas_set_layers)
     std::vector<InstShade> shadeHost;
+    std::vector<uint32_t> remap;  // indices of the instances with geometry (those the tree holds), ascending
     uint32_t depth = 0;
+    uint32_t nodeCount = 0;       // two-child nodes of the tree
     size_t bytes = 0;  // device memory of the TLAS and the meshes/BLASes it references
 };
 

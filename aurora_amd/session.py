@@ -136,6 +136,16 @@ class Renderer:
         self._check(self.lib.aph_tlas_build(self.ctx, capi.ptr(inst), len(inst), C.byref(h)))
         return h.value
 
+    def update_tlas(self, tlas, instances, mode=capi.APH_TLAS_REBUILD):
+        """aph_tlas_update: the handle takes the new instances (BLASes kept)."""
+        inst = np.ascontiguousarray(instances, dtype=capi.INSTANCE_DTYPE)
+        self._check(self.lib.aph_tlas_update(self.ctx, tlas, capi.ptr(inst), len(inst), mode))
+
+    def update_materials(self, first, mats):
+        """aph_materials_update: patches materials first .. first + len(mats) - 1 in place."""
+        arr = (capi.Material * len(mats))(*mats)
+        self._check(self.lib.aph_materials_update(self.ctx, first, C.cast(arr, C.c_void_p), len(mats)))
+
     def set_layers(self, tlas, layers, n):
         """Per-instance material layer lists, bottom to top (aph_tlas_set_layers)."""
         off = np.zeros(n + 1, dtype=np.uint32)
@@ -238,6 +248,7 @@ class Renderer:
         inst = sc.instances.copy()
         inst["mesh"] = np.array([handles[i] for i in sc.instances["mesh"]], dtype=np.uint32)
         t = self.tlas(inst)
+        self.mesh_handles = handles  # the scene's mesh index -> mesh handle, for updates of its instances
         if getattr(sc, "layers", None) is not None:
             self.set_layers(t, sc.layers, len(inst))
         self.materials(sc.materials)

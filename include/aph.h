@@ -218,12 +218,29 @@ int aph_blas_export(aph_ctx_t ctx, uint64_t mesh, void* nodes, uint64_t nodes_by
  * (HGIScene.cpp:377-449) / PTScene::buildTLAS (PTScene.cpp:1460-1552).
  * Instance IDs are the indices in inst[] (active order, HGIScene.cpp:392). */
 int aph_tlas_build(aph_ctx_t ctx, const aph_instance* inst, uint32_t n, uint64_t* tlas);
+/* Incremental scene update (IScene::setInstanceProperties, Aurora.h:1003-1009, addInstance /
+ * removeInstance, Aurora.h:892-1042), in place of the reference's rebuild of the instance list,
+ * acceleration structure and pipeline on any change (HGIScene.cpp:158-171, driven by
+ * resource activation, ResourceStub.cpp:386-470).  The TLAS handle takes the new
+ * instance list; every BLAS is kept.  APH_TLAS_REBUILD builds a new tree over the new
+ * instances (any count); APH_TLAS_REFIT keeps the tree's topology and recomputes its
+ * boxes bottom-up (the same instances with geometry as before; new transforms,
+ * materials or meshes) - faster for small motions, and the hits are the same either
+ * way because boxes only cull.  The handle keeps its environment, and its layers while
+ * the instance count is unchanged.  Synchronizes with renders in flight. */
+enum { APH_TLAS_REBUILD = 0, APH_TLAS_REFIT = 1 };
+int aph_tlas_update(aph_ctx_t ctx, uint64_t tlas, const aph_instance* inst, uint32_t n, int mode);
 /* Row-major 3x4 affine inverse in double, rounded to float (the w2o convention). */
 int aph_inverse3x4(const float m[12], float out[12]);
 
 /* ---- materials: MaterialBase::updateBuiltInMaterial (MaterialBase.cpp:193-214),
  *      uploaded per change like HGIMaterial.cpp:39-59 ---- */
 int aph_materials_upload(aph_ctx_t ctx, const aph_material* mats, uint32_t n);
+/* Patches materials [first, first + n) of the uploaded table in place
+ * (IScene::setMaterialProperties, Aurora.h:998; MaterialBase::updateBuiltInMaterial,
+ * MaterialBase.cpp:193-214, for the changed materials only): unlike a new upload,
+ * texture bindings are kept. */
+int aph_materials_update(aph_ctx_t ctx, uint32_t first, const aph_material* mats, uint32_t n);
 void aph_material_defaults(aph_material* m);
 
 /* ---- material textures: evaluateDefaultMaterial (DefaultMaterial.slang:171-233)
