@@ -152,6 +152,8 @@ int buildRequests(aph_ctx* c, const aph_request* reqs, uint32_t nreq, std::vecto
         Tlas* tl = lookup(c->tlases, r.tlas);
         if (!tg) return fail(c, APH_E_INVALID, "request %u: unknown target", i);
         if (!tl) return fail(c, APH_E_INVALID, "request %u: unknown tlas", i);
+        int rc = ensureTransmit(c, tl);
+        if (rc) return rc;
         if ((uint64_t)r.sample_start + r.sample_count > 0xFFFFFFFFull)
             return fail(c, APH_E_INVALID, "request %u: sample range overflows", i);
         if (r.frame.light_count < 0 || r.frame.light_count > 4)
@@ -684,6 +686,7 @@ int aph_materials_upload(aph_ctx_t c, const aph_material* mats, uint32_t n)
     HIPCALL(c, hipMemcpy(c->mats.p, d.data(), sizeof(DevMaterial) * n, hipMemcpyHostToDevice), "copy materials");
     c->nmat = n;
     c->matsHost.assign(mats, mats + n);
+    c->matVersion++;
     c->texBindings.clear();
     return uploadMaterialTextures(c);
 }
@@ -705,6 +708,7 @@ int aph_materials_update(aph_ctx_t c, uint32_t first, const aph_material* mats, 
     HIPCALL(c, hipMemcpy(c->mats.as<DevMaterial>() + first, d.data(), sizeof(DevMaterial) * n, hipMemcpyHostToDevice),
             "patch materials");
     std::copy(mats, mats + n, c->matsHost.begin() + first);
+    c->matVersion++;
     // texture flags and transforms may have changed; the bindings stay
     return uploadMaterialTextures(c);
 }

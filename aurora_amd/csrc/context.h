@@ -96,6 +96,8 @@ struct Tlas {
     std::vector<uint32_t> remap;  // indices of the instances with geometry (those the tree holds), ascending
     uint32_t depth = 0;
     uint32_t nodeCount = 0;       // two-child nodes of the tree
+    DevBuf transmit;              // shadow transmittance per instance (TraceScene::transmit)
+    uint64_t transmitVersion = 0; // the material table version it was computed from (0: never)
     size_t bytes = 0;  // device memory of the TLAS and the meshes/BLASes it references
 };
 
@@ -151,6 +153,7 @@ struct aph_ctx {
     std::map<uint64_t, std::unique_ptr<aph_denoiser, aph::DenoiserDeleter>> denoisers;
     DevBuf mats;
     uint32_t nmat = 0;
+    uint64_t matVersion = 1;                       // bumped by every change of the material table
     std::vector<aph_material> matsHost;            // the uploaded table (texture flags and transforms)
     std::vector<aph_texture_binding> texBindings;  // APH_TEX_SLOTS per material, or empty
     std::vector<uint32_t> texturesMissing;         // material * APH_TEX_SLOTS + slot: flagged, unbound
@@ -223,7 +226,37 @@ inline RenderScene renderScene(const aph_ctx* c, const Tlas* tl)
     rs.layerMats = tl->layerMats.as<uint32_t>();
     rs.pad_ = 0;
     rs.env = tl->environment ? tl->environment->env : c->env->env;
+    rs.sc.transmit = tl->transmit.as<float4>();
     return rs;
+}
+
+// The per-instance shadow transmittance of a TLAS for the current material table
+// (the shadow any-hit shader's opacity, HitShaderEntryPoints.slang:489-553; opaque as
+// MaterialBase::isOpaque decides: opacity 1, no opacity image, no transmission), recomputed
+// when the table has changed since.  Material textures are not sampled for shadows: an
+// instance with an opacity image transmits by its constants.
+inline int ensureTransmit(aph_ctx* c, Tlas* tl)
+{
+    if (tl->transmitVersion == c->matVersion && tl->transmit.p) return APH_OK;
+    const size_t n = tl->shadeHost.size();
+    std::vector<float4> tm(n ? n : 1, make_float4(1.0f, 1.0f, 1.0f, 0.0f));
+    for (size_t i = 0; i < n; i++) {
+        uint32_t mi = tl->shadeHost[i].material;
+        if (mi >= c->nmat) mi = 0;  // as the shading of an out-of-range index
+        const aph_material& m = c->matsHost[mi];
+        const bool opacityImage = m.texture_block[25] != 0;  // has_opacity_image (the opacity slot's flag word)
+        const bool opaque = m.opacity[0] == 1.0f && m.opacity[1] == 1.0f && m.opacity[2] == 1.0f &&
+                            m.transmission == 0.0f && !opacityImage;
+        float t[3];
+        for (int k = 0; k < 3; k++) t[k] = 1.0f - m.opacity[k] * (1.0f - m.transmission * m.transmission_color[k]);
+        tm[i] = make_float4(t[0], t[1], t[2], opaque ? 1.0f : 0.0f);
+    }
+    HIPCALL(c, hipStreamSynchronize(c->stream), "shadow transmittance");  // renders in flight read the old one
+    HIPCALL(c, tl->transmit.alloc(sizeof(float4) * tm.size()), "alloc shadow transmittance");
+    HIPCALL(c, hipMemcpy(tl->transmit.p, tm.data(), sizeof(float4) * tm.size(), hipMemcpyHostToDevice),
+            "copy shadow transmittance");
+    tl->transmitVersion = c->matVersion;
+    return APH_OK;
 }
 
 // A render reads every flagged texture slot; an unbound one (other than

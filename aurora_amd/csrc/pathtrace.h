@@ -118,12 +118,11 @@ struct PathCounters {
 };
 
 template <bool STATS>
-__device__ __forceinline__ float traceShadow(const RenderScene& rs, f3 org, f3 dir, const StackMem& stk, PathCounters& pc)
+__device__ __forceinline__ f3 traceShadow(const RenderScene& rs, bool transparent, f3 org, f3 dir,
+                                          const StackMem& stk, PathCounters& pc)
 {
-    HitRec h;
     pc.shadow++;
-    bool occ = traceRay<true, STATS>(rs.sc, org, dir, kRayTMin, INFINITY, stk, h, pc.tsS);
-    return occ ? 0.0f : 1.0f;
+    return traceVisibility<STATS>(rs.sc, org, dir, kRayTMin, transparent, stk, pc.tsS);
 }
 
 // computeCameraRay, RayTrace.slang:130-182
@@ -184,6 +183,8 @@ __device__ float4 pathSample(const RenderScene& rs, const aph_frame& fp, uint32_
 {
     const DevEnv& env = rs.env;
     const int maxDepth = fp.trace_depth;
+    // gFrameData.isForceOpaqueShadowsEnabled (HitShaderEntryPoints.slang:42)
+    const bool transparentShadows = fp.force_opaque_shadows == 0 && rs.sc.transmit != nullptr;
     Rng rng = rngInit(sampleIndex, W, px, py);
     f3 contrib = splat(1.0f);
     int rayType = kCameraRay, nextRayType = kCameraRay;
@@ -238,7 +239,7 @@ __device__ float4 pathSample(const RenderScene& rs, const aph_frame& fp, uint32_
                             f3 sd;
                             f3 lr = shadeDirectional(fp.lights[li], m, sh, V, rng, sd);
                             if (isBlack(lr)) continue;
-                            direct = direct + lr * traceShadow<STATS>(rs, o2, sd, stk, pc);
+                            direct = direct + traceShadow<STATS>(rs, transparentShadows, o2, sd, stk, pc) * lr;
                         }
                     } else if (fp.light_count > 0) {
                         // one light: the selection still draws a random number (HitShaderEntryPoints.slang:322-341)
@@ -303,11 +304,11 @@ __device__ float4 pathSample(const RenderScene& rs, const aph_frame& fp, uint32_
                         }
                     }
                     f3 so = dot(V, sh.normal) < 0.0f ? sh.geomPosition : sh.position;
-                    if (hasDir) direct = direct + dirColor * traceShadow<STATS>(rs, so, dirShadow, stk, pc);
+                    if (hasDir) direct = direct + traceShadow<STATS>(rs, transparentShadows, so, dirShadow, stk, pc) * dirColor;
                     if (emitsMat || emitsLight) {
                         f3 mis = splat(0.0f);
-                        if (emitsMat) mis = mis + matRes * traceShadow<STATS>(rs, so, matDir, stk, pc);
-                        if (emitsLight) mis = mis + lightRes * traceShadow<STATS>(rs, so, lightDir, stk, pc);
+                        if (emitsMat) mis = mis + traceShadow<STATS>(rs, transparentShadows, so, matDir, stk, pc) * matRes;
+                        if (emitsLight) mis = mis + traceShadow<STATS>(rs, transparentShadows, so, lightDir, stk, pc) * lightRes;
                         indirect = mis;
                         misLobeID = misLobe;
                     }

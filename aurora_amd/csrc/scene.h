@@ -71,6 +71,11 @@ struct alignas(16) InstShade {
 struct TraceScene {
     const Bvh4Node* tlas;
     const InstTrav* inst;
+    // Shadow transmittance per instance (the DirectX shadow any-hit): xyz = 1 - opacity *
+    // (1 - transmission * transmission colour) of the instance's material, w = 1 when the
+    // material is opaque (blocks outright).  Read by shadow rays of frames without
+    // force_opaque_shadows.
+    const float4* transmit;
     float extent;      // 3 x max |coordinate| of all instance world boxes
     float padScale;    // relative pad of instance boxes (times condition number bound)
     uint32_t ninst;

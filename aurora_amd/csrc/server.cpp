@@ -198,6 +198,7 @@ int stepOnce(aph_server* s)
     std::vector<uint32_t> counts;
     batch.reserve(s->active.size());
     for (Job* j : s->active) {
+        if ((rc = ensureTransmit(c, j->tlas))) return rc;
         uint32_t n = std::min(quantum, j->desc.spp - j->done);
         WavefrontRequest w;
         w.rs = renderScene(c, j->tlas);

@@ -238,6 +238,30 @@ __device__ __forceinline__ bool triWatertight(const WRay& w, const Tri& tr, doub
     return true;
 }
 
+// A hit exactly on an edge or vertex of a triangle (an edge function of zero) counts for
+// only one of the triangles sharing it, as ray-tracing hardware reports it (no double
+// hits on shared edges): with each edge's 2D direction in the sheared ray frame taken in
+// the triangle's winding (times the sign of det), an edge owns the hit when it runs in -y,
+// or in +x at y constant - the top-left rule of rasterization.  Only a transmitting shadow
+// ray, which multiplies every hit, depends on it.
+__device__ __forceinline__ bool edgeOwns(double ex, double ey)
+{
+    return ey < 0.0 || (ey == 0.0 && ex > 0.0);
+}
+__device__ __forceinline__ bool hitOwned(const WRay& w, const Tri& tr)
+{
+#pragma clang fp contract(off)
+    Sheared A = shear(w, tr.v0), B = shear(w, tr.v1), C = shear(w, tr.v2);
+    double U = (C.x * B.y) - (C.y * B.x);
+    double V = (A.x * C.y) - (A.y * C.x);
+    double W = (B.x * A.y) - (B.y * A.x);
+    const double s = ((U + V) + W) > 0.0 ? 1.0 : -1.0;
+    if (U == 0.0 && !edgeOwns(s * (C.x - B.x), s * (C.y - B.y))) return false;
+    if (V == 0.0 && !edgeOwns(s * (A.x - C.x), s * (A.y - C.y))) return false;
+    if (W == 0.0 && !edgeOwns(s * (B.x - A.x), s * (B.y - A.y))) return false;
+    return true;
+}
+
 APH_HD bool hitBetter(double t, uint32_t inst, uint32_t prim, const HitRec& b)
 {
     if (t < b.t) return true;
@@ -265,6 +289,8 @@ struct TraceState {
     uint32_t steps;
     HitRec best;
     bool found;
+    bool transparent;     // any-hit with transmittance (non-opaque shadows): every hit along the ray counts
+    f3 vis;               // transmittance accumulated so far (transparent any-hit)
 };
 
 __device__ __forceinline__ bool stackPush(TraceState& s, const StackMem& m, int32_t v)
@@ -299,6 +325,8 @@ __device__ __forceinline__ bool traceBegin(TraceState& s, const TraceScene& sc, 
     s.best.u = s.best.v = 0.0f;
     s.best.inst = s.best.prim = 0xFFFFFFFFu;
     s.found = false;
+    s.transparent = false;
+    s.vis = f3{1.0f, 1.0f, 1.0f};
     if (sc.ninst == 0) return false;
     if (!(isfinite(org.x) && isfinite(org.y) && isfinite(org.z) && isfinite(dir.x) && isfinite(dir.y) &&
           isfinite(dir.z)))
@@ -423,6 +451,24 @@ __device__ __forceinline__ bool traceRound(TraceState& s, const TraceScene& sc, 
                 TriHit th;
                 if (!triWatertight(s.w, tr, dtmin, s.best.t, th)) continue;
                 uint32_t prim = __float_as_uint(tr.v0.w);
+                if (ANY && s.transparent) {
+                    // INSTANCE_SHADOW_ANY_HIT_SHADER (HitShaderEntryPoints.slang:489-553): an opaque
+                    // instance blocks; otherwise the hit scales the visibility and the search goes on
+                    // (IgnoreHit) until it is black.
+                    if (!hitOwned(s.w, tr)) continue;
+                    const float4 tm = sc.transmit[s.curInst];
+                    if (tm.w != 0.0f) {
+                        s.vis = f3{0.0f, 0.0f, 0.0f};
+                        s.found = true;
+                        return true;
+                    }
+                    s.vis = f3{s.vis.x * tm.x, s.vis.y * tm.y, s.vis.z * tm.z};
+                    if (s.vis.x == 0.0f && s.vis.y == 0.0f && s.vis.z == 0.0f) {
+                        s.found = true;
+                        return true;
+                    }
+                    continue;
+                }
                 if (!s.found || hitBetter(th.t, s.curInst, prim, s.best)) {
                     s.found = true;
                     s.best.t = th.t;
@@ -459,6 +505,21 @@ __device__ __forceinline__ bool traceRay(const TraceScene& sc, f3 org, f3 dir, f
     }
     best = s.best;
     return s.found;
+}
+
+// Shadow visibility (traceShadowRay, RayTrace.slang:221-270): 0 or 1 with opaque shadows,
+// the product of the transmittances of every instance hit along the ray otherwise.
+template <bool STATS>
+__device__ __forceinline__ f3 traceVisibility(const TraceScene& sc, f3 org, f3 dir, float tmin, bool transparent,
+                                              const StackMem& m, TraceStats& st)
+{
+    TraceState s;
+    bool live = traceBegin(s, sc, org, dir, tmin, INFINITY);
+    s.transparent = transparent;
+    while (live && !traceRound<true, STATS>(s, sc, m, st)) {
+    }
+    if (s.found) return f3{0.0f, 0.0f, 0.0f};
+    return s.vis;
 }
 
 }  // namespace aph

@@ -306,9 +306,12 @@ __device__ __forceinline__ uint32_t traceQueue(uint32_t n, uint32_t* cursor, uin
                     }
                     f3 org, dir;
                     float tmin;
-                    sc = decode(a, b, org, dir, tmin);
+                    bool transparent = false;
+                    sc = decode(a, b, org, dir, tmin, transparent);
                     ray = i;
-                    if (!traceBegin(s, *sc, org, dir, tmin, INFINITY)) {
+                    const bool live = traceBegin(s, *sc, org, dir, tmin, INFINITY);
+                    s.transparent = transparent;
+                    if (!live) {
                         store(i, s);
                         ray = kNoRay;
                         done++;
@@ -360,10 +363,11 @@ __global__ __launch_bounds__(kWB) void kExtend(const DevReq* reqs, const SegInfo
     TraceStats ts = {};
     const uint32_t n = Q.counts[cur];
     uint32_t found = 0;
-    auto decode = [&](float4 a, float4 b, f3& org, f3& dir, float& tmin) -> const TraceScene* {
+    auto decode = [&](float4 a, float4 b, f3& org, f3& dir, float& tmin, bool& transparent) -> const TraceScene* {
         org = xyz(a);
         dir = xyz(b);
         tmin = tagTmin(a.w);
+        transparent = false;
         return &reqs[tagReq(a.w)].rs.sc;
     };
     auto store = [&](uint32_t i, const TraceState& s) {
@@ -631,7 +635,7 @@ __global__ __launch_bounds__(kWB) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     }
 }
 
-// ---- shadow: any-hit of the shadow queue (traceShadowRay, forced opaque) ----
+// ---- shadow: any-hit of the shadow queue (traceShadowRay; opaque, or transmitted by non-opaque instances) ----
 template <bool STATS>
 __global__ __launch_bounds__(kWB) void kShadow(const DevReq* reqs, const SegInfo* segs, Paths P, Queues Q,
                                               uint32_t refill, uint32_t leafWait, DevCounters* ctr)
@@ -642,13 +646,25 @@ __global__ __launch_bounds__(kWB) void kShadow(const DevReq* reqs, const SegInfo
     const StackMem mem{sStack + threadIdx.x, spill};
     TraceStats ts = {};
     const uint32_t n = Q.counts[2];
-    auto decode = [&](float4 a, float4 b, f3& org, f3& dir, float& tmin) -> const TraceScene* {
+    auto decode = [&](float4 a, float4 b, f3& org, f3& dir, float& tmin, bool& transparent) -> const TraceScene* {
         org = xyz(a);
         dir = xyz(b);
         tmin = kRayTMin;
-        return &reqs[__float_as_uint(a.w)].rs.sc;
+        const DevReq& rq = reqs[__float_as_uint(a.w)];
+        // gFrameData.isForceOpaqueShadowsEnabled (HitShaderEntryPoints.slang:42)
+        transparent = rq.fp.force_opaque_shadows == 0 && rq.rs.sc.transmit != nullptr;
+        return &rq.rs.sc;
     };
-    auto store = [&](uint32_t i, const TraceState& s) { P.vis[__float_as_uint(Q.shB[i].w)] = s.found ? 0 : 1; };
+    // A transmitted shadow ray scales its slot's light colour by the visibility it
+    // accumulated (vis stays 1), the reference's lightVisibility * colour.
+    auto store = [&](uint32_t i, const TraceState& s) {
+        const uint32_t id = __float_as_uint(Q.shB[i].w);
+        if (s.transparent && !s.found) {
+            const float4 c = P.shColor[id];
+            P.shColor[id] = make_float4(s.vis.x * c.x, s.vis.y * c.y, s.vis.z * c.z, c.w);
+        }
+        P.vis[id] = s.found ? 0 : 1;
+    };
     float4(*ring)[kReserve] = sRing[threadIdx.x >> 6];
     uint32_t done = traceQueue<true, STATS>(n, &Q.counts[4], refill, leafWait, mem, ts, Q.shA, Q.shB, ring[0], ring[1], decode,
                                             store);

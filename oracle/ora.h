@@ -103,6 +103,9 @@ double ora_sah_sweep_cost(const float* lo, const float* hi, uint32_t n, uint32_t
 void ora_trace_any(const ora_scene* s, const ora_ray* rays, uint32_t n, uint8_t* occluded, int nthreads);
 void ora_trace_any_brute(const ora_scene* s, const ora_ray* rays, uint32_t n, uint8_t* occluded);
 /* single triangle, FP64 watertight test; returns 1 on hit */
+/* Accepted and owned by this triangle (shared edges and vertices count once: transmitting shadow rays). */
+int ora_triangle_hit_owned(const float org[3], const float dir[3], const float v0[3], const float v1[3],
+                           const float v2[3]);
 int ora_intersect_triangle(const float org[3], const float dir[3], float tmin, float tmax,
                            const float v0[3], const float v1[3], const float v2[3],
                            double* t, double* u, double* v);

@@ -79,6 +79,35 @@ int ora_wray_triangle(const ora_wray* r, const float* p0, const float* p1, const
     return 1;
 }
 
+/* Whether an accepted hit belongs to this triangle when it lies exactly on an edge or
+ * vertex (an edge function of zero): of the triangles sharing that edge only one counts
+ * it, as ray-tracing hardware reports shared edges once.  The edge's 2D direction in the
+ * sheared ray frame, in the triangle's winding (times the sign of det), must point to -y,
+ * or to +x when horizontal (the top-left rule). */
+static int edge_owns(double ex, double ey) { return ey < 0.0 || (ey == 0.0 && ex > 0.0); }
+
+int ora_wray_hit_owned(const ora_wray* r, const float* p0, const float* p1, const float* p2)
+{
+    int kx = r->kx, ky = r->ky, kz = r->kz;
+    double A[3], B[3], C[3];
+    for (int i = 0; i < 3; i++) {
+        A[i] = (double)p0[i] - r->o[i];
+        B[i] = (double)p1[i] - r->o[i];
+        C[i] = (double)p2[i] - r->o[i];
+    }
+    double Ax = A[kx] - r->Sx * A[kz], Ay = A[ky] - r->Sy * A[kz];
+    double Bx = B[kx] - r->Sx * B[kz], By = B[ky] - r->Sy * B[kz];
+    double Cx = C[kx] - r->Sx * C[kz], Cy = C[ky] - r->Sy * C[kz];
+    double U = Cx * By - Cy * Bx;
+    double V = Ax * Cy - Ay * Cx;
+    double W = Bx * Ay - By * Ax;
+    double s = (U + V + W) > 0.0 ? 1.0 : -1.0;
+    if (U == 0.0 && !edge_owns(s * (Cx - Bx), s * (Cy - By))) return 0;
+    if (V == 0.0 && !edge_owns(s * (Ax - Cx), s * (Ay - Cy))) return 0;
+    if (W == 0.0 && !edge_owns(s * (Bx - Ax), s * (By - Ay))) return 0;
+    return 1;
+}
+
 int ora_intersect_triangle(const float org[3], const float dir[3], float tmin, float tmax,
                            const float v0[3], const float v1[3], const float v2[3],
                            double* t, double* u, double* v)
@@ -87,6 +116,17 @@ int ora_intersect_triangle(const float org[3], const float dir[3], float tmin, f
     ora_wray r;
     ora_wray_setup(&r, o, d, (double)tmin, (double)tmax);
     return ora_wray_triangle(&r, v0, v1, v2, t, u, v);
+}
+
+/* The hit of one triangle counted by a transmitting shadow ray: accepted and owned. */
+int ora_triangle_hit_owned(const float org[3], const float dir[3], const float v0[3], const float v1[3],
+                           const float v2[3])
+{
+    double o[3] = { org[0], org[1], org[2] }, d[3] = { dir[0], dir[1], dir[2] };
+    ora_wray r;
+    ora_wray_setup(&r, o, d, 0.0, INFINITY);
+    double t, u, v;
+    return ora_wray_triangle(&r, v0, v1, v2, &t, &u, &v) && ora_wray_hit_owned(&r, v0, v1, v2);
 }
 
 /* world -> object ray: double evaluation, fixed summation order, rounded to float */
@@ -307,9 +347,15 @@ typedef struct {
     double t, u, v;
     uint32_t inst, prim;
     int found;
+    /* transparent any-hit (mode 2): per-instance transmittance (xyz, w = opaque) and the
+     * visibility accumulated over every hit along the ray */
+    const float* transmit;
+    float vis[3];
 } best_hit;
 
-/* Closest (any_hit = 0) or first found (any_hit = 1) over one instance's BLAS. */
+/* Closest (any_hit = 0), first found (any_hit = 1), or every hit with transmittance
+ * (any_hit = 2: the shadow any-hit shader, HitShaderEntryPoints.slang:489-553) over one
+ * instance's BLAS. */
 static int blas_trace(const ora_scene* s, uint32_t ii, const ora_ray* ray, int any_hit, best_hit* best)
 {
     const ora_instance* in = &s->inst[ii];
@@ -325,7 +371,7 @@ static int blas_trace(const ora_scene* s, uint32_t ii, const ora_ray* ray, int a
     stack[sp++] = 0;
     while (sp) {
         const ora_bvh_node* n = &b->nodes[stack[--sp]];
-        double tlim = best->found ? best->t : wr.tmax;
+        double tlim = (best->found && any_hit != 2) ? best->t : wr.tmax;
         if (!box_hit(&wr, n->lo, n->hi, 0.0, wr.tmin, tlim)) continue;
         if (n->leaf) {
             for (uint32_t i = n->first; i < n->first + n->count; i++) {
@@ -335,6 +381,23 @@ static int blas_trace(const ora_scene* s, uint32_t ii, const ora_ray* ray, int a
                 if (!ora_wray_triangle(&wr, me->pos + 3 * (size_t)tri[0], me->pos + 3 * (size_t)tri[1],
                                        me->pos + 3 * (size_t)tri[2], &t, &u, &v))
                     continue;
+                if (any_hit == 2) {
+                    if (!ora_wray_hit_owned(&wr, me->pos + 3 * (size_t)tri[0], me->pos + 3 * (size_t)tri[1],
+                                            me->pos + 3 * (size_t)tri[2]))
+                        continue;
+                    const float* tm = best->transmit + 4 * (size_t)ii;
+                    if (tm[3] != 0.0f) {
+                        best->vis[0] = best->vis[1] = best->vis[2] = 0.0f;
+                        best->found = 1;
+                        return 1;
+                    }
+                    for (int k = 0; k < 3; k++) best->vis[k] *= tm[k];
+                    if (best->vis[0] == 0.0f && best->vis[1] == 0.0f && best->vis[2] == 0.0f) {
+                        best->found = 1;
+                        return 1;
+                    }
+                    continue;
+                }
                 if (!best->found || better(t, ii, prim, best->t, best->inst, best->prim)) {
                     best->found = 1; best->t = t; best->u = u; best->v = v; best->inst = ii; best->prim = prim;
                     if (any_hit) return 1;
@@ -368,7 +431,7 @@ static void trace_one(const ora_scene* s, const ora_ray* ray, int any_hit, best_
     stack[sp++] = 0;
     while (sp) {
         const ora_bvh_node* n = &s->tlas.nodes[stack[--sp]];
-        double tlim = best->found ? best->t : wr.tmax;
+        double tlim = (best->found && any_hit != 2) ? best->t : wr.tmax;
         if (!box_hit(&wr, n->lo, n->hi, pad, wr.tmin, tlim * (1.0 + 1e-6))) continue;
         if (n->leaf) {
             for (uint32_t i = n->first; i < n->first + n->count; i++) {
@@ -397,6 +460,15 @@ int ora_trace_any_one(const ora_scene* s, const ora_ray* ray)
     best_hit b;
     trace_one(s, ray, 1, &b);
     return b.found;
+}
+
+void ora_trace_transmit_one(const ora_scene* s, const float* transmit, const ora_ray* ray, float vis[3])
+{
+    best_hit b;
+    b.transmit = transmit;
+    b.vis[0] = b.vis[1] = b.vis[2] = 1.0f;
+    trace_one(s, ray, 2, &b);
+    for (int k = 0; k < 3; k++) vis[k] = b.found ? 0.0f : b.vis[k];
 }
 
 /* ---------------- threaded batch entry points ---------------- */

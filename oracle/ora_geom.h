@@ -49,8 +49,12 @@ int ora_tex_slot(const ora_scene* s, const ora_material_constants* mc, uint32_t 
 void ora_wray_setup(ora_wray* r, const double o[3], const double d[3], double tmin, double tmax);
 int ora_wray_triangle(const ora_wray* r, const float* p0, const float* p1, const float* p2,
                       double* t, double* u, double* v);
+int ora_wray_hit_owned(const ora_wray* r, const float* p0, const float* p1, const float* p2);
 void ora_object_ray(const float* w2o, const float org[3], const float dir[3], double o[3], double d[3]);
 void ora_trace_closest_one(const ora_scene* s, const ora_ray* ray, ora_hit* hit);
 int ora_trace_any_one(const ora_scene* s, const ora_ray* ray);
+/* Shadow visibility through non-opaque instances: the product of the transmittances
+ * (transmit: xyz per instance, w = 1 for an opaque instance) of every hit, or 0. */
+void ora_trace_transmit_one(const ora_scene* s, const float* transmit, const ora_ray* ray, float vis[3]);
 
 #endif

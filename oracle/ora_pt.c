@@ -145,10 +145,32 @@ typedef struct {
     const ora_frame_params* fp;
     uint32_t W, H, x, y;
     ora_counters* cnt;
+    const float* transmit; /* per-instance shadow transmittance, or NULL for forced-opaque shadows */
 } pt_ctx;
 
-/* traceShadowRay, RayTrace.slang:221-270, opaque (forced on HGI). */
-static float trace_shadow(const pt_ctx* c, v3 org, v3 dir)
+/* The shadow any-hit shader's opacity per instance (HitShaderEntryPoints.slang:489-553):
+ * xyz = 1 - opacity * (1 - transmission * transmissionColor); w = 1 for an opaque material
+ * (opacity 1, no opacity image, no transmission: MaterialBase::isOpaque).  Constants only:
+ * opacity images are not sampled for shadows. */
+static float* make_transmit(const ora_scene* s, const ora_material_constants* mats, uint32_t nmat)
+{
+    float* t = (float*)malloc(sizeof(float) * 4 * (s->ninst ? s->ninst : 1));
+    for (uint32_t i = 0; i < s->ninst; i++) {
+        uint32_t mi = s->inst[i].material < nmat ? s->inst[i].material : 0;
+        const ora_material_constants* m = &mats[mi];
+        int opaque = m->opacity[0] == 1.0f && m->opacity[1] == 1.0f && m->opacity[2] == 1.0f &&
+                     m->transmission == 0.0f && !ora_tex_flag(m, 3); /* the opacity slot */
+        for (int k = 0; k < 3; k++)
+            t[4 * i + k] = 1.0f - m->opacity[k] * (1.0f - m->transmission * m->transmissionColor[k]);
+        t[4 * i + 3] = opaque ? 1.0f : 0.0f;
+    }
+    return t;
+}
+
+/* traceShadowRay, RayTrace.slang:221-270: occlusion only when shadows are forced opaque
+ * (HGI forces them, HGIRenderer.cpp:490-491), otherwise visibility through non-opaque
+ * instances. */
+static v3 trace_shadow(const pt_ctx* c, v3 org, v3 dir)
 {
     ora_ray r;
     v3store(r.org, org);
@@ -156,7 +178,12 @@ static float trace_shadow(const pt_ctx* c, v3 org, v3 dir)
     r.tmin = ORA_RAY_TMIN;
     r.tmax = INFINITY;
     if (c->cnt) c->cnt->shadow_rays++;
-    return ora_trace_any_one(c->s, &r) ? 0.0f : 1.0f;
+    if (c->transmit) {
+        float vis[3];
+        ora_trace_transmit_one(c->s, c->transmit, &r, vis);
+        return V3(vis[0], vis[1], vis[2]);
+    }
+    return v3s(ora_trace_any_one(c->s, &r) ? 0.0f : 1.0f);
 }
 
 typedef struct {
@@ -283,8 +310,8 @@ static void closest_hit(const pt_ctx* c, payload* pl, v3 rayDir, const ora_hit* 
                 v3 sd;
                 v3 lr = shade_directional(&fp->lights[li], &m, &sh, V, pl->rng, &sd);
                 if (is_black(lr)) continue;
-                float vis = trace_shadow(c, allOrigin, sd);
-                pl->direct = add(pl->direct, scale(lr, vis));
+                v3 vis = trace_shadow(c, allOrigin, sd);
+                pl->direct = add(pl->direct, mul(vis, lr));
             }
         } else if (fp->light_count > 0) {
             float weights[4] = { 0.0f, 0.0f, 0.0f, 0.0f };
@@ -330,18 +357,18 @@ static void closest_hit(const pt_ctx* c, payload* pl, v3 rayDir, const ora_hit* 
                                          &nextRayContribution);
         v3 shadowOrigin = dot(V, sh.normal) < 0.0f ? sh.geomPosition : sh.position;
         if (hasDirLight) {
-            float vis = trace_shadow(c, shadowOrigin, dirShadowDir);
-            pl->direct = add(pl->direct, scale(dirLightColor, vis));
+            v3 vis = trace_shadow(c, shadowOrigin, dirShadowDir);
+            pl->direct = add(pl->direct, mul(vis, dirLightColor));
         }
         if (emitsLight || emitsMat) {
             v3 misResult = v3s(0.0f);
             if (emitsMat) {
-                float vis = trace_shadow(c, shadowOrigin, matDir);
-                misResult = add(misResult, scale(matRes, vis));
+                v3 vis = trace_shadow(c, shadowOrigin, matDir);
+                misResult = add(misResult, mul(vis, matRes));
             }
             if (emitsLight) {
-                float vis = trace_shadow(c, shadowOrigin, lightDir);
-                misResult = add(misResult, scale(lightRes, vis));
+                v3 vis = trace_shadow(c, shadowOrigin, lightDir);
+                misResult = add(misResult, mul(vis, lightRes));
             }
             pl->misLobeID = misLobe;
             pl->indirect = misResult;
@@ -435,11 +462,11 @@ void ora_camera_ray(const ora_frame_params* fp, uint32_t W, uint32_t H, uint32_t
 }
 
 /* RayGenShader, MainEntryPoints.slang:70-334 (HGI: no AOVs, no denoising). */
-void ora_pixel_sample(const ora_scene* s, const ora_material_constants* mats, uint32_t nmat, const ora_env* env,
-                      const ora_frame_params* fp, uint32_t W, uint32_t H, uint32_t x, uint32_t y,
-                      uint32_t sample_index, float out[4], ora_counters* cnt)
+static void pixel_sample(const ora_scene* s, const ora_material_constants* mats, uint32_t nmat, const ora_env* env,
+                         const ora_frame_params* fp, uint32_t W, uint32_t H, uint32_t x, uint32_t y,
+                         uint32_t sample_index, float out[4], ora_counters* cnt, const float* transmit)
 {
-    pt_ctx c = { s, mats, nmat, env, fp, W, H, x, y, cnt };
+    pt_ctx c = { s, mats, nmat, env, fp, W, H, x, y, cnt, transmit };
     int maxDepth = fp->trace_depth;
     payload pl;
     memset(&pl, 0, sizeof(pl));
@@ -505,10 +532,19 @@ void ora_pixel_sample(const ora_scene* s, const ora_material_constants* mats, ui
     out[3] = outAlpha;
 }
 
+void ora_pixel_sample(const ora_scene* s, const ora_material_constants* mats, uint32_t nmat, const ora_env* env,
+                      const ora_frame_params* fp, uint32_t W, uint32_t H, uint32_t x, uint32_t y,
+                      uint32_t sample_index, float out[4], ora_counters* cnt)
+{
+    float* transmit = fp->force_opaque_shadows ? NULL : make_transmit(s, mats, nmat);
+    pixel_sample(s, mats, nmat, env, fp, W, H, x, y, sample_index, out, cnt, transmit);
+    free(transmit);
+}
+
 typedef struct {
     const ora_scene* s; const ora_material_constants* mats; uint32_t nmat; const ora_env* env;
     const ora_frame_params* fp; uint32_t W, H, start, count; const uint32_t* pixels; uint32_t npix;
-    float* accum; ora_counters cnt; int tid, nthreads;
+    float* accum; ora_counters cnt; int tid, nthreads; const float* transmit;
 } render_job;
 
 static void* render_worker(void* p)
@@ -521,7 +557,7 @@ static void* render_worker(void* p)
         float* a = j->accum + 4 * (size_t)pix;
         for (uint32_t si = j->start; si < j->start + j->count; si++) {
             float res[4];
-            ora_pixel_sample(j->s, j->mats, j->nmat, j->env, j->fp, j->W, j->H, x, y, si, res, &j->cnt);
+            pixel_sample(j->s, j->mats, j->nmat, j->env, j->fp, j->W, j->H, x, y, si, res, &j->cnt, j->transmit);
             if (si > 0) {
                 /* Accumulation.glsl:22-30: mix(prev, new, 1/(i+1)) */
                 float t = 1.0f / (float)(si + 1);
@@ -542,13 +578,14 @@ void ora_render(const ora_scene* s, const ora_material_constants* mats, uint32_t
     pthread_t th[256];
     int started[256];
     render_job* jobs = (render_job*)calloc((size_t)nthreads, sizeof(render_job));
+    float* transmit = fp->force_opaque_shadows ? NULL : make_transmit(s, mats, nmat);
     /* explicit stack size: the worker must not depend on the process's RLIMIT_STACK */
     pthread_attr_t attr;
     pthread_attr_init(&attr);
     pthread_attr_setstacksize(&attr, (size_t)16 << 20);
     for (int t = 0; t < nthreads; t++) {
         jobs[t] = (render_job){ s, mats, nmat, env, fp, W, H, start, count, pixels, npix, accum, { 0, 0 }, t,
-                                nthreads };
+                                nthreads, transmit };
         started[t] = pthread_create(&th[t], &attr, render_worker, &jobs[t]) == 0;
         if (!started[t]) render_worker(&jobs[t]); /* no thread available: run this share inline */
     }
@@ -560,6 +597,7 @@ void ora_render(const ora_scene* s, const ora_material_constants* mats, uint32_t
         tot.shadow_rays += jobs[t].cnt.shadow_rays;
     }
     free(jobs);
+    free(transmit);
     if (counters) *counters = tot;
 }
 

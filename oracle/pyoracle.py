@@ -101,6 +101,7 @@ def load():
         "ora_pixel_sample": (None, [P, P, u32, P, P, u32, u32, u32, u32, u32, P, P]),
         "ora_post": (None, [P, u32, P, i, i, P]),
         "ora_camera_ray": (None, [P, u32, u32, u32, u32, u32, P, P]),
+        "ora_triangle_hit_owned": (i, [P, P, P, P, P]),
         "ora_temporal_resolve": (None, [P, P, P, P, P, u32, u32, i, C.c_float, C.c_float, C.c_float]),
         "ora_view_z": (C.c_float, [P, P]),
         "ora_screen_motion": (None, [P, P, P, u32, u32, P]),

@@ -534,3 +534,39 @@ def test_environment_transforms_and_screen_background_match_oracle(renderer, tea
         for mode in (capi.APH_MODE_MEGAKERNEL, capi.APH_MODE_WAVEFRONT):
             gpu, cpu = render_both(renderer, sc, 4, mode=mode)
             compare(gpu, cpu)
+
+
+def test_transparent_shadows_match_oracle(renderer, teapot_path):
+    """Shadows through non-opaque surfaces (force_opaque_shadows = 0: the DirectX shadow
+    any-hit shader, HitShaderEntryPoints.slang:489-553): the stacked-quad scene of the oracle
+    pin, straight-down rays through the quads' shared diagonal included, and config2's
+    transmissive and coated spheres under a distant light, in both modes against the oracle."""
+    from test_oracle_pathtracer import shadow_stack_scene
+
+    def mat(opacity, transmission=0.0, tcolor=(1, 1, 1)):
+        m = capi.default_material()
+        m.opacity[:] = opacity
+        m.transmission = transmission
+        m.transmission_color[:] = tcolor
+        return m
+
+    stack = shadow_stack_scene([mat((0.3, 0.5, 0.7)), mat((1.0, 1.0, 1.0), 0.5, (1.0, 0.5, 0.0))])
+    stack.width = stack.height = 32
+    fr = scenes.look_at_frame((0.0, 0.5, 0.0), (0.0, 0.0, 0.0), (0, 0, -1), 2.0, 1.0, 1)
+    fr.light_count = 1
+    fr.lights[0].direction[:] = [0.0, 1.0, 0.0]
+    fr.lights[0].color_intensity[:] = [1.0, 0.9, 0.8, 3.0]
+    fr.lights[0].cos_radius = 1.0
+    fr.force_opaque_shadows = 0
+    stack.frame = fr
+    c2 = scenes.config2(subdiv=2, width=96, height=64, trace_depth=5, env_w=256, env_h=128)
+    c2.frame.force_opaque_shadows = 0
+    c2.frame.light_count = 1
+    d = np.array([0.3, 0.8, -0.4]) / np.linalg.norm([0.3, 0.8, -0.4])
+    c2.frame.lights[0].direction[:] = d
+    c2.frame.lights[0].color_intensity[:] = [1.0, 0.95, 0.9, 2.0]
+    c2.frame.lights[0].cos_radius = math.cos(math.radians(2.0))
+    for sc, spp, bar in ((stack, 4, 1.0), (c2, 2, 0.98)):
+        for mode in (capi.APH_MODE_MEGAKERNEL, capi.APH_MODE_WAVEFRONT):
+            gpu, cpu = render_both(renderer, sc, spp, mode=mode)
+            compare(gpu, cpu, min_frac=bar)

@@ -157,3 +157,65 @@ def test_sah_sweep_cost_closed_forms():
         return 2 * (w + 1 + w)
     exp = (1.0 * (a(8) + 2 * a(4) + 4 * a(2)) + 1.2 * 8 * a(1)) / a(8)
     assert lib.ora_sah_sweep_cost(pyoracle._p(lo), pyoracle._p(hi), 8, 1, 1.0, 1.2) == pytest.approx(exp, rel=1e-6)  # costs pass as float32
+
+
+@pytest.mark.parametrize("flip", [False, True])
+def test_shared_edges_and_vertices_are_owned_once(flip):
+    """A transmitting shadow ray multiplies every hit, so a hit exactly on a shared edge or
+    vertex must count once (no double hits, as ray-tracing hardware reports it): over the
+    fan's spokes and centre, and a quad's diagonal, exactly one triangle owns each hit - with
+    consistent winding and with every other triangle's winding reversed."""
+    lib = pyoracle.load()
+    n = 12
+    ang = np.linspace(0, 2 * np.pi, n, endpoint=False)
+    pos = np.concatenate([[[0, 0, 0]], np.stack([np.cos(ang), np.sin(ang), np.zeros(n)], 1)]).astype(np.float32)
+    tris = [[0, 1 + i, 1 + (i + 1) % n] for i in range(n)]
+    if flip:
+        tris = [t if i % 2 else [t[0], t[2], t[1]] for i, t in enumerate(tris)]
+    quad = np.array([[-1, -1, 0], [1, -1, 0], [1, 1, 0], [-1, 1, 0]], np.float32)
+    rng = np.random.default_rng(7)
+    cases = [(pos, tris, np.zeros(3))] + [(pos, tris, r * np.array([np.cos(a), np.sin(a), 0]))
+                                         for a in ang for r in (0.25, 0.5, 0.75)]
+    cases += [(quad, [[0, 2, 1], [0, 3, 2]], np.array([x, x, 0.0])) for x in (-0.5, 0.0, 0.25, 0.75)]
+    for verts, tr, p in cases:
+        for _ in range(8):
+            d = np.array([rng.normal() * 0.3, rng.normal() * 0.3, -1.0])
+            d /= np.linalg.norm(d)
+            org = (p - 2.0 * d).astype(np.float32)
+            # the ray must pass the point exactly: aim from a representable origin along -d
+            dirf = (p - org).astype(np.float32)
+            owners = sum(lib.ora_triangle_hit_owned(pyoracle._p(org), pyoracle._p(dirf), pyoracle._p(verts[a]),
+                                                     pyoracle._p(verts[b]), pyoracle._p(verts[c]))
+                         for a, b, c in tr)
+            t, u, v = C.c_double(), C.c_double(), C.c_double()
+            hits = sum(lib.ora_intersect_triangle(pyoracle._p(org), pyoracle._p(dirf), 0.0, np.inf,
+                                                   pyoracle._p(verts[a]), pyoracle._p(verts[b]),
+                                                   pyoracle._p(verts[c]), C.byref(t), C.byref(u), C.byref(v))
+                       for a, b, c in tr)
+            assert hits >= 1
+            assert owners == 1, (p, owners, hits)
+
+
+def test_exact_edge_and_vertex_hits_are_owned_once():
+    """Rays straight down through exactly representable points of shared edges and vertices
+    (a quad's diagonal, a four-triangle fan's axis-aligned spokes and centre): the watertight
+    test accepts the hit in every triangle sharing it (two or four), ownership in exactly one."""
+    lib = pyoracle.load()
+    quad = np.array([[-1, -1, 0], [1, -1, 0], [1, 1, 0], [-1, 1, 0]], np.float32)
+    fan = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [-1, 0, 0], [0, -1, 0]], np.float32)
+    fan_tris = [[0, 1, 2], [0, 2, 3], [0, 3, 4], [0, 4, 1]]
+    cases = [(quad, [[0, 2, 1], [0, 3, 2]], (x, x)) for x in (-0.5, 0.0, 0.25, 0.75)]
+    cases += [(fan, fan_tris, p) for p in ((0.0, 0.0), (0.5, 0.0), (0.0, 0.25), (-0.75, 0.0), (0.0, -0.5))]
+    cases += [(fan, [[0, 2, 1], [0, 2, 3], [0, 4, 3], [0, 4, 1]], p) for p in ((0.0, 0.0), (0.5, 0.0))]  # mixed winding
+    t, u, v = C.c_double(), C.c_double(), C.c_double()
+    for verts, tr, (x, y) in cases:
+        for dz in (-1.0, 1.0):
+            org = np.array([x, y, -3.0 * dz], np.float32)
+            d = np.array([0.0, 0.0, dz], np.float32)
+            hits = sum(lib.ora_intersect_triangle(pyoracle._p(org), pyoracle._p(d), 0.0, np.inf, pyoracle._p(verts[a]),
+                                                   pyoracle._p(verts[b]), pyoracle._p(verts[c]), C.byref(t),
+                                                   C.byref(u), C.byref(v)) for a, b, c in tr)
+            owners = sum(lib.ora_triangle_hit_owned(pyoracle._p(org), pyoracle._p(d), pyoracle._p(verts[a]),
+                                                     pyoracle._p(verts[b]), pyoracle._p(verts[c])) for a, b, c in tr)
+            assert hits >= 2, (x, y, hits)
+            assert owners == 1, (x, y, dz, owners)

@@ -210,3 +210,59 @@ def test_environment_mis_is_unbiased_under_textured_skies(sky):
     px = acc[..., :3].reshape(-1, 3).astype(np.float64)
     se = px.std(axis=0, ddof=1) / math.sqrt(len(px))
     assert np.all(np.abs(px.mean(axis=0) - 3 * integral) <= 4 * se + 2e-3 * 3 * integral), (px.mean(0), 3 * integral)
+
+
+def shadow_stack_scene(layers):
+    """A white diffuse ground at y = 0 under quads at heights 1, 2, ... with the given
+    materials; a camera at y = 0.5 looking straight down, so camera rays never meet the quads."""
+    g = capi.default_material()
+    g.base = 1.0
+    g.specular = 0.0
+    sc = scenes.Scene()
+    gp, gn, gi = scenes.ground_quad(1000.0)
+    sc.meshes = [(gp, gn, gi)]
+    inst = [(0, 0, scenes.affine())]
+    sc.materials = [g]
+    for k, m in enumerate(layers):
+        sc.materials.append(m)
+        inst.append((0, k + 1, scenes.affine(trans=(0.0, 1.0 + k, 0.0))))
+    sc.instances = scenes._instances(inst)
+    sc.env = dict(light_top=(0, 0, 0), light_bottom=(0, 0, 0))
+    return sc
+
+
+def test_transparent_shadows_multiply_transmittances():
+    """The DirectX shadow any-hit shader (HitShaderEntryPoints.slang:489-553, used when shadows
+    are not forced opaque): a shadow ray passing non-opaque surfaces carries the product of
+    their 1 - opacity * (1 - transmission * transmission colour); an opaque surface blocks it.
+    Under a straight-down distant light the ground's radiance behind a stack of quads is the
+    unshadowed radiance times that product, per channel and per pixel."""
+    W = H = 16
+    fr = scenes.look_at_frame((0.0, 0.5, 0.0), (0.0, 0.0, 0.0), (0, 0, -1), 2.0, 1.0, 1)
+    fr.light_count = 1
+    fr.lights[0].direction[:] = [0.0, 1.0, 0.0]
+    fr.lights[0].color_intensity[:] = [1.0, 0.9, 0.8, 3.0]
+    fr.lights[0].cos_radius = 1.0
+    fr.force_opaque_shadows = 0
+
+    def mat(opacity, transmission=0.0, tcolor=(1, 1, 1)):
+        m = capi.default_material()
+        m.opacity[:] = opacity
+        m.transmission = transmission
+        m.transmission_color[:] = tcolor
+        return m
+
+    stack = [mat((0.3, 0.5, 0.7)), mat((1.0, 1.0, 1.0), 0.5, (1.0, 0.5, 0.0)), mat((0.2, 0.2, 0.2))]
+    factor = np.array([0.7, 0.5, 0.3]) * np.array([0.5, 0.25, 0.0]) * np.array([0.8, 0.8, 0.8])
+    clear, _ = pyoracle.OracleScene(shadow_stack_scene([])).render(fr, W, H, 0, 4)
+    lit, cnt = pyoracle.OracleScene(shadow_stack_scene(stack[0:1] + stack[2:])).render(fr, W, H, 0, 4)
+    assert clear[..., :3].min() > 0.1
+    np.testing.assert_allclose(lit[..., :3], clear[..., :3] * np.array([0.7, 0.5, 0.3]) * 0.8, rtol=1e-5)
+    lit3, _ = pyoracle.OracleScene(shadow_stack_scene(stack)).render(fr, W, H, 0, 4)
+    np.testing.assert_allclose(lit3[..., :3], clear[..., :3] * factor, rtol=1e-5, atol=1e-7)
+    # an opaque surface blocks, and forced-opaque shadows block at any surface
+    blocked, _ = pyoracle.OracleScene(shadow_stack_scene(stack[:1] + [capi.default_material()])).render(fr, W, H, 0, 4)
+    np.testing.assert_array_equal(blocked[..., :3], 0.0)
+    fr.force_opaque_shadows = 1
+    forced, _ = pyoracle.OracleScene(shadow_stack_scene(stack[:1])).render(fr, W, H, 0, 4)
+    np.testing.assert_array_equal(forced[..., :3], 0.0)
