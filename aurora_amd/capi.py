@@ -87,6 +87,18 @@ class Request(C.Structure):
                 ("sample_count", C.c_uint32), ("seed_offset", C.c_uint32), ("_pad", C.c_uint32)]
 
 
+class ObjShape(C.Structure):
+    _fields_ = [("nv", C.c_uint32), ("nt", C.c_uint32), ("positions", C.POINTER(C.c_float)),
+                ("normals", C.POINTER(C.c_float)), ("uvs", C.POINTER(C.c_float)),
+                ("indices", C.POINTER(C.c_uint32)), ("material", C.c_int32)]
+
+
+class ObjFile(C.Structure):
+    _fields_ = [("nshape", C.c_uint32), ("nmat", C.c_uint32), ("shapes", C.POINTER(ObjShape)),
+                ("materials", C.POINTER(Material)), ("material_names", C.POINTER(C.c_char_p)),
+                ("textures", C.POINTER(C.c_char_p))]
+
+
 class Stats(C.Structure):
     _fields_ = [("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("node_visits", C.c_uint64),
                 ("tri_tests", C.c_uint64), ("inst_visits", C.c_uint64), ("paths", C.c_uint64),
@@ -260,6 +272,10 @@ SIGNATURES = {
     "aph_server_working_set": (_i, [_P, C.POINTER(_u64), C.POINTER(_u32)]),
     "aph_render_noalloc": (_i, [_P, _P, _u32]),
     "aph_tlas_update": (_i, [_P, _u64, _P, _u32, _i]),
+    "aph_obj_load": (_i, [C.c_char_p, C.POINTER(C.POINTER(ObjFile))]),
+    "aph_obj_free": (None, [C.POINTER(ObjFile)]),
+    "aph_hdr_load": (_i, [C.c_char_p, C.POINTER(C.POINTER(C.c_float)), C.POINTER(_u32), C.POINTER(_u32)]),
+    "aph_host_free": (None, [_P]),
     "aph_materials_update": (_i, [_P, _u32, _P, _u32]),
     "aph_server_replay": (_i, [_P, _P, _P, _u32, _P]),
     "aph_image_create": (_i, [_P, C.POINTER(ImageDesc), C.POINTER(_u64)]),
@@ -308,3 +324,58 @@ def default_material():
     m = Material()
     load().aph_material_defaults(C.byref(m))
     return m
+
+
+TEXTURE_SLOT_NAMES = ("base_color", "specular_roughness", "emission_color", "opacity", "normal")
+
+
+def load_obj(path):
+    """Parses a Wavefront OBJ file (and its MTL libraries) with aph_obj_load.
+
+    Returns a dict: ``shapes`` - list of dicts with float32 ``positions`` (nv, 3),
+    ``normals`` (nv, 3), ``uvs`` (nv, 2) or None, uint32 ``indices`` (nt, 3) and the
+    ``material`` index (-1: default material); ``materials`` - list of Material;
+    ``material_names``; ``textures`` - per material a dict slot name -> file path."""
+    lib = load()
+    f = C.POINTER(ObjFile)()
+    rc = lib.aph_obj_load(os.fsencode(path), C.byref(f))
+    if rc != APH_OK:
+        raise RuntimeError(f"aph_obj_load({path!r}) failed: {rc}")
+    try:
+        o = f.contents
+        shapes = []
+        for k in range(o.nshape):
+            s = o.shapes[k]
+
+            def arr(p, n, c, dt):
+                return np.ctypeslib.as_array(p, shape=(n, c)).astype(dt, copy=True) if n else np.zeros((0, c), dt)
+
+            shapes.append(dict(positions=arr(s.positions, s.nv, 3, np.float32),
+                               normals=arr(s.normals, s.nv, 3, np.float32),
+                               uvs=arr(s.uvs, s.nv, 2, np.float32) if s.uvs else None,
+                               indices=arr(s.indices, s.nt, 3, np.uint32), material=int(s.material)))
+        mats, names, tex = [], [], []
+        for i in range(o.nmat):
+            m = Material()
+            C.memmove(C.byref(m), C.byref(o.materials[i]), C.sizeof(Material))
+            mats.append(m)
+            names.append(o.material_names[i].decode())
+            tex.append({TEXTURE_SLOT_NAMES[s]: o.textures[5 * i + s].decode()
+                        for s in range(5) if o.textures[5 * i + s]})
+        return dict(shapes=shapes, materials=mats, material_names=names, textures=tex)
+    finally:
+        lib.aph_obj_free(f)
+
+
+def load_hdr(path):
+    """Reads a Radiance .hdr image with aph_hdr_load: float32 (height, width, 3), top row first."""
+    lib = load()
+    p = C.POINTER(C.c_float)()
+    w, h = C.c_uint32(), C.c_uint32()
+    rc = lib.aph_hdr_load(os.fsencode(path), C.byref(p), C.byref(w), C.byref(h))
+    if rc != APH_OK:
+        raise RuntimeError(f"aph_hdr_load({path!r}) failed: {rc}")
+    try:
+        return np.ctypeslib.as_array(p, shape=(h.value, w.value, 3)).copy()
+    finally:
+        lib.aph_host_free(p)

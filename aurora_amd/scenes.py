@@ -18,6 +18,7 @@ Workloads:
     (``camera_spline``).
 """
 import math
+import os
 
 import numpy as np
 
@@ -576,6 +577,43 @@ def teapot_scene(path, width=128, height=128, spp=4, trace_depth=5):
     sc.env = dict()  # default gradient environment (EnvironmentBase.cpp:33-45)
     sc.width, sc.height, sc.spp = width, height, spp
     sc.frame = look_at_frame((0.0, 0.0, -5.0), (0.0, 0.0, 0.0), (0, 1, 0), 45.0, width / height, trace_depth)
+    return sc
+
+
+def obj_scene(obj_path, hdr_path=None, width=128, height=128, spp=4, trace_depth=5, eye=(0.0, 0.0, -5.0),
+              target=(0.0, 0.0, 0.0), fov_y_deg=45.0):
+    """A scene read from disk as Plasma reads one (Applications/Plasma/OBJLoader.cpp): one
+    mesh and one identity instance per OBJ shape, the MTL materials translated to Standard
+    Surface (shapes without a material get the default one, appended last), and, given
+    ``hdr_path``, that Radiance image as the lat-long light and background (Plasma.cpp:1341).
+    Texture maps named by the MTL file are listed in ``sc.texture_paths`` but not decoded."""
+    obj = capi.load_obj(obj_path)
+    sc = Scene()
+    sc.name = os.path.basename(obj_path)
+    sc.materials = list(obj["materials"])
+    default = None
+    inst = []
+    for k, sh in enumerate(obj["shapes"]):
+        sc.meshes.append((sh["positions"], sh["normals"], sh["indices"]))
+        if sh["uvs"] is not None:
+            sc.uvs[k] = sh["uvs"]
+        mat = sh["material"]
+        if mat < 0:
+            if default is None:
+                default = len(sc.materials)
+                sc.materials.append(capi.default_material())
+            mat = default
+        inst.append((k, mat, affine()))
+    if not inst:
+        raise ValueError(f"{obj_path}: no faces")
+    sc.instances = _instances(inst)
+    sc.texture_paths = obj["textures"]
+    sc.env = dict()
+    if hdr_path is not None:
+        rgb = capi.load_hdr(hdr_path)
+        sc.env = dict(light_rgb=rgb, background_rgb=rgb)
+    sc.width, sc.height, sc.spp = width, height, spp
+    sc.frame = look_at_frame(eye, target, (0, 1, 0), fov_y_deg, width / height, trace_depth)
     return sc
 
 

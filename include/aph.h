@@ -498,6 +498,35 @@ int aph_server_release(aph_server_t s, uint64_t ticket);
 int aph_server_replay(aph_server_t s, const aph_job* jobs, const double* arrival, uint32_t n, uint64_t* tickets);
 
 
+/* ---- on-disk inputs (host only, no context): the Wavefront OBJ + MTL translation of
+ * Plasma's loader (Applications/Plasma/OBJLoader.cpp:77-470, tinyobjloader's parsing) and
+ * the Radiance HDR lat-long environments Plasma loads (Plasma.cpp:69, :1341).  Texture
+ * maps are reported by path, not decoded.  glTF and MaterialX inputs are not read. ---- */
+typedef struct {
+    uint32_t nv, nt;
+    const float* positions;     /* 3 nv */
+    const float* normals;       /* 3 nv: the file's, normalized, or angle-weighted ones
+                                   (Foundation calculateNormals, Geometry.cpp:29-101) */
+    const float* uvs;           /* 2 nv with V flipped, or NULL */
+    const uint32_t* indices;    /* 3 nt */
+    int32_t material;           /* the shape's first face's material, -1: the default */
+} aph_obj_shape;
+typedef struct {
+    uint32_t nshape, nmat;
+    const aph_obj_shape* shapes;        /* one per o/g group with faces */
+    const aph_material* materials;      /* Standard Surface, OBJLoader.cpp:162-229 */
+    const char* const* material_names;  /* nmat */
+    const char* const* textures;        /* 5 per material (map_Kd, map_Pr, map_Ke, map_d, norm/bump,
+                                           the APH_TEX_* slot order): file path or NULL */
+} aph_obj_file;
+/* Parses an OBJ file and the MTL libraries it names; *out stays valid until aph_obj_free. */
+int aph_obj_load(const char* path, aph_obj_file** out);
+void aph_obj_free(aph_obj_file* f);
+/* Reads a Radiance RGBE (.hdr) image: *rgb is a width x height x 3 float array, top row
+ * first, freed with aph_host_free.  APH_E_UNSUPPORTED for other orientations. */
+int aph_hdr_load(const char* path, float** rgb, uint32_t* width, uint32_t* height);
+void aph_host_free(void* p);
+
 /* ---- Learned denoiser ----
  * A U-Net over the noisy radiance of a target and the guides of an AOV set
  * (aph_render_aovs): log radiance, base colour + metalness, normal +
