@@ -44,7 +44,9 @@ def test_teapot_read_from_obj_matches_the_fixture(renderer, teapot_path, tmp_pat
     for mode in (capi.APH_MODE_MEGAKERNEL, capi.APH_MODE_WAVEFRONT):
         gpu, cpu = render_both(renderer, sc, 4, mode=mode)
         compare(gpu, cpu)
-    # and against the fixture scene itself
+    # and against the fixture scene under the same (default) material: a shape without
+    # usemtl gets the renderer's default material, as in Plasma
+    base.materials = [capi.default_material()]
     ref, _ = render_both(renderer, base, 4)
     compare(gpu, ref)
 
