@@ -87,6 +87,26 @@ class Request(C.Structure):
                 ("sample_count", C.c_uint32), ("seed_offset", C.c_uint32), ("_pad", C.c_uint32)]
 
 
+class GroundPlane(C.Structure):
+    """aph_ground_plane: the properties of PTGroundPlane.cpp."""
+    _fields_ = [("enabled", C.c_int32), ("position", C.c_float * 3), ("normal", C.c_float * 3),
+                ("shadow_opacity", C.c_float), ("shadow_color", C.c_float * 3), ("reflection_opacity", C.c_float),
+                ("reflection_color", C.c_float * 3), ("reflection_roughness", C.c_float)]
+
+
+def ground_plane(props=None):
+    """GroundPlane with the library's defaults, overridden by the dict `props`
+    (keys: the field names)."""
+    g = GroundPlane()
+    load().aph_ground_plane_defaults(C.byref(g))
+    for k, v in (props or {}).items():
+        if isinstance(v, (tuple, list, np.ndarray)):
+            getattr(g, k)[:] = [float(x) for x in v]
+        else:
+            setattr(g, k, type(getattr(g, k))(v))
+    return g
+
+
 class ObjShape(C.Structure):
     _fields_ = [("nv", C.c_uint32), ("nt", C.c_uint32), ("positions", C.POINTER(C.c_float)),
                 ("normals", C.POINTER(C.c_float)), ("uvs", C.POINTER(C.c_float)),
@@ -232,6 +252,8 @@ SIGNATURES = {
     "aph_environment_create": (_i, [_P, C.POINTER(Environment), C.POINTER(C.c_uint64)]),
     "aph_environment_destroy": (_i, [_P, C.c_uint64]),
     "aph_tlas_set_environment": (_i, [_P, C.c_uint64, C.c_uint64]),
+    "aph_ground_plane_defaults": (None, [C.POINTER(GroundPlane)]),
+    "aph_tlas_set_ground_plane": (_i, [_P, C.c_uint64, C.POINTER(GroundPlane)]),
     "aph_alias_build": (_i, [_P, _u32, _u32, _P, C.POINTER(C.c_float)]),
     "aph_trace_closest": (_i, [_P, _u64, _P, _u32, _P]),
     "aph_trace_any": (_i, [_P, _u64, _P, _u32, _P]),

@@ -538,8 +538,10 @@ int aph_tlas_update(aph_ctx_t c, uint64_t tlas, const aph_instance* inst, uint32
     if (mode != APH_TLAS_REBUILD && mode != APH_TLAS_REFIT) return fail(c, APH_E_INVALID, "unknown update mode %d", mode);
     Tlas fresh;
     if ((rc = buildTlas(c, inst, n, &fresh, mode == APH_TLAS_REFIT ? old : nullptr))) return rc;
-    // The handle keeps its environment, and its layers while the instance count is unchanged.
+    // The handle keeps its environment and ground plane, and its layers while the instance
+    // count is unchanged.
     fresh.environment = old->environment;
+    fresh.ground = old->ground;
     if (n == old->n) {
         fresh.layerMats = std::move(old->layerMats);
         for (uint32_t k = 0; k < n && k < old->shadeHost.size(); k++) {
@@ -902,6 +904,44 @@ int aph_tlas_set_environment(aph_ctx_t c, uint64_t tlas, uint64_t env)
         if (!en) return fail(c, APH_E_INVALID, "unknown environment handle %llu", (unsigned long long)env);
     }
     tl->environment = en;
+    return APH_OK;
+}
+
+void aph_ground_plane_defaults(aph_ground_plane* gp)
+{
+    if (!gp) return;
+    *gp = aph_ground_plane{};
+    gp->enabled = 1;
+    gp->normal[1] = 1.0f;
+    gp->shadow_opacity = 1.0f;
+    gp->reflection_opacity = 0.5f;
+    for (int k = 0; k < 3; k++) gp->reflection_color[k] = 0.5f;
+    gp->reflection_roughness = 0.1f;
+}
+
+int aph_tlas_set_ground_plane(aph_ctx_t c, uint64_t tlas, const aph_ground_plane* gp)
+{
+    int rc = enter(c);
+    if (rc) return rc;
+    Tlas* tl = lookup(c->tlases, tlas);
+    if (!tl) return fail(c, APH_E_INVALID, "unknown tlas handle %llu", (unsigned long long)tlas);
+    DevGround g{};
+    if (gp && gp->enabled) {
+        g.enabled = 1;
+        g.position = f3{gp->position[0], gp->position[1], gp->position[2]};
+        g.normal = f3{gp->normal[0], gp->normal[1], gp->normal[2]};
+        // the plane's frame (PTGroundPlane.cpp buildBasis): from +Y, or +X for a normal near +-Y
+        f3 b = std::fabs(g.normal.y) < 0.999f ? f3{0.0f, 1.0f, 0.0f} : f3{1.0f, 0.0f, 0.0f};
+        g.tangent = normalize(cross(b, g.normal));
+        g.bitangent = cross(g.normal, g.tangent);
+        g.shadowOpacity = gp->shadow_opacity;
+        g.shadowColor = f3{gp->shadow_color[0], gp->shadow_color[1], gp->shadow_color[2]};
+        g.reflectionOpacity = gp->reflection_opacity;
+        g.reflectionColor = f3{gp->reflection_color[0], gp->reflection_color[1], gp->reflection_color[2]};
+        g.reflectionRoughness = gp->reflection_roughness;
+    }
+    // renders already enqueued read the TLAS's plane from their own call tables
+    tl->ground = g;
     return APH_OK;
 }
 

@@ -96,6 +96,7 @@ struct Tlas {
     std::vector<uint32_t> remap;  // indices of the instances with geometry (those the tree holds), ascending
     uint32_t depth = 0;
     uint32_t nodeCount = 0;       // two-child nodes of the tree
+    DevGround ground{};           // aph_tlas_set_ground_plane (enabled = 0: none)
     DevBuf transmit;              // shadow transmittance per instance (TraceScene::transmit)
     uint64_t transmitVersion = 0; // the material table version it was computed from (0: never)
     size_t bytes = 0;  // device memory of the TLAS and the meshes/BLASes it references
@@ -227,6 +228,7 @@ inline RenderScene renderScene(const aph_ctx* c, const Tlas* tl)
     rs.pad_ = 0;
     rs.env = tl->environment ? tl->environment->env : c->env->env;
     rs.sc.transmit = tl->transmit.as<float4>();
+    rs.ground = tl->ground;
     return rs;
 }
 

@@ -1,11 +1,13 @@
 // One pixel-sample of the Aurora path tracer on gfx950: the RayGenShader
 // loop (Shaders/MainEntryPoints.slang:70-334) with the closest-hit
 // (Shaders/HitShaderEntryPoints.slang:35-486) and miss (:354-426) programs
-// inlined and traversal from trace.h.  HGI semantics: opaque shadows, no
-// ground plane; material layers (DirectX semantics) when an instance has them.
+// inlined and traversal from trace.h.  Shadows are opaque unless the frame turns
+// forced-opaque shadows off; the DirectX backend's ground plane (ground.h) and
+// material layers apply when the scene has them.
 #pragma once
 
 #include "../../include/aph.h"
+#include "ground.h"
 #include "shade.h"
 #include "trace.h"
 
@@ -21,6 +23,7 @@ struct RenderScene {
     uint32_t nmat;
     uint32_t pad_;
     DevEnv env;
+    DevGround ground;
 };
 
 // Texture coordinate of a hit (Geometry.slang:199-203): the barycentric mix of
@@ -209,8 +212,28 @@ __device__ float4 pathSample(const RenderScene& rs, const aph_frame& fp, uint32_
             HitRec hit;
             pc.ext++;
             bool h = traceRay<false, STATS>(rs.sc, org, dir, minT, INFINITY, stk, hit, pc.ts);
-            bool skip = false;
-            if (!h) {
+            bool skip = false, absorbed = false;
+            // evaluateGroundPlane (GroundPlane.slang:232-290), first in the hit and miss programs
+            f3 gpos;
+            float gdist;
+            if (isPrimary && rs.ground.enabled && groundPlaneHit(rs.ground, org, dir, gpos, gdist) &&
+                gdist < (h ? (float)hit.t : INFINITY)) {
+                f3 gcol, gdir, sdir;
+                bool gshadow;
+                const int gt = groundPlaneSetup(rs.ground, env, dir, rng, gcol, gdir, gshadow, sdir);
+                if (gshadow) gcol = gcol * groundShadowTint(rs.ground, traceShadow<STATS>(rs, false, gpos, sdir, stk, pc));
+                contrib = contrib * gcol;
+                if (gt == kGroundPlaneReflectionRay) {
+                    // the segment ends at the plane; the path goes on as its reflection
+                    org = gpos;
+                    dir = gdir;
+                    nextRayType = kGroundPlaneReflectionRay;
+                    isPrimary = false;
+                    absorbed = true;
+                }
+            }
+            if (absorbed) {
+            } else if (!h) {
                 bool bg = isPrimary || rayType == kTransmissionLobe || rayType == kGroundPlaneReflectionRay;
                 if (bg) direct = evalEnvironment(env, dir, true, px, py, W, H) * contrib;
                 pathEnded = true;

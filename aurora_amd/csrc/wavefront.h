@@ -51,12 +51,13 @@ struct WavefrontState {
         uint32_t capacity;
         uint32_t depth;
         bool stats;
+        bool ground;  // a request has a ground plane: the chunk runs its stage
         unsigned gen, trace, shade, shadow, accum, refill, leafWait;
         bool operator==(const GraphKey& o) const
         {
             return buf == o.buf && tbl == o.tbl && ctr == o.ctr && capacity == o.capacity && depth == o.depth &&
-                   stats == o.stats && gen == o.gen && trace == o.trace && shade == o.shade && shadow == o.shadow &&
-                   accum == o.accum && refill == o.refill && leafWait == o.leafWait;
+                   stats == o.stats && ground == o.ground && gen == o.gen && trace == o.trace && shade == o.shade &&
+                   shadow == o.shadow && accum == o.accum && refill == o.refill && leafWait == o.leafWait;
         }
     };
     GraphKey graphKey{};

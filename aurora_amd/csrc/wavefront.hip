@@ -394,6 +394,67 @@ __global__ __launch_bounds__(kWB) void kExtend(const DevReq* reqs, const SegInfo
     addCounter(&ctr->overflow, ts.overflow);
 }
 
+// ---- ground plane of primary rays (evaluateGroundPlane, GroundPlane.slang:232-290) ----
+// Runs between extend and shade when a request of the call has a ground plane.  For a
+// primary ray that meets the plane in front of its hit, it draws the plane's coin and
+// samples, traces the matte shadow ray (opaque, per lane), and tints the path's
+// contribution.  A reflection ends the segment at the plane: its next ray replaces the
+// queue entry and the hit record is marked kGroundInst, which kShade resolves as an
+// empty segment (no emission, lights or roulette), as the reference's early return.
+constexpr uint32_t kGroundInst = 0xFFFFFFFEu;
+template <bool STATS>
+__global__ __launch_bounds__(kWB) void kGround(const DevReq* reqs, Paths P, Queues Q, int cur, DevCounters* ctr)
+{
+    __shared__ uint32_t sStack[kLdsStack * kWB];
+    uint32_t spill[kSpillStack];
+    const StackMem mem{sStack + threadIdx.x, spill};
+    TraceStats ts = {};
+    const uint32_t n = Q.counts[cur];
+    uint32_t shadows = 0;
+    for (uint32_t i = blockIdx.x * kWB + threadIdx.x; i < n; i += gridDim.x * kWB) {
+        const float4 rb = Q.rayB[cur][i];
+        const uint32_t p = __float_as_uint(rb.w);
+        Flags f{P.flags[p]};
+        if (!f.primary()) continue;
+        const float4 ra = Q.rayA[cur][i];
+        const uint32_t ri = tagReq(ra.w);
+        const DevReq& rq = reqs[ri];
+        const DevGround& g = rq.rs.ground;
+        f3 pos;
+        float dist;
+        if (!g.enabled || !groundPlaneHit(g, xyz(ra), xyz(rb), pos, dist) || !(dist < Q.hitA[i].x)) continue;
+        const uint2 rs = P.rng[p];
+        Rng rng{rs.x, rs.y};
+        f3 color, ndir, sdir;
+        bool shadow;
+        const int type = groundPlaneSetup(g, rq.rs.env, xyz(rb), rng, color, ndir, shadow, sdir);
+        if (shadow) {
+            color = color * groundShadowTint(g, traceVisibility<STATS>(rq.rs.sc, pos, sdir, kRayTMin, false, mem, ts));
+            shadows++;
+        }
+        P.contrib[p] = f4(xyz(P.contrib[p]) * color, 0.0f);
+        P.rng[p] = make_uint2(rng.x, rng.y);
+        if (type == kGroundPlaneReflectionRay) {
+            f.primary(false);
+            f.nextRayType((uint32_t)kGroundPlaneReflectionRay);
+            P.flags[p] = f.v;
+            Q.rayA[cur][i] = f4(pos, rayTag(ri, false));
+            Q.rayB[cur][i] = f4(ndir, __uint_as_float(p));
+            Q.hitA[i] = make_float4(INFINITY, __uint_as_float(kGroundInst), __uint_as_float(0xFFFFFFFFu), 0.0f);
+        }
+    }
+    addCounter(&ctr->shadow, shadows);
+    if (STATS) {
+        addCounter(&ctr->nodes, ts.nodes);
+        addCounter(&ctr->tris, ts.tris);
+        addCounter(&ctr->insts, ts.insts);
+        addCounter(&ctr->snodes, ts.nodes);
+        addCounter(&ctr->stris, ts.tris);
+        addCounter(&ctr->sinsts, ts.insts);
+    }
+    addCounter(&ctr->overflow, ts.overflow);
+}
+
 __device__ __forceinline__ void emitShadow(const Queues& Q, bool emit, f3 org, f3 dir, uint32_t slotId, uint32_t req)
 {
     uint32_t at = waveAppend(emit, &Q.counts[2]);
@@ -429,6 +490,30 @@ __device__ __forceinline__ void shadeMiss(const DevReq* reqs, const SegInfo* seg
     P.shMask[p] = 0;
     P.flags[p] = f.v;
     Q.cont[i] = 0;
+}
+
+// A segment that ended at the ground plane (kGround): nothing is shaded; the path
+// continues with the reflection ray kGround put in entry i, unless the depth runs out.
+__device__ __forceinline__ void shadeGround(const DevReq* reqs, const Paths& P, const Queues& Q, int cur, uint32_t i)
+{
+    const float4 rb = Q.rayB[cur][i];
+    const uint32_t p = __float_as_uint(rb.w);
+    Flags f{P.flags[p]};
+    const DevReq& rq = reqs[tagReq(Q.rayA[cur][i].w)];
+    P.pendDirect[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    P.pendContrib[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    P.shMask[p] = 0;
+    f.event(kEvHit);
+    f.ended(false);
+    bool cont = true;
+    f.terminal(false);
+    if ((int)f.depth() + 1 >= rq.fp.trace_depth) {
+        f.terminal(true);
+        cont = false;
+        P.nextDir[p] = f4(xyz(rb), 0.0f);
+    }
+    P.flags[p] = f.v;
+    Q.cont[i] = cont ? 1 : 0;
 }
 
 // Closest-hit program of queue entry i (live lanes).  A continuing path's next
@@ -610,8 +695,10 @@ __global__ __launch_bounds__(kWB) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
             next += stride;
             bool hit = false;
             if (i < n) {
-                hit = __float_as_uint(Q.hitA[i].y) != 0xFFFFFFFFu;
-                if (!hit) shadeMiss(reqs, segs, P, Q, cur, i);
+                const uint32_t inst = __float_as_uint(Q.hitA[i].y);
+                hit = inst < kGroundInst;
+                if (inst == kGroundInst) shadeGround(reqs, P, Q, cur, i);
+                else if (!hit) shadeMiss(reqs, segs, P, Q, cur, i);
             }
             const uint64_t m = __ballot(hit);
             const int lane = __lane_id();
@@ -955,9 +1042,9 @@ struct Launch {
 };
 
 // Everything one chunk runs: its tables moved into place from the call table,
-// generation, then per bounce the reset, extension, shading, shadow and resolve
-// stages, then accumulation.  Timed per stage with HIP events when timing is given.
-void enqueueChunk(hipStream_t stream, const Launch& L, uint32_t maxDepth, bool stats, uint8_t* tbl,
+// generation, then per bounce the reset, extension, ground plane (when a request
+// has one), shading, shadow and resolve stages, then accumulation.  Timed per stage with HIP events when timing is given.
+void enqueueChunk(hipStream_t stream, const Launch& L, uint32_t maxDepth, bool stats, bool ground, uint8_t* tbl,
                   const DevReq* dreq, SegInfo* gseg, ChunkReq* gcr, ChunkHdr* hdr, const Paths& P, const Queues& Q,
                   DevCounters* ctr, std::vector<PendingEvent>* timing)
 {
@@ -975,6 +1062,11 @@ void enqueueChunk(hipStream_t stream, const Launch& L, uint32_t maxDepth, bool s
             EventScope ev(timing, "wf_extend", stream);
             if (stats) kExtend<true><<<L.trace, kWB, 0, stream>>>(dreq, gseg, P, Q, cur, L.refill, L.leafWait, ctr);
             else kExtend<false><<<L.trace, kWB, 0, stream>>>(dreq, gseg, P, Q, cur, L.refill, L.leafWait, ctr);
+        }
+        if (ground) {
+            EventScope ev(timing, "wf_ground", stream);
+            if (stats) kGround<true><<<L.light, kWB, 0, stream>>>(dreq, P, Q, cur, ctr);
+            else kGround<false><<<L.light, kWB, 0, stream>>>(dreq, P, Q, cur, ctr);
         }
         {
             EventScope ev(timing, "wf_shade", stream);
@@ -1226,6 +1318,7 @@ hipError_t wavefrontRender(WavefrontState& st, const WavefrontRequest* reqs, uin
 
     // the call table's host image, uploaded through kernel arguments
     uint32_t maxDepth = 1;
+    bool ground = false;
     std::vector<uint32_t> image(tblNeed / 4, 0u);
     uint8_t* img = (uint8_t*)image.data();
     CallHdr ch{};
@@ -1246,6 +1339,7 @@ hipError_t wavefrontRender(WavefrontState& st, const WavefrontRequest* reqs, uin
         r.accum = reqs[i].accum;
         std::memcpy(img + kCallHdrBytes + sizeof(DevReq) * i, &r, sizeof(DevReq));
         maxDepth = std::max<uint32_t>(maxDepth, (uint32_t)reqs[i].frame.trace_depth);
+        ground = ground || reqs[i].rs.ground.enabled != 0;
     }
     std::memcpy(img + ch.descOff, plan.desc.data(), sizeof(ChunkDesc) * plan.desc.size());
     std::memcpy(img + ch.segOff, plan.segs.data(), sizeof(SegInfo) * plan.segs.size());
@@ -1274,14 +1368,14 @@ hipError_t wavefrontRender(WavefrontState& st, const WavefrontRequest* reqs, uin
     // eagerly (stage by stage, with HIP events when timed).
     const bool graph = st.useGraph && timing == nullptr && !st.external;
     if (graph) {
-        WavefrontState::GraphKey key{st.buf, st.tbl, ctr, cap, maxDepth, stats, L.gen, L.trace, L.shade, L.shadow,
-                                     L.accum, L.refill, L.leafWait};
+        WavefrontState::GraphKey key{st.buf, st.tbl, ctr, cap, maxDepth, stats, ground, L.gen, L.trace, L.shade,
+                                     L.shadow, L.accum, L.refill, L.leafWait};
         if (!st.graphExec || !(key == st.graphKey)) {
             if (st.graphExec) hipGraphExecDestroy(st.graphExec);
             st.graphExec = nullptr;
             hipGraph_t g = nullptr;
             if ((e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal)) != hipSuccess) return e;
-            enqueueChunk(stream, L, maxDepth, stats, tbl, dreq, gseg, gcr, ghdr, P, Q, ctr, nullptr);
+            enqueueChunk(stream, L, maxDepth, stats, ground, tbl, dreq, gseg, gcr, ghdr, P, Q, ctr, nullptr);
             if ((e = hipStreamEndCapture(stream, &g)) != hipSuccess) return e;
             e = hipGraphInstantiate(&st.graphExec, g, nullptr, nullptr, 0);
             hipGraphDestroy(g);
@@ -1298,7 +1392,7 @@ hipError_t wavefrontRender(WavefrontState& st, const WavefrontRequest* reqs, uin
         if (graph) {
             if ((e = hipGraphLaunch(st.graphExec, stream)) != hipSuccess) return e;
         } else {
-            enqueueChunk(stream, L, maxDepth, stats, tbl, dreq, gseg, gcr, ghdr, P, Q, ctr, timing);
+            enqueueChunk(stream, L, maxDepth, stats, ground, tbl, dreq, gseg, gcr, ghdr, P, Q, ctr, timing);
         }
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }

@@ -292,6 +292,7 @@ class Scene:
         self.materials = []     # list of capi.Material
         self.env = None         # dict
         self.layers = None      # per instance: material indices of its layers, bottom to top
+        self.ground = None      # ground plane properties (dict over aph_ground_plane_defaults) or None
         self.uvs = {}           # mesh index -> (nv, 2) texture coordinates (others: u = v = i / nv)
         self.images = []        # dicts: pixels (h, w, 4) uint8 / (h, w, 4|3) float32, linearize
         self.textures = {}      # (material, slot) -> (image index, address_u, address_v)
@@ -614,6 +615,46 @@ def obj_scene(obj_path, hdr_path=None, width=128, height=128, spp=4, trace_depth
         sc.env = dict(light_rgb=rgb, background_rgb=rgb)
     sc.width, sc.height, sc.spp = width, height, spp
     sc.frame = look_at_frame(eye, target, (0, 1, 0), fov_y_deg, width / height, trace_depth)
+    return sc
+
+
+GROUND_KINDS = ("reference", "mixed")
+
+
+def ground_plane_scene(path, kind="reference", width=96, height=64, spp=4, trace_depth=3, seed=13):
+    """A teapot standing on the ground plane.
+      reference  the reference's TestRendererGroundPlane (Tests/Aurora/Tests/TestRenderer.cpp:476-511):
+                 a red teapot, the plane at the origin facing +Y with shadow and reflection
+                 opacity 1 (other properties at their defaults), camera at (0, 1, -5)
+      mixed      reflection opacity 0.5 at roughness 0.3 in a warm grey, a blue-black shadow at
+                 opacity 0.8, a second (glass) teapot, a seeded sun sky image and a distant
+                 light: both plane branches, the matte shadow against an importance-sampled sky."""
+    if kind not in GROUND_KINDS:
+        raise ValueError(kind)
+    sc = teapot_scene(path, width, height, spp, trace_depth)
+    sc.name = f"ground_{kind}"
+    if kind == "reference":
+        sc.materials = [_material((1.0, 0.0, 0.0))]
+        sc.ground = dict(enabled=1, position=(0.0, 0.0, 0.0), normal=(0.0, 1.0, 0.0), shadow_opacity=1.0,
+                         reflection_opacity=1.0)
+        sc.frame = look_at_frame((0.0, 1.0, -5.0), (0.0, 0.0, 0.0), (0, 1, 0), 45.0, width / height, trace_depth)
+        return sc
+    rng = np.random.default_rng(seed)
+    sc.materials = [_material((0.8, 0.3, 0.1), 0.0, 0.3),
+                    _material((1.0, 1.0, 1.0), transmission=0.9, specular_roughness=0.05)]
+    sc.instances = _instances([(0, 0, affine(trans=(-1.2, 0.0, 0.0))),
+                               (0, 1, affine(rot=None, scale=0.7, trans=(1.6, 0.0, 0.8)))])
+    sky = sun_sky(128, 64, rng, sun_radiance=200.0, sun_radius_deg=3.0)
+    sc.env = dict(light_rgb=sky, background_rgb=sky)
+    sc.ground = dict(enabled=1, position=(0.0, 0.0, 0.0), normal=(0.0, 1.0, 0.0), shadow_opacity=0.8,
+                     shadow_color=(0.05, 0.05, 0.1), reflection_opacity=0.5, reflection_color=(0.8, 0.7, 0.6),
+                     reflection_roughness=0.3)
+    sc.frame = look_at_frame((1.0, 2.5, -6.0), (0.2, 0.3, 0.0), (0, 1, 0), 50.0, width / height, trace_depth)
+    sc.frame.light_count = 1
+    d = np.array([-0.4, 0.8, -0.3]) / np.linalg.norm([-0.4, 0.8, -0.3])
+    sc.frame.lights[0].direction[:] = d
+    sc.frame.lights[0].color_intensity[:] = [1.0, 0.95, 0.9, 2.0]
+    sc.frame.lights[0].cos_radius = math.cos(math.radians(1.0))
     return sc
 
 

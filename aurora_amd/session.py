@@ -212,6 +212,12 @@ class Renderer:
         """Renders of `tlas` use environment `handle` (0: the context's)."""
         self._check(self.lib.aph_tlas_set_environment(self.ctx, tlas, handle))
 
+    def set_ground_plane(self, tlas, props):
+        """Renders of `tlas` see a ground plane with these properties (a dict over the
+        library's defaults); None removes it."""
+        g = None if props is None else C.byref(capi.ground_plane(props))
+        self._check(self.lib.aph_tlas_set_ground_plane(self.ctx, tlas, g))
+
     def load_scenes(self, scs, builder=capi.APH_BUILDER_LBVH):
         """Uploads several scenes into this context, resident together: one material
         table (each scene's instance material indices offset into it) and one
@@ -227,6 +233,8 @@ class Renderer:
             inst["mesh"] = np.array([handles[i] for i in sc.instances["mesh"]], dtype=np.uint32)
             inst["material"] = inst["material"] + len(mats)
             tl.append(self.tlas(inst))
+            if getattr(sc, "ground", None):
+                self.set_ground_plane(tl[-1], sc.ground)
             if getattr(sc, "layers", None) is not None:
                 self.set_layers(tl[-1], [[m + len(mats) for m in (ls or [])] for ls in sc.layers], len(inst))
             slots.update(self._scene_textures(sc, len(mats)))
@@ -249,6 +257,8 @@ class Renderer:
         inst["mesh"] = np.array([handles[i] for i in sc.instances["mesh"]], dtype=np.uint32)
         t = self.tlas(inst)
         self.mesh_handles = handles  # the scene's mesh index -> mesh handle, for updates of its instances
+        if getattr(sc, "ground", None):
+            self.set_ground_plane(t, sc.ground)
         if getattr(sc, "layers", None) is not None:
             self.set_layers(t, sc.layers, len(inst))
         self.materials(sc.materials)

@@ -297,6 +297,30 @@ int aph_environment_upload(aph_ctx_t ctx, const aph_environment* env);
 int aph_environment_create(aph_ctx_t ctx, const aph_environment* env, uint64_t* handle);
 int aph_environment_destroy(aph_ctx_t ctx, uint64_t handle);
 int aph_tlas_set_environment(aph_ctx_t ctx, uint64_t tlas, uint64_t env);
+/* ---- ground plane: IGroundPlane (Aurora.h), the DirectX backend's PTGroundPlane
+ *      (DirectX/PTGroundPlane.cpp) evaluated by Shaders/GroundPlane.slang:24-290 ----
+ * A matte plane seen by primary rays in front of their hit: with probability
+ * reflection_opacity a GGX reflection of reflection_color / reflection_roughness continues
+ * the path from the plane; otherwise the ray goes on, darkened towards shadow_color by
+ * shadow_opacity where one environment sample from the plane is blocked (opaque shadow
+ * ray).  The normal is used as given (not normalized). */
+typedef struct {
+    int32_t enabled;
+    float position[3];
+    float normal[3];
+    float shadow_opacity;
+    float shadow_color[3];
+    float reflection_opacity;
+    float reflection_color[3];
+    float reflection_roughness;
+} aph_ground_plane;
+/* The property defaults of PTGroundPlane.cpp: enabled, at the origin facing +Y, shadow
+ * opacity 1 in black, reflection opacity 0.5 of grey 0.5 at roughness 0.1. */
+void aph_ground_plane_defaults(aph_ground_plane* gp);
+/* IScene::setGroundPlanePointer (PTScene.cpp:660-667): renders of this TLAS use the plane;
+ * NULL restores the default, a disabled plane (PTRenderer.cpp:713-727). */
+int aph_tlas_set_ground_plane(aph_ctx_t ctx, uint64_t tlas, const aph_ground_plane* gp);
+
 /* Alias table + luminance integral for an RGB32F lat-long image (host build,
  * the AliasMap::build algorithm); entries are 16 B {alias, prob, pdf, pad}. */
 int aph_alias_build(const float* rgb, uint32_t w, uint32_t h, void* entries, float* integral);

@@ -124,6 +124,26 @@ typedef struct {
 } ora_material_constants;
 void ora_material_defaults(ora_material_constants* m); /* MaterialBase.cpp:109-167 */
 
+/* ---- ground plane (DirectX: Shaders/GroundPlane.slang:24-290, PTGroundPlane.cpp) ---- */
+typedef struct {
+    int32_t enabled;
+    float position[3];
+    float normal[3];
+    float shadow_opacity;
+    float shadow_color[3];
+    float reflection_opacity;
+    float reflection_color[3];
+    float reflection_roughness;
+} ora_ground_plane;  /* same layout as aph_ground_plane */
+/* The scene's ground plane (NULL or enabled = 0: none), evaluated on primary rays. */
+void ora_scene_set_ground_plane(ora_scene* s, const ora_ground_plane* gp);
+/* intersectGroundPlane (GroundPlane.slang:104-128): 1 and the hit when the ray meets the plane's front. */
+int ora_ground_plane_hit(const ora_ground_plane* gp, const float org[3], const float dir[3], float pos[3],
+                         float* dist);
+/* sampleGGXSmithBRDF (BSDFCommon.slang:331-367) with anisotropy 0, in tangent space. */
+void ora_ggx_smith_sample(const float V[3], const float f0[3], float roughness, const float rnd[2], float L[3],
+                          float* pdf, float out[3]);
+
 /* ---- material textures (DefaultMaterial.slang:84-233; HGISampler.cpp:22-60) ---- */
 #define ORA_TEX_SLOTS 5 /* base colour, specular roughness, emission colour, opacity, normal */
 typedef struct {

@@ -175,6 +175,29 @@ static v3 sample_ggx_vndf(v3 V, float ax, float ay, const float r[2])
     return normalize(V3(ax * Nh.x, ay * Nh.y, maxf(0.0f, Nh.z)));
 }
 
+/* sampleGGXSmithBRDF, BSDFCommon.slang:331-367, with anisotropy 0 (the ground plane's
+ * matte reflection): alpha = max(1e-4, roughness^2) (roughnessToAlpha, :201-210); the
+ * visible normal reflects V into L; pdf = VNDF / (4 V.H); the value is
+ * D G2 F / (4 |V.z| |L.z|) at the half vector of V and L (evaluateGGXSmithBRDF, :303-323),
+ * black when L leaves V's hemisphere. */
+v3 ora_ggx_smith_brdf_sample(v3 V, v3 f0, float roughness, const float r[2], v3* L, float* pdf)
+{
+    float a = maxf(0.0001f, sqrf(roughness));
+    int flip = V.z <= 0.0f;
+    v3 H = sample_ggx_vndf(flip ? neg(V) : V, a, a, r);
+    if (flip) H = neg(H);
+    *L = reflect3(neg(V), H);
+    *pdf = ggx_smith_vndf(V, H, a, a) / (4.0f * dot(V, H));
+    if (!(V.z * L->z > 0.0f)) return v3s(0.0f);
+    v3 hr = normalize(add(V, *L));
+    float D = ggx_ndf(hr, a, a);
+    float G2 = (dot(V, hr) <= 0.0f || dot(*L, hr) < 0.0f)
+                   ? 0.0f
+                   : 1.0f / (1.0f + smith_lambda(V, a, a) + smith_lambda(*L, a, a));
+    v3 F = lerp3(f0, v3s(1.0f), powf(1.0f - dot(V, hr), 5.0f));
+    return divs(scale(F, D * G2), 4.0f * fabsf(V.z) * fabsf(L->z));
+}
+
 /* ---------------- StandardSurfaceBSDF.slang ---------------- */
 
 static int same_hemisphere_n(v3 a, v3 b, v3 n) { return dot(a, n) * dot(b, n) > 0.0f; } /* :22-25 */

@@ -26,6 +26,8 @@ void ora_material_from_constants(const ora_material_constants* c, ora_material* 
 v3 ora_eval_material(const ora_material* m, const ora_shading* sh, v3 Vw, v3 Lw);
 v3 ora_eval_material_pdf(const ora_material* m, const ora_shading* sh, v3 Vw, v3 Lw, const float r[2],
                          float* pdf, int* lobe);
+/* sampleGGXSmithBRDF (BSDFCommon.slang:331-367), anisotropy 0, tangent space. */
+v3 ora_ggx_smith_brdf_sample(v3 V, v3 f0, float roughness, const float r[2], v3* L, float* pdf);
 v3 ora_sample_material(const ora_material* m, const ora_shading* sh, v3 Vw, const float r[2],
                        v3* Lw, float* pdf, int* lobe);
 

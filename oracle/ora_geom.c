@@ -293,6 +293,25 @@ void ora_scene_set_layers(ora_scene* s, const uint32_t* offsets, const uint32_t*
     if (total) memcpy(s->layer_mat, materials, sizeof(uint32_t) * total);
 }
 
+/* The plane's frame as PTGroundPlane.cpp builds it: start from +Y (or +X for a normal
+ * near +-Y), tangent = normalize(cross(that, N)), bitangent = cross(N, tangent). */
+void ora_scene_set_ground_plane(ora_scene* s, const ora_ground_plane* gp)
+{
+    memset(&s->ground, 0, sizeof(s->ground));
+    if (!gp) return;
+    s->ground = *gp;
+    float n[3] = { gp->normal[0], gp->normal[1], gp->normal[2] };
+    float b[3] = { 0.0f, 1.0f, 0.0f };
+    if (!(fabsf(n[1]) < 0.999f)) { b[0] = 1.0f; b[1] = 0.0f; }
+    float t[3] = { b[1] * n[2] - b[2] * n[1], b[2] * n[0] - b[0] * n[2], b[0] * n[1] - b[1] * n[0] };
+    float l = sqrtf(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+    for (int k = 0; k < 3; k++) t[k] /= l;
+    s->ground_t[0] = t[0]; s->ground_t[1] = t[1]; s->ground_t[2] = t[2];
+    s->ground_b[0] = n[1] * t[2] - n[2] * t[1];
+    s->ground_b[1] = n[2] * t[0] - n[0] * t[2];
+    s->ground_b[2] = n[0] * t[1] - n[1] * t[0];
+}
+
 void ora_scene_destroy(ora_scene* s)
 {
     if (!s) return;

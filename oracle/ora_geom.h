@@ -39,6 +39,8 @@ struct ora_scene {
     uint32_t ntex;
     int32_t* slot_tex;   /* ORA_TEX_SLOTS per material: texture index or -1 */
     uint32_t nmat_tex;
+    ora_ground_plane ground;         /* enabled = 0: none */
+    float ground_t[3], ground_b[3];  /* tangent and bitangent of the plane's normal */
 };
 
 /* Material texture slot at a texture coordinate (0 and return 0 when unbound). */

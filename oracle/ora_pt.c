@@ -396,7 +396,8 @@ static void closest_hit(const pt_ctx* c, payload* pl, v3 rayDir, const ora_hit* 
     }
 }
 
-/* InstanceMissShader, MainEntryPoints.slang:354-426 (no ground plane on HGI). */
+/* InstanceMissShader, MainEntryPoints.slang:354-426 (its ground plane step runs before it, in
+ * pixel_sample). */
 static void miss(const pt_ctx* c, payload* pl, v3 rayDir)
 {
     int shadeBg = pl->isPrimaryRay || pl->rayType == ORA_TRANSMISSION_LOBE ||
@@ -407,6 +408,92 @@ static void miss(const pt_ctx* c, payload* pl, v3 rayDir)
         pl->direct = mul(e, pl->rayContribution);
     }
     pl->pathEnded = 1;
+}
+
+/* ---- ground plane (DirectX only: Shaders/GroundPlane.slang:24-290) ---- */
+
+void ora_ggx_smith_sample(const float V[3], const float f0[3], float roughness, const float rnd[2], float L[3],
+                          float* pdf, float out[3])
+{
+    v3 l;
+    v3 f = ora_ggx_smith_brdf_sample(v3load(V), v3load(f0), roughness, rnd, &l, pdf);
+    v3store(L, l);
+    v3store(out, f);
+}
+
+int ora_ground_plane_hit(const ora_ground_plane* gp, const float org[3], const float dir[3], float pos[3],
+                         float* dist)
+{
+    v3 n = v3load(gp->normal), o = v3load(org), d = v3load(dir);
+    float pd = dot(d, n);
+    if (pd >= 0.0f) return 0;
+    float t = dot(sub(v3load(gp->position), o), n) / pd;
+    if (t < 0.0f) return 0;
+    *dist = t;
+    v3store(pos, add(o, scale(d, t)));
+    return 1;
+}
+
+/* evaluateGroundPlane (:232-290) for a primary ray whose closest hit is at tHit (INFINITY
+ * on a miss).  A reflection sample ends this segment ("absorbed", returns 1: the path
+ * continues from the plane as a ground-plane reflection ray); otherwise the contribution
+ * is tinted by the plane (transparency, matte shadow) and the hit or miss is shaded. */
+static int ground_plane(const pt_ctx* c, payload* pl, v3 org, v3 dir, float tHit)
+{
+    const ora_scene* s = c->s;
+    const ora_ground_plane* g = &s->ground;
+    float o[3], d[3], pos[3], dist;
+    v3store(o, org);
+    v3store(d, dir);
+    if (!ora_ground_plane_hit(g, o, d, pos, &dist) || !(dist < tHit)) return 0;
+    v3 N = v3load(g->normal), T = v3load(s->ground_t), B = v3load(s->ground_b), P = v3load(pos);
+    v3 color = v3s(1.0f);
+    float r[2];
+    if (g->reflection_opacity > 0.0f) {
+        v3 transparency = v3s(1.0f - g->reflection_opacity);
+        float Pr = luminance(transparency);
+        ora_random2d(pl->rng, r);
+        if (r[0] >= Pr) {
+            /* setupMatteReflectionIndirectLightRay (:24-65) */
+            v3 V = V3(dot(neg(dir), T), dot(neg(dir), B), dot(neg(dir), N));
+            v3 L;
+            float pdf;
+            ora_random2d(pl->rng, r);
+            v3 bc = ora_ggx_smith_brdf_sample(V, v3load(g->reflection_color), g->reflection_roughness, r, &L, &pdf);
+            bc = scale(bc, fabsf(L.z));
+            int reflect = !is_black(bc);
+            if (reflect) color = divs(bc, pdf);
+            color = scale(color, g->reflection_opacity / (1.0f - Pr));
+            pl->rayContribution = mul(pl->rayContribution, color);
+            if (!reflect) return 0; /* NO_SECONDARY_RAY: on to the hit, tinted, no shadow */
+            pl->nextRayType = ORA_GROUND_PLANE_REFLECTION_RAY;
+            pl->nextRayOrigin = P;
+            pl->nextRayDirection = add(add(scale(T, L.x), scale(B, L.y)), scale(N, L.z));
+            pl->isPrimaryRay = 0;
+            return 1;
+        }
+        color = divs(transparency, Pr);
+    }
+    if (g->shadow_opacity > 0.0f) {
+        /* shadeMatteShadow (:67-101): one environment sample, an opaque shadow ray */
+        float Lf[3], pdf, Le[3];
+        ora_random2d(pl->rng, r);
+        ora_env_sample(c->env, r, Lf, &pdf, Le);
+        v3 L = v3load(Lf);
+        if (!(dot(L, N) < 0.0f)) {
+            ora_ray sr;
+            v3store(sr.org, P);
+            v3store(sr.dir, L);
+            sr.tmin = ORA_RAY_TMIN;
+            sr.tmax = INFINITY;
+            if (c->cnt) c->cnt->shadow_rays++;
+            v3 vis = v3s(ora_trace_any_one(s, &sr) ? 0.0f : 1.0f);
+            float a = luminance(sub(v3s(1.0f), vis)) * g->shadow_opacity;
+            color = mul(color, lerp3(v3s(1.0f), v3load(g->shadow_color), a));
+        }
+    }
+    pl->rayContribution = mul(pl->rayContribution, color);
+    return 0;
 }
 
 /* computeCameraRay, RayTrace.slang:130-182 */
@@ -502,8 +589,16 @@ static void pixel_sample(const ora_scene* s, const ora_material_constants* mats,
             ora_hit hit;
             if (cnt) cnt->extension_rays++;
             ora_trace_closest_one(s, &r, &hit);
-            if (hit.inst != 0xFFFFFFFFu) closest_hit(&c, &pl, rayDir, &hit);
-            else miss(&c, &pl, rayDir);
+            int absorbed = 0;
+            if (pl.isPrimaryRay && s->ground.enabled)
+                absorbed = ground_plane(&c, &pl, rayOrg, rayDir, hit.inst != 0xFFFFFFFFu ? hit.t : INFINITY);
+            if (absorbed) {
+                /* the segment ends at the plane: nothing shaded, no roulette */
+            } else if (hit.inst != 0xFFFFFFFFu) {
+                closest_hit(&c, &pl, rayDir, &hit);
+            } else {
+                miss(&c, &pl, rayDir);
+            }
             rayOrg = pl.nextRayOrigin;
             rayDir = pl.nextRayDirection;
             outAlpha = pl.outputAlpha;

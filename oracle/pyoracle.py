@@ -53,6 +53,29 @@ class Env(C.Structure):
                 ("bg_w", C.c_uint32), ("bg_h", C.c_uint32)]
 
 
+class GroundPlane(C.Structure):
+    _fields_ = [("enabled", C.c_int32), ("position", C.c_float * 3), ("normal", C.c_float * 3),
+                ("shadow_opacity", C.c_float), ("shadow_color", C.c_float * 3), ("reflection_opacity", C.c_float),
+                ("reflection_color", C.c_float * 3), ("reflection_roughness", C.c_float)]
+
+
+# The ground plane's properties and their defaults (PTGroundPlane.cpp propertySet)
+GROUND_DEFAULTS = dict(enabled=1, position=(0.0, 0.0, 0.0), normal=(0.0, 1.0, 0.0), shadow_opacity=1.0,
+                       shadow_color=(0.0, 0.0, 0.0), reflection_opacity=0.5, reflection_color=(0.5, 0.5, 0.5),
+                       reflection_roughness=0.1)
+
+
+def ground_plane_struct(d):
+    g = GroundPlane()
+    for k, dv in GROUND_DEFAULTS.items():
+        v = d.get(k, dv)
+        if isinstance(dv, tuple):
+            getattr(g, k)[:] = [float(x) for x in v]
+        else:
+            setattr(g, k, type(getattr(g, k))(v))
+    return g
+
+
 class Frame3(C.Structure):
     _fields_ = [("N", C.c_float * 3), ("X", C.c_float * 3), ("Y", C.c_float * 3)]
 
@@ -113,6 +136,9 @@ def load():
         "ora_scene_set_textures": (None, [P, P, u32, P, u32]),
         "ora_normal_from_map": (None, [P, P, P, P]),
         "ora_hit_uv": (None, [P, u32, u32, C.c_float, C.c_float, P]),
+        "ora_scene_set_ground_plane": (None, [P, P]),
+        "ora_ground_plane_hit": (i, [P, P, P, P, P]),
+        "ora_ggx_smith_sample": (None, [P, P, C.c_float, P, P, P, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -173,6 +199,10 @@ class OracleScene:
                 tex[n] = Texture(_p(t), t.shape[1], t.shape[0], au, av)
                 slot_tex[mi * 5 + k] = n
             lib.ora_scene_set_textures(self.ptr, C.cast(tex, C.c_void_p), len(textures), _p(slot_tex), nmat)
+        ground = getattr(sc, "ground", None)
+        if ground:
+            g = ground_plane_struct(ground)
+            lib.ora_scene_set_ground_plane(self.ptr, C.cast(C.pointer(g), C.c_void_p))
         self._env = None
 
     def __del__(self):
