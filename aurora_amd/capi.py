@@ -245,6 +245,7 @@ SIGNATURES = {
     "aph_blas_export": (_i, [_P, _u64, _P, _u64, _P, _u64, C.POINTER(_u32), C.POINTER(_u32)]),
     "aph_tlas_build": (_i, [_P, _P, _u32, C.POINTER(_u64)]),
     "aph_tlas_set_layers": (_i, [_P, _u64, _P, _P]),
+    "aph_tlas_set_layer_geometry": (_i, [_P, C.c_uint64, _P]),
     "aph_inverse3x4": (_i, [_P, _P]),
     "aph_materials_upload": (_i, [_P, _P, _u32]),
     "aph_material_defaults": (None, [C.POINTER(Material)]),

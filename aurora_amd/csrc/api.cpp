@@ -544,6 +544,7 @@ int aph_tlas_update(aph_ctx_t c, uint64_t tlas, const aph_instance* inst, uint32
     fresh.ground = old->ground;
     if (n == old->n) {
         fresh.layerMats = std::move(old->layerMats);
+        fresh.layerUV = std::move(old->layerUV);
         for (uint32_t k = 0; k < n && k < old->shadeHost.size(); k++) {
             fresh.shadeHost[k].layerFirst = old->shadeHost[k].layerFirst;
             fresh.shadeHost[k].layerCount = old->shadeHost[k].layerCount;
@@ -574,6 +575,7 @@ int aph_tlas_set_layers(aph_ctx_t c, uint64_t tlas, const uint32_t* offsets, con
     // Indices are resolved at render time against the material table then current;
     // one beyond it reads material 0, like an instance's own material index.
     HIPCALL(c, hipStreamSynchronize(c->stream), "layers");
+    t->layerUV.reset();  // new layers: every layer reads the instance's texture coordinates again
     HIPCALL(c, t->layerMats.alloc((size_t)(total ? total : 1) * 4), "alloc layers");
     if (total) HIPCALL(c, hipMemcpy(t->layerMats.p, materials, (size_t)total * 4, hipMemcpyHostToDevice), "layers");
     // Instance k of the TLAS is the k-th instance given to aph_tlas_build.
@@ -584,6 +586,40 @@ int aph_tlas_set_layers(aph_ctx_t c, uint64_t tlas, const uint32_t* offsets, con
     HIPCALL(c, hipMemcpy(t->shade.p, t->shadeHost.data(), sizeof(InstShade) * t->shadeHost.size(),
                          hipMemcpyHostToDevice),
             "layers");
+    return APH_OK;
+}
+
+int aph_tlas_set_layer_geometry(aph_ctx_t c, uint64_t tlas, const uint64_t* meshes)
+{
+    int rc = enter(c);
+    if (rc) return rc;
+    Tlas* t = lookup(c->tlases, tlas);
+    if (!t) return fail(c, APH_E_INVALID, "unknown tlas handle %llu", (unsigned long long)tlas);
+    uint32_t total = 0;
+    for (const InstShade& sh : t->shadeHost) total = std::max(total, sh.layerFirst + sh.layerCount);
+    if (total == 0) return fail(c, APH_E_STATE, "TLAS %llu has no material layers", (unsigned long long)tlas);
+    if (!meshes) return fail(c, APH_E_INVALID, "null layer geometry");
+    // vertex count of each instance's mesh, found by its texture coordinate array
+    std::map<const float*, uint32_t> nvOf;
+    for (const auto& kv : c->meshes) nvOf[kv.second->uv.as<float>()] = kv.second->nv;
+    std::vector<const float*> uv(total, nullptr);
+    for (size_t k = 0; k < t->shadeHost.size() && k < t->n; k++) {
+        const InstShade& sh = t->shadeHost[k];
+        for (uint32_t j = sh.layerFirst; j < sh.layerFirst + sh.layerCount; j++) {
+            if (!meshes[j]) continue;  // the instance's own texture coordinates
+            Mesh* m = lookup(c->meshes, meshes[j]);
+            if (!m) return fail(c, APH_E_INVALID, "layer %u: unknown mesh handle %llu", j, (unsigned long long)meshes[j]);
+            // PTScene.cpp:1102-1103: a layer's geometry matches its instance's vertex count
+            auto it = nvOf.find(sh.uv);
+            if (it == nvOf.end() || it->second != m->nv)
+                return fail(c, APH_E_INVALID, "layer %u: mesh %llu has %u vertices, its instance's mesh %u", j,
+                            (unsigned long long)meshes[j], m->nv, it == nvOf.end() ? 0u : it->second);
+            uv[j] = m->uv.as<float>();
+        }
+    }
+    HIPCALL(c, hipStreamSynchronize(c->stream), "layer geometry");  // renders in flight read the old table
+    HIPCALL(c, t->layerUV.alloc(sizeof(const float*) * total), "alloc layer geometry");
+    HIPCALL(c, hipMemcpy(t->layerUV.p, uv.data(), sizeof(const float*) * total, hipMemcpyHostToDevice), "layer geometry");
     return APH_OK;
 }
 

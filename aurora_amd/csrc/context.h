@@ -92,6 +92,7 @@ struct Tlas {
     DevBuf nodes, nodes4, trav, shade;
     TraceScene sc{};
     DevBuf layerMats;  // material indices of the instances' layers (aph_tlas_set_layers)
+    DevBuf layerUV;    // per layer: device pointer to its geometry's UVs, or null (aph_tlas_set_layer_geometry)
     std::vector<InstShade> shadeHost;
     std::vector<uint32_t> remap;  // indices of the instances with geometry (those the tree holds), ascending
     uint32_t depth = 0;
@@ -225,6 +226,7 @@ inline RenderScene renderScene(const aph_ctx* c, const Tlas* tl)
     rs.textures = c->textures.as<DevTexture>();
     rs.nmat = c->nmat;
     rs.layerMats = tl->layerMats.as<uint32_t>();
+    rs.layerUV = tl->layerUV.as<const float*>();
     rs.pad_ = 0;
     rs.env = tl->environment ? tl->environment->env : c->env->env;
     rs.sc.transmit = tl->transmit.as<float4>();

@@ -20,6 +20,7 @@ struct RenderScene {
     const DevMatTex* matTex;      // per material: bound textures and UV transforms
     const DevTexture* textures;
     const uint32_t* layerMats;  // material indices of all instances' layers
+    const float* const* layerUV;  // per layer: its geometry's UVs, or null (the instance's); null: none set
     uint32_t nmat;
     uint32_t pad_;
     DevEnv env;
@@ -27,23 +28,38 @@ struct RenderScene {
 };
 
 // Texture coordinate of a hit (Geometry.slang:199-203): the barycentric mix of
-// the vertex UVs, computed on first use (only textured materials read it).
+// the vertex UVs, computed on first use (only textured materials read it).  A
+// material layer with geometry of its own reads that geometry's UVs at the same
+// vertices (computeLayerUV, GlobalBufferAccessors.slang:116-127): source() picks
+// the array, null for the instance's own.
 struct HitUV {
     const InstShade* ins;
     uint32_t prim;
     float bu, bv;
     bool have;
+    const float* src;
     f2 uv;
-    __device__ HitUV(const InstShade& i, uint32_t p, float u, float v) : ins(&i), prim(p), bu(u), bv(v), have(false) {}
+    __device__ HitUV(const InstShade& i, uint32_t p, float u, float v)
+        : ins(&i), prim(p), bu(u), bv(v), have(false), src(nullptr)
+    {
+    }
+    __device__ void source(const float* s)
+    {
+        if (s != src) {
+            src = s;
+            have = false;
+        }
+    }
     __device__ f2 get()
     {
         if (!have) {
+            const float* a = src ? src : ins->uv;
             const float b[3] = {1.0f - bu - bv, bu, bv};
             f2 t = f2{0.0f, 0.0f};
 #pragma unroll
             for (int k = 0; k < 3; k++) {
                 const uint32_t vi = ins->idx[3 * prim + k];
-                t = f2{t.x + ins->uv[2 * (size_t)vi] * b[k], t.y + ins->uv[2 * (size_t)vi + 1] * b[k]};
+                t = f2{t.x + a[2 * (size_t)vi] * b[k], t.y + a[2 * (size_t)vi + 1] * b[k]};
             }
             uv = t;
             have = true;
@@ -69,7 +85,9 @@ __device__ __forceinline__ f3 sampleSlot(const RenderScene& rs, uint32_t mi, int
 // DirectX semantics) are tried from the top layer down to the base material; each
 // draws its opacity coin (its opacity texture, if flagged, in place of the
 // constant), and a layer the coin passes through scales the throughput by
-// transparency / P.  Without layers this is the single coin of the HGI build.
+// transparency / P.  Each layer's textures read its own texture coordinates
+// (HitShaderEntryPoints.slang:98-111), and the base material the instance's.
+// Without layers this is the single coin of the HGI build.
 // Returns the hit layer's material index and opacity, or kNoMaterial when every
 // layer was passed through.
 __device__ __forceinline__ uint32_t selectLayer(const RenderScene& rs, const InstShade& ins, HitUV& huv, Rng& rng,
@@ -77,6 +95,7 @@ __device__ __forceinline__ uint32_t selectLayer(const RenderScene& rs, const Ins
 {
     for (int li = (int)ins.layerCount; li >= 0; li--) {
         uint32_t mi = li == 0 ? ins.material : rs.layerMats[ins.layerFirst + (uint32_t)li - 1u];
+        if (rs.layerUV) huv.source(li == 0 ? nullptr : rs.layerUV[ins.layerFirst + (uint32_t)li - 1u]);
         mi = mi < rs.nmat ? mi : 0u;
         const DevMaterial& m = rs.mats[mi];
         f3 op = m.opacity;

@@ -292,6 +292,7 @@ class Scene:
         self.materials = []     # list of capi.Material
         self.env = None         # dict
         self.layers = None      # per instance: material indices of its layers, bottom to top
+        self.layer_geometry = None  # per instance, per layer: mesh index whose UVs the layer reads, or None
         self.ground = None      # ground plane properties (dict over aph_ground_plane_defaults) or None
         self.uvs = {}           # mesh index -> (nv, 2) texture coordinates (others: u = v = i / nv)
         self.images = []        # dicts: pixels (h, w, 4) uint8 / (h, w, 4|3) float32, linearize
@@ -615,6 +616,46 @@ def obj_scene(obj_path, hdr_path=None, width=128, height=128, spp=4, trace_depth
         sc.env = dict(light_rgb=rgb, background_rgb=rgb)
     sc.width, sc.height, sc.spp = width, height, spp
     sc.frame = look_at_frame(eye, target, (0, 1, 0), fov_y_deg, width / height, trace_depth)
+    return sc
+
+
+def decal_scene(path, width=96, height=96, spp=4, trace_depth=3, seed=17):
+    """A textured decal (the DirectX layer loop with layer geometry, HitShaderEntryPoints.slang:84-113,
+    GlobalBufferAccessors.slang:116-127): the fixture teapot, its base material checkered
+    through the generated u = v = i / nv, under a decal layer whose own geometry (a copy of
+    the teapot carrying a planar projection as texture coordinates) places a colour pattern
+    and a round opacity mask on the teapot's side; outside the mask the layer is clear."""
+    sc = teapot_scene(path, width, height, spp, trace_depth)
+    sc.name = "decal"
+    rng = np.random.default_rng(seed)
+    p, n, f = sc.meshes[0]
+    # planar projection onto the xy plane over the teapot's front: u from x, v from y
+    lo, hi = np.array([-0.9, 0.1]), np.array([0.9, 1.5])
+    proj = np.ascontiguousarray(((p[:, :2] - lo) / (hi - lo)).astype(np.float32))
+    sc.meshes = [(p, n, f), (p.copy(), n.copy(), f.copy())]
+    sc.uvs = {1: proj}
+    y, x = np.mgrid[0:64, 0:64]
+    pattern = np.stack([(x / 63.0), (y / 63.0), 0.5 + 0.5 * np.sin(x * 0.3) * np.cos(y * 0.2),
+                        np.ones_like(x, dtype=np.float64)], -1).astype(np.float32)
+    r = np.hypot(x - 31.5, y - 31.5)
+    mask = np.zeros((64, 64, 4), np.float32)
+    mask[..., :3] = (r < 24)[..., None].astype(np.float32)
+    mask[..., 3] = 1.0
+    a, b = rng.integers(0, 256, 4), rng.integers(0, 256, 4)
+    checker = np.where((((x // 8) + (y // 8)) % 2 == 0)[..., None], a, b).astype(np.uint8)
+    checker[..., 3] = 255
+    sc.images = [dict(pixels=checker, linearize=True), dict(pixels=pattern), dict(pixels=mask)]
+    base = _material((0.8, 0.8, 0.8), 0.0, 0.3)
+    capi.set_texture_slot(base, capi.APH_TEX_BASE_COLOR, scale=(0.05, 0.05))
+    decal = _material((1.0, 1.0, 1.0), 0.0, 0.4)
+    capi.set_texture_slot(decal, capi.APH_TEX_BASE_COLOR)
+    capi.set_texture_slot(decal, capi.APH_TEX_OPACITY)
+    sc.materials = [base, decal]
+    sc.textures = {(0, capi.APH_TEX_BASE_COLOR): (0, capi.APH_ADDRESS_WRAP, capi.APH_ADDRESS_WRAP),
+                   (1, capi.APH_TEX_BASE_COLOR): (1, capi.APH_ADDRESS_CLAMP, capi.APH_ADDRESS_CLAMP),
+                   (1, capi.APH_TEX_OPACITY): (2, capi.APH_ADDRESS_CLAMP, capi.APH_ADDRESS_CLAMP)}
+    sc.layers = [[1]]
+    sc.layer_geometry = [[1]]
     return sc
 
 

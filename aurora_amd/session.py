@@ -157,6 +157,21 @@ class Renderer:
         mats = np.ascontiguousarray(flat if flat else [0], dtype=np.uint32)
         self._check(self.lib.aph_tlas_set_layers(self.ctx, tlas, off.ctypes.data, mats.ctypes.data))
 
+    def set_layer_geometry(self, tlas, geometry, n):
+        """Per instance, per layer (as set_layers): the mesh handle whose texture
+        coordinates the layer reads, or 0 for the instance's (aph_tlas_set_layer_geometry)."""
+        flat = []
+        for i in range(n):
+            flat.extend(geometry[i] if i < len(geometry) and geometry[i] else [])
+        arr = np.ascontiguousarray(flat if flat else [0], dtype=np.uint64)
+        self._check(self.lib.aph_tlas_set_layer_geometry(self.ctx, tlas, arr.ctypes.data))
+
+    def _scene_layer_geometry(self, sc, tlas, handles):
+        lg = getattr(sc, "layer_geometry", None)
+        if lg is not None:
+            per = [[0 if k is None else handles[k] for k in (g or [])] for g in lg]
+            self.set_layer_geometry(tlas, per, len(sc.instances))
+
     def materials(self, mats):
         arr = (capi.Material * len(mats))(*mats)
         self._check(self.lib.aph_materials_upload(self.ctx, C.cast(arr, C.c_void_p), len(mats)))
@@ -237,6 +252,7 @@ class Renderer:
                 self.set_ground_plane(tl[-1], sc.ground)
             if getattr(sc, "layers", None) is not None:
                 self.set_layers(tl[-1], [[m + len(mats) for m in (ls or [])] for ls in sc.layers], len(inst))
+                self._scene_layer_geometry(sc, tl[-1], handles)
             slots.update(self._scene_textures(sc, len(mats)))
             mats.extend(sc.materials)
         self.materials(mats)
@@ -261,6 +277,7 @@ class Renderer:
             self.set_ground_plane(t, sc.ground)
         if getattr(sc, "layers", None) is not None:
             self.set_layers(t, sc.layers, len(inst))
+            self._scene_layer_geometry(sc, t, handles)
         self.materials(sc.materials)
         slots = self._scene_textures(sc)
         if slots:

@@ -204,8 +204,16 @@ int aph_blas_build(aph_ctx_t ctx, uint64_t mesh, int builder);
  * offsets[i+1]) (indices into the material table), ordered from just above the
  * base material to the top; a hit tries the top layer first, each layer's
  * opacity coin passing the ray on (throughput *= transparency / P) or stopping
- * at it.  Layer UV geometry only matters for textures, not supported here. */
+ * at it.  Layers read the instance's texture coordinates until
+ * aph_tlas_set_layer_geometry gives them their own; setting layers clears that. */
 int aph_tlas_set_layers(aph_ctx_t ctx, uint64_t tlas, const uint32_t* offsets, const uint32_t* materials);
+/* The geometry half of each LayerDefinition (IScene::addInstancePointer, Aurora.h:1027-1029;
+ * PTScene.cpp:1051-1066, 1095-1107): meshes[j] supplies the texture coordinates of layer j
+ * (the same indexing as aph_tlas_set_layers' materials), read at the hit's vertices
+ * (computeLayerUV, GlobalBufferAccessors.slang:116-127) by that layer's opacity and material
+ * textures; 0 keeps the instance's own.  A layer mesh has its instance's vertex count
+ * (PTScene.cpp:1102-1103, APH_E_INVALID otherwise); APH_E_STATE without layers. */
+int aph_tlas_set_layer_geometry(aph_ctx_t ctx, uint64_t tlas, const uint64_t* meshes);
 /* Copies a built BLAS to host memory: *node_count 64-byte nodes (two child
  * boxes; child references in a.w / b.w, leaf = bit 31 | (count-1) << 27 |
  * first) and the triangles in leaf order (48 bytes, original primitive index in

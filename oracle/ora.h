@@ -95,6 +95,9 @@ ora_scene* ora_scene_create(const ora_mesh* meshes, uint32_t nmesh,
 void ora_scene_destroy(ora_scene* s);
 /* Material layers per instance (as aph_tlas_set_layers): bottom to top. */
 void ora_scene_set_layers(ora_scene* s, const uint32_t* offsets, const uint32_t* materials);
+/* Per layer (indexed as the materials above): the texture coordinates of the layer's own
+ * geometry, 2 per vertex of its instance's mesh (copied), or NULL for the instance's. */
+void ora_scene_set_layer_uvs(ora_scene* s, const float* const* uvs);
 void ora_trace_closest(const ora_scene* s, const ora_ray* rays, uint32_t n, ora_hit* hits, int nthreads);
 void ora_trace_closest_brute(const ora_scene* s, const ora_ray* rays, uint32_t n, ora_hit* hits);
 /* SAH cost of a full-sweep SAH tree over n boxes (lo/hi float[3n]); checker for the BVH builders. */

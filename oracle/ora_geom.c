@@ -282,8 +282,34 @@ ora_scene* ora_scene_create(const ora_mesh* meshes, uint32_t nmesh, const ora_in
     return s;
 }
 
+static void free_layer_uvs(ora_scene* s)
+{
+    if (!s->layer_uv) return;
+    uint32_t total = s->layer_off ? s->layer_off[s->ninst] : 0;
+    for (uint32_t j = 0; j < total; j++) free(s->layer_uv[j]);
+    free(s->layer_uv);
+    s->layer_uv = NULL;
+}
+
+void ora_scene_set_layer_uvs(ora_scene* s, const float* const* uvs)
+{
+    free_layer_uvs(s);
+    if (!s->layer_off || !uvs) return;
+    uint32_t total = s->layer_off[s->ninst];
+    s->layer_uv = (float**)calloc(total ? total : 1, sizeof(float*));
+    for (uint32_t i = 0; i < s->ninst; i++) {
+        uint32_t nv = s->meshes[s->inst[i].mesh].nv;
+        for (uint32_t j = s->layer_off[i]; j < s->layer_off[i + 1]; j++) {
+            if (!uvs[j]) continue;
+            s->layer_uv[j] = (float*)malloc(sizeof(float) * 2 * (nv ? nv : 1));
+            memcpy(s->layer_uv[j], uvs[j], sizeof(float) * 2 * nv);
+        }
+    }
+}
+
 void ora_scene_set_layers(ora_scene* s, const uint32_t* offsets, const uint32_t* materials)
 {
+    free_layer_uvs(s);
     free(s->layer_off);
     free(s->layer_mat);
     s->layer_off = (uint32_t*)malloc(sizeof(uint32_t) * (s->ninst + 1));
@@ -321,6 +347,7 @@ void ora_scene_destroy(ora_scene* s)
     }
     free(s->tlas.nodes);
     free(s->tlas.prims);
+    free_layer_uvs(s);
     free(s->layer_off);
     free(s->layer_mat);
     for (uint32_t m = 0; m < s->nmesh; m++) free(s->uv_gen[m]);

@@ -34,6 +34,8 @@ struct ora_scene {
     double extent;   /* 3 x max |coordinate| of the instance world boxes */
     uint32_t* layer_off; /* material layers: instance i has layer_mat[layer_off[i] .. layer_off[i+1]) */
     uint32_t* layer_mat;
+    float** layer_uv;    /* per layer: its geometry's texture coordinates (2 per vertex of the instance's
+                            mesh), or NULL for the instance's; the array itself NULL: none set */
     float** uv_gen;      /* per mesh: generated texture coordinates (u = v = i / nv) or NULL */
     ora_texture* tex;    /* material textures (ora_scene_set_textures) */
     uint32_t ntex;

@@ -277,8 +277,24 @@ static void closest_hit(const pt_ctx* c, payload* pl, v3 rayDir, const ora_hit* 
     int hitLayer = 0;
     uint32_t hitMat = 0;
     float r[2];
+    /* computeLayerUV (GlobalBufferAccessors.slang:116-127): a layer with geometry of its own
+     * reads that geometry's texture coordinates at the hit's vertices; the base layer gets
+     * the instance's back (HitShaderEntryPoints.slang:98-111). */
+    const float baseUV[2] = { sh.texCoord[0], sh.texCoord[1] };
+    const uint32_t* tri = s->meshes[s->inst[hit->inst].mesh].idx + 3 * (size_t)hit->prim;
+    const float bw[3] = { 1.0f - hit->u - hit->v, hit->u, hit->v };
     for (int li = (int)lcount; li >= 0; li--) {
         uint32_t mi = li == 0 ? s->inst[hit->inst].material : s->layer_mat[lfirst + (uint32_t)li - 1];
+        const float* luv = (li > 0 && s->layer_uv) ? s->layer_uv[lfirst + (uint32_t)li - 1] : NULL;
+        sh.texCoord[0] = baseUV[0];
+        sh.texCoord[1] = baseUV[1];
+        if (luv) {
+            sh.texCoord[0] = sh.texCoord[1] = 0.0f;
+            for (int k = 0; k < 3; k++) {
+                sh.texCoord[0] = sh.texCoord[0] + luv[2 * (size_t)tri[k]] * bw[k];
+                sh.texCoord[1] = sh.texCoord[1] + luv[2 * (size_t)tri[k] + 1] * bw[k];
+            }
+        }
         if (mi >= c->nmat) mi = 0;
         ora_material_from_constants(&c->mats[mi], &m);
         if (ora_tex_flag(&c->mats[mi], 3)) { /* opacity texture */

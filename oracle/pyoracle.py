@@ -137,6 +137,7 @@ def load():
         "ora_normal_from_map": (None, [P, P, P, P]),
         "ora_hit_uv": (None, [P, u32, u32, C.c_float, C.c_float, P]),
         "ora_scene_set_ground_plane": (None, [P, P]),
+        "ora_scene_set_layer_uvs": (None, [P, P]),
         "ora_ground_plane_hit": (i, [P, P, P, P, P]),
         "ora_ggx_smith_sample": (None, [P, P, C.c_float, P, P, P, P]),
     }
@@ -185,6 +186,23 @@ class OracleScene:
         if layers is not None:
             off, mats = layer_csr(layers, len(inst))
             lib.ora_scene_set_layers(self.ptr, _p(off), _p(mats))
+            geom = getattr(sc, "layer_geometry", None)
+            if geom is not None:
+                # per layer: the texture coordinates of its mesh (given, or u = v = i / nv)
+                ptrs = (C.c_void_p * max(1, int(off[-1])))()
+                for i in range(len(inst)):
+                    g = geom[i] if i < len(geom) and geom[i] else []
+                    for j, k in enumerate(g):
+                        if k is None:
+                            continue
+                        uv = uvs.get(k)
+                        if uv is None:
+                            nv = len(np.asarray(sc.meshes[k][0]).reshape(-1, 3))
+                            uv = np.repeat((np.arange(nv, dtype=np.float32) / np.float32(nv))[:, None], 2, axis=1)
+                        uv = np.ascontiguousarray(uv, dtype=np.float32)
+                        self._keep.append(uv)
+                        ptrs[int(off[i]) + j] = uv.ctypes.data
+                lib.ora_scene_set_layer_uvs(self.ptr, C.cast(ptrs, C.c_void_p))
         textures = getattr(sc, "textures", None)
         if textures:
             # one oracle texture per (material, slot) binding, as the library's table
