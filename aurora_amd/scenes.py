@@ -364,9 +364,23 @@ def config1(seed=1, width=512, height=512, spp=16, trace_depth=3):
     return sc
 
 
+def decal_images(size=64):
+    """A decal's colour pattern (RGBA32F) and its round opacity mask (clear at the border)."""
+    y, x = np.mgrid[0:size, 0:size]
+    t = (np.arange(size) + 0.5) / size
+    pattern = np.stack([np.broadcast_to(t[None, :], (size, size)), np.broadcast_to(t[:, None], (size, size)),
+                        0.5 + 0.5 * np.sin(x * 0.3) * np.cos(y * 0.2), np.ones((size, size))], -1)
+    r = np.hypot(x + 0.5 - size / 2, y + 0.5 - size / 2)
+    mask = np.zeros((size, size, 4), np.float32)
+    mask[..., :3] = (r < 0.4 * size)[..., None]
+    mask[..., 3] = 1.0
+    return [dict(pixels=pattern.astype(np.float32)), dict(pixels=mask)]
+
+
 def config2(seed=2, subdiv=4, width=1920, height=1080, spp=64, trace_depth=5, env_w=2048, env_h=1024,
             instances=200):
-    """200 instances of 5 displaced icospheres, transmission/coat materials, 2048x1024 sun sky."""
+    """200 instances of 5 displaced icospheres, transmission/coat materials, a textured decal
+    layer on every fourth sphere, 2048x1024 sun sky."""
     rng = np.random.Generator(np.random.PCG64(seed))
     sc = Scene()
     sc.name = "config2"
@@ -386,12 +400,24 @@ def config2(seed=2, subdiv=4, width=1920, height=1080, spp=64, trace_depth=5, en
                                       float(rng.uniform(0.05, 0.8)), transmission=float(rng.uniform(0, 1)),
                                       coat=float(rng.uniform(0, 1)), coat_roughness=float(rng.uniform(0.02, 0.3))))
     sc.instances = _instances(inst)
-    # one decal layer (DirectX layer semantics): a partly opaque coat over every
-    # fourth sphere instance; constant opacity stands in for a decal's texture
-    sc.materials.append(_material((0.9, 0.35, 0.1), 0.0, 0.3))
-    sc.materials[-1].opacity[:] = [0.6, 0.6, 0.6]
+    # one decal layer (DirectX layer semantics, with layer geometry): a textured, round-masked
+    # decal over every fourth sphere instance, placed by its own geometry - a copy of the
+    # sphere mesh carrying a spherical projection as texture coordinates
+    decal_mat = _material((1.0, 1.0, 1.0), 0.0, 0.3)
+    capi.set_texture_slot(decal_mat, capi.APH_TEX_BASE_COLOR)
+    capi.set_texture_slot(decal_mat, capi.APH_TEX_OPACITY)
+    sc.materials.append(decal_mat)
     decal = len(sc.materials) - 1
-    sc.layers = [[decal] if (k > 0 and k % 4 == 0) else [] for k in range(len(inst))]
+    for k in range(1, 6):
+        p, n, f = sc.meshes[k]
+        sc.meshes.append((p.copy(), n.copy(), f.copy()))
+        sc.uvs[len(sc.meshes) - 1] = np.ascontiguousarray(sphere_uvs(p) * 3.0 - 1.0, dtype=np.float32)
+    sc.images, sc.textures = decal_images(), {
+        (decal, capi.APH_TEX_BASE_COLOR): (0, capi.APH_ADDRESS_CLAMP, capi.APH_ADDRESS_CLAMP),
+        (decal, capi.APH_TEX_OPACITY): (1, capi.APH_ADDRESS_CLAMP, capi.APH_ADDRESS_CLAMP)}
+    decorated = [k > 0 and k % 4 == 0 for k in range(len(inst))]
+    sc.layers = [[decal] if d else [] for d in decorated]
+    sc.layer_geometry = [[inst[k][0] + 5] if d else [] for k, d in enumerate(decorated)]
     sky = sun_sky(env_w, env_h, rng)
     sc.env = dict(light_rgb=sky, background_rgb=sky)
     sc.width, sc.height, sc.spp = width, height, spp
