@@ -8,8 +8,9 @@
 //     (w2o 3x4 from aph_inverse3x4, products summed left to right) and rounded
 //     to float; every box and triangle test of that instance uses this ray;
 //   * triangles: Woop-Benthin-Wald watertight test in double, no FMA
-//     contraction; U,V,W of one sign (zeros allowed), det != 0,
-//     tmin <= t = T/det <= tmax;
+//     contraction, shear constants Sz = 1/d_kz, Sx = d_kx Sz, Sy = d_ky Sz;
+//     U,V,W of one sign (zeros allowed), det != 0, tmin <= t = T/det <= tmax
+//     (an FP32 pre-test only skips triangles this test certainly rejects);
 //   * closest hit = lexicographic minimum of (t, instance, primitive).
 // Boxes only cull, conservatively: BLAS boxes are tight float boxes tested with
 // the robust slab test of Ize (JCGT 2(2), 2013): sign-selected planes,
@@ -148,9 +149,13 @@ __device__ __forceinline__ bool slab(const BoxRay& r, float4 lo, float4 hi, floa
 struct WRay {
     float ox, oy, oz;    // origin permuted to (kx, ky, kz)
     double Sx, Sy, Sz;
+    float sx, sy;        // Sx, Sy rounded to float, for the FP32 pre-test
     uint32_t perm;       // kx | ky << 2 | kz << 4
 };
 
+// The products of two floats are exact in double, so fusing each into the next
+// addition (fma(m1, oy, m0 * ox) = fl(m0 ox + m1 oy)) rounds exactly as the
+// left-to-right sum of products does.
 __device__ __forceinline__ void objectRay(const float* w2o, f3 o, f3 d, f3& oo, f3& od)
 {
 #pragma clang fp contract(off)
@@ -159,8 +164,8 @@ __device__ __forceinline__ void objectRay(const float* w2o, f3 o, f3 d, f3& oo, 
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         double m0 = w2o[4 * i + 0], m1 = w2o[4 * i + 1], m2 = w2o[4 * i + 2], m3 = w2o[4 * i + 3];
-        r[i] = (float)((((m0 * ox) + (m1 * oy)) + (m2 * oz)) + m3);
-        r[3 + i] = (float)(((m0 * dx) + (m1 * dy)) + (m2 * dz));
+        r[i] = (float)(__builtin_fma(m2, oz, __builtin_fma(m1, oy, m0 * ox)) + m3);
+        r[3 + i] = (float)__builtin_fma(m2, dz, __builtin_fma(m1, dy, m0 * dx));
     }
     oo = f3{r[0], r[1], r[2]};
     od = f3{r[3], r[4], r[5]};
@@ -183,9 +188,12 @@ __device__ __forceinline__ WRay setupWRay(f3 o, f3 d)
     double dkz = (double)comp(d, kz);
     if (dkz < 0.0) { uint32_t t = kx; kx = ky; ky = t; }
     w.perm = kx | (ky << 2) | (kz << 4);
-    w.Sx = (double)comp(d, kx) / dkz;
-    w.Sy = (double)comp(d, ky) / dkz;
+    // one division: Sx and Sy are the direction's kx and ky scaled by the rounded 1 / d_kz
     w.Sz = 1.0 / dkz;
+    w.Sx = (double)comp(d, kx) * w.Sz;
+    w.Sy = (double)comp(d, ky) * w.Sz;
+    w.sx = (float)w.Sx;
+    w.sy = (float)w.Sy;
     w.ox = comp(o, kx);
     w.oy = comp(o, ky);
     w.oz = comp(o, kz);
@@ -213,6 +221,43 @@ __device__ __forceinline__ Sheared shear(const WRay& w, float4 v)
 struct TriHit {
     double t, det, V, W;
 };
+
+// FP32 pre-test of the watertight test: true when the edge functions U, V, W have
+// certainly mixed signs, which the FP64 test would reject.  Each sheared coordinate
+// is computed in float from the float vertex and origin and the float-rounded shear
+// (two roundings and the shear's own), so it lies within 3.01 u M of the exact value
+// (u = 2^-24, M = |v_k - o_k| + |S| |v_z - o_z|, computed alongside); an edge function
+// then lies within 8.1 u (Ma Mb' + Ma' Mb) of its exact value, and the FP64 one within
+// 2^-50 of that.  A sign counts only beyond 10 u (Ma Mb' + Ma' Mb) + 2^-126: tiny,
+// denormal, infinite or NaN quantities never decide, and the FP64 test runs.
+__device__ __forceinline__ bool triRejectF32(const WRay& w, const Tri& tr)
+{
+#pragma clang fp contract(off)
+    uint32_t kx = w.perm & 3u, ky = (w.perm >> 2) & 3u, kz = w.perm >> 4;
+    float x[3], y[3], mx[3], my[3];
+    const float4 v[3] = {tr.v0, tr.v1, tr.v2};
+    const float asx = fabsf(w.sx), asy = fabsf(w.sy);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const float dz = comp4(v[i], kz) - w.oz;
+        const float dx = comp4(v[i], kx) - w.ox, dy = comp4(v[i], ky) - w.oy;
+        x[i] = __builtin_fmaf(-w.sx, dz, dx);
+        y[i] = __builtin_fmaf(-w.sy, dz, dy);
+        mx[i] = __builtin_fmaf(asx, fabsf(dz), fabsf(dx));
+        my[i] = __builtin_fmaf(asy, fabsf(dz), fabsf(dy));
+    }
+    // U from (C, B), V from (A, C), W from (B, A), as the FP64 test forms them
+    const float U = __builtin_fmaf(x[2], y[1], -(y[2] * x[1]));
+    const float V = __builtin_fmaf(x[0], y[2], -(y[0] * x[2]));
+    const float W = __builtin_fmaf(x[1], y[0], -(y[1] * x[0]));
+    const float k = 10.0f * 5.9604645e-8f;
+    const float eU = __builtin_fmaf(k, __builtin_fmaf(mx[2], my[1], my[2] * mx[1]), 1.1754944e-38f);
+    const float eV = __builtin_fmaf(k, __builtin_fmaf(mx[0], my[2], my[0] * mx[2]), 1.1754944e-38f);
+    const float eW = __builtin_fmaf(k, __builtin_fmaf(mx[1], my[0], my[1] * mx[0]), 1.1754944e-38f);
+    const bool neg = U < -eU || V < -eV || W < -eW;
+    const bool pos = U > eU || V > eV || W > eW;
+    return neg && pos;
+}
 
 // Watertight test.  Accepts when U, V, W share a sign (zeros allowed), det != 0
 // and tmin <= t <= tmax for t = T / det.  Before dividing, |T| is compared with
@@ -408,11 +453,21 @@ __device__ __forceinline__ bool traceRound(TraceState& s, const TraceScene& sc, 
         cswap(k1, r1, k3, r3);
         cswap(k1, r1, k2, r2);
         // Far to near: the h - 1 farther hit children are pushed, the nearest goes on.
-        bool ok = true;
-        if (h > 3) ok = stackPush(s, m, r3);
-        if (h > 2) ok = ok && stackPush(s, m, r2);
-        if (h > 1) ok = ok && stackPush(s, m, r1);
-        if (!ok) { st.overflow = 1; return true; }
+        if (s.sp + 3 <= kLdsStack) {
+            // Room for three in the LDS ring, no spill: write the pushed run r_{h-1} .. r1
+            // to slots sp, sp+1, ... unconditionally (slots above the new top are free).
+            uint32_t* slot = m.lds + s.sp * kStackLanes;
+            slot[0] = (uint32_t)(h > 3 ? r3 : (h > 2 ? r2 : r1));
+            slot[kStackLanes] = (uint32_t)(h > 3 ? r2 : r1);
+            slot[2 * kStackLanes] = (uint32_t)r1;
+            s.sp += h > 1 ? h - 1 : 0;
+        } else {
+            bool ok = true;
+            if (h > 3) ok = stackPush(s, m, r3);
+            if (h > 2) ok = ok && stackPush(s, m, r2);
+            if (h > 1) ok = ok && stackPush(s, m, r1);
+            if (!ok) { st.overflow = 1; return true; }
+        }
         s.cur = h > 0 ? r0 : stackPop(s, m);
         if (s.cur < 0 && s.leaf == 0) {
             s.leaf = s.cur;
@@ -449,7 +504,7 @@ __device__ __forceinline__ bool traceRound(TraceState& s, const TraceScene& sc, 
                 Tri tr = gloadTri(s.tris + i);
                 if (STATS) st.tris++;
                 TriHit th;
-                if (!triWatertight(s.w, tr, dtmin, s.best.t, th)) continue;
+                if (triRejectF32(s.w, tr) || !triWatertight(s.w, tr, dtmin, s.best.t, th)) continue;
                 uint32_t prim = __float_as_uint(tr.v0.w);
                 if (ANY && s.transparent) {
                     // INSTANCE_SHADOW_ANY_HIT_SHADER (HitShaderEntryPoints.slang:489-553): an opaque

@@ -41,9 +41,10 @@ void ora_wray_setup(ora_wray* r, const double o[3], const double d[3], double tm
     int ky = kx + 1; if (ky == 3) ky = 0;
     if (d[kz] < 0.0) { int t = kx; kx = ky; ky = t; }
     r->kx = kx; r->ky = ky; r->kz = kz;
-    r->Sx = d[kx] / d[kz];
-    r->Sy = d[ky] / d[kz];
+    /* one division: the shear constants are d[kx] and d[ky] scaled by the rounded 1 / d[kz] */
     r->Sz = 1.0 / d[kz];
+    r->Sx = d[kx] * r->Sz;
+    r->Sy = d[ky] * r->Sz;
     for (int i = 0; i < 3; i++) {
         r->dz[i] = d[i] == 0.0;
         r->inv[i] = r->dz[i] ? 0.0 : 1.0 / d[i];
