@@ -48,7 +48,7 @@ __device__ __forceinline__ void flushCounters(const PathCounters& pc, uint32_t p
 
 // Megakernel: one lane per pixel, samples [start, start+count) in order, running
 // mean accumulated exactly like Accumulation.glsl:22-30.
-template <bool STATS>
+template <bool STATS, bool GROUND>
 __global__ __launch_bounds__(kBlock) void kRenderMega(RenderScene rs, aph_frame fp, uint32_t W, uint32_t H,
                                                      uint32_t start, uint32_t count, uint32_t seedOffset,
                                                      float4* accum, DevCounters* ctr)
@@ -66,7 +66,7 @@ __global__ __launch_bounds__(kBlock) void kRenderMega(RenderScene rs, aph_frame 
         size_t pix = (size_t)py * W + px;
         float4 acc = accum[pix];
         for (uint32_t s = start; s < start + count; s++) {
-            float4 r = pathSample<STATS>(rs, fp, W, H, px, py, s + seedOffset, stk, pc);
+            float4 r = pathSample<STATS, GROUND>(rs, fp, W, H, px, py, s + seedOffset, stk, pc);
             if (s > 0) {
                 float t = 1.0f / (float)(s + 1);
                 float u = 1.0f - t;
@@ -143,8 +143,15 @@ hipError_t launchRenderMega(const RenderScene& rs, const aph_frame& fp, uint32_t
 {
     uint32_t tiles = ((W + 15) / 16) * ((H + 7) / 8);
     if (tiles == 0 || count == 0) return hipSuccess;
-    if (stats) kRenderMega<true><<<tiles, kBlock, 0, st>>>(rs, fp, W, H, start, count, seedOffset, accum, ctr);
-    else kRenderMega<false><<<tiles, kBlock, 0, st>>>(rs, fp, W, H, start, count, seedOffset, accum, ctr);
+    const bool ground = rs.ground.enabled != 0;
+    if (stats && ground)
+        kRenderMega<true, true><<<tiles, kBlock, 0, st>>>(rs, fp, W, H, start, count, seedOffset, accum, ctr);
+    else if (stats)
+        kRenderMega<true, false><<<tiles, kBlock, 0, st>>>(rs, fp, W, H, start, count, seedOffset, accum, ctr);
+    else if (ground)
+        kRenderMega<false, true><<<tiles, kBlock, 0, st>>>(rs, fp, W, H, start, count, seedOffset, accum, ctr);
+    else
+        kRenderMega<false, false><<<tiles, kBlock, 0, st>>>(rs, fp, W, H, start, count, seedOffset, accum, ctr);
     return hipGetLastError();
 }
 

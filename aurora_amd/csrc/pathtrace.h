@@ -198,8 +198,10 @@ __device__ __forceinline__ f3 shadeDirectional(const aph_distant_light& l, const
     return f3{l.color_intensity[0], l.color_intensity[1], l.color_intensity[2]} * l.color_intensity[3] * f;
 }
 
-// The whole path of one pixel-sample; returns RGBA before accumulation.
-template <bool STATS>
+// The whole path of one pixel-sample; returns RGBA before accumulation. GROUND compiles
+// the ground-plane step in: a scene without a ground plane runs the variant without it,
+// whose register pressure (and so its spills) the plane's state does not raise.
+template <bool STATS, bool GROUND>
 __device__ float4 pathSample(const RenderScene& rs, const aph_frame& fp, uint32_t W, uint32_t H, uint32_t px,
                              uint32_t py, uint32_t sampleIndex, const StackMem& stk, PathCounters& pc)
 {
@@ -235,7 +237,7 @@ __device__ float4 pathSample(const RenderScene& rs, const aph_frame& fp, uint32_
             // evaluateGroundPlane (GroundPlane.slang:232-290), first in the hit and miss programs
             f3 gpos;
             float gdist;
-            if (isPrimary && rs.ground.enabled && groundPlaneHit(rs.ground, org, dir, gpos, gdist) &&
+            if (GROUND && isPrimary && rs.ground.enabled && groundPlaneHit(rs.ground, org, dir, gpos, gdist) &&
                 gdist < (h ? (float)hit.t : INFINITY)) {
                 f3 gcol, gdir, sdir;
                 bool gshadow;
