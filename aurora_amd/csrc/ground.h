@@ -23,6 +23,15 @@ struct DevGround {
     float reflectionRoughness;
 };
 
+// A plane with neither a reflection nor a shadow changes nothing: its setup draws no
+// random number and scales the path by exactly 1.  Such a plane runs the kernels of a
+// scene without one (the render kernels are compiled with and without the plane's
+// step, and the two compilations need not round alike).
+inline bool groundActive(const DevGround& g)
+{
+    return g.enabled != 0 && (g.reflectionOpacity > 0.0f || g.shadowOpacity > 0.0f);
+}
+
 // intersectGroundPlane (GroundPlane.slang:104-128): the plane faces its normal; rays
 // travelling along it, or with the plane behind them, miss.
 APH_HD bool groundPlaneHit(const DevGround& g, f3 org, f3 dir, f3& pos, float& dist)

@@ -143,7 +143,7 @@ hipError_t launchRenderMega(const RenderScene& rs, const aph_frame& fp, uint32_t
 {
     uint32_t tiles = ((W + 15) / 16) * ((H + 7) / 8);
     if (tiles == 0 || count == 0) return hipSuccess;
-    const bool ground = rs.ground.enabled != 0;
+    const bool ground = groundActive(rs.ground);
     if (stats && ground)
         kRenderMega<true, true><<<tiles, kBlock, 0, st>>>(rs, fp, W, H, start, count, seedOffset, accum, ctr);
     else if (stats)

@@ -1339,7 +1339,7 @@ hipError_t wavefrontRender(WavefrontState& st, const WavefrontRequest* reqs, uin
         r.accum = reqs[i].accum;
         std::memcpy(img + kCallHdrBytes + sizeof(DevReq) * i, &r, sizeof(DevReq));
         maxDepth = std::max<uint32_t>(maxDepth, (uint32_t)reqs[i].frame.trace_depth);
-        ground = ground || reqs[i].rs.ground.enabled != 0;
+        ground = ground || groundActive(reqs[i].rs.ground);
     }
     std::memcpy(img + ch.descOff, plan.desc.data(), sizeof(ChunkDesc) * plan.desc.size());
     std::memcpy(img + ch.segOff, plan.segs.data(), sizeof(SegInfo) * plan.segs.size());
