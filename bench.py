@@ -790,8 +790,14 @@ def main():
                 e["valu_lanes_active"] = round(c["SQ_THREAD_CYCLES_VALU"] / c["SQ_ACTIVE_INST_VALU"], 1)
             e["counters_per_launch"] = {n: int(c[n]) for n in ISSUE_COUNTERS if n in c}
         if "hbm_frac" in e and "valu_issue_frac" in e:
-            e["limiter"] = ("VALU issue (divergent waves)" if e["valu_issue_frac"] > e["hbm_frac"]
-                            else "HBM bandwidth")
+            # a resource more than 60 % busy is the limiter; below that on both, the
+            # waves mostly wait on memory latency
+            if e["valu_issue_frac"] >= 0.6 and e["valu_issue_frac"] >= e["hbm_frac"]:
+                e["limiter"] = "VALU issue (divergent waves)"
+            elif e["hbm_frac"] >= 0.6:
+                e["limiter"] = "HBM bandwidth"
+            else:
+                e["limiter"] = "memory latency (VALU and HBM both under 60 % busy)"
         kernels[k] = e
     dom = kernels[kname]
     avg_launch_ms = dom["avg_launch_ms"]
