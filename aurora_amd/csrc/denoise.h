@@ -16,7 +16,10 @@ constexpr int kDnLossL2Tonemapped = 1; // L2 of x / (1 + x)
 // 3x3 convolution, stride 1, zero padding 1, NHWC.  Input channels [0, cinP)
 // of a buffer with stride inStride (cinP a multiple of 32); weights
 // [9][coutP][cinP] bf16 (coutP a multiple of 16); output channels [0,
-// coutStore) written at outOffset of a buffer with stride outStride.
+// coutStore) written at outOffset of a buffer with stride outStride.  With
+// pool set (bf16 output, even H and W), the 2x2 average pooling of the stored
+// output is also written at poolOffset of a buffer of (H/2) x (W/2) pixels with
+// stride poolStride, exactly as launchAvgPool computes it from the output.
 struct ConvArgs {
     const __bf16* in;
     const __bf16* w;
@@ -25,6 +28,8 @@ struct ConvArgs {
     float* outF;        // fp32 output
     int N, H, W;
     int inStride, cinP, coutP, coutStore, outStride, outOffset;
+    __bf16* pool;       // optional pooled output
+    int poolStride, poolOffset;
 };
 
 // dW[9][coutP][cinP] += sum over pixels of dY (x) shifted X; db[coutP] +=

@@ -145,6 +145,64 @@ __global__ __launch_bounds__(256) void kConv3x3(ConvArgs a)
     }
 }
 
+// Fused 2x2 average pooling of a convolution's stored output.  Rows y0 + i and
+// y0 + i + 1 (i even) are two accumulators of the same lane and columns x, x + 1
+// are neighbouring lanes (swapped with a DPP quad permutation), so each even lane
+// forms one pooled pixel from the bf16-rounded outputs, summed in the order of
+// kAvgPool2 ((y, x), (y, x+1), (y+1, x), (y+1, x+1)) and rounded once.
+template <bool RELU, int P, int T>
+__device__ __forceinline__ void poolQuads(const ConvArgs& a, const f32x4 (&acc)[P][T], int nTiles, int coBase, int n,
+                                          int y0, int x, int lane, int kg)
+{
+    static_assert(P % 2 == 0, "row pairs within a wave");
+    const int Ho = a.H / 2, Wo = a.W / 2;
+#pragma unroll
+    for (int j = 0; j < T; j++) {
+        if (j >= nTiles) continue;
+        const int co = coBase + j * 16 + kg * 4;
+        float b[4];
+#pragma unroll
+        for (int e = 0; e < 4; e++) b[e] = a.bias ? a.bias[co + e] : 0.0f;
+#pragma unroll
+        for (int i = 0; i < P; i += 2) {
+            float s[4];
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                float v0 = acc[i][j][e] + b[e], v1 = acc[i + 1][j][e] + b[e];
+                if (RELU) {
+                    v0 = fmaxf(v0, 0.0f);
+                    v1 = fmaxf(v1, 0.0f);
+                }
+                v0 = fromBf(toBf(v0));
+                v1 = fromBf(toBf(v1));
+                // quad_perm [1, 0, 3, 2]: the value of lane l ^ 1
+                const float w0 = __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v0), __float_as_int(v0),
+                                                                            0xB1, 0xF, 0xF, false));
+                const float w1 = __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v1), __float_as_int(v1),
+                                                                            0xB1, 0xF, 0xF, false));
+                float t = 0.0f;
+                t += v0;
+                t += w0;
+                t += v1;
+                t += w1;
+                s[e] = t;
+            }
+            const int y = y0 + i;
+            if ((lane & 1) == 0 && y < a.H && x < a.W && co < a.coutStore) {
+                __bf16* q = a.pool + ((int64_t)(n * Ho + y / 2) * Wo + x / 2) * a.poolStride + a.poolOffset + co;
+                if (co + 4 <= a.coutStore) {
+                    *reinterpret_cast<bf16x4*>(q) =
+                        bf16x4{toBf(0.25f * s[0]), toBf(0.25f * s[1]), toBf(0.25f * s[2]), toBf(0.25f * s[3])};
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 4; e++)
+                        if (co + e < a.coutStore) q[e] = toBf(0.25f * s[e]);
+                }
+            }
+        }
+    }
+}
+
 // ---- convolution through LDS (32 or 64 output channels) ----
 // Block: 4 waves over a tile of 4P rows x 16 columns of one image and 16*T
 // output channels.  For each chunk of 32 input channels the block stages the
@@ -241,6 +299,7 @@ __global__ __launch_bounds__(256) void kConvLds(ConvArgs a)
     // pixels of tile row wave*P + i; lane (row, kg) holds channels 4kg..4kg+3 of
     // pixel tx0 + row.
     const int x = tx0 + row;
+    if (!OUT_F32 && a.pool) poolQuads<RELU, P, T>(a, acc, nTiles, coBase, n, ty0 + wave * P, x, lane, kg);
 #pragma unroll
     for (int j = 0; j < T; j++) {
         if (j >= nTiles) continue;
@@ -646,9 +705,19 @@ hipError_t launchConv(const ConvArgs& a, bool relu, bool outF32, hipStream_t st)
     // wider layers: 2 pixel x 4 channel subtiles per wave from global memory.
     // Measured on a 1080p frame against 32 x 16 tiles for 32 channels and 2 x 8
     // subtiles: 5 % faster full-resolution layers, 10 % faster wide ones
+    if (a.pool && (outF32 || (a.H & 1) || (a.W & 1))) return hipErrorInvalidValue;
     if (a.coutP <= 32) convLdsLaunch<4, 2>(a, relu, outF32, st);
     else if (a.coutP <= 64) convLdsLaunch<4, 4>(a, relu, outF32, st);
-    else convLaunch<2, 4>(a, relu, outF32, st);
+    else {
+        // pooling as its own pass after the global-memory kernel
+        convLaunch<2, 4>(a, relu, outF32, st);
+        if (a.pool) {
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return e;
+            return launchAvgPool(a.out, a.outStride, a.outOffset, a.pool, a.poolStride, a.poolOffset, a.N, a.H, a.W,
+                                 a.coutStore, st);
+        }
+    }
     return hipGetLastError();
 }
 

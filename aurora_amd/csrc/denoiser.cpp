@@ -248,6 +248,18 @@ hipError_t forward(aph_denoiser* d, hipStream_t st)
                 a.outStride = kBufs[L.out].channels;
                 a.outOffset = L.outOff;
             }
+            // A pooling of exactly this output right after it is fused into the
+            // convolution's epilogue (the same bf16 result as the separate pass).
+            if (!isPred && i + 1 < kNumOps) {
+                const aph_dn_op& nx = kOps[i + 1];
+                if (nx.kind == APH_DN_OP_POOL && nx.src == L.out && nx.src_offset == L.outOff &&
+                    nx.channels == L.cout) {
+                    a.pool = d->act(nx.dst);
+                    a.poolStride = kBufs[nx.dst].channels;
+                    a.poolOffset = nx.dst_offset;
+                    i++;
+                }
+            }
             e = launchConv(a, L.relu, isPred, st);
         } else if (op.kind == APH_DN_OP_POOL) {
             const int lv = kBufs[op.src].level;
