@@ -30,6 +30,11 @@ struct ConvArgs {
     int inStride, cinP, coutP, coutStore, outStride, outOffset;
     __bf16* pool;       // optional pooled output
     int poolStride, poolOffset;
+    // Optional nearest 2x upsampling folded into the input: with in2 set, input
+    // channels [in2From, cinP) (in2From a multiple of 32) are read from in2, a
+    // buffer of (H/2) x (W/2) pixels with stride in2Stride, at pixel (y/2, x/2).
+    const __bf16* in2;
+    int in2Stride, in2From;
 };
 
 // dW[9][coutP][cinP] += sum over pixels of dY (x) shifted X; db[coutP] +=

@@ -103,18 +103,24 @@ __global__ __launch_bounds__(256) void kConv3x3(ConvArgs a)
     for (int t = 0; t < 9; t++) {
         const int ty = t / 3 - 1, tx = t % 3 - 1;
         const __bf16* src[P];
+        const __bf16* src2[P];
         bool ok[P];
 #pragma unroll
         for (int i = 0; i < P; i++) {
             const int sy = y[i] + ty, sx = x[i] + tx;
             ok[i] = sy >= 0 && sy < a.H && sx >= 0 && sx < a.W;
-            src[i] = a.in + (((int64_t)n[i] * a.H + (ok[i] ? sy : 0)) * a.W + (ok[i] ? sx : 0)) * a.inStride + kg * 8;
+            const int cy = ok[i] ? sy : 0, cx = ok[i] ? sx : 0;
+            src[i] = a.in + (((int64_t)n[i] * a.H + cy) * a.W + cx) * a.inStride + kg * 8;
+            // the upsampled channels: pixel (y/2, x/2) of the half-resolution buffer
+            src2[i] = a.in2 ? a.in2 + (((int64_t)n[i] * (a.H / 2) + cy / 2) * (a.W / 2) + cx / 2) * a.in2Stride + kg * 8
+                            : src[i];
         }
+        const int split = a.in2 ? a.in2From : a.cinP;
         const __bf16* wt = a.w + ((int64_t)t * a.coutP + coBase + row) * a.cinP + kg * 8;
         for (int c0 = 0; c0 < a.cinP; c0 += 32) {
             bf16x8 av[P], bv[T];
 #pragma unroll
-            for (int i = 0; i < P; i++) av[i] = ok[i] ? load8(src[i] + c0) : bf16x8{};
+            for (int i = 0; i < P; i++) av[i] = ok[i] ? load8(c0 < split ? src[i] + c0 : src2[i] + (c0 - split)) : bf16x8{};
 #pragma unroll
             for (int j = 0; j < T; j++) bv[j] = j < nTiles ? load8(wt + (int64_t)j * 16 * a.cinP + c0) : bf16x8{};
 #pragma unroll
@@ -246,8 +252,13 @@ __global__ __launch_bounds__(256) void kConvLds(ConvArgs a)
             const int hy = px / HC, hx = px - hy * HC;
             const int gy = ty0 + hy - 1, gx = tx0 + hx - 1;
             sv[k] = bf16x8{};
-            if (sgi < NSEG && gy >= 0 && gy < a.H && gx >= 0 && gx < a.W)
-                sv[k] = load8(a.in + (((int64_t)n * a.H + gy) * a.W + gx) * a.inStride + c0 + part * 8);
+            if (sgi < NSEG && gy >= 0 && gy < a.H && gx >= 0 && gx < a.W) {
+                if (a.in2 && c0 >= a.in2From)
+                    sv[k] = load8(a.in2 + (((int64_t)n * (a.H / 2) + gy / 2) * (a.W / 2) + gx / 2) * a.in2Stride +
+                                  (c0 - a.in2From) + part * 8);
+                else
+                    sv[k] = load8(a.in + (((int64_t)n * a.H + gy) * a.W + gx) * a.inStride + c0 + part * 8);
+            }
         }
 #pragma unroll
         for (int k = 0; k < NWIT; k++) {

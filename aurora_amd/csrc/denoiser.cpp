@@ -219,16 +219,34 @@ hipError_t refreshWeights(aph_denoiser* d, hipStream_t st)
     return hipSuccess;
 }
 
-hipError_t forward(aph_denoiser* d, hipStream_t st)
+// Inference folds each upsampling into the decoder convolution that reads it
+// (the upsampled channels of its input are read from the lower level directly and
+// never stored); training stores them, as its gradients read them.
+hipError_t forward(aph_denoiser* d, hipStream_t st, bool inference)
 {
     const Offsets& o = offsets();
+    const aph_dn_op* up = nullptr;  // an upsampling folded into the next convolution
     for (int i = 0; i < kNumOps; i++) {
         const aph_dn_op& op = kOps[i];
         hipError_t e = hipSuccess;
+        if (op.kind == APH_DN_OP_UPSAMPLE && inference && i + 1 < kNumOps && kOps[i + 1].kind == APH_DN_OP_CONV) {
+            const LayerDef& Ln = kLayers[kOps[i + 1].layer];
+            if (Ln.in == op.dst && op.dst_offset % 32 == 0 && op.dst_offset + op.channels == Ln.cinP &&
+                op.src_offset == 0) {
+                up = &op;
+                continue;
+            }
+        }
         if (op.kind == APH_DN_OP_CONV) {
             const LayerDef& L = kLayers[op.layer];
             const int lv = kBufs[L.in].level;
             ConvArgs a{};
+            if (up) {
+                a.in2 = d->act(up->src);
+                a.in2Stride = kBufs[up->src].channels;
+                a.in2From = up->dst_offset;
+                up = nullptr;
+            }
             a.in = d->act(L.in);
             a.w = d->wf.as<__bf16>() + o.wb[op.layer];
             a.bias = d->params.as<float>() + o.b[op.layer];
@@ -603,7 +621,7 @@ int aph_denoiser_train_step(aph_ctx_t c, uint64_t dn, const aph_denoise_example*
     TimedScope ts(c, "denoiser_train_step");
     HIPCALL(c, reserveShape(d, (int)n, padded(H), padded(W), true), "alloc denoiser activations");
     HIPCALL(c, packInputs(d, tg, av, W, H, xf, st), "denoiser pack");
-    HIPCALL(c, forward(d, st), "denoiser forward");
+    HIPCALL(c, forward(d, st, false), "denoiser forward");
     HIPCALL(c, hipMemsetAsync(d->loss.p, 0, 4, st), "clear loss");
     HIPCALL(c, hipMemsetAsync(d->grads.p, 0, o.nParams * 4, st), "clear gradients");
     for (uint32_t i = 0; i < n; i++)
@@ -673,7 +691,7 @@ int aph_denoise(aph_ctx_t c, uint64_t dn, const aph_denoise_item* items, uint32_
     TimedScope ts(c, "denoise");
     HIPCALL(c, reserveShape(d, (int)nb, padded(H), padded(W), false), "alloc denoiser activations");
     HIPCALL(c, packInputs(d, btg, bav, W, H, bxf, st), "denoiser pack");
-    HIPCALL(c, forward(d, st), "denoiser forward");
+    HIPCALL(c, forward(d, st, true), "denoiser forward");
     for (uint32_t b = 0, k = 0; b < nb; b++) {
         const uint32_t i = bitem[b];
         k = (b > 0 && bitem[b - 1] == i) ? k + 1 : 0;

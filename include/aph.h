@@ -594,7 +594,10 @@ int aph_denoiser_set_params(aph_ctx_t ctx, uint64_t denoiser, const float* param
 enum {
     APH_DN_PARAMS = 0,           /* fp32 parameter vector */
     APH_DN_PARAM_GRADS = 1,      /* its gradient from the last training step */
-    APH_DN_ACTIVATIONS = 2,      /* buffer `index` of the last forward pass, NHWC */
+    APH_DN_ACTIVATIONS = 2,      /* buffer `index` of the last forward pass, NHWC; after
+                                  * aph_denoise the channels an upsampling op writes are
+                                  * not stored (the decoder convolutions read the lower
+                                  * level directly), after a training step they are */
     APH_DN_ACTIVATION_GRADS = 3, /* its gradient from the last training step */
     APH_DN_PREDICTION = 4        /* fp32 [n][Hp][Wp][4] predicted log1p radiance */
 };

@@ -128,10 +128,10 @@ def test_forward_every_layer_matches_torch(renderer, setup):
             got = acts[op.dst][..., op.dst_offset:op.dst_offset + op.channels]
             np.testing.assert_allclose(got, ref, rtol=BF, atol=1e-6)
         else:
+            # aph_denoise does not store upsampled channels: the decoder convolution
+            # reads the lower level itself, so its check runs on the upsampled input
             src = acts[op.src][..., op.src_offset:op.src_offset + op.channels]
-            ref = src.repeat(2, axis=1).repeat(2, axis=2)
-            got = acts[op.dst][..., op.dst_offset:op.dst_offset + op.channels]
-            np.testing.assert_array_equal(got, ref)
+            acts[op.dst][..., op.dst_offset:op.dst_offset + op.channels] = src.repeat(2, axis=1).repeat(2, axis=2)
     assert n_conv == len(dn.layers) == 19
     # output image: expm1 of the clamped prediction, alpha from the noisy input
     for k, (nz, out) in enumerate(((na, out_a), (nb, out_b))):
@@ -139,6 +139,33 @@ def test_forward_every_layer_matches_torch(renderer, setup):
         ref = np.expm1(np.maximum(pred[k, :h, :w, :3], 0))
         np.testing.assert_allclose(got[..., :3], ref, rtol=1e-5, atol=1e-6)
         np.testing.assert_array_equal(got[..., 3], r.readback(nz, w, h)[..., 3])
+    dn.close()
+
+
+def test_inference_equals_the_training_forward_pass(renderer, setup):
+    """aph_denoise folds each upsampling into the decoder convolution that reads it;
+    a training step runs the upsamplings as their own ops and stores them.  Both
+    forward passes give the same prediction bit for bit, and the training pass's
+    stored upsampled channels are the nearest upsampling of their source."""
+    r, w, h = renderer, setup["w"], setup["h"]
+    dn = Denoiser(r, seed=13)
+    (na, aa), (nb, ab) = setup["inputs"]
+    ra, rb = setup["refs"]
+    dn.train_step([(na, aa, ra), (nb, ab, rb)], lr=0.0)
+    hp, wp = padded(h), padded(w)
+    trained = dn.tensor(capi.APH_DN_PREDICTION)
+    acts = {i: dn.activation(i, 2, hp, wp) for i in range(len(dn.buffers) - 1)}
+    n_up = 0
+    for op in dn.ops:
+        if op.kind == capi.APH_DN_OP_UPSAMPLE:
+            src = acts[op.src][..., op.src_offset:op.src_offset + op.channels]
+            got = acts[op.dst][..., op.dst_offset:op.dst_offset + op.channels]
+            np.testing.assert_array_equal(got, src.repeat(2, axis=1).repeat(2, axis=2))
+            n_up += 1
+    assert n_up == 4
+    oa, ob = r.target(w, h), r.target(w, h)
+    dn.denoise([(na, aa, oa), (nb, ab, ob)])
+    np.testing.assert_array_equal(dn.tensor(capi.APH_DN_PREDICTION), trained)
     dn.close()
 
 
