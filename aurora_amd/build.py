@@ -14,7 +14,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libaph.so")
 ARCH = os.environ.get("APH_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["api.cpp", "server.cpp", "assets.cpp", "guides.cpp", "denoiser.cpp", "alias.cpp", "bvh.hip", "sah.hip", "kernels.hip",
+SOURCES = ["api.cpp", "server.cpp", "assets.cpp", "gltf.cpp", "guides.cpp", "denoiser.cpp", "alias.cpp", "bvh.hip", "sah.hip", "kernels.hip",
            "wavefront.hip", "temporal.hip", "denoise.hip"]
 
 DEVICE_FLAGS = [

@@ -119,6 +119,29 @@ class ObjFile(C.Structure):
                 ("textures", C.POINTER(C.c_char_p))]
 
 
+class GltfMesh(C.Structure):
+    _fields_ = [("nv", C.c_uint32), ("nt", C.c_uint32), ("positions", C.POINTER(C.c_float)),
+                ("normals", C.POINTER(C.c_float)), ("uvs", C.POINTER(C.c_float)),
+                ("indices", C.POINTER(C.c_uint32))]
+
+
+class GltfInstance(C.Structure):
+    _fields_ = [("mesh", C.c_uint32), ("material", C.c_int32), ("o2w", C.c_float * 12), ("node", C.c_int32),
+                ("materialx", C.c_char_p)]
+
+
+class GltfCamera(C.Structure):
+    _fields_ = [("type", C.c_int32), ("node", C.c_int32), ("view", C.c_float * 16), ("yfov", C.c_float),
+                ("aspect", C.c_float), ("znear", C.c_float), ("zfar", C.c_float), ("xmag", C.c_float),
+                ("ymag", C.c_float)]
+
+
+class GltfFile(C.Structure):
+    _fields_ = [("nmesh", C.c_uint32), ("ninst", C.c_uint32), ("nmat", C.c_uint32), ("ncam", C.c_uint32),
+                ("meshes", C.POINTER(GltfMesh)), ("instances", C.POINTER(GltfInstance)),
+                ("materials", C.POINTER(Material)), ("cameras", C.POINTER(GltfCamera))]
+
+
 class Stats(C.Structure):
     _fields_ = [("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("node_visits", C.c_uint64),
                 ("tri_tests", C.c_uint64), ("inst_visits", C.c_uint64), ("paths", C.c_uint64),
@@ -297,6 +320,8 @@ SIGNATURES = {
     "aph_tlas_update": (_i, [_P, _u64, _P, _u32, _i]),
     "aph_obj_load": (_i, [C.c_char_p, C.POINTER(C.POINTER(ObjFile))]),
     "aph_obj_free": (None, [C.POINTER(ObjFile)]),
+    "aph_gltf_load": (_i, [C.c_char_p, C.POINTER(C.POINTER(GltfFile))]),
+    "aph_gltf_free": (None, [C.POINTER(GltfFile)]),
     "aph_hdr_load": (_i, [C.c_char_p, C.POINTER(C.POINTER(C.c_float)), C.POINTER(_u32), C.POINTER(_u32)]),
     "aph_host_free": (None, [_P]),
     "aph_materials_update": (_i, [_P, _u32, _P, _u32]),
@@ -388,6 +413,55 @@ def load_obj(path):
         return dict(shapes=shapes, materials=mats, material_names=names, textures=tex)
     finally:
         lib.aph_obj_free(f)
+
+
+def load_gltf(path):
+    """Parses a glTF 2.0 file (.gltf or .glb) with aph_gltf_load.
+
+    Returns a dict: ``meshes`` - list of dicts with float32 ``positions`` (nv, 3),
+    ``normals`` (nv, 3), ``uvs`` (nv, 2) or None and uint32 ``indices`` (nt, 3);
+    ``instances`` - list of dicts with ``mesh``, ``material`` (-1: default), float32
+    ``o2w`` (12,), ``node`` and ``materialx`` (path or None); ``materials`` - list of
+    Material; ``cameras`` - list of dicts with ``type`` (0 perspective, 1 orthographic),
+    ``node``, float32 ``view`` (4, 4) row-major world-to-view and the projection fields."""
+    lib = load()
+    f = C.POINTER(GltfFile)()
+    rc = lib.aph_gltf_load(os.fsencode(path), C.byref(f))
+    if rc != APH_OK:
+        raise RuntimeError(f"aph_gltf_load({path!r}) failed: {rc}")
+    try:
+        g = f.contents
+
+        def arr(p, n, c, dt):
+            return np.ctypeslib.as_array(p, shape=(n, c)).astype(dt, copy=True) if n else np.zeros((0, c), dt)
+
+        meshes = []
+        for k in range(g.nmesh):
+            m = g.meshes[k]
+            meshes.append(dict(positions=arr(m.positions, m.nv, 3, np.float32),
+                               normals=arr(m.normals, m.nv, 3, np.float32),
+                               uvs=arr(m.uvs, m.nv, 2, np.float32) if m.uvs else None,
+                               indices=arr(m.indices, m.nt, 3, np.uint32)))
+        insts = []
+        for k in range(g.ninst):
+            i = g.instances[k]
+            insts.append(dict(mesh=int(i.mesh), material=int(i.material), node=int(i.node),
+                              o2w=np.array(i.o2w[:], dtype=np.float32),
+                              materialx=i.materialx.decode() if i.materialx else None))
+        mats = []
+        for k in range(g.nmat):
+            m = Material()
+            C.memmove(C.byref(m), C.byref(g.materials[k]), C.sizeof(Material))
+            mats.append(m)
+        cams = []
+        for k in range(g.ncam):
+            c = g.cameras[k]
+            cams.append(dict(type=int(c.type), node=int(c.node),
+                             view=np.array(c.view[:], dtype=np.float32).reshape(4, 4), yfov=c.yfov,
+                             aspect=c.aspect, znear=c.znear, zfar=c.zfar, xmag=c.xmag, ymag=c.ymag))
+        return dict(meshes=meshes, instances=insts, materials=mats, cameras=cams)
+    finally:
+        lib.aph_gltf_free(f)
 
 
 def load_hdr(path):

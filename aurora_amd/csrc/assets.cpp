@@ -37,6 +37,7 @@
 #include <vector>
 
 #include "../../include/aph.h"
+#include "assets.h"
 
 namespace {
 
@@ -111,6 +112,27 @@ bool parseMtl(const std::string& path, std::vector<ObjMaterial>& mats, std::map<
     return true;
 }
 
+struct ShapeBuild {
+    std::string name;
+    int material = -1;
+    bool materialSet = false;
+    std::vector<std::tuple<int, int, int>> corners;  // (v, vt, vn), 0-based, -1 = none; 3 per triangle
+};
+
+struct ObjFile {
+    aph_obj_file pub{};
+    std::vector<aph_obj_shape> shapes;
+    std::vector<aph_material> materials;
+    std::vector<std::vector<float>> pos, nrm, uv;
+    std::vector<std::vector<uint32_t>> idx;
+    std::vector<std::string> names, textures;
+    std::vector<const char*> namePtr, texturePtr;
+};
+
+}  // namespace
+
+namespace aph {
+
 // Foundation::calculateNormals (Geometry.cpp:29-101): per face, the normal of
 // cross(p2 - p1, p3 - p1) weighted at each corner by acos of the edge directions' dot
 // product as that code forms it, summed per vertex and normalized.
@@ -153,24 +175,9 @@ void angleWeightedNormals(const std::vector<float>& pos, const std::vector<uint3
     for (size_t v = 0; v < out.size(); v += 3) norm(out[v], out[v + 1], out[v + 2]);
 }
 
-struct ShapeBuild {
-    std::string name;
-    int material = -1;
-    bool materialSet = false;
-    std::vector<std::tuple<int, int, int>> corners;  // (v, vt, vn), 0-based, -1 = none; 3 per triangle
-};
+}  // namespace aph
 
-struct ObjFile {
-    aph_obj_file pub{};
-    std::vector<aph_obj_shape> shapes;
-    std::vector<aph_material> materials;
-    std::vector<std::vector<float>> pos, nrm, uv;
-    std::vector<std::vector<uint32_t>> idx;
-    std::vector<std::string> names, textures;
-    std::vector<const char*> namePtr, texturePtr;
-};
-
-}  // namespace
+using aph::angleWeightedNormals;
 
 extern "C" {
 

@@ -645,6 +645,69 @@ def obj_scene(obj_path, hdr_path=None, width=128, height=128, spp=4, trace_depth
     return sc
 
 
+def camera_frame(cam, width, height, trace_depth=5):
+    """aph_frame of a glTF camera (aph_gltf_camera): the inverse of its view matrix as the
+    camera-to-world transform; a perspective yfov (its aspect ratio, or the image's when
+    the file gives none), or an orthographic xmag / ymag."""
+    fr = look_at_frame((0, 0, 1), (0, 0, 0), (0, 1, 0), 45.0, width / height, trace_depth)
+    c2w = np.linalg.inv(np.asarray(cam["view"], dtype=np.float64))
+    for c in range(4):
+        for r in range(4):
+            fr.inv_view[4 * c + r] = float(np.float32(c2w[r, c]))
+    if cam["type"] == 1:
+        fr.is_ortho = 1
+        fr.view_size[0] = float(np.float32(2.0 * cam["xmag"]))
+        fr.view_size[1] = float(np.float32(2.0 * cam["ymag"]))
+    else:
+        aspect = cam["aspect"] if cam["aspect"] > 0 else width / height
+        f = 1.0 / math.tan(cam["yfov"] / 2.0)
+        p00, p11 = np.float32(f / aspect), np.float32(f)
+        fr.view_size[0] = float(np.float32(2.0) / p00)
+        fr.view_size[1] = float(np.float32(2.0) / p11)
+    return fr
+
+
+def gltf_scene(gltf_path, hdr_path=None, width=128, height=128, spp=4, trace_depth=5, camera=None,
+               eye=(0.0, 0.0, 5.0), target=(0.0, 0.0, 0.0), fov_y_deg=45.0):
+    """A glTF scene as Plasma loads one (Applications/Plasma/glTFLoader.cpp): one mesh per
+    (mesh, triangle primitive), one instance per node using it at the node's world
+    transform, the glTF materials translated to Standard Surface (primitives without one
+    get the default material, appended last); ``camera`` picks one of the file's cameras,
+    otherwise the look-at camera; ``hdr_path`` as in obj_scene."""
+    g = capi.load_gltf(gltf_path)
+    sc = Scene()
+    sc.name = os.path.basename(gltf_path)
+    sc.materials = list(g["materials"])
+    for k, m in enumerate(g["meshes"]):
+        sc.meshes.append((m["positions"], m["normals"], m["indices"]))
+        if m["uvs"] is not None:
+            sc.uvs[k] = m["uvs"]
+    default = None
+    inst = []
+    for i in g["instances"]:
+        mat = i["material"]
+        if mat < 0:
+            if default is None:
+                default = len(sc.materials)
+                sc.materials.append(capi.default_material())
+            mat = default
+        inst.append((i["mesh"], mat, i["o2w"]))
+    if not inst:
+        raise ValueError(f"{gltf_path}: no triangles")
+    sc.instances = _instances(inst)
+    sc.cameras = g["cameras"]
+    sc.env = dict()
+    if hdr_path is not None:
+        rgb = capi.load_hdr(hdr_path)
+        sc.env = dict(light_rgb=rgb, background_rgb=rgb)
+    sc.width, sc.height, sc.spp = width, height, spp
+    if camera is not None:
+        sc.frame = camera_frame(g["cameras"][camera], width, height, trace_depth)
+    else:
+        sc.frame = look_at_frame(eye, target, (0, 1, 0), fov_y_deg, width / height, trace_depth)
+    return sc
+
+
 def decal_scene(path, width=96, height=96, spp=4, trace_depth=3, seed=17):
     """A textured decal (the DirectX layer loop with layer geometry, HitShaderEntryPoints.slang:84-113,
     GlobalBufferAccessors.slang:116-127): the fixture teapot, its base material checkered

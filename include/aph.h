@@ -531,9 +531,10 @@ int aph_server_replay(aph_server_t s, const aph_job* jobs, const double* arrival
 
 
 /* ---- on-disk inputs (host only, no context): the Wavefront OBJ + MTL translation of
- * Plasma's loader (Applications/Plasma/OBJLoader.cpp:77-470, tinyobjloader's parsing) and
+ * Plasma's loader (Applications/Plasma/OBJLoader.cpp:77-470, tinyobjloader's parsing), the
+ * glTF 2.0 translation of its glTF loader (glTFLoader.cpp:26-386, tinygltf's parsing) and
  * the Radiance HDR lat-long environments Plasma loads (Plasma.cpp:69, :1341).  Texture
- * maps are reported by path, not decoded.  glTF and MaterialX inputs are not read. ---- */
+ * maps are reported by path, not decoded; MaterialX documents are reported, not read. ---- */
 typedef struct {
     uint32_t nv, nt;
     const float* positions;     /* 3 nv */
@@ -554,6 +555,39 @@ typedef struct {
 /* Parses an OBJ file and the MTL libraries it names; *out stays valid until aph_obj_free. */
 int aph_obj_load(const char* path, aph_obj_file** out);
 void aph_obj_free(aph_obj_file* f);
+/* glTF: one mesh per (mesh, TRIANGLES primitive) with a float POSITION accessor, one
+ * instance per node that uses it (every scene's node tree, world = parent x local). */
+typedef struct {
+    uint32_t nv, nt;
+    const float* positions;     /* 3 nv */
+    const float* normals;       /* 3 nv: NORMAL as stored, or Foundation calculateNormals' */
+    const float* uvs;           /* 2 nv: TEXCOORD_0 as stored, or NULL */
+    const uint32_t* indices;    /* 3 nt: the primitive's, or sequential */
+} aph_gltf_mesh;
+typedef struct {
+    uint32_t mesh;              /* index into meshes */
+    int32_t material;           /* index into materials; -1: the default material */
+    float o2w[12];              /* row-major 3x4 world transform of the node */
+    int32_t node;               /* glTF node index */
+    const char* materialx;      /* the node's AURORA_materials_materialX file, or NULL */
+} aph_gltf_instance;
+typedef struct {
+    int32_t type;               /* 0 perspective, 1 orthographic */
+    int32_t node;               /* glTF node index */
+    float view[16];             /* row-major world-to-view: inverse of the node's own transform */
+    float yfov, aspect, znear, zfar, xmag, ymag;
+} aph_gltf_camera;
+typedef struct {
+    uint32_t nmesh, ninst, nmat, ncam;
+    const aph_gltf_mesh* meshes;
+    const aph_gltf_instance* instances;
+    const aph_material* materials;   /* base_color, metalness, specular_roughness from
+                                        pbrMetallicRoughness (glTFLoader.cpp:400-414) */
+    const aph_gltf_camera* cameras;
+} aph_gltf_file;
+/* Parses a .gltf (data: or file buffers) or .glb file; *out stays valid until aph_gltf_free. */
+int aph_gltf_load(const char* path, aph_gltf_file** out);
+void aph_gltf_free(aph_gltf_file* f);
 /* Reads a Radiance RGBE (.hdr) image: *rgb is a width x height x 3 float array, top row
  * first, freed with aph_host_free.  APH_E_UNSUPPORTED for other orientations. */
 int aph_hdr_load(const char* path, float** rgb, uint32_t* width, uint32_t* height);

@@ -101,3 +101,54 @@ def test_obj_materials_and_hdr_environment_match_oracle(renderer, teapot_path, t
         gpu, cpu = render_both(renderer, sc, 4, mode=mode)
         compare(gpu, cpu, min_frac=0.98)
         assert gpu[..., 3].mean() > 0.05
+
+
+def test_gltf_scene_with_camera_matches_oracle(renderer, teapot_path, tmp_path):
+    """A binary glTF (aph_gltf_load, Plasma's glTFLoader.cpp translation): the teapot under
+    two nodes of a hierarchy (one mesh, two instances), a ground quad without normals,
+    pbrMetallicRoughness materials and a primitive without one, seen from the file's
+    perspective camera under an HDR sky, in both render modes against the oracle."""
+    from test_gltf import Doc
+
+    base = scenes.teapot_scene(teapot_path)
+    pos, nrm, idx = base.meshes[0]
+    d = Doc()
+    d.j["materials"] = [{"pbrMetallicRoughness": {"baseColorFactor": [0.9, 0.3, 0.2, 1],
+                                                  "metallicFactor": 0.0, "roughnessFactor": 0.4}},
+                        {"pbrMetallicRoughness": {"baseColorFactor": [0.95, 0.8, 0.4, 1],
+                                                  "metallicFactor": 1.0, "roughnessFactor": 0.25}}]
+    tea = d.mesh([{"attributes": {"POSITION": d.accessor(pos.astype(np.float32)),
+                                  "NORMAL": d.accessor(nrm.astype(np.float32))},
+                   "indices": d.accessor(idx.astype(np.uint16).reshape(-1), ctype=5123, typ="SCALAR"),
+                   "material": 0}])
+    tea_gold = d.mesh([{"attributes": {"POSITION": d.j["meshes"][0]["primitives"][0]["attributes"]["POSITION"],
+                                       "NORMAL": d.j["meshes"][0]["primitives"][0]["attributes"]["NORMAL"]},
+                        "indices": d.j["meshes"][0]["primitives"][0]["indices"], "material": 1}])
+    g = np.array([[-6, 0, -6], [6, 0, -6], [6, 0, 6], [-6, 0, 6]], np.float32)
+    floor = d.mesh([{"attributes": {"POSITION": d.accessor(g)},
+                     "indices": d.accessor(np.array([0, 2, 1, 0, 3, 2], np.uint8), ctype=5121, typ="SCALAR")}])
+    s = float(np.sqrt(0.5))
+    left = d.node(root=False, mesh=tea, translation=[-1.6, 0, 0])
+    right = d.node(root=False, mesh=tea_gold, translation=[1.6, 0, 0], rotation=[0, s, 0, s], scale=[1.2, 1.2, 1.2])
+    d.node(children=[left, right], translation=[0, 0.0, 0.5])
+    d.node(mesh=floor)
+    d.j["cameras"] = [{"type": "perspective", "perspective": {"yfov": 0.7, "aspectRatio": 2.0, "znear": 0.1,
+                                                                "zfar": 100}}]
+    # camera at (0, 2, 8) pitched down by atan(1.5 / 8) about x
+    a = -np.arctan2(1.5, 8.0) / 2
+    d.node(camera=0, translation=[0, 2, 8], rotation=[float(np.sin(a)), 0, 0, float(np.cos(a))])
+    d.write_glb(str(tmp_path / "scene.glb"))
+    h, w = 32, 64
+    theta = (np.arange(h) + 0.5) / h * np.pi
+    sky = np.zeros((h, w, 3), np.float32)
+    sky[...] = (0.3 + 0.7 * np.cos(theta / 2) ** 2)[:, None, None] * np.array([0.6, 0.7, 1.0])
+    sky[4:7, 40:44] = [30.0, 28.0, 25.0]
+    write_hdr(tmp_path / "sky.hdr", sky)
+    sc = scenes.gltf_scene(str(tmp_path / "scene.glb"), str(tmp_path / "sky.hdr"), width=128, height=64, spp=4,
+                           camera=0)
+    assert len(sc.meshes) == 3 and len(sc.instances) == 3
+    assert len(sc.materials) == 3  # two glTF materials + the default for the floor
+    for mode in (capi.APH_MODE_MEGAKERNEL, capi.APH_MODE_WAVEFRONT):
+        gpu, cpu = render_both(renderer, sc, 4, mode=mode)
+        compare(gpu, cpu, min_frac=0.98)
+        assert gpu[..., 3].mean() > 0.3  # teapots and floor in view
