@@ -156,6 +156,22 @@ __device__ __forceinline__ uint32_t waveAppend(bool keep, uint32_t* counter)
     return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
 }
 
+// Appends n (0, 1 or 2) entries per active lane with one atomic per wave; returns the
+// lane's first position.
+__device__ __forceinline__ uint32_t waveAppend2(uint32_t n, uint32_t* counter)
+{
+    const uint64_t active = __ballot(1);
+    const uint64_t m1 = __ballot(n >= 1), m2 = __ballot(n >= 2);
+    const int lane = __lane_id();
+    const uint64_t below = (1ull << lane) - 1ull;
+    const int leader = __ffsll((unsigned long long)active) - 1;
+    const uint32_t total = (uint32_t)(__popcll(m1) + __popcll(m2));
+    uint32_t base = 0;
+    if (lane == leader && total) base = atomicAdd(counter, total);
+    base = __shfl(base, leader, 64);
+    return base + (uint32_t)(__popcll(m1 & below) + __popcll(m2 & below));
+}
+
 __device__ __forceinline__ uint32_t waveSum(uint32_t v)
 {
 #pragma unroll
@@ -593,25 +609,47 @@ __device__ __forceinline__ void shadeHit(const DevReq* reqs, const SegInfo* segs
                         shadow(0, so, sd, dc);
                     }
                 }
-                // environment MIS (ShadeFunctions.slang:135-259)
+                // environment MIS (ShadeFunctions.slang:135-259).  Both shadow rays are queued
+                // together, with one atomic per wave.
                 {
                     f2 r = random2D(rng);
                     f3 L;
                     float matPdf;
                     int lobe;
                     f3 fs = sampleMaterial(m, shd, V, r, L, matPdf, lobe);
+                    bool emitMat = false, emitLight = false;
+                    f3 matDir = splat(0.0f), matColor = splat(0.0f), lightDir = splat(0.0f), lightColor = splat(0.0f);
                     if (lobe != kTransmissionLobe) {
                         f3 Le = evalEnvironment(env, L, false, px, py, rq.W, rq.H);
                         if (!isBlack(Le) && !isBlack(fs)) {
                             float lightPdf = env.hasLightTex ? luminance(Le) / env.lightIntegral : 0.25f * kPiInv;
-                            shadow(kSlotMisMat, so, L, Le * misWeight(matPdf, lightPdf) * fs / matPdf);
+                            emitMat = true;
+                            matDir = L;
+                            matColor = Le * misWeight(matPdf, lightPdf) * fs / matPdf;
                         }
                         float lightPdf;
                         Le = sampleEnvironment(env, r, L, lightPdf);
                         int evLobe;
                         fs = evalMaterialPdf(m, shd, V, L, r, matPdf, evLobe);
-                        if (!isBlack(Le) && !isBlack(fs))
-                            shadow(kSlotMisLight, so, L, Le * misWeight(lightPdf, matPdf) * fs / lightPdf);
+                        if (!isBlack(Le) && !isBlack(fs)) {
+                            emitLight = true;
+                            lightDir = L;
+                            lightColor = Le * misWeight(lightPdf, matPdf) * fs / lightPdf;
+                        }
+                    }
+                    uint32_t at = waveAppend2((uint32_t)emitMat + (uint32_t)emitLight, &Q.counts[2]);
+                    if (emitMat) {
+                        P.shColor[(size_t)p * kSlots + kSlotMisMat] = f4(matColor, 0.0f);
+                        Q.shA[at] = f4(so, __uint_as_float(ri));
+                        Q.shB[at] = f4(matDir, __uint_as_float(p * kSlots + (uint32_t)kSlotMisMat));
+                        mask |= (uint8_t)(1u << kSlotMisMat);
+                        at++;
+                    }
+                    if (emitLight) {
+                        P.shColor[(size_t)p * kSlots + kSlotMisLight] = f4(lightColor, 0.0f);
+                        Q.shA[at] = f4(so, __uint_as_float(ri));
+                        Q.shB[at] = f4(lightDir, __uint_as_float(p * kSlots + (uint32_t)kSlotMisLight));
+                        mask |= (uint8_t)(1u << kSlotMisLight);
                     }
                     if (mask & ((1u << kSlotMisMat) | (1u << kSlotMisLight))) f.misLobe((uint32_t)lobe);
                 }
