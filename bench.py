@@ -328,50 +328,57 @@ def c3_cameras(r, builder, seed=3, autotune=True):
             "launch_config_source": "autotuner on these 64 cameras at 1/8 spp" if tuned else "context"}
 
 
-def serving(r, builder, jobs=32, rates=(10.0, 40.0), max_spp=64, seed=5, budget_gib=48.0, width=1920, height=1080):
-    """Config 4 at reduced length: Poisson arrivals of 1080p jobs with spp U{1..max_spp} over the
-    four resident serving scenes (about 10k, 100k, 1M and 5M triangles), HBM-budget admission
-    (scenes, job buffers and the wavefront working set).  Batched (continuous batching in
-    shared wavefronts) vs plain (one job at a time, megakernel).  Both rates saturate the plain
-    form; the batched jobs/s at the higher rate is its saturation throughput."""
+def serving(r, builder, jobs=32, rates=(10.0, 40.0), max_spp=64, seed=5, budget_gib=48.0, width=1920, height=1080,
+            plain_jobs=None):
+    """Config 4 (by default at reduced length): Poisson arrivals of 1080p jobs with spp
+    U{1..max_spp} over the four resident serving scenes (about 10k, 100k, 1M and 5M
+    triangles), HBM-budget admission (scenes, job buffers and the wavefront working set).
+    Batched (continuous batching in shared wavefronts) vs plain (one job at a time,
+    megakernel).  Both default rates saturate the plain form; the batched jobs/s at the
+    higher rate is its saturation throughput.  plain_jobs: replay only the first that many
+    jobs of each trace in the plain form (its rate is then measured on that prefix)."""
     from aurora_amd.session import Server
     scs = scenes.serving_scenes()
     tl = r.load_scenes(scs, builder)
     rng = np.random.default_rng(seed)
     out = {"scenes_triangles": [s.instanced_triangles for s in scs], "jobs": jobs, "resolution": f"{width}x{height}",
-           "spp": f"U{{1..{max_spp}}}", "hbm_budget_gib": budget_gib, "rates": {},
-           "differs_from_config4": f"{jobs} jobs per rate (config 4: 1,024) at rates {list(rates)} /s (config 4: "
-                                   "10-500 /s); every rate above about 2 jobs/s saturates one GPU, so longer traces "
-                                   "only lengthen the queue"}
+           "spp": f"U{{1..{max_spp}}}", "hbm_budget_gib": budget_gib, "rates": {}}
+    if jobs != 1024:
+        out["differs_from_config4"] = (f"{jobs} jobs per rate (config 4: 1,024) at rates {list(rates)} /s (config 4: "
+                                       "10-500 /s); every rate above about 2 jobs/s saturates one GPU, so longer "
+                                       "traces only lengthen the queue")
+    if plain_jobs is not None:
+        out["plain_jobs"] = plain_jobs
     js = []
     for _ in range(jobs):
         k = int(rng.integers(0, 4))
         js.append(Server.job(tl[k], scenes.orbit_frame(rng, width, height, radius=30.0 if k else 14.0), width,
                              height, int(rng.integers(1, max_spp + 1))))
-    samples = sum(j.width * j.height * j.spp for j in js)
     for rate in rates:
         arr = np.cumsum(rng.exponential(1.0 / rate, jobs))
         res = {}
         for name, mode in (("batched", capi.APH_MODE_WAVEFRONT), ("plain", capi.APH_MODE_MEGAKERNEL)):
+            n = jobs if name == "batched" or plain_jobs is None else min(plain_jobs, jobs)
             srv = Server(r, hbm_budget=int(budget_gib * (1 << 30)), mode=mode, keep_results=False)
             r.reset_stats()
             t = time.perf_counter()
-            tickets = srv.replay(js, arr)
+            tickets = srv.replay(js[:n], arr[:n])
             dt = time.perf_counter() - t
             lat = np.array([srv.status(tk).t_done - srv.status(tk).t_arrival for tk in tickets])
             resident = srv.usage()[0]
             wf_bytes, wf_paths = srv.working_set()
             srv.close()
-            res[name] = {"p50_s": round(float(np.percentile(lat, 50)), 3),
+            ns = sum(j.width * j.height * j.spp for j in js[:n])
+            res[name] = {"jobs": n, "p50_s": round(float(np.percentile(lat, 50)), 3),
                          "p99_s": round(float(np.percentile(lat, 99)), 3),
-                         "jobs_per_s": round(jobs / dt, 2), "mrays_s": round(rays_of(r.stats()) / dt / 1e6, 2),
-                         "samples_per_s": round(samples / dt, 1), "wall_s": round(dt, 2),
-                         "last_arrival_s": round(float(arr[-1]), 2)}
+                         "jobs_per_s": round(n / dt, 2), "mrays_s": round(rays_of(r.stats()) / dt / 1e6, 2),
+                         "samples_per_s": round(ns / dt, 1), "wall_s": round(dt, 2),
+                         "last_arrival_s": round(float(arr[n - 1]), 2)}
             if mode == capi.APH_MODE_WAVEFRONT:
                 res[name]["working_set_gib"] = round(wf_bytes / (1 << 30), 2)
                 res[name]["paths_in_flight"] = wf_paths
             log(f"serving rate {rate}/s {name}: {res[name]}")
-        res["batched_vs_plain_throughput"] = round(res["batched"]["jobs_per_s"] / res["plain"]["jobs_per_s"], 2)
+        res["batched_vs_plain_throughput"] = round(res["batched"]["samples_per_s"] / res["plain"]["samples_per_s"], 2)
         out["rates"][str(rate)] = res
     out["resident_scene_bytes"] = int(resident)
     return out
@@ -655,8 +662,9 @@ def main():
                     help="extra measurements after the headline line's frame: literal (config2 at subdiv 5), "
                          "c3 (64 cameras), serving (Poisson replay), rmse (equal-time RMSE), denoise (denoised "
                          "4 spp vs plain 256 spp at 1080p), config5 (24.7M-triangle 120-frame sequence with "
-                         "denoiser and temporal resolve); config5q (not by default: that sequence's frames "
-                         "against 1024-spp references); '' for none")
+                         "denoiser and temporal resolve); not by default: config5q (that sequence's frames "
+                         "against 1024-spp references), c4full (config 4's 1,024 jobs at 10 and 500 /s, the "
+                         "plain form on 64 of them); '' for none")
     ap.add_argument("--tuning", default="",
                     help="wavefront launch configuration 'chunk_paths,trace_blocks_per_cu,shade_blocks_per_cu"
                          "[,trace_refill_lanes[,trace_leaf_wait_lanes]]'; "
@@ -873,6 +881,10 @@ def main():
         log(f"c3: {out['c3_cameras']}")
     if "serving" in sections:
         out["serving"] = serving(r, builder)
+    if "c4full" in sections:
+        # config 4 at its length: 1,024 jobs per rate; the plain form on each trace's first 64 jobs
+        out["serving_config4_full"] = serving(r, builder, jobs=1024, rates=(10.0, 500.0), plain_jobs=64)
+        log(f"config4 full: {out['serving_config4_full']}")
     if "rmse" in sections:
         out["equal_time_rmse"] = equal_time_rmse(r, builder)
         log(f"rmse: {out['equal_time_rmse']}")
