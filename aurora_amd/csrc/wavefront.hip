@@ -156,20 +156,20 @@ __device__ __forceinline__ uint32_t waveAppend(bool keep, uint32_t* counter)
     return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
 }
 
-// Appends n (0, 1 or 2) entries per active lane with one atomic per wave; returns the
+// Appends n (0 to 3) entries per active lane with one atomic per wave; returns the
 // lane's first position.
-__device__ __forceinline__ uint32_t waveAppend2(uint32_t n, uint32_t* counter)
+__device__ __forceinline__ uint32_t waveAppend3(uint32_t n, uint32_t* counter)
 {
     const uint64_t active = __ballot(1);
-    const uint64_t m1 = __ballot(n >= 1), m2 = __ballot(n >= 2);
+    const uint64_t b0 = __ballot(n & 1u), b1 = __ballot(n & 2u);
     const int lane = __lane_id();
     const uint64_t below = (1ull << lane) - 1ull;
     const int leader = __ffsll((unsigned long long)active) - 1;
-    const uint32_t total = (uint32_t)(__popcll(m1) + __popcll(m2));
+    const uint32_t total = (uint32_t)(__popcll(b0) + 2 * __popcll(b1));
     uint32_t base = 0;
     if (lane == leader && total) base = atomicAdd(counter, total);
     base = __shfl(base, leader, 64);
-    return base + (uint32_t)(__popcll(m1 & below) + __popcll(m2 & below));
+    return base + (uint32_t)(__popcll(b0 & below) + 2 * __popcll(b1 & below));
 }
 
 __device__ __forceinline__ uint32_t waveSum(uint32_t v)
@@ -596,7 +596,11 @@ __device__ __forceinline__ void shadeHit(const DevReq* reqs, const SegInfo* segs
                         if (isBlack(lr)) continue;
                         shadow(li, so, sd, lr);
                     }
-                } else if (fp.light_count > 0) {
+                }
+                // the single light's shadow ray is queued with the environment's below
+                bool emitL0 = false;
+                f3 l0Dir = splat(0.0f);
+                if (fp.light_count == 1) {
                     const aph_distant_light& l0 = fp.lights[0];
                     float w0 = luminance(f3{l0.color_intensity[0], l0.color_intensity[1], l0.color_intensity[2]}) *
                                l0.color_intensity[3];
@@ -604,13 +608,14 @@ __device__ __forceinline__ void shadeHit(const DevReq* reqs, const SegInfo* segs
                     random2D(rng);
                     if (w0 > 0.0f) selPdf = w0 / w0;
                     if (l0.color_intensity[3] > 0.0f) {
-                        f3 sd;
-                        f3 dc = shadeDirectional(l0, m, shd, V, rng, sd) / fmaxf(selPdf, 1e-4f);
-                        shadow(0, so, sd, dc);
+                        f3 dc = shadeDirectional(l0, m, shd, V, rng, l0Dir) / fmaxf(selPdf, 1e-4f);
+                        P.shColor[(size_t)p * kSlots] = f4(dc, 0.0f);
+                        mask |= 1u;
+                        emitL0 = true;
                     }
                 }
-                // environment MIS (ShadeFunctions.slang:135-259).  Both shadow rays are queued
-                // together, with one atomic per wave.
+                // environment MIS (ShadeFunctions.slang:135-259).  Its shadow rays and the
+                // single light's are queued together, with one atomic per wave.
                 {
                     f2 r = random2D(rng);
                     f3 L;
@@ -637,7 +642,12 @@ __device__ __forceinline__ void shadeHit(const DevReq* reqs, const SegInfo* segs
                             lightColor = Le * misWeight(lightPdf, matPdf) * fs / lightPdf;
                         }
                     }
-                    uint32_t at = waveAppend2((uint32_t)emitMat + (uint32_t)emitLight, &Q.counts[2]);
+                    uint32_t at = waveAppend3((uint32_t)emitL0 + (uint32_t)emitMat + (uint32_t)emitLight, &Q.counts[2]);
+                    if (emitL0) {
+                        Q.shA[at] = f4(so, __uint_as_float(ri));
+                        Q.shB[at] = f4(l0Dir, __uint_as_float(p * kSlots));
+                        at++;
+                    }
                     if (emitMat) {
                         P.shColor[(size_t)p * kSlots + kSlotMisMat] = f4(matColor, 0.0f);
                         Q.shA[at] = f4(so, __uint_as_float(ri));
