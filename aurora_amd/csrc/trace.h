@@ -12,10 +12,10 @@
 //     U,V,W of one sign (zeros allowed), det != 0, tmin <= t = T/det <= tmax
 //     (an FP32 pre-test only skips triangles this test certainly rejects);
 //   * closest hit = lexicographic minimum of (t, instance, primitive).
-// Boxes only cull, conservatively: BLAS boxes are tight float boxes tested with a
-// robust slab test (one fused multiply-add per plane, origin terms rounded
-// outward, far distances widened to cover the remaining rounding; NaN-ignoring
-// min/max); instance boxes are widened by a pad that bounds the world/object
+// Boxes only cull, conservatively: BLAS boxes are tight float boxes tested with
+// the robust slab test of Ize (JCGT 2(2), 2013): sign-selected planes,
+// NaN-ignoring min/max, far distance widened to cover the rounding of the
+// distances; instance boxes are widened by a pad that bounds the world/object
 // rounding mismatch.
 #pragma once
 
@@ -26,12 +26,11 @@ namespace aph {
 constexpr int kLdsStack = 16;    // stack slots per lane held in LDS
 constexpr int kSpillStack = 112; // deeper slots spill to private (scratch) memory
 constexpr int kStackCap = kLdsStack + kSpillStack;
-// Far-distance widening of the slab test.  A plane's distance fma(p, i, -n) carries
-// the reciprocal's error (v_rcp_f32: 1 ulp <= 2u, u = 2^-24), the fma's rounding and,
-// on the far side, the rounding of the scaled coefficients (i kRobustFar and the
-// offset times kRobustFar), about 5u relative in all besides the absolute terms the
-// offsets' margins cover; near and far together need less than 1 + 11u.  Taken as
-// 1 + 13.4u:
+// Far-distance widening of the slab test.  Each distance (b - o) * inv carries a
+// subtraction and a product rounding (u each, u = 2^-24) and the reciprocal's
+// error (v_rcp_f32: 1 ulp <= 2u), so |relative error| <= gamma4; near and far,
+// plus the rounding of this product, need 1 + 2*gamma5 = 1 + 5.96e-7.  Taken
+// with margin:
 constexpr float kRobustFar = 1.0000008f;
 constexpr float kTRoundUp = 1.0000002f;    // rounds a float-converted double t up
 constexpr uint32_t kMaxSteps = 1u << 22;   // traversal step cap: a corrupt BVH cannot hang a wave
@@ -87,70 +86,61 @@ __device__ __forceinline__ InstTrav gloadInst(const InstTrav* p)
     return t;
 }
 
-// Box-test ray.  A plane p's distance along the ray is (p - o) / d, computed as
-// one fused multiply-add p * i - o * i with i = v_rcp_f32(d) (1 ulp) and the
-// product o * i precomputed per ray, rounded outward: the near plane subtracts
-// n >= o i (+ pad |i|) and the far plane f <= o i (- pad |i|), so neither the
-// rounding of o * i nor the pad of instance boxes makes a distance cross its true
-// value.  The far planes' coefficients carry the factor kRobustFar, which covers
-// the remaining relative errors (the reciprocal's, the fma's and the scaling's).
-// The sign of i selects the near plane of each axis; the node loads pick the
-// planes by byte offset (ox, oy, oz), so no per-child selects are needed.
+// Box-test ray.  Instance boxes are widened by a pad; instead of padding every
+// box, the pad moves the origin once per ray: (lo - pad) - o is computed as
+// lo - oLo with oLo = o + pad', hi + pad - o as hi - oHi with oHi = o - pad',
+// where pad' exceeds pad by more than the rounding of o +- pad'.  Inside a BLAS
+// both origins are o.  The reciprocal direction comes from v_rcp_f32 (1 ulp).
 struct BoxRay {
-    float ix, iy, iz;  // reciprocal direction: near planes
-    float jx, jy, jz;  // reciprocal direction times kRobustFar: far planes
-    float nx, ny, nz;  // near offsets, >= o i + pad |i|
-    float fx, fy, fz;  // far offsets, (<= o i - pad |i|) times kRobustFar
-    uint32_t ox, oy, oz; // byte offset in a Bvh4Node of each axis' near planes (lo or hi)
+    float lx, ly, lz;  // origin for the low planes
+    float hx, hy, hz;  // origin for the high planes
+    float ix, iy, iz;  // reciprocal direction; its sign selects the near plane
 };
 
-// Margins: with u = 2^-24, fl(o i) is within u |o i| of o i, and the far side's
-// scaled coefficients add about 2u |o i| of absolute error, so m = 8u |o i| + 2^-120
-// keeps n above and f below their exact values (the absolute term covers
-// underflow).  The pad term carries 2^-16.6 of relative margin.
-__device__ __forceinline__ void boxAxis(float o, float d, float pad, float& i, float& j, float& n, float& f)
-{
-    i = __builtin_amdgcn_rcpf(d);  // +-0 -> +-inf
-    j = i * kRobustFar;
-    const float oi = o * i;
-    const float m = __builtin_fmaf(fabsf(oi), 4.8e-7f, 7.5e-37f) + (pad > 0.0f ? pad * fabsf(i) * 1.00001f : 0.0f);
-    n = oi + m;
-    f = (oi - m) * kRobustFar;
-}
-
-__device__ __forceinline__ BoxRay makeBoxRayPadded(f3 o, f3 d, float pad)
+__device__ __forceinline__ BoxRay makeBoxRay(f3 o, f3 d)
 {
     BoxRay r;
-    boxAxis(o.x, d.x, pad, r.ix, r.jx, r.nx, r.fx);
-    boxAxis(o.y, d.y, pad, r.iy, r.jy, r.ny, r.fy);
-    boxAxis(o.z, d.z, pad, r.iz, r.jz, r.nz, r.fz);
-    // Bvh4Node: lox, loy, loz at 0, 16, 32; hix, hiy, hiz at 48, 64, 80
-    r.ox = r.ix >= 0.0f ? 0u : 48u;
-    r.oy = r.iy >= 0.0f ? 16u : 64u;
-    r.oz = r.iz >= 0.0f ? 32u : 80u;
+    r.lx = r.hx = o.x; r.ly = r.hy = o.y; r.lz = r.hz = o.z;
+    r.ix = __builtin_amdgcn_rcpf(d.x);  // +-0 -> +-inf
+    r.iy = __builtin_amdgcn_rcpf(d.y);
+    r.iz = __builtin_amdgcn_rcpf(d.z);
     return r;
 }
-__device__ __forceinline__ BoxRay makeBoxRay(f3 o, f3 d) { return makeBoxRayPadded(o, d, 0.0f); }
-// byte offsets of the far planes
-__device__ __forceinline__ uint32_t farX(uint32_t ox) { return ox ^ 48u; }
-__device__ __forceinline__ uint32_t farY(uint32_t oy) { return oy ^ 80u; }
-__device__ __forceinline__ uint32_t farZ(uint32_t oz) { return oz ^ 112u; }
 
-// Robust slab test of one child on its near (n*) and far (f*) planes.  A zero
-// direction component gives infinite or NaN distances; fmaxf/fminf ignore NaN, so
-// such an axis is not limiting, which can only accept more (boxes only cull).
-// tmaxK is the cull distance times kRobustFar.  The caller rejects unused slots
-// by their reference (kEmptyChild), whatever their inverted box gives.
-__device__ __forceinline__ bool slab(const BoxRay& r, float nxp, float nyp, float nzp, float fxp, float fyp, float fzp,
-                                     float tmin, float tmaxK, float& tnear)
+// fl(o + p) >= o + pad holds for p >= (pad + u|o|) / (1 - u), u = 2^-24; the
+// margins below (2^-20 relative on the pad, about 4u on |o|) cover that and the
+// rounding of p itself.
+__device__ __forceinline__ BoxRay makeBoxRayPadded(f3 o, f3 d, float pad)
 {
-    const float nx = __builtin_fmaf(nxp, r.ix, -r.nx), fx = __builtin_fmaf(fxp, r.jx, -r.fx);
-    const float ny = __builtin_fmaf(nyp, r.iy, -r.ny), fy = __builtin_fmaf(fyp, r.jy, -r.fy);
-    const float nz = __builtin_fmaf(nzp, r.iz, -r.nz), fz = __builtin_fmaf(fzp, r.jz, -r.fz);
-    const float tn = fmaxf(fmaxf(nx, ny), fmaxf(nz, tmin));
-    const float tf = fminf(fminf(fx, fy), fminf(fz, tmaxK));
+    BoxRay r = makeBoxRay(o, d);
+    float px = pad * 1.000001f + fabsf(o.x) * 2.4e-7f;
+    float py = pad * 1.000001f + fabsf(o.y) * 2.4e-7f;
+    float pz = pad * 1.000001f + fabsf(o.z) * 2.4e-7f;
+    r.lx = o.x + px; r.hx = o.x - px;
+    r.ly = o.y + py; r.hy = o.y - py;
+    r.lz = o.z + pz; r.hz = o.z - pz;
+    return r;
+}
+
+// Robust slab test.  A zero direction component gives +-inf distances, or NaN
+// when the origin lies on the plane; fmaxf/fminf ignore NaN, which treats that
+// plane as not limiting - the correct answer for a ray inside the slab.  The
+// inverted box of an unused four-child slot (lo = 3e38 > hi = -3e38) gives a
+// near distance of +huge or +inf on every axis for any finite origin, so it is
+// never accepted and its kEmptyChild reference never followed.
+__device__ __forceinline__ bool slab(const BoxRay& r, float4 lo, float4 hi, float tmin, float tmax, float& tnear)
+{
+    float tlx = (lo.x - r.lx) * r.ix, thx = (hi.x - r.hx) * r.ix;
+    float tly = (lo.y - r.ly) * r.iy, thy = (hi.y - r.hy) * r.iy;
+    float tlz = (lo.z - r.lz) * r.iz, thz = (hi.z - r.hz) * r.iz;
+    bool sx = r.ix >= 0.0f, sy = r.iy >= 0.0f, sz = r.iz >= 0.0f;
+    float nx = sx ? tlx : thx, fx = sx ? thx : tlx;
+    float ny = sy ? tly : thy, fy = sy ? thy : tly;
+    float nz = sz ? tlz : thz, fz = sz ? thz : tlz;
+    float tn = fmaxf(fmaxf(nx, ny), fmaxf(nz, tmin));
+    float tf = fminf(fminf(fx, fy), fminf(fz, tmax));
     tnear = tn;
-    return tn <= tf;
+    return tn <= tf * kRobustFar;
 }
 
 // Watertight ray (Woop, Benthin, Wald 2013) built from a float ray: the
@@ -424,25 +414,24 @@ __device__ __forceinline__ bool traceRound(TraceState& s, const TraceScene& sc, 
 {
     while (isInner(s.cur)) {
         if (++s.steps > kMaxSteps) { st.overflow = 1; return true; }
-        const char* nb = reinterpret_cast<const char*>(s.nodes + s.cur);
-        const float4 nx = gload4(nb + s.br.ox), ny = gload4(nb + s.br.oy), nz = gload4(nb + s.br.oz);
-        const float4 fx = gload4(nb + farX(s.br.ox)), fy = gload4(nb + farY(s.br.oy)), fz = gload4(nb + farZ(s.br.oz));
-        const float4 rf = gload4(nb + 96);
+        const float4* np = reinterpret_cast<const float4*>(s.nodes + s.cur);
+        const float4 lx = gload4(np), ly = gload4(np + 1), lz = gload4(np + 2);
+        const float4 hx = gload4(np + 3), hy = gload4(np + 4), hz = gload4(np + 5), rf = gload4(np + 6);
         if (STATS) st.nodes++;
         // Four box tests.  A missed child gets key +inf and a hit one its entry
         // distance capped at the largest float, so after sorting the h hit children come
         // first, nearest first.
         float k0, k1, k2, k3;
-        const float tmaxK = s.tlim * kRobustFar;
+        const bool b0 = slab(s.br, make_float4(lx.x, ly.x, lz.x, 0.0f), make_float4(hx.x, hy.x, hz.x, 0.0f), s.tmin,
+                             s.tlim, k0);
+        const bool b1 = slab(s.br, make_float4(lx.y, ly.y, lz.y, 0.0f), make_float4(hx.y, hy.y, hz.y, 0.0f), s.tmin,
+                             s.tlim, k1);
+        const bool b2 = slab(s.br, make_float4(lx.z, ly.z, lz.z, 0.0f), make_float4(hx.z, hy.z, hz.z, 0.0f), s.tmin,
+                             s.tlim, k2);
+        const bool b3 = slab(s.br, make_float4(lx.w, ly.w, lz.w, 0.0f), make_float4(hx.w, hy.w, hz.w, 0.0f), s.tmin,
+                             s.tlim, k3);
         int32_t r0 = __float_as_int(rf.x), r1 = __float_as_int(rf.y), r2 = __float_as_int(rf.z),
                 r3 = __float_as_int(rf.w);
-        // Slots 0 and 1 always hold a child (each child of the two-child node fills
-        // at least one; a one-primitive tree repeats its leaf under an inverted box),
-        // so only slots 2 and 3 can be unused.
-        const bool b0 = slab(s.br, nx.x, ny.x, nz.x, fx.x, fy.x, fz.x, s.tmin, tmaxK, k0);
-        const bool b1 = slab(s.br, nx.y, ny.y, nz.y, fx.y, fy.y, fz.y, s.tmin, tmaxK, k1);
-        const bool b2 = slab(s.br, nx.z, ny.z, nz.z, fx.z, fy.z, fz.z, s.tmin, tmaxK, k2) && r2 != kEmptyChild;
-        const bool b3 = slab(s.br, nx.w, ny.w, nz.w, fx.w, fy.w, fz.w, s.tmin, tmaxK, k3) && r3 != kEmptyChild;
         const int h = (int)b0 + (int)b1 + (int)b2 + (int)b3;
         if (ANY) {
             // Any hit: the order decides how soon an occluder is found, not whether.

@@ -369,7 +369,7 @@ __device__ __forceinline__ uint32_t traceQueue(uint32_t n, uint32_t* cursor, uin
 
 // ---- extend: closest hit of the live queue ----
 template <bool STATS>
-__global__ __launch_bounds__(kWB) __attribute__((amdgpu_waves_per_eu(4, 8))) void kExtend(const DevReq* reqs, const SegInfo* segs, Paths P, Queues Q, int cur,
+__global__ __launch_bounds__(kWB) void kExtend(const DevReq* reqs, const SegInfo* segs, Paths P, Queues Q, int cur,
                                               uint32_t refill, uint32_t leafWait, DevCounters* ctr)
 {
     __shared__ uint32_t sStack[kLdsStack * kWB];
@@ -772,7 +772,7 @@ __global__ __launch_bounds__(kWB) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
 
 // ---- shadow: any-hit of the shadow queue (traceShadowRay; opaque, or transmitted by non-opaque instances) ----
 template <bool STATS>
-__global__ __launch_bounds__(kWB) __attribute__((amdgpu_waves_per_eu(4, 8))) void kShadow(const DevReq* reqs, const SegInfo* segs, Paths P, Queues Q,
+__global__ __launch_bounds__(kWB) void kShadow(const DevReq* reqs, const SegInfo* segs, Paths P, Queues Q,
                                               uint32_t refill, uint32_t leafWait, DevCounters* ctr)
 {
     __shared__ uint32_t sStack[kLdsStack * kWB];
