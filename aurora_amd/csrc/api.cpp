@@ -859,13 +859,28 @@ static int buildEnvironment(aph_ctx_t c, const aph_environment* e, Environment& 
         std::vector<AliasEntry> alias(n);
         float integral = 0.0f;
         aliasBuild(e->light_rgb, w, h, alias.data(), &integral);
+        // the light sampling records: each entry with its alias's pdf and both radiances
+        std::vector<EnvSample> samples(n);
+        for (size_t i = 0; i < n; i++) {
+            const AliasEntry& ae = alias[i];
+            const size_t j = ae.alias < n ? ae.alias : i;
+            EnvSample& sr = samples[i];
+            sr.alias = ae.alias;
+            sr.prob = ae.prob;
+            sr.pdf = ae.pdf;
+            sr.pdfAlias = alias[j].pdf;
+            sr.r = tex[i].x; sr.g = tex[i].y; sr.b = tex[i].z;
+            sr.ra = tex[j].x; sr.ga = tex[j].y; sr.ba = tex[j].z;
+            sr.pad[0] = sr.pad[1] = 0.0f;
+        }
         HIPCALL(c, out.light.alloc(n * 16), "alloc light image");
-        HIPCALL(c, out.alias.alloc(n * 16), "alloc alias map");
+        HIPCALL(c, out.alias.alloc(n * sizeof(EnvSample)), "alloc light sampling records");
         HIPCALL(c, hipMemcpy(out.light.p, tex.data(), n * 16, hipMemcpyHostToDevice), "copy light image");
-        HIPCALL(c, hipMemcpy(out.alias.p, alias.data(), n * 16, hipMemcpyHostToDevice), "copy alias map");
+        HIPCALL(c, hipMemcpy(out.alias.p, samples.data(), n * sizeof(EnvSample), hipMemcpyHostToDevice),
+                "copy light sampling records");
         d.hasLightTex = 1;
         d.lightTex = out.light.as<float4>();
-        d.alias = out.alias.as<AliasEntry>();
+        d.samples = out.alias.as<EnvSample>();
         d.lw = w;
         d.lh = h;
         d.lightIntegral = integral;

@@ -10,7 +10,7 @@
 //                   InstTrav[m]   64 B: world->object 3x4 + BLAS pointers
 //                   InstShade[m]  object->world 3x4 + shading pointers
 //   materials       DevMaterial[k]
-//   environment     float4 texels, AliasEntry (16 B) per texel
+//   environment     float4 texels, EnvSample (48 B) per texel
 #pragma once
 
 #include "dmath.h"
@@ -126,12 +126,26 @@ struct alignas(16) AliasEntry {
     float pad;
 };
 
+// One record per light-image texel for light sampling: the texel's alias-map entry with
+// its alias's pdf and both texels' radiance, so a light sample reads one record (three
+// 16-byte loads of one line) instead of the entry, possibly its alias's entry, and then a
+// texel, each waiting on the last.
+struct alignas(16) EnvSample {
+    uint32_t alias;
+    float prob;
+    float pdf, pdfAlias;
+    float r, g, b;      // radiance of this texel
+    float ra, ga, ba;   // radiance of the alias texel
+    float pad[2];
+};
+static_assert(sizeof(EnvSample) == 48, "three 16-byte loads");
+
 struct DevEnv {
     f3 lightTop, lightBottom, bgTop, bgBottom;
     float lightXf[16], lightXfInv[16], bgXf[16];
     int bgUseScreen, filter, hasLightTex, hasBgTex;
     const float4* lightTex;
-    const AliasEntry* alias;
+    const EnvSample* samples;  // light sampling records, one per light-image texel
     const float4* bgTex;
     uint32_t lw, lh, bw, bh;
     float lightIntegral;

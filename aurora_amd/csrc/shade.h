@@ -148,16 +148,17 @@ __device__ __forceinline__ f3 sampleEnvironment(const DevEnv& e, f2 r, f3& L, fl
         uint32_t count = e.lw * e.lh;
         uint32_t index = (uint32_t)((float)count * r.x);
         index = index > count - 1 ? count - 1 : index;
-        AliasEntry ent = e.alias[index];
-        if (ent.prob < r.y) {
-            index = ent.alias;
-            ent = e.alias[index];
-        }
-        pdf = ent.pdf;
+        // the alias table's two-way choice, with the chosen texel's pdf and radiance
+        // from the same record
+        const float4* rec = reinterpret_cast<const float4*>(e.samples + index);
+        const float4 a = rec[0], b = rec[1], c = rec[2];
+        const bool useAlias = a.y < r.y;
+        index = useAlias ? __float_as_uint(a.x) : index;
+        pdf = useAlias ? a.w : a.z;
         uint32_t xi = index % e.lw, yi = index / e.lw;
         f3 d = latLongToDir(f2{((float)xi + 0.5f) / (float)e.lw, ((float)yi + 0.5f) / (float)e.lh});
         L = normalize(xf4Dir(e.lightXfInv, d));
-        return texel(e.lightTex, e.lw, (int)xi, (int)yi);
+        return useAlias ? f3{b.w, c.x, c.y} : f3{b.x, b.y, b.z};
     }
     L = sampleUniformDirection(r, pdf);
     L = normalize(xf4Dir(e.lightXfInv, L));

@@ -234,11 +234,10 @@ __global__ __launch_bounds__(kWB) void kGen(const DevReq* reqs, const SegInfo* s
         f.misLobe(kNoSecondaryRay);
         f.primary(true);
         P.flags[p] = f.v;
+        // outDirect and lastIndirect are first written by the first resolve
         float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        P.outDirect[p] = z;
         P.outDiffuse[p] = z;
         P.outGlossy[p] = z;
-        P.lastIndirect[p] = z;
         Q.rayA[0][p] = f4(org, rayTag(sg.req, true));
         Q.rayB[0][p] = f4(dir, __uint_as_float(p));
         started++;
@@ -924,6 +923,8 @@ __device__ __forceinline__ void resolvePath(const DevReq* reqs, const Paths& P, 
         if (ev == kEvHit) f.rayType(f.nextRayType());
         P.lastIndirect[p] = f4(indirect, 0.0f);
     } else {
+        // a segment through a fully transparent surface: nothing is added
+        if (depth == 0) P.outDirect[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         P.lastIndirect[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
     bool done = f.ended();
