@@ -25,6 +25,7 @@
 //     orthographic parameters.
 // A .glb holds a JSON chunk and an optional BIN chunk that a buffer without a uri
 // refers to; a .gltf's buffers are data: URIs (base64) or files next to it.
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -32,6 +33,7 @@
 #include <fstream>
 #include <map>
 #include <memory>
+#include <new>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -62,7 +64,21 @@ struct Json {
         const Json* v = get(key);
         return v && v->kind == Num ? v->n : dflt;
     }
-    int integer(const char* key, int dflt) const { return (int)num(key, dflt); }
+    // an int field; a value outside int's range (or NaN) reads as INT_MIN, which every
+    // index check rejects
+    int integer(const char* key, int dflt) const
+    {
+        const double v = num(key, dflt);
+        return v >= -2147483648.0 && v <= 2147483647.0 ? (int)v : INT_MIN;
+    }
+    // a byte count or offset: absent reads as 0; false unless a whole number in [0, 2^53]
+    bool size(const char* key, size_t& out) const
+    {
+        const double v = num(key, 0.0);
+        if (!(v >= 0.0 && v <= 9007199254740992.0) || v != std::floor(v)) return false;
+        out = (size_t)v;
+        return true;
+    }
     const std::vector<Json>& arr(const char* key) const
     {
         static const std::vector<Json> none;
@@ -430,19 +446,25 @@ public:
         const Json& a = acc[i];
         ctype = a.integer("componentType", 0);
         ncomp = componentCount(a.str("type"));
-        count = (size_t)a.num("count", 0);
+        if (!a.size("count", count)) return false;
         const int bvi = a.integer("bufferView", -1);
         const std::vector<Json>& bvs = doc.arr("bufferViews");
         if (ncomp == 0 || bvi < 0 || bvi >= (int)bvs.size()) return false;
         const Json& bv = bvs[bvi];
         const int bi = bv.integer("buffer", -1);
         if (bi < 0 || bi >= (int)buffers.size()) return false;
-        const size_t off = (size_t)bv.num("byteOffset", 0) + (size_t)a.num("byteOffset", 0);
-        const size_t bvEnd = (size_t)bv.num("byteOffset", 0) + (size_t)bv.num("byteLength", 0);
-        const size_t bs = (size_t)bv.num("byteStride", 0);
+        size_t bvOff, aOff, bvLen, bs;
+        if (!bv.size("byteOffset", bvOff) || !a.size("byteOffset", aOff) || !bv.size("byteLength", bvLen) ||
+            !bv.size("byteStride", bs))
+            return false;
+        // each term is at most 2^53, so the sums cannot wrap
+        const size_t off = bvOff + aOff, bvEnd = bvOff + bvLen;
         stride = bs ? bs : (size_t)ncomp * csize;
         const size_t elem = (size_t)ncomp * csize;
-        if (count > 0 && (off + (count - 1) * stride + elem > bvEnd || bvEnd > buffers[bi].size())) return false;
+        if (off > bvEnd || bvEnd > buffers[bi].size()) return false;
+        // the last element ends inside the view: off + (count - 1) stride + elem <= bvEnd,
+        // tested without forming the product
+        if (count > 0 && (elem > bvEnd - off || (count - 1) > (bvEnd - off - elem) / stride)) return false;
         ptr = buffers[bi].data() + off;
         return true;
     }
@@ -526,10 +548,19 @@ public:
         return result;
     }
 
+    // node visits over all scenes: a file whose nodes share children (not a tree) could
+    // otherwise ask for exponentially many
+    size_t visits = 0;
+    static constexpr size_t kMaxVisits = size_t(1) << 24;
+    static int index(const Json& v)
+    {
+        return v.kind == Json::Num && v.n >= 0.0 && v.n <= 2147483647.0 ? (int)v.n : -1;
+    }
+
     bool node(int ni, const Mat4& parent, int depth)
     {
         const std::vector<Json>& nodes = doc.arr("nodes");
-        if (ni < 0 || ni >= (int)nodes.size() || depth > 64) return false;
+        if (ni < 0 || ni >= (int)nodes.size() || depth > 64 || ++visits > kMaxVisits) return false;
         const Json& nd = nodes[ni];
         const Mat4 world = mul(parent, nodeTransform(nd));
         const int mi = nd.integer("mesh", -1);
@@ -562,7 +593,7 @@ public:
             }
         }
         for (const Json& c : nd.arr("children"))
-            if (!node((int)c.n, world, depth + 1)) return false;
+            if (!node(index(c), world, depth + 1)) return false;
         return true;
     }
 
@@ -572,8 +603,8 @@ public:
         const std::vector<Json>& cams = doc.arr("cameras");
         if (depth > 64) return false;
         for (const Json& r : roots) {
-            const int ni = (int)r.n;
-            if (ni < 0 || ni >= (int)nodes.size()) return false;
+            const int ni = index(r);
+            if (ni < 0 || ni >= (int)nodes.size() || ++visits > kMaxVisits) return false;
             const Json& nd = nodes[ni];
             const int ci = nd.integer("camera", -1);
             if (ci >= 0 && ci < (int)cams.size()) {
@@ -612,10 +643,25 @@ public:
 
 extern "C" {
 
+static int gltfLoad(const char* path, aph_gltf_file** out);
+
 int aph_gltf_load(const char* path, aph_gltf_file** out)
 {
     if (!path || !out) return APH_E_INVALID;
     *out = nullptr;
+    // no exception leaves the C boundary: a file whose arrays do not fit in host memory
+    // reports an allocation failure
+    try {
+        return gltfLoad(path, out);
+    } catch (const std::bad_alloc&) {
+        return APH_E_MEMORY;
+    } catch (...) {
+        return APH_E_INVALID;
+    }
+}
+
+static int gltfLoad(const char* path, aph_gltf_file** out)
+{
     std::vector<uint8_t> bytes;
     if (!readFile(path, bytes)) return APH_E_INVALID;
     Loader L;
@@ -661,7 +707,8 @@ int aph_gltf_load(const char* path, aph_gltf_file** out)
         } else if (!readFile(uriPath(uri, L.base), data)) {
             return APH_E_INVALID;
         }
-        if (data.size() < (size_t)b.num("byteLength", 0)) return APH_E_INVALID;
+        size_t len;
+        if (!b.size("byteLength", len) || data.size() < len) return APH_E_INVALID;
         L.buffers.push_back(std::move(data));
     }
     std::unique_ptr<GltfFile> f(new GltfFile());
@@ -686,7 +733,7 @@ int aph_gltf_load(const char* path, aph_gltf_file** out)
     for (const Json& sc : L.doc.arr("scenes")) {
         if (!L.cameras(sc.arr("nodes"), 0)) return APH_E_INVALID;
         for (const Json& r : sc.arr("nodes"))
-            if (!L.node((int)r.n, identity(), 0)) return APH_E_INVALID;
+            if (!L.node(Loader::index(r), identity(), 0)) return APH_E_INVALID;
     }
     for (size_t k = 0; k < f->pos.size(); k++) {
         aph_gltf_mesh m{};

@@ -318,6 +318,50 @@ def test_malformed_files_are_rejected(tmp_path):
         capi.load_gltf(str(tmp_path / "v3.glb"))
 
 
+def test_hostile_sizes_and_node_graphs_are_rejected(tmp_path):
+    """Counts, offsets and lengths that are negative, fractional, beyond 2^53 or large
+    enough to wrap the bounds arithmetic are errors, as are node indices beyond int; a node
+    graph that shares children (not a tree) fails at the visit cap instead of walking
+    2^64 paths."""
+    d = Doc()
+    d.node(mesh=d.mesh([d.quad()]))
+    text = open(d.write_gltf(str(tmp_path / "g.gltf"))).read()
+    edits = [
+        ("accessors", 0, "count", 2 ** 62),  # (count - 1) * stride wraps a 64-bit size
+        ("accessors", 0, "count", 1.5),
+        ("accessors", 0, "count", -4),
+        ("accessors", 0, "byteOffset", -16),
+        ("accessors", 0, "byteOffset", 1e300),
+        ("bufferViews", 0, "byteStride", 2 ** 61),
+        ("bufferViews", 0, "byteLength", 2 ** 64),
+        ("accessors", 0, "bufferView", 1e20),
+        ("buffers", 0, "byteLength", -1),
+    ]
+    for k, (table, i, key, value) in enumerate(edits):
+        j = json.loads(text)
+        j[table][i][key] = value
+        p = tmp_path / f"e{k}.gltf"
+        p.write_text(json.dumps(j))
+        with pytest.raises(RuntimeError):
+            capi.load_gltf(str(p))
+    j = json.loads(text)
+    j["scenes"][0]["nodes"] = [1e20]
+    (tmp_path / "root.gltf").write_text(json.dumps(j))
+    with pytest.raises(RuntimeError):
+        capi.load_gltf(str(tmp_path / "root.gltf"))
+    # 60 levels, each node listing the next twice: 2^60 paths through 61 nodes
+    j = json.loads(text)
+    j["nodes"] = [{"children": [k + 1, k + 1]} for k in range(60)] + [{"mesh": 0}]
+    j["scenes"][0]["nodes"] = [0]
+    (tmp_path / "dag.gltf").write_text(json.dumps(j))
+    with pytest.raises(RuntimeError):
+        capi.load_gltf(str(tmp_path / "dag.gltf"))
+    # the same chain as a tree loads, with its one instance
+    j["nodes"] = [{"children": [k + 1]} for k in range(60)] + [{"mesh": 0}]
+    (tmp_path / "chain.gltf").write_text(json.dumps(j))
+    assert len(capi.load_gltf(str(tmp_path / "chain.gltf"))["instances"]) == 1
+
+
 def test_json_escapes_and_numbers(tmp_path):
     # string escapes (\\u, surrogate pairs) and exponent numbers parse as JSON defines them
     d = Doc()
