@@ -24,6 +24,7 @@
 //   - HDR: "#?RADIANCE" / "#?RGBE" header, 32-bit_rle_rgbe, "-Y h +X w" orientation,
 //     new-style run-length scanlines or flat pixels; RGBE to float as stb_image does
 //     (mantissa * 2^(exponent - 136), exponent 0 -> black).
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -31,6 +32,7 @@
 #include <fstream>
 #include <map>
 #include <memory>
+#include <new>
 #include <sstream>
 #include <string>
 #include <tuple>
@@ -181,7 +183,34 @@ using aph::angleWeightedNormals;
 
 extern "C" {
 
+static int objLoad(const char* path, aph_obj_file** out);
+static int hdrLoad(const char* path, float** rgb, uint32_t* width, uint32_t* height);
+
+// No exception leaves the C boundary: input whose arrays do not fit in host memory
+// reports an allocation failure.
 int aph_obj_load(const char* path, aph_obj_file** out)
+{
+    try {
+        return objLoad(path, out);
+    } catch (const std::bad_alloc&) {
+        return APH_E_MEMORY;
+    } catch (...) {
+        return APH_E_INVALID;
+    }
+}
+
+int aph_hdr_load(const char* path, float** rgb, uint32_t* width, uint32_t* height)
+{
+    try {
+        return hdrLoad(path, rgb, width, height);
+    } catch (const std::bad_alloc&) {
+        return APH_E_MEMORY;
+    } catch (...) {
+        return APH_E_INVALID;
+    }
+}
+
+static int objLoad(const char* path, aph_obj_file** out)
 {
     if (!path || !out) return APH_E_INVALID;
     *out = nullptr;
@@ -195,9 +224,10 @@ int aph_obj_load(const char* path, aph_obj_file** out)
     int curMat = -1;
     std::string line;
     auto resolve = [](long i, size_t n) -> int {
-        if (i > 0) return (int)(i - 1);
-        if (i < 0) return (int)((long)n + i);
-        return -2;  // 0 is not a valid OBJ index
+        // 0 is not a valid OBJ index; neither is one beyond the arrays' int range
+        if (i > 0) return i <= INT_MAX ? (int)(i - 1) : -2;
+        if (i < 0) return i >= -(long)n ? (int)((long)n + i) : -2;
+        return -2;
     };
     while (std::getline(in, line)) {
         std::istringstream ls(line);
@@ -375,7 +405,7 @@ int aph_obj_load(const char* path, aph_obj_file** out)
 
 void aph_obj_free(aph_obj_file* f) { delete reinterpret_cast<ObjFile*>(f); }
 
-int aph_hdr_load(const char* path, float** rgb, uint32_t* width, uint32_t* height)
+static int hdrLoad(const char* path, float** rgb, uint32_t* width, uint32_t* height)
 {
     if (!path || !rgb || !width || !height) return APH_E_INVALID;
     *rgb = nullptr;
@@ -395,6 +425,15 @@ int aph_hdr_load(const char* path, float** rgb, uint32_t* width, uint32_t* heigh
     if (!rgbe || std::fscanf(f, "-Y %d +X %d", &h, &w) != 2 || w <= 0 || h <= 0 || w > 65535 || h > 65535)
         return APH_E_UNSUPPORTED;  // only the standard orientation and pixel format
     std::fgetc(f);  // the line end
+    // the image must be there before it is allocated: a run-length coded scanline takes
+    // at least its 4-byte header and 2 bytes per run of up to 127 values in each of the
+    // four channels, a flat one 4 bytes per pixel
+    const long at = std::ftell(f);
+    if (at < 0 || std::fseek(f, 0, SEEK_END) != 0) return APH_E_INVALID;
+    const long end = std::ftell(f);
+    if (end < at || std::fseek(f, at, SEEK_SET) != 0) return APH_E_INVALID;
+    const uint64_t minLine = (w >= 8 && w < 32768) ? 4 + 8 * (uint64_t)((w + 126) / 127) : 4 * (uint64_t)w;
+    if ((uint64_t)(end - at) < minLine * (uint64_t)h) return APH_E_INVALID;
     std::vector<float> out((size_t)w * h * 3);
     std::vector<uint8_t> scan((size_t)w * 4);
     for (int y = 0; y < h; y++) {

@@ -184,7 +184,8 @@ def test_mixed_channels_are_rejected(tmp_path):
         capi.load_obj(write(tmp_path, "m.obj", text))
 
 
-@pytest.mark.parametrize("face", ["f 1 2 4", "f 0 1 2", "f 1 2 -4", "f 1/5 2/1 3/1", "f 1//2 2//1 3//1"])
+@pytest.mark.parametrize("face", ["f 1 2 4", "f 0 1 2", "f 1 2 -4", "f 1/5 2/1 3/1", "f 1//2 2//1 3//1",
+                                  "f 1 2 -9223372036854775808", "f 1 2 4294967298", "f 1/-9223372036854775807 2 3"])
 def test_out_of_range_indices_are_rejected(tmp_path, face):
     text = "v 0 0 0\nv 1 0 0\nv 0 1 0\nvt 0 0\nvn 0 0 1\n" + face + "\n"
     with pytest.raises(RuntimeError):
@@ -352,8 +353,11 @@ def test_hdr_run_length_scanlines(tmp_path):
     b"#?RADIANCE\nFORMAT=32-bit_rle_xyze\n\n-Y 1 +X 1\n\x80\x80\x80\x80",   # XYZ pixels
     b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n+Y 1 +X 1\n\x80\x80\x80\x80",   # flipped rows
     b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y 2 +X 1\n\x80\x80\x80\x80",   # truncated
-    b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y 1 +X 8\n\x02\x02\x00\x09",   # scanline width mismatch
-    b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y 1 +X 8\n\x02\x02\x00\x08\x8a\x01",  # run past the end
+    b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y 1 +X 8\n\x02\x02\x00\x09" + bytes(8),   # scanline width mismatch
+    b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y 1 +X 8\n\x02\x02\x00\x08\x8a\x01" + bytes(6),  # run past the end
+    # 65535 x 65535 claimed by 16 bytes: shorter than the smallest coding of that image, so
+    # rejected before 51 GB are allocated
+    b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y 65535 +X 65535\n" + b"\x02\x02\xff\xff" * 4,
 ])
 def test_hdr_malformed_files_are_rejected(tmp_path, data):
     p = tmp_path / "bad.hdr"
