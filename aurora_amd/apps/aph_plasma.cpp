@@ -61,6 +61,15 @@ struct Args {
         return it == opt.end() ? d : it->second;
     }
     double num(const char* k, double d) const { return has(k) ? std::atof(str(k).c_str()) : d; }
+    // a whole-number option in [lo, hi]; anything else is an error
+    static uint32_t checked(const char* k, double v, uint32_t lo, uint32_t hi)
+    {
+        if (!(v >= lo && v <= hi) || v != std::floor(v))
+            throw std::runtime_error(std::string("--") + k + " takes a whole number in [" + std::to_string(lo) +
+                                     ", " + std::to_string(hi) + "]");
+        return (uint32_t)v;
+    }
+    uint32_t count(const char* k, uint32_t d, uint32_t lo, uint32_t hi) const { return checked(k, num(k, d), lo, hi); }
     bool vec(const char* k, std::vector<double>& v, size_t n) const
     {
         if (!has(k)) return false;
@@ -464,8 +473,8 @@ int run(const Args& a)
     uint32_t w = 1280, h = 720;
     std::vector<double> v;
     if (a.vec("dims", v, 2)) {
-        w = (uint32_t)v[0];
-        h = (uint32_t)v[1];
+        w = Args::checked("dims", v[0], 1, 16384);
+        h = Args::checked("dims", v[1], 1, 16384);
     }
     const HostScene scene = loadScene(a.scene);
     Renderer r;
@@ -560,11 +569,13 @@ int run(const Args& a)
     fr.lights[0].cos_radius = std::cos(0.5f * 0.1f);
 
     const int mode = a.str("mode", "wavefront") == "megakernel" ? APH_MODE_MEGAKERNEL : APH_MODE_WAVEFRONT;
-    const uint32_t spp = (uint32_t)a.num("output_spp", 1000);
+    const uint32_t spp = a.count("output_spp", 1000, 1, 1u << 20);
     aph_app::SteadyTimer timer;
     if (a.has("progressive")) {
         // Plasma's interactive update: as many samples per frame as fit the frame time
-        aph_app::SampleCounter<> counter((float)a.num("progressive", 33.3), 200.0f, spp);
+        const double frameMs = a.num("progressive", 33.3);
+        if (!(frameMs > 0.0 && frameMs <= 1e6)) throw std::runtime_error("--progressive takes a frame time in (0, 1e6] ms");
+        aph_app::SampleCounter<> counter((float)frameMs, 200.0f, spp);
         uint32_t frame = 0;
         while (!counter.isComplete()) {
             uint32_t start = 0;
@@ -575,7 +586,7 @@ int run(const Args& a)
             std::printf("frame %u: samples %u..%u\n", frame++, start, start + n);
         }
     } else if (a.has("output_frames")) {
-        const int frames = std::max((int)a.num("output_frames", 32), 1);
+        const int frames = (int)a.count("output_frames", 32, 1, 100000);
         const double deg = a.num("rotate_deg_per_frame", 0.0);
         const std::string dump = a.str("dump_frames_dir");
         for (int k = 0; k < frames; k++) {
