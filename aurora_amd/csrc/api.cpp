@@ -874,13 +874,13 @@ static int buildEnvironment(aph_ctx_t c, const aph_environment* e, Environment& 
             sr.pad[0] = sr.pad[1] = 0.0f;
         }
         HIPCALL(c, out.light.alloc(n * 16), "alloc light image");
-        HIPCALL(c, out.alias.alloc(n * sizeof(EnvSample)), "alloc light sampling records");
+        HIPCALL(c, out.samples.alloc(n * sizeof(EnvSample)), "alloc light sampling records");
         HIPCALL(c, hipMemcpy(out.light.p, tex.data(), n * 16, hipMemcpyHostToDevice), "copy light image");
-        HIPCALL(c, hipMemcpy(out.alias.p, samples.data(), n * sizeof(EnvSample), hipMemcpyHostToDevice),
+        HIPCALL(c, hipMemcpy(out.samples.p, samples.data(), n * sizeof(EnvSample), hipMemcpyHostToDevice),
                 "copy light sampling records");
         d.hasLightTex = 1;
         d.lightTex = out.light.as<float4>();
-        d.samples = out.alias.as<EnvSample>();
+        d.samples = out.samples.as<EnvSample>();
         d.lw = w;
         d.lh = h;
         d.lightIntegral = integral;

@@ -81,9 +81,9 @@ struct Mesh {
 // A lat-long environment resident on the device: light image + alias map,
 // background image, and the DevEnv record that points at them.
 struct Environment {
-    DevBuf light, alias, bg;
+    DevBuf light, samples, bg;  // light image, light sampling records (EnvSample), background
     DevEnv env{};
-    size_t bytes() const { return light.bytes + alias.bytes + bg.bytes; }
+    size_t bytes() const { return light.bytes + samples.bytes + bg.bytes; }
 };
 
 struct Tlas {
